@@ -1,0 +1,86 @@
+/*
+ * kspir_oracle.h — C API of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+ *
+ * The oracle is a plain C++ restatement of the reference KsPIR algorithm for
+ * the online server answer and the client/offline steps that feed it.  It is
+ * used only by tests/, __graft_entry__.smoke() and the cpu_baseline leg of
+ * bench.py, as the checker and as the reported CPU baseline.  The product
+ * (kspir_amd) never links, imports or calls it.
+ *
+ * Array conventions (all uint64_t words hold values mod the stated modulus):
+ *   poly            : [N] with N = 4096
+ *   RLWE            : [2][N]        index 0 = a, 1 = b
+ *   query           : [2][2][N]     [0] = mod Q part, [1] = mod bsMod part; coefficient form
+ *   rgsw            : [4][2][N]     rows 0..3 of (a[i], b[i]) of RGSWCiphertext, NTT form
+ *   baby keys       : [N1/2][6][2][N]   keyMap.at(5^i), i = 1..N1/2; rows 0..2 mod Q, 3..5 mod bsMod
+ *   giant keys      : [N2-1][2][2][N]   keyMap.at(5^(N1*i)), i = 1..N2-1
+ *   packing keys    : [log2 r][4][2][N] keyMap.at(N/i + 1), i = 1, 2, 4, ... < r
+ *   permutations    : int32 [N1][N]
+ *   database words  : [r][N][N2][N1]  (the reference's datacrt layout)
+ */
+#ifndef KSPIR_ORACLE_H
+#define KSPIR_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* constants */
+uint64_t ko_const(int which);   /* 0:N 1:q1 2:q2 3:Q 4:bsMod 5:p 6:Delta 7:rootQ 8:rootBs 9:rootP */
+uint64_t ko_qinv(int r);        /* utils.cpp:118 QInv(r, crtMod) table (0 when absent) */
+void ko_set_threads(int n);     /* OpenMP threads for offline preprocessing and multi-core timing */
+
+/* arithmetic building blocks */
+void ko_ntt_forward(int modsel, uint64_t* a);   /* modsel 0: Q (root crt.h:81), 1: bsMod, 2: p, 3: q1, 4: q2 */
+void ko_ntt_inverse(int modsel, uint64_t* a);
+uint64_t ko_min_root(uint64_t q);               /* minimal primitive 2N-th root of unity mod prime q */
+uint64_t ko_crt_compose(uint64_t x, uint64_t y);            /* crt.cpp:368 */
+void ko_decompose(uint64_t* out, const uint64_t* in, int ell, uint64_t base, uint64_t bg, uint64_t q);
+void ko_decompose_rlwe(uint64_t* out, const uint64_t* in_a, const uint64_t* in_b, int ell,
+                       uint64_t base, uint64_t bg, uint64_t q);
+void ko_decompose_bsgs(uint64_t* out_q, uint64_t* out_bs, const uint64_t* in_q, const uint64_t* in_bs,
+                       int ell, uint64_t base, uint64_t bg);
+void ko_crt_inv_bsgs(uint64_t* out_lohi /*[N][2]*/, const uint64_t* in_q, const uint64_t* in_bs);
+void ko_hexl_ntt_index(int32_t* out);          /* compute_hexl_rotate_indexes first output */
+void ko_query_encode(int32_t* out);            /* utils.cpp:1657 */
+void ko_permutation_matrix(int32_t* out, int n1);   /* utils.cpp:1591 */
+void ko_automorphic_rns(uint64_t* out, const uint64_t* in, int index, uint64_t q);
+/* one call of fastMultiplyQueryByDatabaseDim1InvCRT: out [n2][2N] */
+void ko_db_product(uint64_t* out, const uint64_t* db, const uint64_t* cipherbuf, int n1, int n2);
+
+/* instance generation in test-pir order (srand(seed) first; seed 1 == glibc default) */
+int ko_gen_instance(int r, int n1, unsigned seed, int plant,
+                    uint64_t* secret, int32_t* target, uint64_t* db, uint64_t* query,
+                    uint64_t* rgsw, uint64_t* baby, uint64_t* giant, uint64_t* packing,
+                    int32_t* perms);
+/* client-only generation: srand(seed), secret, then query/RGSW/keys for the given target */
+int ko_gen_client(int r, int n1, unsigned seed, int target_col, int target_packing,
+                  uint64_t* secret, uint64_t* query, uint64_t* rgsw, uint64_t* baby,
+                  uint64_t* giant, uint64_t* packing);
+/* database-only generation: srand(seed); r blocks, optional KAT plant at (col, packing) */
+int ko_gen_db(int r, int n1, unsigned seed, int plant, int target_col, int target_packing,
+              uint64_t* db);
+
+/* online answer: matrix_vector_mul_bsgs_rns_crt_large + r externalProduct + packingRLWEs.
+   Optional outputs (may be NULL):
+     cipherbuf  [N][N1][2]  the reorientCipher buffer (packed q1 | q2<<32)
+     firstdim   [r][2][N]   result[] after the first dimension (NTT form)
+     ext        [r][2][N]   after externalProduct (coefficient form)
+   response [2][N] coefficient form. */
+int ko_answer(int r, int n1, const uint64_t* db, const uint64_t* query, const uint64_t* rgsw,
+              const uint64_t* baby, const uint64_t* giant, const uint64_t* packing,
+              const int32_t* perms, uint64_t* response, uint64_t* cipherbuf,
+              uint64_t* firstdim, uint64_t* ext);
+/* timing split of the last ko_answer call in microseconds:
+   [0] baby steps [1] DB pass [2] giant steps [3] external products [4] packing */
+void ko_last_stage_us(double* out5);
+
+/* decrypt_bsgs_total (twosteps.cpp:388) on a copy of the response */
+int ko_decrypt(int r, const uint64_t* secret, const uint64_t* response, uint64_t* msg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
