@@ -1,0 +1,168 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X KsPIR online answer.
+
+One step = one online server response (test-pir.cpp:107-116: BSGS first
+dimension over the r database blocks, r external products, packing) for one
+query against a 256 MB database (32768 records x 8 KB: N 4096, r 16, N1 128,
+N2 16), with the database, keys and query resident in HBM before timing.
+Inputs are synthetic (uniform residues of the right moduli; the answer's
+cost does not depend on their values).  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--db-mb 256] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N = 4096
+Q1, Q2, BS = 268369921, 249561089, 16760833
+Q = Q1 * Q2
+HBM_PEAK_GBPS = 8000.0
+README_SECONDS_256MB = 0.236635     # README.md:62 online response, 1 query, 256 MB (CPU, HEXL)
+
+
+def synthetic_inputs(r, n1, seed=1234):
+    """uniform residues in the reference layouts (see include/kspir_amd.h)"""
+    rng = np.random.default_rng(seed)
+    n2 = N // 2 // n1
+    words = r * N * N // 2
+    lo = rng.integers(0, Q1, words, dtype=np.uint64)
+    hi = rng.integers(0, Q2, words, dtype=np.uint64)
+    db = (lo | (hi << np.uint64(32))).reshape(r, -1)
+    del lo, hi
+    query = np.empty((2, 2, N), dtype=np.uint64)
+    query[0] = rng.integers(0, Q, (2, N), dtype=np.uint64)
+    query[1] = rng.integers(0, BS, (2, N), dtype=np.uint64)
+    baby = np.empty((n1 // 2, 6, 2, N), dtype=np.uint64)
+    baby[:, :3] = rng.integers(0, Q, (n1 // 2, 3, 2, N), dtype=np.uint64)
+    baby[:, 3:] = rng.integers(0, BS, (n1 // 2, 3, 2, N), dtype=np.uint64)
+    giant = rng.integers(0, Q, (n2 - 1, 2, 2, N), dtype=np.uint64)
+    packing = rng.integers(0, Q, (max(r.bit_length() - 1, 0), 4, 2, N), dtype=np.uint64)
+    rgsw = rng.integers(0, Q, (4, 2, N), dtype=np.uint64)
+    return db, query, baby, giant, packing, rgsw
+
+
+def traffic_from_profile(kernel="k_db_product"):
+    """HBM bytes per launch of the DB-pass kernel from the committed rocprof
+    PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)"""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_traffic.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f).get("bytes_per_launch")
+    return None
+
+
+def cpu_baseline(r, n1, reps):
+    """the CPU oracle (C++ restatement of the reference path) timed on this
+    host: single thread, like test-pir; inputs generated in test-pir order"""
+    from oracle import oracle as O
+    if not os.path.exists(O.LIB):
+        O.build()
+    O.set_threads(min(16, os.cpu_count() or 1))
+    inst = O.gen_instance(r=r, n1=n1, seed=1)
+    O.set_threads(1)
+    O.answer(inst)   # warm-up
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        O.answer(inst)
+        times.append(time.perf_counter() - t0)
+    sec = float(np.median(times))
+    return {"value": round(1.0 / sec, 3), "unit": "queries/s", "cores": 1, "kind": "port",
+            "sample": f"{reps} single-query online answers (median {sec * 1e3:.1f} ms) at r={r}, N1={n1} "
+                      f"on oracle-generated test-pir inputs, seed 1; host {platform.processor() or platform.machine()}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--db-mb", type=int, default=256, help="database size: 16 MB per block (r = MB/16)")
+    ap.add_argument("--n1", type=int, default=128)
+    ap.add_argument("--lat-samples", type=int, default=100)
+    ap.add_argument("--cpu-reps", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the KsPIR answer here runs on a single MI355X "
+              "(multi-GPU sharding by database block is a design note, not built).", file=sys.stderr)
+        sys.exit(2)
+
+    import kspir_amd
+    r, n1 = args.db_mb // 16, args.n1
+    n2 = N // 2 // n1
+    print(f"[bench] generating synthetic inputs r={r} n1={n1}", file=sys.stderr, flush=True)
+    db_h, query, baby, giant, packing, rgsw = synthetic_inputs(r, n1)
+    perms = kspir_amd.compute_permutation_matrix(n1)
+    ctx = kspir_amd.Context(0)
+    db = ctx.database(db_h, r, n1)
+    del db_h
+    keys = ctx.keys(r, n1, baby, giant, packing, rgsw, perms)
+    qbuf = ctx.alloc(query.nbytes)
+    qbuf.upload(query)
+    rbuf = ctx.alloc(2 * N * 8)
+
+    print("[bench] timing", file=sys.stderr, flush=True)
+    ctx.synchronize()
+    total_ms = ctx.time_answer(db, keys, qbuf, rbuf, args.warmup, args.steps)
+    ctx.synchronize()
+    ms_per_step = total_ms / args.steps
+    qps = 1000.0 / ms_per_step
+
+    # per-query latency (submit -> response in HBM), one query in flight
+    lats = [ctx.time_answer(db, keys, qbuf, rbuf, 0, 1) for _ in range(args.lat_samples)]
+    # dominant kernel: the database pass, timed alone with HIP events on the
+    # context stream; algorithmic bytes = 8 B per database word, r*N*N/2 words
+    db_ms = ctx.time_db_pass(db, keys, qbuf, 20)
+    db_bytes = 8 * r * N * (N // 2)
+    achieved = db_bytes / (db_ms * 1e-3) / 1e9
+
+    out = {
+        "metric": "PIR queries/sec + DB GB/s at 256 MB, batched vs plain 1-query; p50/p99 latency",
+        "value": round(qps, 2),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": round(qps / (1.0 / README_SECONDS_256MB), 1) if args.db_mb == 256 and n1 == 128 else None,
+        "dtype": "uint64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain single-query online answer, {args.db_mb} MB database "
+                        f"({r * N // 2} records x 8 KB), N=4096, r={r}, N1={n1}, N2={n2}, 1 query/step",
+            "records": r * N // 2, "record_bytes": 8192, "r": r, "N1": n1, "N2": n2, "queries_per_step": 1,
+        },
+        "db_GBps_raw": round(r * N * (N // 2) * 2 * qps / 1e9, 2),
+        "db_GBps_streamed": round(db_bytes * qps / 1e9, 2),
+        "latency_ms": {"p50": round(float(np.percentile(lats, 50)), 4),
+                       "p99": round(float(np.percentile(lats, 99)), 4)},
+        "roofline": {"kernel": "k_db_product", "bound": "hbm", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                     "traffic": traffic_from_profile(), "algorithmic_bytes": db_bytes,
+                     "avg_launch_ms": round(db_ms, 4)},
+    }
+    keys.close()
+    db.close()
+    qbuf.free()
+    rbuf.free()
+    ctx.close()
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(r, n1, args.cpu_reps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,122 @@
+/*
+ * kspir_amd.h — C ABI of the MI355X (gfx950) KsPIR online-answer library.
+ *
+ * The library replaces the online server response of the reference
+ * (tests/test-pir.cpp:107-116): matrix_vector_mul_bsgs_rns_crt_large
+ * (src/twosteps.h:55), externalProduct (src/lwe.h:292) and packingRLWEs
+ * (src/answer.h:79).  Arguments are the reference's own data, flattened:
+ *
+ *   N = 4096, Q = 268369921 * 249561089 (crtMod), bsMod = 16760833.
+ *   RLWE           uint64_t [2][N]            a then b, words mod Q
+ *   query          uint64_t [2][2][N]         input[0] (mod Q) a,b then input[1] (mod bsMod) a,b,
+ *                                             coefficient form (query_bsgs_rns, twosteps.cpp:279)
+ *   database       uint64_t [r][N][N2][N1]    database_tocrt words (crt.cpp:859-885):
+ *                                             (x mod q1) | (x mod q2) << 32
+ *   baby keys      uint64_t [N1/2][6][2][N]   AutoKeyBSGSRNS::keyMap.at(5^i), i = 1..N1/2
+ *                                             (rows 0..2 mod Q, rows 3..5 mod bsMod), NTT form
+ *   giant keys     uint64_t [N2-1][2][2][N]   AutoKeyBSGSRNS::keyMap.at(5^(N1*i)), i = 1..N2-1
+ *   packing keys   uint64_t [log2 r][4][2][N] AutoKey::keyMap.at(N/i + 1), i = 1,2,4,..,r/2
+ *   rgsw           uint64_t [4][2][N]         RGSWCiphertext rows (a[k], b[k]), NTT form
+ *   permutations   int32_t  [N1][N]           compute_permutation_matrix(N1, N) (utils.cpp:1591)
+ *
+ * Every value is exact modular integer arithmetic; results are word-for-word
+ * those of the reference algorithm on the same inputs.  Functions return 0 on
+ * success or a negative KSPIR_E* code; kspir_last_error() describes the last
+ * failure on the calling thread.  A context is bound to one device and one
+ * HIP stream and is not safe for concurrent use from several threads.
+ */
+#ifndef KSPIR_AMD_H
+#define KSPIR_AMD_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KSPIR_OK 0
+#define KSPIR_EINVAL (-1)   /* bad argument or shape */
+#define KSPIR_EHIP (-2)     /* HIP runtime error (message carries hipGetErrorString) */
+#define KSPIR_ENOMEM (-3)   /* device allocation failed */
+
+#define KSPIR_N 4096
+
+typedef struct kspir_context kspir_context;
+typedef struct kspir_database kspir_database;
+typedef struct kspir_keys kspir_keys;
+
+const char* kspir_last_error(void);
+
+/* compute_permutation_matrix (utils.h:129, utils.cpp:1591-1619): out [n1][N],
+ * row i maps NTT slots so that NTT(tau_{5^i}(a))[s] = NTT(a)[out[i][s]].
+ * Host-only; needs no device. */
+int kspir_compute_permutation_matrix(int32_t* out, int32_t n1);
+
+/* device context: twiddle tables, stream and scratch for one device */
+int kspir_context_create(int device, kspir_context** out);
+void kspir_context_destroy(kspir_context* ctx);
+
+/* Preprocessed database, resident in HBM for the lifetime of the handle.
+ * Replaces the `data` argument of matrix_vector_mul_bsgs_rns_crt_large
+ * (twosteps.h:55, twosteps.cpp:1527); r blocks of N*N/2 words each.
+ * `datacrt` is a host pointer, or a device pointer when on_device != 0. */
+int kspir_database_create(kspir_context* ctx, const uint64_t* datacrt, int32_t r, int32_t n1,
+                          int on_device, kspir_database** out);
+void kspir_database_destroy(kspir_database* db);
+
+/* Per-client evaluation keys, resident in HBM.  Replaces the autoKey and
+ * permutations arguments of matrix_vector_mul_bsgs_rns_crt_large
+ * (twosteps.cpp:1527), the query argument of externalProduct (lwe.cpp:761)
+ * and the autokey argument of packingRLWEs (answer.cpp:800).  Host pointers.
+ * packing may be NULL when r == 1; rgsw may be NULL when only the first
+ * dimension is evaluated. */
+int kspir_keys_create(kspir_context* ctx, int32_t r, int32_t n1, const uint64_t* baby,
+                      const uint64_t* giant, const uint64_t* packing, const uint64_t* rgsw,
+                      const int32_t* permutations, kspir_keys** out);
+void kspir_keys_destroy(kspir_keys* keys);
+
+/* Online answer of test-pir.cpp:107-116: first dimension over the r database
+ * blocks, r external products, packing.  query (host) -> response (host),
+ * coefficient form.  The query is not modified. */
+int kspir_answer(kspir_context* ctx, const kspir_database* db, const kspir_keys* keys,
+                 const uint64_t* query, uint64_t* response);
+
+/* Same, with query and response in device memory, enqueued on `stream`
+ * (a hipStream_t; NULL selects the context's stream).  Asynchronous. */
+int kspir_answer_device(kspir_context* ctx, const kspir_database* db, const kspir_keys* keys,
+                        const uint64_t* query_dev, uint64_t* response_dev, void* stream);
+
+/* matrix_vector_mul_bsgs_rns_crt_large (twosteps.h:55).  result: [r][2][N]
+ * NTT form.  Like the reference, query[0] (a,b mod Q) is replaced by the
+ * modulus-switched query in NTT form (twosteps.cpp:1074-1086); query[1] is
+ * left as given.  Host pointers. */
+int kspir_matrix_vector_mul_bsgs_rns_crt_large(kspir_context* ctx, const kspir_database* db,
+                                               const kspir_keys* keys, uint64_t* query,
+                                               uint64_t* result);
+
+/* externalProduct (lwe.h:292).  result [2][N] is accumulated into, as NTT-
+ * domain words, and returned in coefficient form.  first_dim [2][N] is read
+ * in NTT form when first_dim_is_ntt != 0 (and then overwritten with its
+ * coefficient form, as the reference does), else in coefficient form. */
+int kspir_external_product(kspir_context* ctx, const kspir_keys* keys, uint64_t* result,
+                           uint64_t* first_dim, int first_dim_is_ntt);
+
+/* packingRLWEs (answer.h:79).  rlwes [r][2][N] in coefficient form (NTT form
+ * when rlwes_are_ntt != 0), r = the keys' r; their contents are unspecified
+ * afterwards (the reference also mutates them).  result [2][N] coefficient. */
+int kspir_packing_rlwes(kspir_context* ctx, const kspir_keys* keys, uint64_t* result,
+                        uint64_t* rlwes, int rlwes_are_ntt);
+
+/* Inspection of one answer (test and profiling aid): optional outputs, each
+ * may be NULL.  cipherbuf [N][N1][2] is reorientCipher's buffer
+ * (twosteps.cpp:1386), firstdim [r][2][N] the first-dimension results (NTT
+ * form), ext [r][2][N] the external-product outputs (coefficient form). */
+int kspir_answer_inspect(kspir_context* ctx, const kspir_database* db, const kspir_keys* keys,
+                         const uint64_t* query, uint64_t* response, uint64_t* cipherbuf,
+                         uint64_t* firstdim, uint64_t* ext);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

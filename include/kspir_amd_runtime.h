@@ -1,0 +1,47 @@
+/*
+ * kspir_amd_runtime.h — device-memory, timing and test plumbing of the
+ * MI355X KsPIR library.  These entry points have no counterpart in the
+ * reference (which is host-only); they exist so that callers without a GPU
+ * framework (the ctypes tests, the benchmark) can keep inputs resident in HBM
+ * and time kernels with HIP events on the context's stream.
+ */
+#ifndef KSPIR_AMD_RUNTIME_H
+#define KSPIR_AMD_RUNTIME_H
+
+#include <stdint.h>
+#include <stddef.h>
+#include "kspir_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int kspir_device_count(int* count);
+int kspir_device_alloc(kspir_context* ctx, size_t bytes, void** out);
+int kspir_device_free(kspir_context* ctx, void* ptr);
+int kspir_memcpy_h2d(kspir_context* ctx, void* dst, const void* src, size_t bytes);
+int kspir_memcpy_d2h(kspir_context* ctx, void* dst, const void* src, size_t bytes);
+int kspir_synchronize(kspir_context* ctx);
+/* hipStream_t of the context */
+void* kspir_context_stream(kspir_context* ctx);
+
+/* HIP-event timing on the context stream: `warmup` untimed then `iters`
+ * timed kspir_answer_device calls; *ms = total milliseconds of the timed
+ * calls. */
+int kspir_time_answer(kspir_context* ctx, const kspir_database* db, const kspir_keys* keys,
+                      const uint64_t* query_dev, uint64_t* response_dev, int warmup, int iters,
+                      float* ms);
+
+/* HIP-event timing of the database-pass kernel alone (after one answer has
+ * filled the rotation words): *ms_per_launch = average milliseconds. */
+int kspir_time_db_pass(kspir_context* ctx, const kspir_database* db, const kspir_keys* keys,
+                       const uint64_t* query_dev, int iters, float* ms_per_launch);
+
+/* Test hook: `count` polynomials of N u32 residues in prime p (0: q1,
+ * 1: q2, 2: bsMod), forward (inverse != 0: inverse) NTT in place, host memory. */
+int kspir_test_ntt(kspir_context* ctx, uint32_t* polys, int count, int prime, int inverse);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,234 @@
+"""kspir_amd — MI355X-native KsPIR online server answer (ctypes over the C ABI).
+
+The compute path is libkspir_amd.so (hand-written HIP kernels for gfx950)
+built in-tree by kspir_amd/build.py.  There is no CPU fallback: importing
+this package without the built library, or using it without a GPU, raises.
+Host arrays are numpy uint64 / int32 in the reference's layouts (see
+include/kspir_amd.h).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libkspir_amd.so")
+N = 4096
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_vp = ctypes.c_void_p
+
+
+class KspirError(RuntimeError):
+    pass
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise KspirError(f"{LIB_PATH} is not built; run python kspir_amd/build.py")
+    L = ctypes.CDLL(LIB_PATH)
+    L.kspir_last_error.restype = ctypes.c_char_p
+    L.kspir_context_stream.restype = _vp
+    L.kspir_context_stream.argtypes = [_vp]
+    for name in ("kspir_context_create", "kspir_database_create", "kspir_keys_create", "kspir_answer",
+                 "kspir_answer_device", "kspir_answer_inspect", "kspir_matrix_vector_mul_bsgs_rns_crt_large",
+                 "kspir_external_product", "kspir_packing_rlwes", "kspir_device_alloc", "kspir_device_free",
+                 "kspir_memcpy_h2d", "kspir_memcpy_d2h", "kspir_synchronize", "kspir_time_answer",
+                 "kspir_time_db_pass", "kspir_test_ntt", "kspir_device_count",
+                 "kspir_compute_permutation_matrix"):
+        getattr(L, name).restype = ctypes.c_int
+    L.kspir_context_destroy.argtypes = [_vp]
+    L.kspir_database_destroy.argtypes = [_vp]
+    L.kspir_keys_destroy.argtypes = [_vp]
+    return L
+
+
+lib = _load()
+
+# every symbol include/kspir_amd.h and include/kspir_amd_runtime.h declare
+EXPORTS = (
+    "kspir_last_error", "kspir_context_create", "kspir_context_destroy", "kspir_database_create",
+    "kspir_database_destroy", "kspir_keys_create", "kspir_keys_destroy", "kspir_answer",
+    "kspir_answer_device", "kspir_matrix_vector_mul_bsgs_rns_crt_large", "kspir_external_product",
+    "kspir_packing_rlwes", "kspir_answer_inspect", "kspir_compute_permutation_matrix",
+    "kspir_device_count", "kspir_device_alloc", "kspir_device_free", "kspir_memcpy_h2d", "kspir_memcpy_d2h",
+    "kspir_synchronize", "kspir_context_stream", "kspir_time_answer", "kspir_time_db_pass", "kspir_test_ntt",
+)
+
+
+def _check(rc):
+    if rc != 0:
+        raise KspirError(f"kspir error {rc}: {lib.kspir_last_error().decode()}")
+
+
+def _p64(a):
+    if a is None:
+        return ctypes.cast(None, _u64p)
+    assert a.dtype == np.uint64 and a.flags["C_CONTIGUOUS"], "expected C-contiguous uint64"
+    return a.ctypes.data_as(_u64p)
+
+
+def _p32(a):
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"], "expected C-contiguous int32"
+    return a.ctypes.data_as(_i32p)
+
+
+def compute_permutation_matrix(n1):
+    """utils.cpp:1591 compute_permutation_matrix(n1, N), host-side."""
+    out = np.zeros((n1, N), dtype=np.int32)
+    _check(lib.kspir_compute_permutation_matrix(_p32(out), n1))
+    return out
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    _check(lib.kspir_device_count(ctypes.byref(n)))
+    return n.value
+
+
+class DeviceBuffer:
+    def __init__(self, ctx, nbytes):
+        self.ctx, self.nbytes = ctx, nbytes
+        p = _vp()
+        _check(lib.kspir_device_alloc(ctx.h, ctypes.c_size_t(nbytes), ctypes.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        _check(lib.kspir_memcpy_h2d(self.ctx.h, _vp(self.ptr), arr.ctypes.data_as(_vp), ctypes.c_size_t(arr.nbytes)))
+
+    def download(self, arr):
+        assert arr.flags["C_CONTIGUOUS"] and arr.nbytes <= self.nbytes
+        _check(lib.kspir_memcpy_d2h(self.ctx.h, arr.ctypes.data_as(_vp), _vp(self.ptr), ctypes.c_size_t(arr.nbytes)))
+        return arr
+
+    def free(self):
+        if self.ptr:
+            _check(lib.kspir_device_free(self.ctx.h, _vp(self.ptr)))
+            self.ptr = None
+
+
+class Database:
+    """Preprocessed database resident in HBM (database_tocrt words, r blocks)."""
+
+    def __init__(self, ctx, datacrt, r, n1, on_device=False):
+        self.ctx, self.r, self.n1 = ctx, r, n1
+        h = _vp()
+        ptr = ctypes.cast(_vp(datacrt), _u64p) if on_device else _p64(datacrt)
+        _check(lib.kspir_database_create(ctx.h, ptr, r, n1, int(on_device), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib.kspir_database_destroy(self.h)
+            self.h = None
+
+
+class Keys:
+    """Per-client evaluation keys resident in HBM."""
+
+    def __init__(self, ctx, r, n1, baby, giant, packing, rgsw, perms):
+        self.ctx, self.r, self.n1 = ctx, r, n1
+        h = _vp()
+        giant = giant if giant is not None and giant.size else None
+        packing = packing if packing is not None and packing.size else None
+        _check(lib.kspir_keys_create(ctx.h, r, n1, _p64(baby), _p64(giant), _p64(packing), _p64(rgsw),
+                                     _p32(perms), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib.kspir_keys_destroy(self.h)
+            self.h = None
+
+
+class Context:
+    def __init__(self, device=0):
+        h = _vp()
+        _check(lib.kspir_context_create(device, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib.kspir_context_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def stream(self):
+        return lib.kspir_context_stream(self.h)
+
+    def database(self, datacrt, r, n1, on_device=False):
+        return Database(self, datacrt, r, n1, on_device)
+
+    def keys(self, r, n1, baby, giant, packing, rgsw, perms):
+        return Keys(self, r, n1, baby, giant, packing, rgsw, perms)
+
+    def alloc(self, nbytes):
+        return DeviceBuffer(self, nbytes)
+
+    def synchronize(self):
+        _check(lib.kspir_synchronize(self.h))
+
+    def answer(self, db, keys, query):
+        resp = np.zeros((2, N), dtype=np.uint64)
+        _check(lib.kspir_answer(self.h, db.h, keys.h, _p64(np.ascontiguousarray(query)), _p64(resp)))
+        return resp
+
+    def answer_device(self, db, keys, query_buf, resp_buf, stream=None):
+        _check(lib.kspir_answer_device(self.h, db.h, keys.h, ctypes.cast(_vp(query_buf.ptr), _u64p),
+                                       ctypes.cast(_vp(resp_buf.ptr), _u64p), _vp(stream)))
+
+    def answer_inspect(self, db, keys, query):
+        r, n1 = db.r, db.n1
+        resp = np.zeros((2, N), dtype=np.uint64)
+        cbuf = np.zeros((N, n1, 2), dtype=np.uint64)
+        fd = np.zeros((r, 2, N), dtype=np.uint64)
+        ext = np.zeros((r, 2, N), dtype=np.uint64)
+        _check(lib.kspir_answer_inspect(self.h, db.h, keys.h, _p64(np.ascontiguousarray(query)), _p64(resp),
+                                        _p64(cbuf), _p64(fd), _p64(ext)))
+        return resp, {"cipherbuf": cbuf, "firstdim": fd, "ext": ext}
+
+    def matrix_vector_mul(self, db, keys, query):
+        """matrix_vector_mul_bsgs_rns_crt_large: returns (result [r][2][N], mutated query)."""
+        q = np.ascontiguousarray(query, dtype=np.uint64).copy()
+        res = np.zeros((db.r, 2, N), dtype=np.uint64)
+        _check(lib.kspir_matrix_vector_mul_bsgs_rns_crt_large(self.h, db.h, keys.h, _p64(q), _p64(res)))
+        return res, q
+
+    def external_product(self, keys, result, first_dim, first_dim_is_ntt):
+        res = np.ascontiguousarray(result, dtype=np.uint64).copy()
+        fd = np.ascontiguousarray(first_dim, dtype=np.uint64).copy()
+        _check(lib.kspir_external_product(self.h, keys.h, _p64(res), _p64(fd), int(first_dim_is_ntt)))
+        return res, fd
+
+    def packing_rlwes(self, keys, rlwes, rlwes_are_ntt=False):
+        rl = np.ascontiguousarray(rlwes, dtype=np.uint64).copy()
+        res = np.zeros((2, N), dtype=np.uint64)
+        _check(lib.kspir_packing_rlwes(self.h, keys.h, _p64(res), _p64(rl), int(rlwes_are_ntt)))
+        return res
+
+    def time_answer(self, db, keys, query_buf, resp_buf, warmup, iters):
+        ms = ctypes.c_float(0)
+        _check(lib.kspir_time_answer(self.h, db.h, keys.h, ctypes.cast(_vp(query_buf.ptr), _u64p),
+                                     ctypes.cast(_vp(resp_buf.ptr), _u64p), warmup, iters, ctypes.byref(ms)))
+        return ms.value
+
+    def time_db_pass(self, db, keys, query_buf, iters):
+        ms = ctypes.c_float(0)
+        _check(lib.kspir_time_db_pass(self.h, db.h, keys.h, ctypes.cast(_vp(query_buf.ptr), _u64p), iters,
+                                      ctypes.byref(ms)))
+        return ms.value
+
+    def test_ntt(self, polys, prime, inverse):
+        x = np.ascontiguousarray(polys, dtype=np.uint32).copy()
+        _check(lib.kspir_test_ntt(self.h, x.ctypes.data_as(_u32p), x.shape[0], prime, int(inverse)))
+        return x

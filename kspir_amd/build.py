@@ -1,0 +1,36 @@
+"""Build libkspir_amd.so (HIP kernels + C ABI) for gfx950 with hipcc, in-tree."""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+OUT = os.path.join(HERE, "libkspir_amd.so")
+SOURCES = [os.path.join(HERE, "csrc", "kspir_capi.hip")]
+DEPS = SOURCES + [os.path.join(HERE, "csrc", f) for f in ("kspir_device.h", "kspir_kernels.h")] + [
+    os.path.join(ROOT, "include", f) for f in ("kspir_amd.h", "kspir_amd_runtime.h")]
+
+
+def hipcc():
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def build(verbose=False):
+    cmd = [hipcc(), "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           "-Wno-unused-result", "-Wno-unused-value", "-I", os.path.join(ROOT, "include"),
+           *SOURCES, "-o", OUT + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    os.replace(OUT + ".tmp", OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    print("built", OUT)

@@ -1,0 +1,258 @@
+// kspir_device.h — device-side arithmetic for the KsPIR online answer on
+// CDNA4 (gfx950): exact modular arithmetic on the two 28-bit CRT primes of
+// the ciphertext modulus Q = q1*q2 and the 24-bit baby-step prime bsMod, all
+// in 32-bit lanes (mul_lo / mul_hi), plus a whole-polynomial negacyclic NTT
+// staged in LDS.
+//
+// Residue number system: a value x mod Q is held as (x mod q1, x mod q2); a
+// baby-step value additionally as x mod bsMod.  Every product is either
+//   * Shoup:      x*w mod q with a precomputed w' = floor(w*2^32/q)
+//                 (NTT twiddles and scalar constants), or
+//   * Montgomery: lazily accumulated sum of up to 4 products of a residue with
+//                 a key word stored as key*2^32 mod q, reduced once.
+// Both return canonical residues in [0, q).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace kspir {
+
+constexpr int kN = 4096;       // ring dimension (params.h:16)
+constexpr int kLogN = 12;
+constexpr int kNttThreads = 256;   // threads per workgroup of every NTT kernel
+
+// the three primes of the residue system (crt.h:16-17, :32)
+constexpr uint32_t kQ1 = 268369921u;
+constexpr uint32_t kQ2 = 249561089u;
+constexpr uint32_t kBs = 16760833u;
+constexpr uint64_t kQ = (uint64_t)kQ1 * kQ2;
+
+enum Prime { P_Q1 = 0, P_Q2 = 1, P_BS = 2 };
+
+// per-prime constants and twiddle tables resident in HBM
+struct PrimeTab {
+    uint32_t q;
+    uint32_t mont_qinv;    // -q^-1 mod 2^32
+    uint32_t ninv, ninv_sh;
+    const uint32_t* fw;    // fw[bitrev(i)] = psi^i
+    const uint32_t* fw_sh;
+    const uint32_t* iv;    // iv[bitrev(i)] = psi^-i
+    const uint32_t* iv_sh;
+    const uint32_t* psi_pow;   // psi^e, e in [0, 2N)
+};
+
+struct Tabs {
+    PrimeTab p[3];
+    const uint16_t* slot_exp;   // NTT slot s holds psi^(slot_exp[s]) (odd exponent)
+    const uint16_t* exp_slot;   // exp_slot[e >> 1] = slot holding psi^e
+    // CRT / modulus-switch constants (Shoup pairs)
+    uint32_t q1inv_mod_q2, q1inv_mod_q2_sh;       // q1^-1 mod q2
+    uint32_t bsinv_q[2], bsinv_q_sh[2];          // bsMod^-1 mod q1, q2
+    uint32_t qinv_bs, qinv_bs_sh;                // Q^-1 mod bsMod
+    uint32_t q_mod_bs;                           // Q mod bsMod
+};
+
+__device__ __forceinline__ uint32_t mulhi32(uint32_t a, uint32_t b) { return __umulhi(a, b); }
+
+__device__ __forceinline__ uint32_t add_mod(uint32_t a, uint32_t b, uint32_t q) {
+    uint32_t s = a + b;
+    return s >= q ? s - q : s;
+}
+__device__ __forceinline__ uint32_t sub_mod(uint32_t a, uint32_t b, uint32_t q) {
+    return a >= b ? a - b : a + q - b;
+}
+// x * w mod q, x < 2^32, w < q, wp = floor(w * 2^32 / q)
+__device__ __forceinline__ uint32_t mul_shoup(uint32_t x, uint32_t w, uint32_t wp, uint32_t q) {
+    uint32_t qh = mulhi32(x, wp);
+    uint32_t r = x * w - qh * q;
+    return r >= q ? r - q : r;
+}
+// T * 2^-32 mod q for T < q * 2^32
+__device__ __forceinline__ uint32_t mont_reduce(uint64_t T, uint32_t q, uint32_t qinv) {
+    uint32_t m = (uint32_t)T * qinv;
+    uint64_t t = T + (uint64_t)m * q;
+    uint32_t r = (uint32_t)(t >> 32);
+    return r >= q ? r - q : r;
+}
+__device__ __forceinline__ uint32_t neg_mod(uint32_t a, uint32_t q) { return a ? q - a : 0; }
+
+// value in [0, Q) from its residues (Garner): x1 + q1 * ((x2 - x1) * q1^-1 mod q2)
+__device__ __forceinline__ uint64_t crt_q(uint32_t x1, uint32_t x2, const Tabs& T) {
+    uint32_t x1m = x1 >= kQ2 ? x1 - kQ2 : x1;
+    uint32_t t = mul_shoup(sub_mod(x2, x1m, kQ2), T.q1inv_mod_q2, T.q1inv_mod_q2_sh, kQ2);
+    return (uint64_t)x1 + (uint64_t)kQ1 * t;
+}
+
+// residue mod q of a signed gadget digit r with |r| < q
+__device__ __forceinline__ uint32_t digit_residue(int32_t r, uint32_t q) {
+    return r < 0 ? (uint32_t)(r + (int32_t)q) : (uint32_t)r;
+}
+
+// ---------------------------------------------------------------- NTT in LDS
+// s: 4096 residues in LDS.  Forward: Cooley-Tukey over fw[m + i] (natural
+// order in, slot order out: slot i = a(psi^(2*bitrev(i)+1))).  Inverse:
+// Gentleman-Sande over iv[m + i], then scaled by N^-1.  All values stay in
+// [0, q).  Caller synchronises before and after.
+__device__ __forceinline__ void ntt_fwd_lds(uint32_t* s, const PrimeTab& P) {
+    const uint32_t q = P.q;
+    const int tid = threadIdx.x;
+#pragma unroll 1
+    for (int lg = 0; lg < kLogN; ++lg) {
+        const int m = 1 << lg;
+        const int lt = kLogN - 1 - lg;    // log2 t
+        const int t = 1 << lt;
+#pragma unroll
+        for (int k = 0; k < kN / 2 / kNttThreads; ++k) {
+            const int b = tid + k * kNttThreads;
+            const int i = b >> lt;
+            const int j = b & (t - 1);
+            const int x = (i << (lt + 1)) + j;
+            const uint32_t w = P.fw[m + i], wp = P.fw_sh[m + i];
+            uint32_t u = s[x];
+            uint32_t v = mul_shoup(s[x + t], w, wp, q);
+            s[x] = add_mod(u, v, q);
+            s[x + t] = sub_mod(u, v, q);
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ void ntt_inv_lds(uint32_t* s, const PrimeTab& P) {
+    const uint32_t q = P.q;
+    const int tid = threadIdx.x;
+#pragma unroll 1
+    for (int lg = kLogN - 1; lg >= 0; --lg) {
+        const int m = 1 << lg;
+        const int lt = kLogN - 1 - lg;
+        const int t = 1 << lt;
+#pragma unroll
+        for (int k = 0; k < kN / 2 / kNttThreads; ++k) {
+            const int b = tid + k * kNttThreads;
+            const int i = b >> lt;
+            const int j = b & (t - 1);
+            const int x = (i << (lt + 1)) + j;
+            const uint32_t w = P.iv[m + i], wp = P.iv_sh[m + i];
+            uint32_t u = s[x], v = s[x + t];
+            s[x] = add_mod(u, v, q);
+            s[x + t] = mul_shoup(u + q - v, w, wp, q);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < kN / kNttThreads; ++k) {
+        const int x = threadIdx.x + k * kNttThreads;
+        s[x] = mul_shoup(s[x], P.ninv, P.ninv_sh, q);
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------- register-blocked NTT
+// The same transforms as ntt_fwd_lds / ntt_inv_lds, organised as three
+// passes of four radix-2 stages.  In pass P a thread holds 16 elements whose
+// indices differ only in bits 4P..4P+3 reversed into its register index k
+// (pass 0: e = tid + 256k, pass 1: e = (tid>>4)*256 + (tid&15) + 16k,
+// pass 2: e = 16*tid + k), so the four stages of a pass run in registers and
+// the passes exchange through LDS: two barriers per transform instead of
+// twelve.  LDS words are padded by one per 16 (kPadN words) so that every
+// pass reads and writes without bank conflicts.
+constexpr int kPadN = kN + kN / 16;
+__host__ __device__ __forceinline__ int lds_pad(int e) { return e + (e >> 4); }
+
+__host__ __device__ __forceinline__ int pass_elem(int pass, int tid, int k) {
+    return pass == 0 ? tid + 256 * k : pass == 1 ? ((tid >> 4) << 8) + (tid & 15) + 16 * k : 16 * tid + k;
+}
+
+__host__ __device__ __forceinline__ uint32_t mulhi32_hd(uint32_t a, uint32_t b) {
+    return (uint32_t)(((uint64_t)a * b) >> 32);
+}
+__host__ __device__ __forceinline__ uint32_t mul_shoup_hd(uint32_t x, uint32_t w, uint32_t wp, uint32_t q) {
+    uint32_t qh = mulhi32_hd(x, wp);
+    uint32_t r = x * w - qh * q;
+    return r >= q ? r - q : r;
+}
+
+// forward pass (stages 4P .. 4P+3) on registers v[16] of thread tid
+__host__ __device__ __forceinline__ void ntt_fwd_pass(uint32_t v[16], int pass, int tid, const uint32_t* fw,
+                                                      const uint32_t* fwsh, uint32_t q) {
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+        const int lg = 4 * pass + st;
+        const int lt = kLogN - 1 - lg;
+        const int kb = 8 >> st;   // register-index bit flipped by this stage
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            if (k & kb) continue;
+            const int e = pass_elem(pass, tid, k);
+            const int w_i = (1 << lg) + (e >> (lt + 1));
+            const uint32_t u = v[k];
+            const uint32_t t = mul_shoup_hd(v[k | kb], fw[w_i], fwsh[w_i], q);
+            const uint32_t a = u + t, b = u + q - t;
+            v[k] = a >= q ? a - q : a;
+            v[k | kb] = b >= q ? b - q : b;
+        }
+    }
+}
+
+// inverse pass (stages 4P+3 .. 4P, Gentleman-Sande)
+__host__ __device__ __forceinline__ void ntt_inv_pass(uint32_t v[16], int pass, int tid, const uint32_t* iv,
+                                                      const uint32_t* ivsh, uint32_t q) {
+#pragma unroll
+    for (int st = 3; st >= 0; --st) {
+        const int lg = 4 * pass + st;
+        const int lt = kLogN - 1 - lg;
+        const int kb = 8 >> st;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            if (k & kb) continue;
+            const int e = pass_elem(pass, tid, k);
+            const int w_i = (1 << lg) + (e >> (lt + 1));
+            const uint32_t u = v[k], w = v[k | kb];
+            const uint32_t a = u + w;
+            v[k] = a >= q ? a - q : a;
+            v[k | kb] = mul_shoup_hd(u + q - w, iv[w_i], ivsh[w_i], q);
+        }
+    }
+}
+
+// whole transforms on a padded LDS buffer s[kPadN]; 256 threads; the caller
+// synchronises before (data in s) and after (result in s)
+__device__ __forceinline__ void ntt_fwd_r16(uint32_t* s, const PrimeTab& P) {
+    const int tid = threadIdx.x;
+    uint32_t v[16];
+#pragma unroll 1
+    for (int pass = 0; pass < 3; ++pass) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] = s[lds_pad(pass_elem(pass, tid, k))];
+        ntt_fwd_pass(v, pass, tid, P.fw, P.fw_sh, P.q);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) s[lds_pad(pass_elem(pass, tid, k))] = v[k];
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ void ntt_inv_r16(uint32_t* s, const PrimeTab& P) {
+    const int tid = threadIdx.x;
+    uint32_t v[16];
+#pragma unroll 1
+    for (int pass = 2; pass >= 0; --pass) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] = s[lds_pad(pass_elem(pass, tid, k))];
+        ntt_inv_pass(v, pass, tid, P.iv, P.iv_sh, P.q);
+        if (pass == 0) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) v[k] = mul_shoup_hd(v[k], P.ninv, P.ninv_sh, P.q);
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) s[lds_pad(pass_elem(pass, tid, k))] = v[k];
+        __syncthreads();
+    }
+}
+
+// slot permutation realising the automorphism X -> X^g in the NTT domain:
+// NTT(tau_g(a))[s] = NTT(a)[perm_g(s)]
+__device__ __forceinline__ int perm_slot(int s, uint32_t g, const Tabs& T) {
+    uint32_t e = ((uint32_t)T.slot_exp[s] * g) & (2 * kN - 1);
+    return T.exp_slot[e >> 1];
+}
+
+}  // namespace kspir

@@ -1,0 +1,597 @@
+// kspir_kernels.h — the HIP kernels of the KsPIR online answer (gfx950).
+//
+// Pipeline for one query (r database blocks, BSGS split N1 x N2, N1*N2 = N/2):
+//   baby_prep      decompose the query's a (80-bit CRT, 3 digits), NTT the
+//                  digits and b, modulus-switch the query (rotation 0)
+//   baby_rot       one workgroup per (rotation, a|b): permuted digit x key
+//                  products in 3 primes, INTT mod bsMod, NTT of the bsMod
+//                  part in q1/q2, modulus switch -> packed rotation words
+//   baby_mid_*     second half: decompose the un-switched rotation N1/2-1
+//   db_product     out[i][z] = sum_k db[i][k][z] * ct[k][z] per residue,
+//                  u64 accumulation, one lane per NTT slot
+//   giant_coef / giant_ks     INTT, automorphism, gadget decomposition and
+//                  key switch of the N2-1 giant-step rows of every block
+//   ext_coef / ext_digit / ext_sum    RGSW external product
+//   pack_coef / pack_digit / pack_combine   one level of the packing tree
+//   response       INTT + CRT to the u64 response words
+// All polynomials are u32 residue planes of N = 4096 words.
+#pragma once
+#include "kspir_device.h"
+
+namespace kspir {
+
+// ------------------------------------------------------------------ helpers
+
+// value in [0, Q*bsMod) congruent to xq mod Q and xb mod bsMod (crt_inv_bsgs)
+__device__ __forceinline__ unsigned __int128 crt80(uint64_t xq, uint32_t xb, const Tabs& T) {
+    uint32_t xq_bs = (uint32_t)(xq % kBs);
+    uint32_t t = mul_shoup(sub_mod(xb, xq_bs, kBs), T.qinv_bs, T.qinv_bs_sh, kBs);
+    return (unsigned __int128)xq + (unsigned __int128)kQ * t;
+}
+
+// utils.cpp:1120-1189 decompose_bsgs digit j (ell 3, PP = Bg = 2^20)
+__device__ __forceinline__ int32_t bsgs_digit(unsigned __int128 v, int j) {
+    const unsigned __int128 M = (unsigned __int128)kQ * kBs;
+    const unsigned __int128 offset = (((unsigned __int128)1 << 20) + ((unsigned __int128)1 << 40) +
+                                      ((unsigned __int128)1 << 60)) << 19 | ((unsigned __int128)1 << 19);
+    unsigned __int128 d = v + offset;
+    if (d > M) d -= M;
+    d >>= 20;
+    uint32_t r = (uint32_t)(d >> (20 * j)) & ((1u << 20) - 1);
+    return (int32_t)r - (1 << 19);
+}
+
+// utils.cpp:754-811 decompose (tfhe variant): digit j of x in [0, Q]
+template <int GBITS, int BASEBITS, int ELL>
+__device__ __forceinline__ int32_t tfhe_digit(uint64_t x, int j) {
+    uint64_t offset = 0;
+#pragma unroll
+    for (int i = 0; i < ELL; ++i) offset += (uint64_t)1 << (BASEBITS + i * GBITS);
+    offset <<= (GBITS - 1);
+    offset += BASEBITS ? ((uint64_t)1 << (BASEBITS - 1)) : 0;
+    uint64_t d = x + offset;
+    if (d > kQ) d -= kQ;
+    d >>= BASEBITS;
+    uint32_t r = (uint32_t)(d >> (GBITS * j)) & ((1u << GBITS) - 1);
+    return (int32_t)r - (1 << (GBITS - 1));
+}
+
+// utils.cpp:974-1044 decompose_rlwe (signed digits): digit j of x in [0, Q)
+template <int GBITS, int BASEBITS>
+__device__ __forceinline__ int32_t signed_digit(uint64_t x, int j) {
+    int64_t d = x > (kQ >> 1) ? (int64_t)(x - kQ) : (int64_t)x;
+    d >>= BASEBITS;
+    int32_t r = 0;
+    for (int k = 0; k <= j; ++k) {
+        r = (int32_t)((int64_t)((uint64_t)d << (64 - GBITS)) >> (64 - GBITS));
+        d -= r;
+        d >>= GBITS;
+    }
+    return r;
+}
+
+// coefficient s of tau_g(a), a given by its residues in two planes; returns
+// the canonical value in [0, Q).  ginv = g^-1 mod 2N.
+__device__ __forceinline__ uint64_t auto_coeff(const uint32_t* a1, const uint32_t* a2, int s,
+                                               uint32_t ginv, const Tabs& T) {
+    uint32_t i0 = ((uint32_t)s * ginv) & (2 * kN - 1);
+    bool neg = i0 >= (uint32_t)kN;
+    uint32_t i = neg ? i0 - kN : i0;
+    uint64_t x = crt_q(a1[i], a2[i], T);
+    return neg ? (x ? kQ - x : 0) : x;
+}
+
+// 5^e mod 2N
+__device__ __forceinline__ uint32_t pow5_mod2n(uint32_t e) {
+    uint32_t r = 1, b = 5;
+    while (e) {
+        if (e & 1) r = (r * b) & (2 * kN - 1);
+        b = (b * b) & (2 * kN - 1);
+        e >>= 1;
+    }
+    return r;
+}
+
+// X^x at NTT slot s, in Montgomery form (psi_pow holds psi^e * 2^32 mod q)
+__device__ __forceinline__ uint32_t xpow_m(int s, uint32_t x, const PrimeTab& P, const Tabs& T) {
+    return P.psi_pow[((uint32_t)T.slot_exp[s] * x) & (2 * kN - 1)];
+}
+
+__device__ __forceinline__ uint32_t mont_mul(uint32_t a, uint32_t bm, const PrimeTab& P) {
+    return mont_reduce((uint64_t)a * bm, P.q, P.mont_qinv);
+}
+
+// ------------------------------------------------------------ key uploads
+
+// Q-keys: rows of reference RLWE words [row][2][N] (mod Q) -> [row][2][2 primes][N] Montgomery
+__global__ void __launch_bounds__(256) k_keys_q(uint32_t* __restrict__ out, const uint64_t* __restrict__ in,
+                                                int rows, Tabs T) {
+    const size_t total = (size_t)rows * 2 * kN;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (size_t)gridDim.x * blockDim.x) {
+        const size_t rab = idx / kN;
+        const int s = (int)(idx % kN);
+        const uint64_t v = in[idx];
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const uint32_t q = T.p[p].q;
+            const uint64_t m = ((v % q) << 32) % q;
+            out[(rab * 2 + p) * kN + s] = (uint32_t)m;
+        }
+    }
+}
+
+// baby keys: [nb][6][2][N] (rows 0..2 mod Q, 3..5 mod bsMod) -> [nb][3][2][3 primes][N]
+__global__ void __launch_bounds__(256) k_keys_baby(uint32_t* __restrict__ out, const uint64_t* __restrict__ in,
+                                                   int nb, Tabs T) {
+    const size_t total = (size_t)nb * 3 * 2 * kN;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (size_t)gridDim.x * blockDim.x) {
+        const int s = (int)(idx % kN);
+        const int ab = (int)((idx / kN) % 2);
+        const int j = (int)((idx / (2 * kN)) % 3);
+        const size_t key = idx / (6 * (size_t)kN);
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            const int row = p < 2 ? j : j + 3;
+            const uint64_t v = in[((key * 6 + row) * 2 + ab) * kN + s];
+            const uint32_t q = T.p[p].q;
+            const uint64_t m = ((v % q) << 32) % q;
+            out[(((key * 3 + j) * 2 + ab) * 3 + p) * kN + s] = (uint32_t)m;
+        }
+    }
+}
+
+// database words, reference layout [z][N2][N1] -> slot-minor layout [N2][N1][z]
+__global__ void __launch_bounds__(256) k_db_transpose(uint64_t* __restrict__ out, const uint64_t* __restrict__ in,
+                                                      int rows /* N2*N1 */) {
+    __shared__ uint64_t tile[32][33];
+    const int z0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+    for (int k = ty; k < 32; k += 8) tile[k][tx] = in[(size_t)(z0 + k) * rows + r0 + tx];
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) out[(size_t)(r0 + k) * kN + z0 + tx] = tile[tx][k];
+}
+
+// ---------------------------------------------------------- baby steps
+
+// rotation-0 and first-half digit preparation from the host query.
+// q: [2][2][N] u64 (mod Q part a,b; mod bsMod part a,b), coefficient form.
+// blocks 0..8   digit (j = b/3, prime p = b%3) of decompose_bsgs(a) -> D[j][p]
+// blocks 9..11  NTT of b in prime p -> bhat[p]
+// blocks 12..15 modulus-switched query (a|b, prime) -> ctw[z][0][ab][p]
+__global__ void __launch_bounds__(kNttThreads) k_baby_prep(const uint64_t* __restrict__ q,
+        uint32_t* __restrict__ D, uint32_t* __restrict__ bhat, uint32_t* __restrict__ ctw, int n1, Tabs T) {
+    __shared__ uint32_t s[kPadN];
+    const int b = blockIdx.x;
+    const uint64_t* qa = q;
+    const uint64_t* qb = q + kN;
+    const uint64_t* ba = q + 2 * kN;
+    const uint64_t* bb = q + 3 * kN;
+    if (b < 9) {
+        const int j = b / 3, p = b % 3;
+        const uint32_t qq = T.p[p].q;
+        for (int x = threadIdx.x; x < kN; x += kNttThreads) {
+            unsigned __int128 v = crt80(qa[x], (uint32_t)ba[x], T);
+            s[lds_pad(x)] = digit_residue(bsgs_digit(v, j), qq);
+        }
+        __syncthreads();
+        ntt_fwd_r16(s, T.p[p]);
+        for (int x = threadIdx.x; x < kN; x += kNttThreads) D[(j * 3 + p) * kN + x] = s[lds_pad(x)];
+    } else if (b < 12) {
+        const int p = b - 9;
+        const uint32_t qq = T.p[p].q;
+        for (int x = threadIdx.x; x < kN; x += kNttThreads)
+            s[lds_pad(x)] = p < 2 ? (uint32_t)(qb[x] % qq) : (uint32_t)bb[x];
+        __syncthreads();
+        ntt_fwd_r16(s, T.p[p]);
+        for (int x = threadIdx.x; x < kN; x += kNttThreads) bhat[p * kN + x] = s[lds_pad(x)];
+    } else {
+        // twosteps.cpp:1074-1086: (c_Q - c_bs) * bsMod^-1 mod Q, then NTT
+        const int ab = (b - 12) >> 1, p = (b - 12) & 1;
+        const uint32_t qq = T.p[p].q;
+        const uint64_t* xq = ab ? qb : qa;
+        const uint64_t* xb = ab ? bb : ba;
+        for (int x = threadIdx.x; x < kN; x += kNttThreads) {
+            uint32_t v = sub_mod((uint32_t)(xq[x] % qq), (uint32_t)xb[x], qq);
+            s[lds_pad(x)] = mul_shoup(v, T.bsinv_q[p], T.bsinv_q_sh[p], qq);
+        }
+        __syncthreads();
+        ntt_fwd_r16(s, T.p[p]);
+        for (int x = threadIdx.x; x < kN; x += kNttThreads)
+            ctw[(((size_t)x * n1 + 0) * 2 + ab) * 2 + p] = s[lds_pad(x)];
+    }
+}
+
+// One baby-step rotation per workgroup (blockIdx.x = rot, blockIdx.y = a|b).
+// rotation rot applies tau_{5^(rot+1)}: key index rot, permutation row rot+1.
+//   acc_p = -sum_j D[j][p][perm(s)] * key[rot][j][ab][p][s]       (p = q1,q2,bs)
+//   b-part adds bhat[p][perm(s)]  (tau(b) in the NTT domain)
+//   c = INTT_bs(acc_bs) in [0,bsMod);  out_p = (acc_p - NTT_p(c)) * bsMod^-1
+// out goes to ctw[z][k_base + rot][ab][p].  Rotation `mid` also stores its
+// un-switched NTT-domain accumulators in mid_out[ab][p] (evalAutoSRNSFirst's
+// intermediate).  twosteps.cpp:937-1204.
+__global__ void __launch_bounds__(kNttThreads) k_baby_rot(const uint32_t* __restrict__ D,
+        const uint32_t* __restrict__ bhat, const uint32_t* __restrict__ keys, const uint16_t* __restrict__ perms,
+        uint32_t* __restrict__ ctw, uint32_t* __restrict__ mid_out, int n1, int k_base, int mid, Tabs T) {
+    __shared__ uint32_t sb[kPadN];
+    __shared__ uint32_t s1[kPadN];
+    __shared__ uint32_t s2[kPadN];
+    const int rot = blockIdx.x, ab = blockIdx.y;
+    const uint16_t* pm = perms + (size_t)(rot + 1) * kN;
+    const uint32_t* key = keys + (size_t)rot * 3 * 2 * 3 * kN;
+    constexpr int PER = kN / kNttThreads;
+    uint32_t acc1[PER], acc2[PER];
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int x = threadIdx.x + e * kNttThreads;
+        const int ps = pm[x];
+        uint32_t res[3];
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+            uint64_t t = 0;
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                t += (uint64_t)D[(j * 3 + p) * kN + ps] * key[((j * 2 + ab) * 3 + p) * kN + x];
+            uint32_t r = neg_mod(mont_reduce(t, T.p[p].q, T.p[p].mont_qinv), T.p[p].q);
+            if (ab) r = add_mod(r, bhat[p * kN + ps], T.p[p].q);
+            res[p] = r;
+        }
+        if (rot == mid) {
+#pragma unroll
+            for (int p = 0; p < 3; ++p) mid_out[(ab * 3 + p) * kN + x] = res[p];
+        }
+        acc1[e] = res[0];
+        acc2[e] = res[1];
+        sb[lds_pad(x)] = res[2];
+    }
+    __syncthreads();
+    ntt_inv_r16(sb, T.p[2]);
+    for (int x = threadIdx.x; x < kN; x += kNttThreads) { s1[lds_pad(x)] = sb[lds_pad(x)]; s2[lds_pad(x)] = sb[lds_pad(x)]; }
+    __syncthreads();
+    ntt_fwd_r16(s1, T.p[0]);
+    ntt_fwd_r16(s2, T.p[1]);
+    const int k = k_base + rot;
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int x = threadIdx.x + e * kNttThreads;
+        uint32_t o1 = mul_shoup(sub_mod(acc1[e], s1[lds_pad(x)], kQ1), T.bsinv_q[0], T.bsinv_q_sh[0], kQ1);
+        uint32_t o2 = mul_shoup(sub_mod(acc2[e], s2[lds_pad(x)], kQ2), T.bsinv_q[1], T.bsinv_q_sh[1], kQ2);
+        uint2 w = make_uint2(o1, o2);
+        *reinterpret_cast<uint2*>(ctw + (((size_t)x * n1 + k) * 2 + ab) * 2) = w;
+    }
+}
+
+// second half, step 1: INTT of the intermediate's a in prime p (blockIdx.x)
+__global__ void __launch_bounds__(kNttThreads) k_baby_mid_coef(const uint32_t* __restrict__ mid,
+                                                               uint32_t* __restrict__ coef, Tabs T) {
+    __shared__ uint32_t s[kPadN];
+    const int p = blockIdx.x;
+    for (int x = threadIdx.x; x < kN; x += kNttThreads) s[lds_pad(x)] = mid[(0 * 3 + p) * kN + x];
+    __syncthreads();
+    ntt_inv_r16(s, T.p[p]);
+    for (int x = threadIdx.x; x < kN; x += kNttThreads) coef[p * kN + x] = s[lds_pad(x)];
+}
+
+// second half, step 2: digits of the intermediate's a (blockIdx.x = j*3 + p)
+__global__ void __launch_bounds__(kNttThreads) k_baby_mid_digits(const uint32_t* __restrict__ coef,
+                                                                 uint32_t* __restrict__ D, Tabs T) {
+    __shared__ uint32_t s[kPadN];
+    const int j = blockIdx.x / 3, p = blockIdx.x % 3;
+    const uint32_t qq = T.p[p].q;
+    for (int x = threadIdx.x; x < kN; x += kNttThreads) {
+        uint64_t xq = crt_q(coef[0 * kN + x], coef[1 * kN + x], T);
+        unsigned __int128 v = crt80(xq, coef[2 * kN + x], T);
+        s[lds_pad(x)] = digit_residue(bsgs_digit(v, j), qq);
+    }
+    __syncthreads();
+    ntt_fwd_r16(s, T.p[p]);
+    for (int x = threadIdx.x; x < kN; x += kNttThreads) D[(j * 3 + p) * kN + x] = s[lds_pad(x)];
+}
+
+// ------------------------------------------------------------- database
+
+// crt.cpp:643-768 restated for a slot-minor layout: one lane per NTT slot z.
+// db: [N2][N1][N] packed (x mod q1) | (x mod q2) << 32 for one block
+// ctw: [N][N1][2] packed rotation words (k = 0..N1-1, a|b)
+// out: [N2][2 ab][2 primes][N] residues
+template <int RG>
+__global__ void __launch_bounds__(64) k_db_product(const uint64_t* __restrict__ db, const uint64_t* __restrict__ ctw,
+                                                   uint32_t* __restrict__ out, int n1, int n2,
+                                                   size_t block_words, size_t out_block_words, Tabs T) {
+    const int z = blockIdx.x * 64 + threadIdx.x;
+    const int blk = blockIdx.y;
+    const uint64_t* dbb = db + blk * block_words;
+    uint32_t* ob = out + blk * out_block_words;
+    const uint64_t* ct = ctw + (size_t)z * n1 * 2;
+    for (int r0 = 0; r0 < n2; r0 += RG) {
+        uint64_t a1[RG], a2[RG], b1[RG], b2[RG];
+#pragma unroll
+        for (int i = 0; i < RG; ++i) a1[i] = a2[i] = b1[i] = b2[i] = 0;
+        for (int k = 0; k < n1; ++k) {
+            const uint64_t ca = ct[2 * k], cb = ct[2 * k + 1];
+            const uint32_t ca1 = (uint32_t)ca, ca2 = (uint32_t)(ca >> 32);
+            const uint32_t cb1 = (uint32_t)cb, cb2 = (uint32_t)(cb >> 32);
+#pragma unroll
+            for (int i = 0; i < RG; ++i) {
+                const uint64_t d = __builtin_nontemporal_load(dbb + ((size_t)(r0 + i) * n1 + k) * kN + z);
+                const uint32_t d1 = (uint32_t)d, d2 = (uint32_t)(d >> 32);
+                a1[i] += (uint64_t)d1 * ca1;
+                a2[i] += (uint64_t)d2 * ca2;
+                b1[i] += (uint64_t)d1 * cb1;
+                b2[i] += (uint64_t)d2 * cb2;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < RG; ++i) {
+            uint32_t* o = ob + (size_t)(r0 + i) * 4 * kN + z;
+            o[0 * kN] = (uint32_t)(a1[i] % kQ1);
+            o[1 * kN] = (uint32_t)(a2[i] % kQ2);
+            o[2 * kN] = (uint32_t)(b1[i] % kQ1);
+            o[3 * kN] = (uint32_t)(b2[i] % kQ2);
+        }
+    }
+}
+
+// ------------------------------------------------------------ giant steps
+
+// INTT of row c >= 1 (a part) of every block: grid (N2-1, r, 2 primes)
+// dbout: [r][N2][2][2][N]  ->  gcoef: [r][N2][2][N]
+__global__ void __launch_bounds__(kNttThreads) k_giant_coef(const uint32_t* __restrict__ dbout,
+                                                            uint32_t* __restrict__ gcoef, int n2, Tabs T) {
+    __shared__ uint32_t s[kPadN];
+    const int c = blockIdx.x + 1, blk = blockIdx.y, p = blockIdx.z;
+    const uint32_t* src = dbout + (((size_t)blk * n2 + c) * 4 + p) * kN;
+    for (int x = threadIdx.x; x < kN; x += kNttThreads) s[lds_pad(x)] = src[x];
+    __syncthreads();
+    ntt_inv_r16(s, T.p[p]);
+    uint32_t* dst = gcoef + (((size_t)blk * n2 + c) * 2 + p) * kN;
+    for (int x = threadIdx.x; x < kN; x += kNttThreads) dst[x] = s[lds_pad(x)];
+}
+
+// Key switch of row c (twosteps.cpp:599-677), prime p: grid (N2-1, r, 2)
+//   a' = -(d0*K0.a + d1*K1.a),  b' = perm_g(bhat) - (d0*K0.b + d1*K1.b)
+// with d = decompose(tau_g(a)) (ell 2, PP 2^20, Bg 2^18), g = 5^(N1*c).
+// gks: [r][N2][2 ab][2 p][N] (row 0 unused)
+__global__ void __launch_bounds__(kNttThreads) k_giant_ks(const uint32_t* __restrict__ dbout,
+        const uint32_t* __restrict__ gcoef, const uint32_t* __restrict__ gkeys, uint32_t* __restrict__ gks,
+        int n1, int n2, Tabs T) {
+    __shared__ uint32_t s0[kPadN];
+    __shared__ uint32_t s1[kPadN];
+    const int c = blockIdx.x + 1, blk = blockIdx.y, p = blockIdx.z;
+    const PrimeTab& P = T.p[p];
+    // g = 5^(N1*c); 5 has order N/2 mod 2N, so g^-1 = 5^(N/2 - N1*c mod N/2)
+    const uint32_t e = (uint32_t)(n1 * c) % (kN / 2);
+    const uint32_t g = pow5_mod2n(e);
+    const uint32_t ginv = pow5_mod2n((kN / 2 - e) % (kN / 2));
+    const uint32_t* a1 = gcoef + (((size_t)blk * n2 + c) * 2 + 0) * kN;
+    const uint32_t* a2 = a1 + kN;
+    for (int x = threadIdx.x; x < kN; x += kNttThreads) {
+        uint64_t v = auto_coeff(a1, a2, x, ginv, T);
+        s0[lds_pad(x)] = digit_residue(tfhe_digit<18, 20, 2>(v, 0), P.q);
+        s1[lds_pad(x)] = digit_residue(tfhe_digit<18, 20, 2>(v, 1), P.q);
+    }
+    __syncthreads();
+    ntt_fwd_r16(s0, P);
+    ntt_fwd_r16(s1, P);
+    const uint32_t* key = gkeys + (size_t)(c - 1) * 2 * 2 * 2 * kN;
+    const uint32_t* bh = dbout + (((size_t)blk * n2 + c) * 4 + 2 + p) * kN;
+    uint32_t* o = gks + (((size_t)blk * n2 + c) * 4) * kN;
+    for (int x = threadIdx.x; x < kN; x += kNttThreads) {
+        const uint64_t d0 = s0[lds_pad(x)], d1 = s1[lds_pad(x)];
+        uint64_t ta = d0 * key[((0 * 2 + 0) * 2 + p) * kN + x] + d1 * key[((1 * 2 + 0) * 2 + p) * kN + x];
+        uint64_t tb = d0 * key[((0 * 2 + 1) * 2 + p) * kN + x] + d1 * key[((1 * 2 + 1) * 2 + p) * kN + x];
+        o[(0 * 2 + p) * kN + x] = neg_mod(mont_reduce(ta, P.q, P.mont_qinv), P.q);
+        uint32_t rb = neg_mod(mont_reduce(tb, P.q, P.mont_qinv), P.q);
+        o[(1 * 2 + p) * kN + x] = add_mod(rb, bh[perm_slot(x, g, T)], P.q);
+    }
+}
+
+// --------------------------------------------------- external product
+
+// first-dimension result of each block (row 0 + key-switched rows), NTT form,
+// written to fd (optional) and INTT'd to coefficients: grid (r, 2 ab, 2 p)
+__global__ void __launch_bounds__(kNttThreads) k_ext_coef(const uint32_t* __restrict__ dbout,
+        const uint32_t* __restrict__ gks, uint32_t* __restrict__ fd, uint32_t* __restrict__ ecoef, int n2, Tabs T) {
+    __shared__ uint32_t s[kPadN];
+    const int blk = blockIdx.x, ab = blockIdx.y, p = blockIdx.z;
+    const uint32_t q = T.p[p].q;
+    for (int x = threadIdx.x; x < kN; x += kNttThreads) {
+        uint32_t v = dbout[(((size_t)blk * n2 + 0) * 4 + ab * 2 + p) * kN + x];
+        for (int c = 1; c < n2; ++c) v = add_mod(v, gks[(((size_t)blk * n2 + c) * 4 + ab * 2 + p) * kN + x], q);
+        s[lds_pad(x)] = v;
+        if (fd) fd[(((size_t)blk * 2 + ab) * 2 + p) * kN + x] = v;
+    }
+    __syncthreads();
+    ntt_inv_r16(s, T.p[p]);
+    for (int x = threadIdx.x; x < kN; x += kNttThreads) ecoef[(((size_t)blk * 2 + ab) * 2 + p) * kN + x] = s[lds_pad(x)];
+}
+
+// lwe.cpp:761-808: digit k (0,1 from a; 2,3 from b) of decompose_rlwe
+// (ell 2, PP 2^20, Bg 2^18), NTT, times RGSW row k: grid (r, 4, 2 p)
+// epart: [r][4][2 ab][2 p][N]
+__global__ void __launch_bounds__(kNttThreads) k_ext_digit(const uint32_t* __restrict__ ecoef,
+        const uint32_t* __restrict__ rgsw, uint32_t* __restrict__ epart, Tabs T) {
+    __shared__ uint32_t s[kPadN];
+    const int blk = blockIdx.x, k = blockIdx.y, p = blockIdx.z;
+    const PrimeTab& P = T.p[p];
+    const int src = k >> 1, j = k & 1;
+    const uint32_t* c1 = ecoef + (((size_t)blk * 2 + src) * 2 + 0) * kN;
+    const uint32_t* c2 = c1 + kN;
+    for (int x = threadIdx.x; x < kN; x += kNttThreads) {
+        uint64_t v = crt_q(c1[x], c2[x], T);
+        s[lds_pad(x)] = digit_residue(signed_digit<18, 20>(v, j), P.q);
+    }
+    __syncthreads();
+    ntt_fwd_r16(s, P);
+    uint32_t* o = epart + (((size_t)blk * 4 + k) * 4) * kN;
+    for (int x = threadIdx.x; x < kN; x += kNttThreads) {
+        const uint64_t d = s[lds_pad(x)];
+        o[(0 * 2 + p) * kN + x] = mont_reduce(d * rgsw[((k * 2 + 0) * 2 + p) * kN + x], P.q, P.mont_qinv);
+        o[(1 * 2 + p) * kN + x] = mont_reduce(d * rgsw[((k * 2 + 1) * 2 + p) * kN + x], P.q, P.mont_qinv);
+    }
+}
+
+// ext[blk][ab][p] = sum_k epart[blk][k][ab][p]  (elementwise, 4*r*N lanes)
+__global__ void __launch_bounds__(256) k_ext_sum(const uint32_t* __restrict__ epart, uint32_t* __restrict__ ext,
+                                                 int r, Tabs T) {
+    const size_t total = (size_t)r * 4 * kN;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (size_t)gridDim.x * blockDim.x) {
+        const size_t blk = idx / (4 * kN);
+        const size_t w = idx % (4 * kN);     // (ab, p, x)
+        const int p = (int)((w / kN) & 1);
+        const uint32_t q = T.p[p].q;
+        uint32_t v = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v = add_mod(v, epart[(blk * 4 + k) * 4 * kN + w], q);
+        ext[idx] = v;
+    }
+}
+
+// ---------------------------------------------------------------- packing
+// one level of answer.cpp:800-835 with i pairs (j, i+j), shift X^i,
+// automorphism index g = N/i + 1, key level li.  ext: [r][2 ab][2 p][N].
+
+// (ext[j].a - ext[i+j].a * X^i) -> INTT -> pcoef[j][p]: grid (i, 2 p)
+__global__ void __launch_bounds__(kNttThreads) k_pack_coef(const uint32_t* __restrict__ ext,
+        uint32_t* __restrict__ pcoef, int i, Tabs T) {
+    __shared__ uint32_t s[kPadN];
+    const int j = blockIdx.x, p = blockIdx.y;
+    const PrimeTab& P = T.p[p];
+    const uint32_t* ej = ext + ((size_t)j * 4 + p) * kN;
+    const uint32_t* ei = ext + ((size_t)(i + j) * 4 + p) * kN;
+    for (int x = threadIdx.x; x < kN; x += kNttThreads)
+        s[lds_pad(x)] = sub_mod(ej[x], mont_mul(ei[x], xpow_m(x, i, P, T), P), P.q);
+    __syncthreads();
+    ntt_inv_r16(s, P);
+    for (int x = threadIdx.x; x < kN; x += kNttThreads) pcoef[((size_t)j * 2 + p) * kN + x] = s[lds_pad(x)];
+}
+
+// answer.cpp:576-648 evalAuto: digit k of decompose(tau_g(a)) (ell 4, PP 0,
+// Bg 2^14), NTT, times packing key: grid (i, 4, 2 p);  ppart: [i][4][2][2][N]
+__global__ void __launch_bounds__(kNttThreads) k_pack_digit(const uint32_t* __restrict__ pcoef,
+        const uint32_t* __restrict__ pkey, uint32_t* __restrict__ ppart, uint32_t ginv, Tabs T) {
+    __shared__ uint32_t s[kPadN];
+    const int j = blockIdx.x, k = blockIdx.y, p = blockIdx.z;
+    const PrimeTab& P = T.p[p];
+    const uint32_t* a1 = pcoef + ((size_t)j * 2 + 0) * kN;
+    const uint32_t* a2 = a1 + kN;
+    for (int x = threadIdx.x; x < kN; x += kNttThreads) {
+        uint64_t v = auto_coeff(a1, a2, x, ginv, T);
+        s[lds_pad(x)] = digit_residue(tfhe_digit<14, 0, 4>(v, k), P.q);
+    }
+    __syncthreads();
+    ntt_fwd_r16(s, P);
+    uint32_t* o = ppart + (((size_t)j * 4 + k) * 4) * kN;
+    for (int x = threadIdx.x; x < kN; x += kNttThreads) {
+        const uint64_t d = s[lds_pad(x)];
+        o[(0 * 2 + p) * kN + x] = mont_reduce(d * pkey[((k * 2 + 0) * 2 + p) * kN + x], P.q, P.mont_qinv);
+        o[(1 * 2 + p) * kN + x] = mont_reduce(d * pkey[((k * 2 + 1) * 2 + p) * kN + x], P.q, P.mont_qinv);
+    }
+}
+
+// answer.cpp:720-739 convertTwoRLWEs combine: grid (i, 2 ab, 2 p)
+//   c' = ext[i+j]*X^i + ext[j];  new = -sum_k ppart  (+ perm_g(ext[j].b - ext[i+j].b*X^i) for b)
+//   next[j] = new + c'
+__global__ void __launch_bounds__(kNttThreads) k_pack_combine(const uint32_t* __restrict__ ext,
+        const uint32_t* __restrict__ ppart, uint32_t* __restrict__ next, int i, uint32_t g, Tabs T) {
+    const int j = blockIdx.x, ab = blockIdx.y, p = blockIdx.z;
+    const PrimeTab& P = T.p[p];
+    const uint32_t* ej = ext + ((size_t)j * 4 + ab * 2 + p) * kN;
+    const uint32_t* ei = ext + ((size_t)(i + j) * 4 + ab * 2 + p) * kN;
+    const uint32_t* pp = ppart + ((size_t)j * 4 * 4 + ab * 2 + p) * kN;
+    uint32_t* o = next + ((size_t)j * 4 + ab * 2 + p) * kN;
+    for (int x = threadIdx.x; x < kN; x += kNttThreads) {
+        uint32_t c = add_mod(mont_mul(ei[x], xpow_m(x, i, P, T), P), ej[x], P.q);
+        uint32_t t = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) t = add_mod(t, pp[(size_t)k * 4 * kN + x], P.q);
+        uint32_t v = neg_mod(t, P.q);
+        if (ab) {
+            const int ps = perm_slot(x, g, T);
+            v = add_mod(v, sub_mod(ej[ps], mont_mul(ei[ps], xpow_m(ps, i, P, T), P), P.q), P.q);
+        }
+        o[x] = add_mod(v, c, P.q);
+    }
+}
+
+// INTT of both primes and CRT to u64 words in [0, Q): grid (2 ab, cnt)
+// ext: [cnt][2 ab][2 p][N] NTT domain -> resp: [cnt][2][N] coefficient form
+__global__ void __launch_bounds__(kNttThreads) k_response(const uint32_t* __restrict__ ext,
+                                                          uint64_t* __restrict__ resp, Tabs T) {
+    __shared__ uint32_t s1[kPadN];
+    __shared__ uint32_t s2[kPadN];
+    const int ab = blockIdx.x;
+    const uint32_t* e = ext + (size_t)blockIdx.y * 4 * kN;
+    for (int x = threadIdx.x; x < kN; x += kNttThreads) {
+        s1[lds_pad(x)] = e[(ab * 2 + 0) * kN + x];
+        s2[lds_pad(x)] = e[(ab * 2 + 1) * kN + x];
+    }
+    __syncthreads();
+    ntt_inv_r16(s1, T.p[0]);
+    ntt_inv_r16(s2, T.p[1]);
+    uint64_t* o = resp + (size_t)blockIdx.y * 2 * kN + ab * kN;
+    for (int x = threadIdx.x; x < kN; x += kNttThreads) o[x] = crt_q(s1[lds_pad(x)], s2[lds_pad(x)], T);
+}
+
+// u64 words mod Q [cnt][2][N] -> RNS planes [cnt][2 ab][2 p][N]; forward NTT
+// per plane when ntt != 0: grid (cnt, 2 ab, 2 p)
+__global__ void __launch_bounds__(kNttThreads) k_u64_to_rns(const uint64_t* __restrict__ in,
+                                                            uint32_t* __restrict__ out, int ntt, Tabs T) {
+    __shared__ uint32_t s[kPadN];
+    const int c = blockIdx.x, ab = blockIdx.y, p = blockIdx.z;
+    const uint32_t q = T.p[p].q;
+    const uint64_t* src = in + ((size_t)c * 2 + ab) * kN;
+    uint32_t* dst = out + (((size_t)c * 2 + ab) * 2 + p) * kN;
+    for (int x = threadIdx.x; x < kN; x += kNttThreads) s[lds_pad(x)] = (uint32_t)(src[x] % q);
+    __syncthreads();
+    if (ntt) ntt_fwd_r16(s, T.p[p]);
+    for (int x = threadIdx.x; x < kN; x += kNttThreads) dst[x] = s[lds_pad(x)];
+}
+
+// rotation-0 words of the packed rotation buffer -> u64 [2][N] mod Q
+__global__ void __launch_bounds__(256) k_ctw0_to_u64(const uint32_t* __restrict__ ctw, uint64_t* __restrict__ out,
+                                                     int n1, Tabs T) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= 2 * kN) return;
+    const int ab = idx / kN, x = idx % kN;
+    const uint32_t* w = ctw + (((size_t)x * n1 + 0) * 2 + ab) * 2;
+    out[idx] = crt_q(w[0], w[1], T);
+}
+
+// RLWE words mod Q, RNS planes [cnt][2 ab][2 p][N] -> u64 [cnt][2][N]
+__global__ void __launch_bounds__(256) k_rns_to_u64(const uint32_t* __restrict__ in, uint64_t* __restrict__ out,
+                                                    int cnt, Tabs T) {
+    const size_t total = (size_t)cnt * 2 * kN;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (size_t)gridDim.x * blockDim.x) {
+        const size_t ca = idx / kN;
+        const int x = (int)(idx % kN);
+        out[idx] = crt_q(in[(ca * 2 + 0) * kN + x], in[(ca * 2 + 1) * kN + x], T);
+    }
+}
+
+// a[i] += b[i] over RNS planes [cnt][2 ab][2 p][N]
+__global__ void __launch_bounds__(256) k_add_planes(uint32_t* __restrict__ a, const uint32_t* __restrict__ b,
+                                                    int cnt, Tabs T) {
+    const size_t total = (size_t)cnt * 4 * kN;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (size_t)gridDim.x * blockDim.x) {
+        const int p = (int)((idx / kN) & 1);
+        a[idx] = add_mod(a[idx], b[idx], T.p[p].q);
+    }
+}
+
+// test hooks: batched forward / inverse NTT of polynomials in prime p
+__global__ void __launch_bounds__(kNttThreads) k_ntt_test(uint32_t* __restrict__ data, int p, int inverse, Tabs T) {
+    __shared__ uint32_t s[kPadN];
+    uint32_t* d = data + (size_t)blockIdx.x * kN;
+    for (int x = threadIdx.x; x < kN; x += kNttThreads) s[lds_pad(x)] = d[x];
+    __syncthreads();
+    if (inverse) ntt_inv_r16(s, T.p[p]);
+    else ntt_fwd_r16(s, T.p[p]);
+    for (int x = threadIdx.x; x < kN; x += kNttThreads) d[x] = s[lds_pad(x)];
+}
+
+}  // namespace kspir

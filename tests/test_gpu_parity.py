@@ -1,0 +1,136 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle, word
+for word, on the same seeded inputs.  Exact integer arithmetic: tolerance 0."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N = 4096
+_cache = {}
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import kspir_amd
+    c = kspir_amd.Context(0)
+    yield c
+    c.close()
+
+
+def instance(oracle, r, n1, seed):
+    key = (r, n1, seed)
+    if key not in _cache:
+        inst = oracle.gen_instance(r=r, n1=n1, seed=seed)
+        resp, st = oracle.answer(inst, want_stages=True)
+        _cache.clear()
+        _cache[key] = (inst, resp, st)
+    return _cache[key]
+
+
+def upload(ctx, inst):
+    db = ctx.database(inst.db, inst.r, inst.n1)
+    keys = ctx.keys(inst.r, inst.n1, inst.baby, inst.giant, inst.packing, inst.rgsw, inst.perms)
+    return db, keys
+
+
+@pytest.mark.parametrize("prime,modsel,q", [(0, 3, 268369921), (1, 4, 249561089), (2, 1, 16760833)])
+def test_ntt_matches_oracle(ctx, oracle, prime, modsel, q):
+    say(f"[gpu] ntt prime={prime}")
+    rng = np.random.default_rng(prime)
+    polys = rng.integers(0, q, (4, N), dtype=np.uint32)
+    polys[0, :] = 0
+    polys[1, :] = q - 1
+    fwd = ctx.test_ntt(polys, prime, inverse=False)
+    for i in range(4):
+        assert np.array_equal(fwd[i].astype(np.uint64), oracle.ntt_forward(modsel, polys[i].astype(np.uint64))), i
+    back = ctx.test_ntt(fwd, prime, inverse=True)
+    assert np.array_equal(back, polys)
+
+
+def test_stages_r2(ctx, oracle):
+    """every stage of one answer at r 2, N1 128 against the oracle's"""
+    say("[gpu] stages r=2 n1=128 seed=3")
+    inst, resp, st = instance(oracle, 2, 128, 3)
+    db, keys = upload(ctx, inst)
+    got, gst = ctx.answer_inspect(db, keys, inst.query)
+    assert np.array_equal(gst["cipherbuf"], st["cipherbuf"]), "rotated query words (reorientCipher)"
+    assert np.array_equal(gst["firstdim"], st["firstdim"]), "first-dimension result"
+    assert np.array_equal(gst["ext"], st["ext"]), "external product"
+    assert np.array_equal(got, resp), "response words"
+    msg = oracle.decrypt(2, inst.secret, got)
+    assert np.array_equal(msg, np.arange(1, N + 1, dtype=np.uint64))
+    db.close()
+    keys.close()
+
+
+@pytest.mark.parametrize("r,n1,seed", [(1, 256, 2), (4, 64, 4), (8, 512, 6), (16, 128, 1)])
+def test_answer_bit_exact(ctx, oracle, r, n1, seed):
+    """response words identical to the oracle's; planted record decrypts to [1..N]"""
+    say(f"[gpu] answer r={r} n1={n1} seed={seed}")
+    inst, resp, _ = instance(oracle, r, n1, seed)
+    db, keys = upload(ctx, inst)
+    got = ctx.answer(db, keys, inst.query)
+    assert np.array_equal(got, resp)
+    msg = oracle.decrypt(r, inst.secret, got)
+    assert np.array_equal(msg, np.arange(1, N + 1, dtype=np.uint64))
+    # repeated calls on the resident handles give the same words
+    assert np.array_equal(ctx.answer(db, keys, inst.query), resp)
+    db.close()
+    keys.close()
+
+
+def test_stage_entry_points(ctx, oracle):
+    """matrix_vector_mul_bsgs_rns_crt_large / externalProduct / packingRLWEs
+    called one at a time, as test-pir.cpp:111-115 does"""
+    say("[gpu] stage entry points r=4 n1=128 seed=8")
+    inst, resp, st = instance(oracle, 4, 128, 8)
+    db, keys = upload(ctx, inst)
+    res, q_after = ctx.matrix_vector_mul(db, keys, inst.query)
+    assert np.array_equal(res, st["firstdim"])
+    # input[0] becomes the modulus-switched query in NTT form (twosteps.cpp:1074-1086)
+    cb = st["cipherbuf"][:, 0, :]
+    lo, hi = (cb & np.uint64(0xffffffff)), (cb >> np.uint64(32))
+    want = np.array([[int(oracle.lib().ko_crt_compose(int(lo[z, ab]), int(hi[z, ab]))) for z in range(N)]
+                     for ab in range(2)], dtype=np.uint64)
+    assert np.array_equal(q_after[0], want)
+    assert np.array_equal(q_after[1], inst.query[1])
+    ext = []
+    for j in range(inst.r):
+        e, fd_coef = ctx.external_product(keys, np.zeros((2, N), dtype=np.uint64), res[j], True)
+        assert np.array_equal(e, st["ext"][j]), j
+        # like the reference, the first-dimension input comes back in coefficient form
+        assert np.array_equal(fd_coef[0], oracle.ntt_inverse(0, res[j][0]))
+        assert np.array_equal(fd_coef[1], oracle.ntt_inverse(0, res[j][1]))
+        ext.append(e)
+    out = ctx.packing_rlwes(keys, np.stack(ext))
+    assert np.array_equal(out, resp)
+    db.close()
+    keys.close()
+
+
+def test_keys_and_db_shape_checks(ctx, oracle):
+    """mismatched handles and bad shapes are rejected with an error, not run"""
+    import kspir_amd
+    say("[gpu] shape checks")
+    inst, _, _ = instance(oracle, 4, 128, 8)
+    with pytest.raises(kspir_amd.KspirError):
+        ctx.database(inst.db, 3, 128)
+    with pytest.raises(kspir_amd.KspirError):
+        ctx.database(inst.db, 4, 96)
+    bad = inst.perms.copy()
+    bad[1, 5] = N
+    with pytest.raises(kspir_amd.KspirError):
+        ctx.keys(4, 128, inst.baby, inst.giant, inst.packing, inst.rgsw, bad)
+    db = ctx.database(inst.db[:2].copy(), 2, 128)
+    keys = ctx.keys(4, 128, inst.baby, inst.giant, inst.packing, inst.rgsw, inst.perms)
+    with pytest.raises(kspir_amd.KspirError):
+        ctx.answer(db, keys, inst.query)
+    db.close()
+    keys.close()

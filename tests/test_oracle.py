@@ -1,0 +1,290 @@
+"""Pins of the CPU oracle (oracle/kspir_oracle.cpp) against the reference's own
+known answers and against what the mathematics fixes.  CPU only."""
+import numpy as np
+import pytest
+
+N = 4096
+
+
+def bitrev(x, bits=12):
+    return int(f"{x:0{bits}b}"[::-1], 2)
+
+
+# ---------------------------------------------------------------- constants
+
+def test_parameter_constants(oracle):
+    O = oracle
+    q1, q2, Q, bs, p, delta = O.Q1, O.Q2, O.Q, O.BS, O.P, O.DELTA
+    assert (q1, q2, bs, p) == (268369921, 249561089, 16760833, 65537)
+    assert Q == 66974689739603969
+    assert delta == round(Q / p) == (Q + p // 2) // p        # params.h:98 bsgsDelta
+    bs_inv_q = 51309461009346113                              # crt.h:33 bsModInv
+    assert bs_inv_q * bs % Q == 1
+    assert 3920321 * Q % bs == 1                              # crt.h:34 crtModInv
+    # crt.h:60-61: 2^128 mod (Q * bsMod) used by compute_mult_qinvqi_bsgs
+    assert (54824 << 63) + 1764040975123512121 == pow(2, 128, Q * bs)
+    # crt.cpp:25-28 Barrett / CRT constants
+    assert 163640210 * q2 % q1 == 1 and 97389680 * q1 % q2 == 1
+    assert (275 << 64) + 7906011006380390721 == (1 << 128) // Q
+    assert 68736257792 == (1 << 64) // q1 and 73916747789 == (1 << 64) // q2   # crt.h:20-21
+
+
+def test_qinv_table(oracle):
+    # utils.cpp:172-197: QInv(r, crtMod) * r == 1 mod Q where the table has an entry
+    for r in (1, 2, 4, 8, 16, 32, 64, 256, 512, 4096):
+        v = int(oracle.lib().ko_qinv(r))
+        assert v * r % oracle.Q == 1, r
+    for r in (128, 1024, 2048):   # absent from the table: the reference returns 0 (cannot decrypt)
+        assert int(oracle.lib().ko_qinv(r)) == 0
+
+
+# --------------------------------------------------------------------- roots
+
+def test_ntt_roots(oracle):
+    O = oracle
+    root_q = 3375402822066082                                  # crt.h:81
+    assert pow(root_q, N, O.Q) == O.Q - 1
+    # HEXL's composite-modulus NTT is the CRT of the minimal-root prime NTTs
+    assert int(O.lib().ko_min_root(O.Q1)) == root_q % O.Q1 == 62736
+    assert int(O.lib().ko_min_root(O.Q2)) == root_q % O.Q2 == 32309
+    assert O.const(8) == 3689 and O.const(9) == 13             # bsMod, p minimal roots
+    for q, w in ((O.BS, 3689), (O.P, 13)):
+        assert pow(w, N, q) == q - 1
+        assert all(pow(x, N, q) != q - 1 for x in range(2, w))
+
+
+@pytest.mark.parametrize("modsel,q,root", [(0, 66974689739603969, 3375402822066082),
+                                           (1, 16760833, 3689), (2, 65537, 13),
+                                           (3, 268369921, 62736), (4, 249561089, 32309)])
+def test_ntt_closed_form(oracle, modsel, q, root):
+    """slot i of ComputeForward holds a(root^(2*bitrev(i)+1)) (utils.cpp:1476-1490)."""
+    rng = np.random.default_rng(modsel)
+    a = rng.integers(0, q, N, dtype=np.uint64)
+    fa = oracle.ntt_forward(modsel, a)
+    ai = [int(v) for v in a]
+    for i in list(range(4)) + [1000, 2048, 4095]:
+        x = pow(root, 2 * bitrev(i) + 1, q)
+        acc, xp = 0, 1
+        for c in ai:
+            acc += c * xp
+            xp = xp * x % q
+        assert int(fa[i]) == acc % q, (modsel, i)
+    assert np.array_equal(oracle.ntt_inverse(modsel, fa), a)
+
+
+def test_ntt_negacyclic_convolution(oracle):
+    """INTT(NTT(a) * NTT(b)) equals the schoolbook product mod X^N + 1 (sparse b)."""
+    Q = oracle.Q
+    rng = np.random.default_rng(7)
+    a = rng.integers(0, Q, N, dtype=np.uint64)
+    b = np.zeros(N, dtype=np.uint64)
+    terms = {0: 5, 17: Q - 3, 4095: 123456789}
+    for k, v in terms.items():
+        b[k] = v
+    fa, fb = oracle.ntt_forward(0, a), oracle.ntt_forward(0, b)
+    prod = np.array([int(x) * int(y) % Q for x, y in zip(fa, fb)], dtype=np.uint64)
+    got = oracle.ntt_inverse(0, prod)
+    want = [0] * N
+    for k, v in terms.items():
+        for i in range(N):
+            j = i + k
+            if j < N:
+                want[j] = (want[j] + int(a[i]) * v) % Q
+            else:
+                want[j - N] = (want[j - N] - int(a[i]) * v) % Q
+    assert [int(x) for x in got] == want
+
+
+def test_hexl_ntt_index_closed_form(oracle):
+    h = oracle.hexl_ntt_index()
+    assert list(h[:4]) == [1, 4097, 2049, 6145]                    # utils.cpp:1478 comment
+    assert all(int(h[i]) == 2 * bitrev(i) + 1 for i in range(N))
+
+
+def test_permutation_realises_automorphism(oracle):
+    """NTT(tau_{5^i}(a)) == NTT(a)[perm[i]] (test-utils.cpp:361-404)."""
+    Q = oracle.Q
+    perms = oracle.perm_matrix(128)
+    rng = np.random.default_rng(3)
+    a = rng.integers(0, Q, N, dtype=np.uint64)
+    fa = oracle.ntt_forward(0, a)
+    for i in (0, 1, 2, 63, 64, 127):
+        g = pow(5, i, 2 * N)
+        ta = oracle.automorphic_rns(a, g, Q)
+        assert np.array_equal(oracle.ntt_forward(0, ta), fa[perms[i]]), i
+    assert np.array_equal(perms[0], np.arange(N))
+
+
+def test_query_encode_is_slot_of_rotation_exponent(oracle):
+    enc = oracle.query_encode()
+    h = oracle.hexl_ntt_index()
+    slot_of = {int(e): s for s, e in enumerate(h)}
+    for i in range(N):
+        e = pow(5, i, 2 * N) if i < N // 2 else (2 * N - 1) * pow(5, N - 1 - i, 2 * N) % (2 * N)
+        assert int(enc[i]) == slot_of[e]
+
+
+# ----------------------------------------------------------- decompositions
+
+def _signed(v, q):
+    v = int(v)
+    return v - q if v > q // 2 else v
+
+
+@pytest.mark.parametrize("ell,base,bg", [(2, 1 << 20, 1 << 18), (4, 0, 1 << 14)])
+def test_decompose_reconstructs(oracle, ell, base, bg):
+    """utils.cpp:815-867 recontruct: sum_j d_j * PP * Bg^j == x up to < PP/2."""
+    Q = oracle.Q
+    rng = np.random.default_rng(11)
+    x = rng.integers(0, Q, N, dtype=np.uint64)
+    bb = base.bit_length() - 1 if base else 0
+    gb = bg.bit_length() - 1
+    # special values: 0, q-1, the `d > q` boundary x = q - offset, and q itself
+    offset = sum(1 << (bb + i * gb) for i in range(ell)) << (gb - 1)
+    offset += base // 2
+    specials = [0, Q - 1, Q - offset, Q - offset - 1, Q - offset + 1, Q]
+    x[:len(specials)] = specials
+    d = oracle.decompose(x, ell, base, bg, Q)
+    pp = max(base, 1)
+    for i in range(N):
+        digits = [_signed(d[j, i], Q) for j in range(ell)]
+        assert all(-bg // 2 <= v < bg // 2 for v in digits)
+        rec = sum(v * pp * bg ** j for j, v in enumerate(digits))
+        err = (rec - int(x[i])) % Q
+        err = err - Q if err > Q // 2 else err
+        assert abs(err) <= max(base // 2, 1) + (1 if i < len(specials) else 0), (i, err)
+    # x = q behaves exactly like x = 0 (reference automorphisms emit q for 0)
+    assert np.array_equal(d[:, 0], d[:, 5])
+
+
+def test_decompose_gt_not_ge(oracle):
+    """utils.cpp:784 reduces with d > q: at x = q - offset, d == q stays unreduced."""
+    Q = oracle.Q
+    ell, base, bg = 2, 1 << 20, 1 << 18
+    offset = (((1 << 20) + (1 << 38)) << 17) + (1 << 19)
+    x = np.zeros(N, dtype=np.uint64)
+    x[0] = Q - offset          # d == Q: not reduced
+    x[1] = Q - offset + 1      # d == Q + 1: reduced to 1
+    d = oracle.decompose(x, ell, base, bg, Q)
+    D0 = Q >> 20
+    want0 = [((D0 >> (18 * j)) & ((1 << 18) - 1)) - (1 << 17) for j in range(2)]
+    assert [_signed(d[j, 0], Q) for j in range(2)] == want0
+    assert [_signed(d[j, 1], Q) for j in range(2)] == [-(1 << 17)] * 2
+
+
+def test_decompose_rlwe_signed(oracle):
+    Q = oracle.Q
+    rng = np.random.default_rng(5)
+    a = rng.integers(0, Q, N, dtype=np.uint64)
+    b = rng.integers(0, Q, N, dtype=np.uint64)
+    d = oracle.decompose_rlwe(a, b, 2, 1 << 20, 1 << 18, Q)
+    for src, base in ((a, 0), (b, 2)):
+        for i in range(0, N, 97):
+            digits = [_signed(d[base + j, i], Q) for j in range(2)]
+            assert all(-(1 << 17) <= v < (1 << 17) for v in digits)
+            xs = _signed(src[i], Q) >> 20
+            # two signed digits cover the value up to the dropped top carry
+            rec = digits[0] + digits[1] * (1 << 18)
+            assert (xs - rec) % (1 << 36) == 0
+
+
+def test_crt_inv_bsgs_and_digits(oracle):
+    Q, bs = oracle.Q, oracle.BS
+    rng = np.random.default_rng(9)
+    xq = rng.integers(0, Q, N, dtype=np.uint64)
+    xb = rng.integers(0, bs, N, dtype=np.uint64)
+    v = oracle.crt_inv_bsgs(xq, xb)
+    for i in range(0, N, 131):
+        assert v[i] < Q * bs and v[i] % Q == int(xq[i]) and v[i] % bs == int(xb[i])
+    dq, db = oracle.decompose_bsgs(xq, xb)
+    for i in range(0, N, 131):
+        dig = [_signed(dq[j, i], Q) for j in range(3)]
+        assert dig == [_signed(db[j, i], bs) for j in range(3)]
+        rec = sum(dv * (1 << 20) * (1 << 20) ** j for j, dv in enumerate(dig))
+        err = (rec - v[i]) % (Q * bs)
+        err = err - Q * bs if err > Q * bs // 2 else err
+        assert abs(err) <= 1 << 19
+
+
+def test_crt_compose(oracle):
+    Q, q1, q2 = oracle.Q, oracle.Q1, oracle.Q2
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        x, y = (int(v) for v in rng.integers(0, 1 << 63, 2, dtype=np.uint64))
+        z = int(oracle.lib().ko_crt_compose(x, y))
+        assert z < Q and z % q1 == x % q1 and z % q2 == y % q2
+        R = oracle.ref_lib()
+        if R is not None:
+            assert int(R.ref_crt_compose(x, y)) == z
+
+
+def test_db_product_matches_reference_kernel(oracle):
+    """oracle db_product == the reference's own AVX-512 fastMultiplyQueryByDatabaseDim1InvCRT."""
+    R = oracle.ref_lib()
+    if R is None:
+        pytest.skip("oracle/_ref not built (needs /root/reference and an AVX-512 host)")
+    n1, n2 = 128, 16
+    rng = np.random.default_rng(2)
+    q1, q2 = oracle.Q1, oracle.Q2
+
+    def packed(shape):
+        lo = rng.integers(0, q1, shape, dtype=np.uint64)
+        hi = rng.integers(0, q2, shape, dtype=np.uint64)
+        return lo | (hi << np.uint64(32))
+
+    db = packed(N * n2 * n1)
+    cbuf = packed(N * n1 * 2)
+    got = oracle.db_product(db, cbuf, n1, n2)
+    want = oracle.ref_db_product(db, cbuf, n1, n2)
+    assert np.array_equal(got, want)
+
+
+# ------------------------------------------------------------ end to end
+
+def test_readme_target_indices(oracle):
+    """README.md:56 prints target_col 177, target_packing 4 for the default seed."""
+    # glibc rand() with seed 1: 4096 secret draws, then the two target draws
+    import ctypes
+    libc = ctypes.CDLL("libc.so.6")
+    libc.srand(1)
+    for _ in range(N):
+        libc.rand()
+    assert (libc.rand() % 2048, libc.rand() % 16) == (177, 4)
+    sec = np.zeros(N, dtype=np.uint64)
+    q = np.zeros((2, 2, N), dtype=np.uint64)
+    sh = oracle.shapes(16, 128)
+    arrs = [np.zeros(sh[k], dtype=np.uint64) for k in ("rgsw", "baby", "giant", "packing")]
+    rc = oracle.lib().ko_gen_client(16, 128, 1, 177, 4, oracle.p64(sec), oracle.p64(q), *(oracle.p64(a) for a in arrs))
+    assert rc == 0
+    # the secret is ternary and is the first 4096 draws of the stream
+    vals = set(int(v) for v in sec)
+    assert vals <= {0, 1, oracle.Q - 1}
+
+
+@pytest.mark.parametrize("r,n1", [(2, 128), (4, 64), (1, 256)])
+def test_kat_small(oracle, r, n1):
+    """test-pir flow at reduced r: the planted record decrypts to [1..N]."""
+    inst = oracle.gen_instance(r=r, n1=n1, seed=1)
+    resp = oracle.answer(inst)
+    msg = oracle.decrypt(r, inst.secret, resp)
+    assert np.array_equal(msg, np.arange(1, N + 1, dtype=np.uint64))
+
+
+@pytest.mark.slow
+def test_kat_test_pir_default(oracle):
+    """test-pir.cpp defaults (r 16, N1 128, seed 1) reproduce README.md:56-65."""
+    inst = oracle.gen_instance(r=16, n1=128, seed=1)
+    assert inst.target == (177, 4)
+    resp = oracle.answer(inst)
+    msg = oracle.decrypt(16, inst.secret, resp)
+    assert np.array_equal(msg, np.arange(1, N + 1, dtype=np.uint64))
+
+
+def test_planted_mistake_breaks_kat(oracle):
+    """A wrong key (one flipped word in a baby key) must break the KAT, so the
+    KAT is sensitive to the key-switch path."""
+    inst = oracle.gen_instance(r=2, n1=128, seed=5)
+    inst.baby[3, 1, 1, 100] = (int(inst.baby[3, 1, 1, 100]) + 12345) % oracle.Q
+    resp = oracle.answer(inst)
+    msg = oracle.decrypt(2, inst.secret, resp)
+    assert not np.array_equal(msg, np.arange(1, N + 1, dtype=np.uint64))
