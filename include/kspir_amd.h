@@ -87,6 +87,16 @@ int kspir_answer(kspir_context* ctx, const kspir_database* db, const kspir_keys*
 int kspir_answer_device(kspir_context* ctx, const kspir_database* db, const kspir_keys* keys,
                         const uint64_t* query_dev, uint64_t* response_dev, void* stream);
 
+/* Batched online answer: nq queries, each with its own client keys (all for
+ * this database's r and N1), against one resident database.  Every database
+ * tile read from HBM serves the whole batch.  queries [nq][2][2][N] ->
+ * responses [nq][2][N]; response i equals kspir_answer on query i alone.
+ * No counterpart in the reference, which answers one query per call. */
+int kspir_answer_batch(kspir_context* ctx, const kspir_database* db, const kspir_keys* const* keys,
+                       int32_t nq, const uint64_t* queries, uint64_t* responses);
+int kspir_answer_batch_device(kspir_context* ctx, const kspir_database* db, const kspir_keys* const* keys,
+                              int32_t nq, const uint64_t* queries_dev, uint64_t* responses_dev, void* stream);
+
 /* matrix_vector_mul_bsgs_rns_crt_large (twosteps.h:55).  result: [r][2][N]
  * NTT form.  Like the reference, query[0] (a,b mod Q) is replaced by the
  * modulus-switched query in NTT form (twosteps.cpp:1074-1086); query[1] is

@@ -32,6 +32,11 @@ int kspir_time_answer(kspir_context* ctx, const kspir_database* db, const kspir_
                       const uint64_t* query_dev, uint64_t* response_dev, int warmup, int iters,
                       float* ms);
 
+/* same for a batch of nq queries (kspir_answer_batch_device per call) */
+int kspir_time_answer_batch(kspir_context* ctx, const kspir_database* db, const kspir_keys* const* keys,
+                            int32_t nq, const uint64_t* queries_dev, uint64_t* responses_dev, int warmup,
+                            int iters, float* ms);
+
 /* HIP-event timing of the database-pass kernel alone (after one answer has
  * filled the rotation words): *ms_per_launch = average milliseconds. */
 int kspir_time_db_pass(kspir_context* ctx, const kspir_database* db, const kspir_keys* keys,

@@ -37,7 +37,8 @@ def _load():
                  "kspir_external_product", "kspir_packing_rlwes", "kspir_device_alloc", "kspir_device_free",
                  "kspir_memcpy_h2d", "kspir_memcpy_d2h", "kspir_synchronize", "kspir_time_answer",
                  "kspir_time_db_pass", "kspir_test_ntt", "kspir_device_count",
-                 "kspir_compute_permutation_matrix"):
+                 "kspir_compute_permutation_matrix", "kspir_answer_batch", "kspir_answer_batch_device",
+                 "kspir_time_answer_batch"):
         getattr(L, name).restype = ctypes.c_int
     L.kspir_context_destroy.argtypes = [_vp]
     L.kspir_database_destroy.argtypes = [_vp]
@@ -53,6 +54,7 @@ EXPORTS = (
     "kspir_database_destroy", "kspir_keys_create", "kspir_keys_destroy", "kspir_answer",
     "kspir_answer_device", "kspir_matrix_vector_mul_bsgs_rns_crt_large", "kspir_external_product",
     "kspir_packing_rlwes", "kspir_answer_inspect", "kspir_compute_permutation_matrix",
+    "kspir_answer_batch", "kspir_answer_batch_device", "kspir_time_answer_batch",
     "kspir_device_count", "kspir_device_alloc", "kspir_device_free", "kspir_memcpy_h2d", "kspir_memcpy_d2h",
     "kspir_synchronize", "kspir_context_stream", "kspir_time_answer", "kspir_time_db_pass", "kspir_test_ntt",
 )
@@ -186,6 +188,24 @@ class Context:
     def answer_device(self, db, keys, query_buf, resp_buf, stream=None):
         _check(lib.kspir_answer_device(self.h, db.h, keys.h, ctypes.cast(_vp(query_buf.ptr), _u64p),
                                        ctypes.cast(_vp(resp_buf.ptr), _u64p), _vp(stream)))
+
+    def answer_batch(self, db, keys_list, queries):
+        """queries [nq][2][2][N] with one Keys per query -> responses [nq][2][N]"""
+        nq = len(keys_list)
+        q = np.ascontiguousarray(queries, dtype=np.uint64)
+        assert q.shape == (nq, 2, 2, N)
+        arr = (_vp * nq)(*[k.h.value for k in keys_list])
+        resp = np.zeros((nq, 2, N), dtype=np.uint64)
+        _check(lib.kspir_answer_batch(self.h, db.h, arr, nq, _p64(q), _p64(resp)))
+        return resp
+
+    def time_answer_batch(self, db, keys_list, queries_buf, resp_buf, warmup, iters):
+        nq = len(keys_list)
+        arr = (_vp * nq)(*[k.h.value for k in keys_list])
+        ms = ctypes.c_float(0)
+        _check(lib.kspir_time_answer_batch(self.h, db.h, arr, nq, ctypes.cast(_vp(queries_buf.ptr), _u64p),
+                                           ctypes.cast(_vp(resp_buf.ptr), _u64p), warmup, iters, ctypes.byref(ms)))
+        return ms.value
 
     def answer_inspect(self, db, keys, query):
         r, n1 = db.r, db.n1

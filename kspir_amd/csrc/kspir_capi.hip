@@ -81,15 +81,20 @@ struct kspir_context {
     hipStream_t stream = nullptr;
     Tabs T{};
     std::vector<void*> tables;
-    // per-query workspace (grown on demand)
-    int ws_r = 0, ws_n1 = 0, ws_n2 = 0;
+    // per-call workspace: one allocation, carved for (r, N1, nq) on each call
     void* ws = nullptr;
     size_t ws_bytes = 0;
-    uint64_t* q_dev = nullptr;      // [4][N]
-    uint64_t* resp_dev = nullptr;   // [max(r,1)][2][N]
+    QS S{};
+    uint64_t* q_dev = nullptr;      // [nq][4][N]
+    uint64_t* resp_dev = nullptr;   // [nq][2][N]
     uint32_t *D0, *D1, *bhat0, *mid, *icoef, *ctw, *dbout, *gcoef, *gks, *fd, *ecoef, *epart;
     uint32_t *extA, *extB, *pcoef, *ppart;
-    uint64_t* u64tmp = nullptr;     // [max(r,1)][2][N]
+    uint64_t* u64tmp = nullptr;     // [max(r,2)][2][N]
+    uint64_t* u64tmp2 = nullptr;    // [N][N1][2] packed rotation words
+    // per-batch key pointer table
+    KeyPtrs* kp_dev = nullptr;
+    KeyPtrs* kp_host = nullptr;     // pinned staging
+    int kp_cap = 0, kp_count = 0;
 };
 
 struct kspir_database {
@@ -101,11 +106,13 @@ struct kspir_database {
 struct kspir_keys {
     kspir_context* ctx;
     int r, n1, n2;
-    uint32_t* baby = nullptr;      // [N1/2][3][2][3][N] Montgomery
-    uint32_t* giant = nullptr;     // [N2-1][2][2][2][N]
-    uint32_t* packing = nullptr;   // [log2 r][4][2][2][N]
-    uint32_t* rgsw = nullptr;      // [4][2][2][N]
-    uint16_t* perms = nullptr;     // [N1][N]
+    KeyPtrs ptrs{};
+    KeyPtrs* dev_ptrs = nullptr;   // one-entry device copy of ptrs
+    uint32_t* baby = nullptr;
+    uint32_t* giant = nullptr;
+    uint32_t* packing = nullptr;
+    uint32_t* rgsw = nullptr;
+    uint16_t* perms = nullptr;
 };
 
 namespace {
@@ -181,42 +188,46 @@ int build_tables(kspir_context* c) {
     return KSPIR_OK;
 }
 
-// carve the per-query workspace; grows to the largest (r, N1, N2) seen
-int ensure_workspace(kspir_context* c, int r, int n1) {
+
+// carve the workspace for nq queries of shape (r, n1); grows on demand
+int carve_workspace(kspir_context* c, int r, int n1, int nq) {
     const int n2 = kN / 2 / n1;
-    if (c->ws && r <= c->ws_r && n1 <= c->ws_n1 && n2 <= c->ws_n2) return KSPIR_OK;
-    const int R = r > c->ws_r ? r : c->ws_r;
-    const int N1 = n1 > c->ws_n1 ? n1 : c->ws_n1;
-    const int N2 = n2 > c->ws_n2 ? n2 : c->ws_n2;
     const size_t n = kN;
-    const size_t rh = R / 2 > 0 ? R / 2 : 1;
-    // sizes in u32 words
-    const size_t sz_q = 2 * 4 * n, sz_resp = 2 * (size_t)(R > 2 ? R : 2) * 2 * n;   // u64 [max(R,2)][2][N]
-    std::vector<std::pair<size_t, uint32_t**>> parts = {
-        {9 * n, &c->D0}, {9 * n, &c->D1}, {3 * n, &c->bhat0}, {6 * n, &c->mid}, {3 * n, &c->icoef},
-        {(size_t)N1 * 2 * 2 * n, &c->ctw},
-        {(size_t)R * N2 * 4 * n, &c->dbout}, {(size_t)R * N2 * 2 * n, &c->gcoef},
-        {(size_t)R * N2 * 4 * n, &c->gks}, {(size_t)R * 4 * n, &c->fd}, {(size_t)R * 4 * n, &c->ecoef},
-        {(size_t)R * 16 * n, &c->epart}, {(size_t)R * 4 * n, &c->extA}, {(size_t)R * 4 * n, &c->extB},
-        {rh * 2 * n, &c->pcoef}, {rh * 16 * n, &c->ppart},
+    const size_t rh = r / 2 > 0 ? r / 2 : 1;
+    const size_t R2 = r > 2 ? r : 2;
+    QS S;
+    S.D = 9 * n; S.bhat = 3 * n; S.mid = 6 * n; S.icoef = 3 * n;
+    S.ctw = (size_t)n1 * 4 * n;
+    S.dbout = (size_t)r * n2 * 4 * n; S.gcoef = (size_t)r * n2 * 2 * n; S.gks = (size_t)r * n2 * 4 * n;
+    S.fd = (size_t)r * 4 * n; S.ecoef = (size_t)r * 4 * n; S.epart = (size_t)r * 16 * n;
+    S.ext = (size_t)r * 4 * n; S.pcoef = rh * 2 * n; S.ppart = rh * 16 * n;
+    S.qin64 = 4 * n; S.resp64 = 2 * n;
+    auto up = [](size_t w) { return (w + 63) / 64 * 64; };
+    struct Part { size_t words; uint32_t** dst; };
+    const size_t Q = (size_t)nq;
+    std::vector<Part> parts = {
+        {Q * S.D, &c->D0}, {Q * S.D, &c->D1}, {Q * S.bhat, &c->bhat0}, {Q * S.mid, &c->mid},
+        {Q * S.icoef, &c->icoef}, {Q * S.ctw, &c->ctw}, {Q * S.dbout, &c->dbout}, {Q * S.gcoef, &c->gcoef},
+        {Q * S.gks, &c->gks}, {Q * S.fd, &c->fd}, {Q * S.ecoef, &c->ecoef}, {Q * S.epart, &c->epart},
+        {Q * S.ext, &c->extA}, {Q * S.ext, &c->extB}, {Q * S.pcoef, &c->pcoef}, {Q * S.ppart, &c->ppart},
+        {Q * 2 * S.qin64, (uint32_t**)&c->q_dev}, {Q * 2 * S.resp64, (uint32_t**)&c->resp_dev},
+        {R2 * 4 * n, (uint32_t**)&c->u64tmp}, {(size_t)n1 * 4 * n, (uint32_t**)&c->u64tmp2},
     };
-    size_t total = sz_q + 2 * sz_resp;
-    for (auto& p : parts) total += (p.first + 63) / 64 * 64;
-    if (c->ws) {
-        HIP_OK(hipStreamSynchronize(c->stream));
-        HIP_OK(hipFree(c->ws));
-        c->ws = nullptr;
+    size_t total = 0;
+    for (auto& p : parts) total += up(p.words);
+    if (total * 4 > c->ws_bytes) {
+        if (c->ws) {
+            HIP_OK(hipStreamSynchronize(c->stream));
+            HIP_OK(hipFree(c->ws));
+            c->ws = nullptr;
+            c->ws_bytes = 0;
+        }
+        HIP_OK(hipMalloc(&c->ws, total * 4));
+        c->ws_bytes = total * 4;
     }
-    HIP_OK(hipMalloc(&c->ws, total * 4));
-    c->ws_bytes = total * 4;
     uint32_t* base = (uint32_t*)c->ws;
-    c->q_dev = (uint64_t*)base; base += sz_q;
-    c->resp_dev = (uint64_t*)base; base += sz_resp;
-    c->u64tmp = (uint64_t*)base; base += sz_resp;
-    for (auto& p : parts) { *p.second = base; base += (p.first + 63) / 64 * 64; }
-    c->ws_r = R;
-    c->ws_n1 = N1;
-    c->ws_n2 = N2;
+    for (auto& p : parts) { *p.dst = base; base += up(p.words); }
+    c->S = S;
     return KSPIR_OK;
 }
 
@@ -225,50 +236,87 @@ int grid1d(size_t total) {
     return (int)(g > 65535 ? 65535 : (g ? g : 1));
 }
 
-// The online answer for one query (test-pir.cpp:107-116) enqueued on `st`.
-// q_dev: [4][N] u64 device; resp: [2][N] u64 device (may be null when only
-// the first dimension is wanted).  Optional inspection outputs (device):
-// fd_u64 [r][2][N] first dimension (NTT form), ext_u64 [r][2][N].
-int enqueue_answer(kspir_context* c, const kspir_database* db, const kspir_keys* k, const uint64_t* q_dev,
+int launch_db_product(kspir_context* c, const kspir_database* db, int nq, hipStream_t st) {
+    const int n1 = db->n1, n2 = db->n2, r = db->r;
+    const size_t bw = (size_t)n2 * n1 * kN, obw = (size_t)n2 * 4 * kN;
+    const dim3 dg(kN / 64, r, nq);
+    const QS& S = c->S;
+    const Tabs& T = c->T;
+    switch (n2 < 16 ? n2 : 16) {
+        case 1: hipLaunchKernelGGL(k_db_product<1>, dg, dim3(64), 0, st, db->words, c->ctw, c->dbout, n1, n2, bw, obw, S, T); break;
+        case 2: hipLaunchKernelGGL(k_db_product<2>, dg, dim3(64), 0, st, db->words, c->ctw, c->dbout, n1, n2, bw, obw, S, T); break;
+        case 4: hipLaunchKernelGGL(k_db_product<4>, dg, dim3(64), 0, st, db->words, c->ctw, c->dbout, n1, n2, bw, obw, S, T); break;
+        case 8: hipLaunchKernelGGL(k_db_product<8>, dg, dim3(64), 0, st, db->words, c->ctw, c->dbout, n1, n2, bw, obw, S, T); break;
+        default: hipLaunchKernelGGL(k_db_product<16>, dg, dim3(64), 0, st, db->words, c->ctw, c->dbout, n1, n2, bw, obw, S, T); break;
+    }
+    LAUNCH_OK("k_db_product");
+    return KSPIR_OK;
+}
+
+// packing tree on the NTT-domain planes at `cur` for nq queries; returns the
+// buffer that holds the packed RLWE of each query (answer.cpp:800-835)
+int enqueue_packing(kspir_context* c, const KeyPtrs* kp, int r, int nq, hipStream_t st, uint32_t** out) {
+    const Tabs& T = c->T;
+    const QS& S = c->S;
+    uint32_t* cur = c->extA;
+    uint32_t* nxt = c->extB;
+    for (int i = r / 2; i > 0; i >>= 1) {
+        const int li = ilog2(i);
+        const uint32_t g = (uint32_t)(kN / i + 1);
+        const uint32_t ginv = (uint32_t)powmod(g, kN - 1, 2 * kN);   // odd g has order | N in (Z/2N)*
+        hipLaunchKernelGGL(k_pack_coef, dim3(i, 2 * nq), dim3(kNttThreads), 0, st, cur, c->pcoef, i, S, T);
+        LAUNCH_OK("k_pack_coef");
+        hipLaunchKernelGGL(k_pack_digit, dim3(i, 4, 2 * nq), dim3(kNttThreads), 0, st, c->pcoef, kp, li, c->ppart,
+                           ginv, S, T);
+        LAUNCH_OK("k_pack_digit");
+        hipLaunchKernelGGL(k_pack_combine, dim3(i, 2, 2 * nq), dim3(kNttThreads), 0, st, cur, c->ppart, nxt, i, g,
+                           S, T);
+        LAUNCH_OK("k_pack_combine");
+        uint32_t* t = cur; cur = nxt; nxt = t;
+    }
+    *out = cur;
+    return KSPIR_OK;
+}
+
+// The online answer (test-pir.cpp:107-116) for nq queries against one
+// database, enqueued on `st`.  q_dev: [nq][4][N] u64; resp: [nq][2][N] u64
+// (unused when first_dim_only).  kp: device array of nq key-pointer sets.
+// Optional inspection outputs for query 0: fd_u64 [r][2][N] (NTT form),
+// ext_u64 [r][2][N] (coefficient form).
+int enqueue_answer(kspir_context* c, const kspir_database* db, const KeyPtrs* kp, int nq, const uint64_t* q_dev,
                    uint64_t* resp, hipStream_t st, uint64_t* fd_u64, uint64_t* ext_u64, bool first_dim_only) {
     const int r = db->r, n1 = db->n1, n2 = db->n2, n1h = n1 / 2;
     const Tabs& T = c->T;
+    const QS& S = c->S;
     // baby steps (twosteps.cpp:1206-1214, evalAutoSRNSSmallKeys)
-    hipLaunchKernelGGL(k_baby_prep, dim3(16), dim3(kNttThreads), 0, st, q_dev, c->D0, c->bhat0, c->ctw, n1, T);
+    hipLaunchKernelGGL(k_baby_prep, dim3(16, nq), dim3(kNttThreads), 0, st, q_dev, c->D0, c->bhat0, c->ctw, n1, S, T);
     LAUNCH_OK("k_baby_prep");
     if (n1h - 1 > 0) {
-        hipLaunchKernelGGL(k_baby_rot, dim3(n1h - 1, 2), dim3(kNttThreads), 0, st, c->D0, c->bhat0, k->baby,
-                           k->perms, c->ctw, c->mid, n1, 1, n1h - 2, T);
+        hipLaunchKernelGGL(k_baby_rot, dim3(n1h - 1, 2, nq), dim3(kNttThreads), 0, st, c->D0, c->bhat0, kp, c->ctw,
+                           c->mid, n1, 1, n1h - 2, S.D, S.bhat, S, T);
         LAUNCH_OK("k_baby_rot(first)");
     }
-    hipLaunchKernelGGL(k_baby_mid_coef, dim3(3), dim3(kNttThreads), 0, st, c->mid, c->icoef, T);
+    hipLaunchKernelGGL(k_baby_mid_coef, dim3(3, nq), dim3(kNttThreads), 0, st, c->mid, c->icoef, S, T);
     LAUNCH_OK("k_baby_mid_coef");
-    hipLaunchKernelGGL(k_baby_mid_digits, dim3(9), dim3(kNttThreads), 0, st, c->icoef, c->D1, T);
+    hipLaunchKernelGGL(k_baby_mid_digits, dim3(9, nq), dim3(kNttThreads), 0, st, c->icoef, c->D1, S, T);
     LAUNCH_OK("k_baby_mid_digits");
-    hipLaunchKernelGGL(k_baby_rot, dim3(n1h, 2), dim3(kNttThreads), 0, st, c->D1, c->mid + 3 * kN, k->baby,
-                       k->perms, c->ctw, (uint32_t*)nullptr, n1, n1h, -1, T);
+    hipLaunchKernelGGL(k_baby_rot, dim3(n1h, 2, nq), dim3(kNttThreads), 0, st, c->D1, c->mid + 3 * kN, kp, c->ctw,
+                       (uint32_t*)nullptr, n1, n1h, -1, S.D, S.mid, S, T);
     LAUNCH_OK("k_baby_rot(second)");
-    // database pass (crt.cpp:643-768), one launch over all r blocks
-    const size_t bw = (size_t)n2 * n1 * kN, obw = (size_t)n2 * 4 * kN;
-    const dim3 dg(kN / 64, r);
-    const uint64_t* ctw64 = (const uint64_t*)c->ctw;
-    switch (n2 < 16 ? n2 : 16) {
-        case 1: hipLaunchKernelGGL(k_db_product<1>, dg, dim3(64), 0, st, db->words, ctw64, c->dbout, n1, n2, bw, obw, T); break;
-        case 2: hipLaunchKernelGGL(k_db_product<2>, dg, dim3(64), 0, st, db->words, ctw64, c->dbout, n1, n2, bw, obw, T); break;
-        case 4: hipLaunchKernelGGL(k_db_product<4>, dg, dim3(64), 0, st, db->words, ctw64, c->dbout, n1, n2, bw, obw, T); break;
-        case 8: hipLaunchKernelGGL(k_db_product<8>, dg, dim3(64), 0, st, db->words, ctw64, c->dbout, n1, n2, bw, obw, T); break;
-        default: hipLaunchKernelGGL(k_db_product<16>, dg, dim3(64), 0, st, db->words, ctw64, c->dbout, n1, n2, bw, obw, T); break;
-    }
-    LAUNCH_OK("k_db_product");
+    // database pass (crt.cpp:643-768)
+    int rc = launch_db_product(c, db, nq, st);
+    if (rc) return rc;
     // giant steps (twosteps.cpp:599-677)
     if (n2 > 1) {
-        hipLaunchKernelGGL(k_giant_coef, dim3(n2 - 1, r, 2), dim3(kNttThreads), 0, st, c->dbout, c->gcoef, n2, T);
+        hipLaunchKernelGGL(k_giant_coef, dim3(n2 - 1, r, 2 * nq), dim3(kNttThreads), 0, st, c->dbout, c->gcoef, n2,
+                           S, T);
         LAUNCH_OK("k_giant_coef");
-        hipLaunchKernelGGL(k_giant_ks, dim3(n2 - 1, r, 2), dim3(kNttThreads), 0, st, c->dbout, c->gcoef, k->giant,
-                           c->gks, n1, n2, T);
+        hipLaunchKernelGGL(k_giant_ks, dim3(n2 - 1, r, 2 * nq), dim3(kNttThreads), 0, st, c->dbout, c->gcoef, kp,
+                           c->gks, n1, n2, S, T);
         LAUNCH_OK("k_giant_ks");
     }
-    hipLaunchKernelGGL(k_ext_coef, dim3(r, 2, 2), dim3(kNttThreads), 0, st, c->dbout, c->gks, c->fd, c->ecoef, n2, T);
+    hipLaunchKernelGGL(k_ext_coef, dim3(r, 2, 2 * nq), dim3(kNttThreads), 0, st, c->dbout, c->gks, c->fd, c->ecoef,
+                       n2, S, T);
     LAUNCH_OK("k_ext_coef");
     if (fd_u64) {
         hipLaunchKernelGGL(k_rns_to_u64, dim3(grid1d((size_t)r * 2 * kN)), dim3(256), 0, st, c->fd, fd_u64, r, T);
@@ -276,31 +324,20 @@ int enqueue_answer(kspir_context* c, const kspir_database* db, const kspir_keys*
     }
     if (first_dim_only) return KSPIR_OK;
     // external products (lwe.cpp:761-808)
-    hipLaunchKernelGGL(k_ext_digit, dim3(r, 4, 2), dim3(kNttThreads), 0, st, c->ecoef, k->rgsw, c->epart, T);
+    hipLaunchKernelGGL(k_ext_digit, dim3(r, 4, 2 * nq), dim3(kNttThreads), 0, st, c->ecoef, kp, c->epart, S, T);
     LAUNCH_OK("k_ext_digit");
-    hipLaunchKernelGGL(k_ext_sum, dim3(grid1d((size_t)r * 4 * kN)), dim3(256), 0, st, c->epart, c->extA, r, T);
+    hipLaunchKernelGGL(k_ext_sum, dim3(grid1d((size_t)nq * r * 4 * kN)), dim3(256), 0, st, c->epart, c->extA, r, nq,
+                       S, T);
     LAUNCH_OK("k_ext_sum");
     if (ext_u64) {
-        hipLaunchKernelGGL(k_response, dim3(2, r), dim3(kNttThreads), 0, st, c->extA, ext_u64, T);
+        hipLaunchKernelGGL(k_response, dim3(2, r), dim3(kNttThreads), 0, st, c->extA, ext_u64, (size_t)4 * kN, T);
         LAUNCH_OK("k_response(ext)");
     }
     // packing (answer.cpp:800-835)
-    uint32_t* cur = c->extA;
-    uint32_t* nxt = c->extB;
-    for (int i = r / 2; i > 0; i >>= 1) {
-        const int li = ilog2(i);
-        const uint32_t g = (uint32_t)(kN / i + 1);
-        const uint32_t ginv = (uint32_t)powmod(g, kN - 1, 2 * kN);   // odd g has order | N in (Z/2N)*
-        const uint32_t* pkey = k->packing + (size_t)li * 4 * 2 * 2 * kN;
-        hipLaunchKernelGGL(k_pack_coef, dim3(i, 2), dim3(kNttThreads), 0, st, cur, c->pcoef, i, T);
-        LAUNCH_OK("k_pack_coef");
-        hipLaunchKernelGGL(k_pack_digit, dim3(i, 4, 2), dim3(kNttThreads), 0, st, c->pcoef, pkey, c->ppart, ginv, T);
-        LAUNCH_OK("k_pack_digit");
-        hipLaunchKernelGGL(k_pack_combine, dim3(i, 2, 2), dim3(kNttThreads), 0, st, cur, c->ppart, nxt, i, g, T);
-        LAUNCH_OK("k_pack_combine");
-        uint32_t* t = cur; cur = nxt; nxt = t;
-    }
-    hipLaunchKernelGGL(k_response, dim3(2, 1), dim3(kNttThreads), 0, st, cur, resp, T);
+    uint32_t* packed = nullptr;
+    rc = enqueue_packing(c, kp, r, nq, st, &packed);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_response, dim3(2, nq), dim3(kNttThreads), 0, st, packed, resp, S.ext, T);
     LAUNCH_OK("k_response");
     return KSPIR_OK;
 }
@@ -311,6 +348,30 @@ int check_pair(const kspir_context* c, const kspir_database* db, const kspir_key
     if (db->r != k->r || db->n1 != k->n1)
         return fail(KSPIR_EINVAL, "database and keys disagree on (r, N1)");
     if (!k->rgsw || (k->r > 1 && !k->packing)) return fail(KSPIR_EINVAL, "keys lack RGSW or packing keys");
+    return KSPIR_OK;
+}
+
+// device key-pointer table for a batch; re-uploaded only when it changes
+int stage_batch_keys(kspir_context* c, const kspir_keys* const* keys, int nq) {
+    if (nq > c->kp_cap) {
+        HIP_OK(hipStreamSynchronize(c->stream));
+        if (c->kp_dev) HIP_OK(hipFree(c->kp_dev));
+        if (c->kp_host) HIP_OK(hipHostFree(c->kp_host));
+        c->kp_dev = nullptr;
+        c->kp_host = nullptr;
+        HIP_OK(hipMalloc(&c->kp_dev, sizeof(KeyPtrs) * nq));
+        HIP_OK(hipHostMalloc(&c->kp_host, sizeof(KeyPtrs) * nq, hipHostMallocDefault));
+        c->kp_cap = nq;
+        c->kp_count = 0;
+    }
+    bool same = c->kp_count == nq;
+    for (int i = 0; i < nq && same; ++i)
+        same = std::memcmp(&c->kp_host[i], &keys[i]->ptrs, sizeof(KeyPtrs)) == 0;
+    if (same) return KSPIR_OK;
+    HIP_OK(hipStreamSynchronize(c->stream));   // the previous table may still be in flight
+    for (int i = 0; i < nq; ++i) c->kp_host[i] = keys[i]->ptrs;
+    HIP_OK(hipMemcpyAsync(c->kp_dev, c->kp_host, sizeof(KeyPtrs) * nq, hipMemcpyHostToDevice, c->stream));
+    c->kp_count = nq;
     return KSPIR_OK;
 }
 
@@ -366,6 +427,8 @@ void kspir_context_destroy(kspir_context* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     for (void* p : c->tables) hipFree(p);
     if (c->ws) hipFree(c->ws);
+    if (c->kp_dev) hipFree(c->kp_dev);
+    if (c->kp_host) hipHostFree(c->kp_host);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -462,6 +525,9 @@ int kspir_keys_create(kspir_context* c, int32_t r, int32_t n1, const uint64_t* b
     if (rgsw) K_OK(upload_q(rgsw, 4, &k->rgsw));
     K_OK(hipMalloc(&k->perms, p16.size() * 2));
     K_OK(hipMemcpy(k->perms, p16.data(), p16.size() * 2, hipMemcpyHostToDevice));
+    k->ptrs = KeyPtrs{k->baby, k->giant, k->packing, k->rgsw, k->perms};
+    K_OK(hipMalloc(&k->dev_ptrs, sizeof(KeyPtrs)));
+    K_OK(hipMemcpy(k->dev_ptrs, &k->ptrs, sizeof(KeyPtrs), hipMemcpyHostToDevice));
 #undef K_OK
     *out = k;
     return cleanup(KSPIR_OK);
@@ -471,11 +537,14 @@ void kspir_keys_destroy(kspir_keys* k) {
     if (!k) return;
     hipSetDevice(k->ctx->device);
     hipStreamSynchronize(k->ctx->stream);
+    // a batch table naming these keys must not be reused
+    k->ctx->kp_count = 0;
     hipFree(k->baby);
     hipFree(k->giant);
     hipFree(k->packing);
     hipFree(k->rgsw);
     hipFree(k->perms);
+    hipFree(k->dev_ptrs);
     delete k;
 }
 
@@ -485,10 +554,45 @@ int kspir_answer_device(kspir_context* c, const kspir_database* db, const kspir_
     if (rc) return rc;
     if (!query_dev || !response_dev) return fail(KSPIR_EINVAL, "null buffer");
     HIP_OK(hipSetDevice(c->device));
-    rc = ensure_workspace(c, db->r, db->n1);
+    rc = carve_workspace(c, db->r, db->n1, 1);
     if (rc) return rc;
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
-    return enqueue_answer(c, db, k, query_dev, response_dev, st, nullptr, nullptr, false);
+    return enqueue_answer(c, db, k->dev_ptrs, 1, query_dev, response_dev, st, nullptr, nullptr, false);
+}
+
+int kspir_answer_batch_device(kspir_context* c, const kspir_database* db, const kspir_keys* const* keys, int32_t nq,
+                              const uint64_t* queries_dev, uint64_t* responses_dev, void* stream) {
+    if (!keys || nq < 1) return fail(KSPIR_EINVAL, "empty batch");
+    for (int i = 0; i < nq; ++i) {
+        int rc = check_pair(c, db, keys[i]);
+        if (rc) return rc;
+    }
+    if (!queries_dev || !responses_dev) return fail(KSPIR_EINVAL, "null buffer");
+    HIP_OK(hipSetDevice(c->device));
+    int rc = carve_workspace(c, db->r, db->n1, nq);
+    if (rc) return rc;
+    rc = stage_batch_keys(c, keys, nq);
+    if (rc) return rc;
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    if (st != c->stream) HIP_OK(hipStreamSynchronize(c->stream));   // key table upload runs on the context stream
+    return enqueue_answer(c, db, c->kp_dev, nq, queries_dev, responses_dev, st, nullptr, nullptr, false);
+}
+
+int kspir_answer_batch(kspir_context* c, const kspir_database* db, const kspir_keys* const* keys, int32_t nq,
+                       const uint64_t* queries, uint64_t* responses) {
+    if (!keys || nq < 1 || !queries || !responses) return fail(KSPIR_EINVAL, "bad batch arguments");
+    if (!c || !db) return fail(KSPIR_EINVAL, "null handle");
+    HIP_OK(hipSetDevice(c->device));
+    int rc = carve_workspace(c, db->r, db->n1, nq);
+    if (rc) return rc;
+    uint64_t* qd = c->q_dev + (size_t)nq * c->S.qin64;   // second half of the query area
+    HIP_OK(hipMemcpyAsync(qd, queries, (size_t)nq * 4 * kN * 8, hipMemcpyHostToDevice, c->stream));
+    uint64_t* rd = c->resp_dev + (size_t)nq * c->S.resp64;
+    rc = kspir_answer_batch_device(c, db, keys, nq, qd, rd, nullptr);
+    if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(responses, rd, (size_t)nq * 2 * kN * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return KSPIR_OK;
 }
 
 int kspir_answer_inspect(kspir_context* c, const kspir_database* db, const kspir_keys* k, const uint64_t* query,
@@ -497,24 +601,27 @@ int kspir_answer_inspect(kspir_context* c, const kspir_database* db, const kspir
     if (rc) return rc;
     if (!query || !response) return fail(KSPIR_EINVAL, "null buffer");
     HIP_OK(hipSetDevice(c->device));
-    rc = ensure_workspace(c, db->r, db->n1);
+    rc = carve_workspace(c, db->r, db->n1, 1);
     if (rc) return rc;
     const int r = db->r;
     uint64_t* fd_dev = c->u64tmp;
-    uint64_t* ext_buf = nullptr;
-    if (ext) HIP_OK(hipMalloc(&ext_buf, (size_t)r * 2 * kN * 8));
-    uint64_t* ext_dev = ext_buf;
+    uint64_t* ext_dev = nullptr;
+    if (ext) HIP_OK(hipMalloc(&ext_dev, (size_t)r * 2 * kN * 8));
     HIP_OK(hipMemcpyAsync(c->q_dev, query, 4 * kN * 8, hipMemcpyHostToDevice, c->stream));
-    rc = enqueue_answer(c, db, k, c->q_dev, c->resp_dev, c->stream, firstdim ? fd_dev : nullptr,
-                        ext ? ext_dev : nullptr, false);
-    if (rc) { if (ext_buf) hipFree(ext_buf); return rc; }
+    rc = enqueue_answer(c, db, k->dev_ptrs, 1, c->q_dev, c->resp_dev, c->stream, firstdim ? fd_dev : nullptr,
+                        ext_dev, false);
+    if (rc) { if (ext_dev) hipFree(ext_dev); return rc; }
     HIP_OK(hipMemcpyAsync(response, c->resp_dev, 2 * kN * 8, hipMemcpyDeviceToHost, c->stream));
-    if (cipherbuf)
-        HIP_OK(hipMemcpyAsync(cipherbuf, c->ctw, (size_t)kN * db->n1 * 2 * 8, hipMemcpyDeviceToHost, c->stream));
+    if (cipherbuf) {
+        hipLaunchKernelGGL(k_ctw_pack, dim3(grid1d((size_t)kN * db->n1 * 2)), dim3(256), 0, c->stream, c->ctw,
+                           c->u64tmp2, db->n1);
+        LAUNCH_OK("k_ctw_pack");
+        HIP_OK(hipMemcpyAsync(cipherbuf, c->u64tmp2, (size_t)kN * db->n1 * 2 * 8, hipMemcpyDeviceToHost, c->stream));
+    }
     if (firstdim) HIP_OK(hipMemcpyAsync(firstdim, fd_dev, (size_t)r * 2 * kN * 8, hipMemcpyDeviceToHost, c->stream));
     if (ext) HIP_OK(hipMemcpyAsync(ext, ext_dev, (size_t)r * 2 * kN * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
-    if (ext_buf) HIP_OK(hipFree(ext_buf));
+    if (ext_dev) HIP_OK(hipFree(ext_dev));
     return KSPIR_OK;
 }
 
@@ -530,10 +637,10 @@ int kspir_matrix_vector_mul_bsgs_rns_crt_large(kspir_context* c, const kspir_dat
     if (db->r != k->r || db->n1 != k->n1) return fail(KSPIR_EINVAL, "database and keys disagree on (r, N1)");
     if (!query || !result) return fail(KSPIR_EINVAL, "null buffer");
     HIP_OK(hipSetDevice(c->device));
-    int rc = ensure_workspace(c, db->r, db->n1);
+    int rc = carve_workspace(c, db->r, db->n1, 1);
     if (rc) return rc;
     HIP_OK(hipMemcpyAsync(c->q_dev, query, 4 * kN * 8, hipMemcpyHostToDevice, c->stream));
-    rc = enqueue_answer(c, db, k, c->q_dev, nullptr, c->stream, c->u64tmp, nullptr, true);
+    rc = enqueue_answer(c, db, k->dev_ptrs, 1, c->q_dev, nullptr, c->stream, c->u64tmp, nullptr, true);
     if (rc) return rc;
     hipLaunchKernelGGL(k_ctw0_to_u64, dim3(2 * kN / 256), dim3(256), 0, c->stream, c->ctw, c->resp_dev, db->n1, c->T);
     LAUNCH_OK("k_ctw0_to_u64");
@@ -548,32 +655,32 @@ int kspir_external_product(kspir_context* c, const kspir_keys* k, uint64_t* resu
     if (!c || !k || !result || !first_dim) return fail(KSPIR_EINVAL, "null argument");
     if (!k->rgsw) return fail(KSPIR_EINVAL, "keys lack the RGSW ciphertext");
     HIP_OK(hipSetDevice(c->device));
-    int rc = ensure_workspace(c, k->r, k->n1);
+    int rc = carve_workspace(c, 1, k->n1, 1);
     if (rc) return rc;
     hipStream_t st = c->stream;
     const Tabs& T = c->T;
+    const QS& S = c->S;
     uint64_t* io = c->u64tmp;   // [0]: first_dim, [1]: result words
     HIP_OK(hipMemcpyAsync(io, first_dim, 2 * kN * 8, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(io + 2 * kN, result, 2 * kN * 8, hipMemcpyHostToDevice, st));
-    // first_dim in coefficient form -> ecoef planes [1][2][2][N]
     if (first_dim_is_ntt) {
         hipLaunchKernelGGL(k_u64_to_rns, dim3(1, 2, 2), dim3(kNttThreads), 0, st, io, c->fd, 0, T);
         LAUNCH_OK("k_u64_to_rns");
-        hipLaunchKernelGGL(k_response, dim3(2, 1), dim3(kNttThreads), 0, st, c->fd, io, T);   // INTT -> u64
+        hipLaunchKernelGGL(k_response, dim3(2, 1), dim3(kNttThreads), 0, st, c->fd, io, (size_t)4 * kN, T);
         LAUNCH_OK("k_response");
     }
     hipLaunchKernelGGL(k_u64_to_rns, dim3(1, 2, 2), dim3(kNttThreads), 0, st, io, c->ecoef, 0, T);
     LAUNCH_OK("k_u64_to_rns");
-    hipLaunchKernelGGL(k_ext_digit, dim3(1, 4, 2), dim3(kNttThreads), 0, st, c->ecoef, k->rgsw, c->epart, T);
+    hipLaunchKernelGGL(k_ext_digit, dim3(1, 4, 2), dim3(kNttThreads), 0, st, c->ecoef, k->dev_ptrs, c->epart, S, T);
     LAUNCH_OK("k_ext_digit");
-    hipLaunchKernelGGL(k_ext_sum, dim3(grid1d(4 * kN)), dim3(256), 0, st, c->epart, c->extA, 1, T);
+    hipLaunchKernelGGL(k_ext_sum, dim3(grid1d(4 * kN)), dim3(256), 0, st, c->epart, c->extA, 1, 1, S, T);
     LAUNCH_OK("k_ext_sum");
     // result words are NTT-domain accumulators: add them
     hipLaunchKernelGGL(k_u64_to_rns, dim3(1, 2, 2), dim3(kNttThreads), 0, st, io + 2 * kN, c->extB, 0, T);
     LAUNCH_OK("k_u64_to_rns");
     hipLaunchKernelGGL(k_add_planes, dim3(grid1d(4 * kN)), dim3(256), 0, st, c->extA, c->extB, 1, T);
     LAUNCH_OK("k_add_planes");
-    hipLaunchKernelGGL(k_response, dim3(2, 1), dim3(kNttThreads), 0, st, c->extA, io + 2 * kN, T);
+    hipLaunchKernelGGL(k_response, dim3(2, 1), dim3(kNttThreads), 0, st, c->extA, io + 2 * kN, (size_t)4 * kN, T);
     LAUNCH_OK("k_response");
     HIP_OK(hipMemcpyAsync(result, io + 2 * kN, 2 * kN * 8, hipMemcpyDeviceToHost, st));
     if (first_dim_is_ntt) HIP_OK(hipMemcpyAsync(first_dim, io, 2 * kN * 8, hipMemcpyDeviceToHost, st));
@@ -586,7 +693,7 @@ int kspir_packing_rlwes(kspir_context* c, const kspir_keys* k, uint64_t* result,
     const int r = k->r;
     if (r > 1 && !k->packing) return fail(KSPIR_EINVAL, "keys lack packing keys");
     HIP_OK(hipSetDevice(c->device));
-    int rc = ensure_workspace(c, r, k->n1);
+    int rc = carve_workspace(c, r, k->n1, 1);
     if (rc) return rc;
     hipStream_t st = c->stream;
     const Tabs& T = c->T;
@@ -595,22 +702,10 @@ int kspir_packing_rlwes(kspir_context* c, const kspir_keys* k, uint64_t* result,
     hipLaunchKernelGGL(k_u64_to_rns, dim3(r, 2, 2), dim3(kNttThreads), 0, st, c->u64tmp, c->extA,
                        rlwes_are_ntt ? 0 : 1, T);
     LAUNCH_OK("k_u64_to_rns");
-    uint32_t* cur = c->extA;
-    uint32_t* nxt = c->extB;
-    for (int i = r / 2; i > 0; i >>= 1) {
-        const int li = ilog2(i);
-        const uint32_t g = (uint32_t)(kN / i + 1);
-        const uint32_t ginv = (uint32_t)powmod(g, kN - 1, 2 * kN);
-        const uint32_t* pkey = k->packing + (size_t)li * 4 * 2 * 2 * kN;
-        hipLaunchKernelGGL(k_pack_coef, dim3(i, 2), dim3(kNttThreads), 0, st, cur, c->pcoef, i, T);
-        LAUNCH_OK("k_pack_coef");
-        hipLaunchKernelGGL(k_pack_digit, dim3(i, 4, 2), dim3(kNttThreads), 0, st, c->pcoef, pkey, c->ppart, ginv, T);
-        LAUNCH_OK("k_pack_digit");
-        hipLaunchKernelGGL(k_pack_combine, dim3(i, 2, 2), dim3(kNttThreads), 0, st, cur, c->ppart, nxt, i, g, T);
-        LAUNCH_OK("k_pack_combine");
-        uint32_t* t = cur; cur = nxt; nxt = t;
-    }
-    hipLaunchKernelGGL(k_response, dim3(2, 1), dim3(kNttThreads), 0, st, cur, c->resp_dev, T);
+    uint32_t* packed = nullptr;
+    rc = enqueue_packing(c, k->dev_ptrs, r, 1, st, &packed);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_response, dim3(2, 1), dim3(kNttThreads), 0, st, packed, c->resp_dev, c->S.ext, T);
     LAUNCH_OK("k_response");
     HIP_OK(hipMemcpyAsync(result, c->resp_dev, 2 * kN * 8, hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
@@ -646,11 +741,12 @@ int kspir_synchronize(kspir_context* c) {
     return KSPIR_OK;
 }
 
-int kspir_time_answer(kspir_context* c, const kspir_database* db, const kspir_keys* k, const uint64_t* query_dev,
-                      uint64_t* response_dev, int warmup, int iters, float* ms) {
+int kspir_time_answer_batch(kspir_context* c, const kspir_database* db, const kspir_keys* const* keys, int32_t nq,
+                            const uint64_t* queries_dev, uint64_t* responses_dev, int warmup, int iters, float* ms) {
     if (!ms || iters < 1) return fail(KSPIR_EINVAL, "bad timing arguments");
     for (int i = 0; i < warmup; ++i) {
-        int rc = kspir_answer_device(c, db, k, query_dev, response_dev, nullptr);
+        int rc = nq == 1 ? kspir_answer_device(c, db, keys[0], queries_dev, responses_dev, nullptr)
+                         : kspir_answer_batch_device(c, db, keys, nq, queries_dev, responses_dev, nullptr);
         if (rc) return rc;
     }
     hipEvent_t e0, e1;
@@ -658,7 +754,8 @@ int kspir_time_answer(kspir_context* c, const kspir_database* db, const kspir_ke
     HIP_OK(hipEventCreate(&e1));
     HIP_OK(hipEventRecord(e0, c->stream));
     for (int i = 0; i < iters; ++i) {
-        int rc = kspir_answer_device(c, db, k, query_dev, response_dev, nullptr);
+        int rc = nq == 1 ? kspir_answer_device(c, db, keys[0], queries_dev, responses_dev, nullptr)
+                         : kspir_answer_batch_device(c, db, keys, nq, queries_dev, responses_dev, nullptr);
         if (rc) return rc;
     }
     HIP_OK(hipEventRecord(e1, c->stream));
@@ -669,33 +766,28 @@ int kspir_time_answer(kspir_context* c, const kspir_database* db, const kspir_ke
     return KSPIR_OK;
 }
 
+int kspir_time_answer(kspir_context* c, const kspir_database* db, const kspir_keys* k, const uint64_t* query_dev,
+                      uint64_t* response_dev, int warmup, int iters, float* ms) {
+    return kspir_time_answer_batch(c, db, &k, 1, query_dev, response_dev, warmup, iters, ms);
+}
+
 int kspir_time_db_pass(kspir_context* c, const kspir_database* db, const kspir_keys* k, const uint64_t* query_dev,
                        int iters, float* ms_per_launch) {
     if (!ms_per_launch || iters < 1) return fail(KSPIR_EINVAL, "bad timing arguments");
     int rc = check_pair(c, db, k);
     if (rc) return rc;
     HIP_OK(hipSetDevice(c->device));
-    rc = ensure_workspace(c, db->r, db->n1);
+    rc = carve_workspace(c, db->r, db->n1, 1);
     if (rc) return rc;
-    rc = enqueue_answer(c, db, k, query_dev, c->resp_dev, c->stream, nullptr, nullptr, true);
+    rc = enqueue_answer(c, db, k->dev_ptrs, 1, query_dev, c->resp_dev, c->stream, nullptr, nullptr, true);
     if (rc) return rc;
-    const int n1 = db->n1, n2 = db->n2, r = db->r;
-    const size_t bw = (size_t)n2 * n1 * kN, obw = (size_t)n2 * 4 * kN;
-    const dim3 dg(kN / 64, r);
-    const uint64_t* ctw64 = (const uint64_t*)c->ctw;
     hipEvent_t e0, e1;
     HIP_OK(hipEventCreate(&e0));
     HIP_OK(hipEventCreate(&e1));
     HIP_OK(hipEventRecord(e0, c->stream));
     for (int i = 0; i < iters; ++i) {
-        switch (n2 < 16 ? n2 : 16) {
-            case 1: hipLaunchKernelGGL(k_db_product<1>, dg, dim3(64), 0, c->stream, db->words, ctw64, c->dbout, n1, n2, bw, obw, c->T); break;
-            case 2: hipLaunchKernelGGL(k_db_product<2>, dg, dim3(64), 0, c->stream, db->words, ctw64, c->dbout, n1, n2, bw, obw, c->T); break;
-            case 4: hipLaunchKernelGGL(k_db_product<4>, dg, dim3(64), 0, c->stream, db->words, ctw64, c->dbout, n1, n2, bw, obw, c->T); break;
-            case 8: hipLaunchKernelGGL(k_db_product<8>, dg, dim3(64), 0, c->stream, db->words, ctw64, c->dbout, n1, n2, bw, obw, c->T); break;
-            default: hipLaunchKernelGGL(k_db_product<16>, dg, dim3(64), 0, c->stream, db->words, ctw64, c->dbout, n1, n2, bw, obw, c->T); break;
-        }
-        LAUNCH_OK("k_db_product");
+        rc = launch_db_product(c, db, 1, c->stream);
+        if (rc) return rc;
     }
     HIP_OK(hipEventRecord(e1, c->stream));
     HIP_OK(hipEventSynchronize(e1));

@@ -20,6 +20,24 @@
 
 namespace kspir {
 
+// ------------------------------------------------------------ batch layout
+
+// device pointers of one client's resident keys
+struct KeyPtrs {
+    const uint32_t* baby;      // [N1/2][3][2][3][N] Montgomery residues
+    const uint32_t* giant;     // [N2-1][2][2][2][N]
+    const uint32_t* packing;   // [log2 r][4][2][2][N]
+    const uint32_t* rgsw;      // [4][2][2][N]
+    const uint16_t* perms;     // [N1][N]
+};
+
+// per-query strides (in 32-bit words unless noted) of the answer workspace;
+// query q of a batch uses base + q * stride for every buffer
+struct QS {
+    size_t D, bhat, mid, icoef, ctw, dbout, gcoef, gks, fd, ecoef, epart, ext, pcoef, ppart;
+    size_t qin64, resp64;   // 64-bit words
+};
+
 // ------------------------------------------------------------------ helpers
 
 // value in [0, Q*bsMod) congruent to xq mod Q and xb mod bsMod (crt_inv_bsgs)
@@ -159,11 +177,16 @@ __global__ void __launch_bounds__(256) k_db_transpose(uint64_t* __restrict__ out
 // q: [2][2][N] u64 (mod Q part a,b; mod bsMod part a,b), coefficient form.
 // blocks 0..8   digit (j = b/3, prime p = b%3) of decompose_bsgs(a) -> D[j][p]
 // blocks 9..11  NTT of b in prime p -> bhat[p]
-// blocks 12..15 modulus-switched query (a|b, prime) -> ctw[z][0][ab][p]
+// blocks 12..15 modulus-switched query (a|b, prime) -> ctw[0][ab][p][z]
 __global__ void __launch_bounds__(kNttThreads) k_baby_prep(const uint64_t* __restrict__ q,
-        uint32_t* __restrict__ D, uint32_t* __restrict__ bhat, uint32_t* __restrict__ ctw, int n1, Tabs T) {
+        uint32_t* __restrict__ D, uint32_t* __restrict__ bhat, uint32_t* __restrict__ ctw, int n1, QS S, Tabs T) {
     __shared__ uint32_t s[kPadN];
     const int b = blockIdx.x;
+    const size_t qi = blockIdx.y;
+    q += qi * S.qin64;
+    D += qi * S.D;
+    bhat += qi * S.bhat;
+    ctw += qi * S.ctw;
     const uint64_t* qa = q;
     const uint64_t* qb = q + kN;
     const uint64_t* ba = q + 2 * kN;
@@ -199,7 +222,7 @@ __global__ void __launch_bounds__(kNttThreads) k_baby_prep(const uint64_t* __res
         __syncthreads();
         ntt_fwd_r16(s, T.p[p]);
         for (int x = threadIdx.x; x < kN; x += kNttThreads)
-            ctw[(((size_t)x * n1 + 0) * 2 + ab) * 2 + p] = s[lds_pad(x)];
+            ctw[((size_t)(0 * 2 + ab) * 2 + p) * kN + x] = s[lds_pad(x)];
     }
 }
 
@@ -208,18 +231,24 @@ __global__ void __launch_bounds__(kNttThreads) k_baby_prep(const uint64_t* __res
 //   acc_p = -sum_j D[j][p][perm(s)] * key[rot][j][ab][p][s]       (p = q1,q2,bs)
 //   b-part adds bhat[p][perm(s)]  (tau(b) in the NTT domain)
 //   c = INTT_bs(acc_bs) in [0,bsMod);  out_p = (acc_p - NTT_p(c)) * bsMod^-1
-// out goes to ctw[z][k_base + rot][ab][p].  Rotation `mid` also stores its
+// out goes to ctw[k_base + rot][ab][p][z].  Rotation `mid` also stores its
 // un-switched NTT-domain accumulators in mid_out[ab][p] (evalAutoSRNSFirst's
 // intermediate).  twosteps.cpp:937-1204.
 __global__ void __launch_bounds__(kNttThreads) k_baby_rot(const uint32_t* __restrict__ D,
-        const uint32_t* __restrict__ bhat, const uint32_t* __restrict__ keys, const uint16_t* __restrict__ perms,
-        uint32_t* __restrict__ ctw, uint32_t* __restrict__ mid_out, int n1, int k_base, int mid, Tabs T) {
+        const uint32_t* __restrict__ bhat, const KeyPtrs* __restrict__ kp,
+        uint32_t* __restrict__ ctw, uint32_t* __restrict__ mid_out, int n1, int k_base, int mid,
+        size_t D_stride, size_t bhat_stride, QS S, Tabs T) {
     __shared__ uint32_t sb[kPadN];
     __shared__ uint32_t s1[kPadN];
     __shared__ uint32_t s2[kPadN];
     const int rot = blockIdx.x, ab = blockIdx.y;
-    const uint16_t* pm = perms + (size_t)(rot + 1) * kN;
-    const uint32_t* key = keys + (size_t)rot * 3 * 2 * 3 * kN;
+    const size_t qi = blockIdx.z;
+    D += qi * D_stride;
+    bhat += qi * bhat_stride;
+    ctw += qi * S.ctw;
+    if (mid_out) mid_out += qi * S.mid;
+    const uint16_t* pm = kp[qi].perms + (size_t)(rot + 1) * kN;
+    const uint32_t* key = kp[qi].baby + (size_t)rot * 3 * 2 * 3 * kN;
     constexpr int PER = kN / kNttThreads;
     uint32_t acc1[PER], acc2[PER];
 #pragma unroll
@@ -257,16 +286,18 @@ __global__ void __launch_bounds__(kNttThreads) k_baby_rot(const uint32_t* __rest
         const int x = threadIdx.x + e * kNttThreads;
         uint32_t o1 = mul_shoup(sub_mod(acc1[e], s1[lds_pad(x)], kQ1), T.bsinv_q[0], T.bsinv_q_sh[0], kQ1);
         uint32_t o2 = mul_shoup(sub_mod(acc2[e], s2[lds_pad(x)], kQ2), T.bsinv_q[1], T.bsinv_q_sh[1], kQ2);
-        uint2 w = make_uint2(o1, o2);
-        *reinterpret_cast<uint2*>(ctw + (((size_t)x * n1 + k) * 2 + ab) * 2) = w;
+        ctw[((size_t)(k * 2 + ab) * 2 + 0) * kN + x] = o1;
+        ctw[((size_t)(k * 2 + ab) * 2 + 1) * kN + x] = o2;
     }
 }
 
 // second half, step 1: INTT of the intermediate's a in prime p (blockIdx.x)
 __global__ void __launch_bounds__(kNttThreads) k_baby_mid_coef(const uint32_t* __restrict__ mid,
-                                                               uint32_t* __restrict__ coef, Tabs T) {
+                                                               uint32_t* __restrict__ coef, QS S, Tabs T) {
     __shared__ uint32_t s[kPadN];
     const int p = blockIdx.x;
+    mid += blockIdx.y * S.mid;
+    coef += blockIdx.y * S.icoef;
     for (int x = threadIdx.x; x < kN; x += kNttThreads) s[lds_pad(x)] = mid[(0 * 3 + p) * kN + x];
     __syncthreads();
     ntt_inv_r16(s, T.p[p]);
@@ -275,9 +306,11 @@ __global__ void __launch_bounds__(kNttThreads) k_baby_mid_coef(const uint32_t* _
 
 // second half, step 2: digits of the intermediate's a (blockIdx.x = j*3 + p)
 __global__ void __launch_bounds__(kNttThreads) k_baby_mid_digits(const uint32_t* __restrict__ coef,
-                                                                 uint32_t* __restrict__ D, Tabs T) {
+                                                                 uint32_t* __restrict__ D, QS S, Tabs T) {
     __shared__ uint32_t s[kPadN];
     const int j = blockIdx.x / 3, p = blockIdx.x % 3;
+    coef += blockIdx.y * S.icoef;
+    D += blockIdx.y * S.D;
     const uint32_t qq = T.p[p].q;
     for (int x = threadIdx.x; x < kN; x += kNttThreads) {
         uint64_t xq = crt_q(coef[0 * kN + x], coef[1 * kN + x], T);
@@ -293,25 +326,26 @@ __global__ void __launch_bounds__(kNttThreads) k_baby_mid_digits(const uint32_t*
 
 // crt.cpp:643-768 restated for a slot-minor layout: one lane per NTT slot z.
 // db: [N2][N1][N] packed (x mod q1) | (x mod q2) << 32 for one block
-// ctw: [N][N1][2] packed rotation words (k = 0..N1-1, a|b)
+// ctw: [N1][2 ab][2 p][N] rotation residues, slot-minor (k = 0..N1-1)
 // out: [N2][2 ab][2 primes][N] residues
 template <int RG>
-__global__ void __launch_bounds__(64) k_db_product(const uint64_t* __restrict__ db, const uint64_t* __restrict__ ctw,
+__global__ void __launch_bounds__(64) k_db_product(const uint64_t* __restrict__ db, const uint32_t* __restrict__ ctw,
                                                    uint32_t* __restrict__ out, int n1, int n2,
-                                                   size_t block_words, size_t out_block_words, Tabs T) {
+                                                   size_t block_words, size_t out_block_words, QS S, Tabs T) {
     const int z = blockIdx.x * 64 + threadIdx.x;
     const int blk = blockIdx.y;
+    const size_t qi = blockIdx.z;
+    ctw += qi * S.ctw;
     const uint64_t* dbb = db + blk * block_words;
-    uint32_t* ob = out + blk * out_block_words;
-    const uint64_t* ct = ctw + (size_t)z * n1 * 2;
+    uint32_t* ob = out + qi * S.dbout + blk * out_block_words;
+    const uint32_t* ct = ctw + z;
     for (int r0 = 0; r0 < n2; r0 += RG) {
         uint64_t a1[RG], a2[RG], b1[RG], b2[RG];
 #pragma unroll
         for (int i = 0; i < RG; ++i) a1[i] = a2[i] = b1[i] = b2[i] = 0;
         for (int k = 0; k < n1; ++k) {
-            const uint64_t ca = ct[2 * k], cb = ct[2 * k + 1];
-            const uint32_t ca1 = (uint32_t)ca, ca2 = (uint32_t)(ca >> 32);
-            const uint32_t cb1 = (uint32_t)cb, cb2 = (uint32_t)(cb >> 32);
+            const uint32_t* c = ct + (size_t)k * 4 * kN;
+            const uint32_t ca1 = c[0], ca2 = c[kN], cb1 = c[2 * kN], cb2 = c[3 * kN];
 #pragma unroll
             for (int i = 0; i < RG; ++i) {
                 const uint64_t d = __builtin_nontemporal_load(dbb + ((size_t)(r0 + i) * n1 + k) * kN + z);
@@ -338,9 +372,12 @@ __global__ void __launch_bounds__(64) k_db_product(const uint64_t* __restrict__ 
 // INTT of row c >= 1 (a part) of every block: grid (N2-1, r, 2 primes)
 // dbout: [r][N2][2][2][N]  ->  gcoef: [r][N2][2][N]
 __global__ void __launch_bounds__(kNttThreads) k_giant_coef(const uint32_t* __restrict__ dbout,
-                                                            uint32_t* __restrict__ gcoef, int n2, Tabs T) {
+                                                            uint32_t* __restrict__ gcoef, int n2, QS S, Tabs T) {
     __shared__ uint32_t s[kPadN];
-    const int c = blockIdx.x + 1, blk = blockIdx.y, p = blockIdx.z;
+    const int c = blockIdx.x + 1, blk = blockIdx.y, p = blockIdx.z & 1;
+    const size_t qi = blockIdx.z >> 1;
+    dbout += qi * S.dbout;
+    gcoef += qi * S.gcoef;
     const uint32_t* src = dbout + (((size_t)blk * n2 + c) * 4 + p) * kN;
     for (int x = threadIdx.x; x < kN; x += kNttThreads) s[lds_pad(x)] = src[x];
     __syncthreads();
@@ -354,11 +391,16 @@ __global__ void __launch_bounds__(kNttThreads) k_giant_coef(const uint32_t* __re
 // with d = decompose(tau_g(a)) (ell 2, PP 2^20, Bg 2^18), g = 5^(N1*c).
 // gks: [r][N2][2 ab][2 p][N] (row 0 unused)
 __global__ void __launch_bounds__(kNttThreads) k_giant_ks(const uint32_t* __restrict__ dbout,
-        const uint32_t* __restrict__ gcoef, const uint32_t* __restrict__ gkeys, uint32_t* __restrict__ gks,
-        int n1, int n2, Tabs T) {
+        const uint32_t* __restrict__ gcoef, const KeyPtrs* __restrict__ kp, uint32_t* __restrict__ gks,
+        int n1, int n2, QS S, Tabs T) {
     __shared__ uint32_t s0[kPadN];
     __shared__ uint32_t s1[kPadN];
-    const int c = blockIdx.x + 1, blk = blockIdx.y, p = blockIdx.z;
+    const int c = blockIdx.x + 1, blk = blockIdx.y, p = blockIdx.z & 1;
+    const size_t qi = blockIdx.z >> 1;
+    dbout += qi * S.dbout;
+    gcoef += qi * S.gcoef;
+    gks += qi * S.gks;
+    const uint32_t* gkeys = kp[qi].giant;
     const PrimeTab& P = T.p[p];
     // g = 5^(N1*c); 5 has order N/2 mod 2N, so g^-1 = 5^(N/2 - N1*c mod N/2)
     const uint32_t e = (uint32_t)(n1 * c) % (kN / 2);
@@ -392,9 +434,15 @@ __global__ void __launch_bounds__(kNttThreads) k_giant_ks(const uint32_t* __rest
 // first-dimension result of each block (row 0 + key-switched rows), NTT form,
 // written to fd (optional) and INTT'd to coefficients: grid (r, 2 ab, 2 p)
 __global__ void __launch_bounds__(kNttThreads) k_ext_coef(const uint32_t* __restrict__ dbout,
-        const uint32_t* __restrict__ gks, uint32_t* __restrict__ fd, uint32_t* __restrict__ ecoef, int n2, Tabs T) {
+        const uint32_t* __restrict__ gks, uint32_t* __restrict__ fd, uint32_t* __restrict__ ecoef, int n2, QS S,
+        Tabs T) {
     __shared__ uint32_t s[kPadN];
-    const int blk = blockIdx.x, ab = blockIdx.y, p = blockIdx.z;
+    const int blk = blockIdx.x, ab = blockIdx.y, p = blockIdx.z & 1;
+    const size_t qi = blockIdx.z >> 1;
+    dbout += qi * S.dbout;
+    gks += qi * S.gks;
+    if (fd) fd += qi * S.fd;
+    ecoef += qi * S.ecoef;
     const uint32_t q = T.p[p].q;
     for (int x = threadIdx.x; x < kN; x += kNttThreads) {
         uint32_t v = dbout[(((size_t)blk * n2 + 0) * 4 + ab * 2 + p) * kN + x];
@@ -411,9 +459,13 @@ __global__ void __launch_bounds__(kNttThreads) k_ext_coef(const uint32_t* __rest
 // (ell 2, PP 2^20, Bg 2^18), NTT, times RGSW row k: grid (r, 4, 2 p)
 // epart: [r][4][2 ab][2 p][N]
 __global__ void __launch_bounds__(kNttThreads) k_ext_digit(const uint32_t* __restrict__ ecoef,
-        const uint32_t* __restrict__ rgsw, uint32_t* __restrict__ epart, Tabs T) {
+        const KeyPtrs* __restrict__ kp, uint32_t* __restrict__ epart, QS S, Tabs T) {
     __shared__ uint32_t s[kPadN];
-    const int blk = blockIdx.x, k = blockIdx.y, p = blockIdx.z;
+    const int blk = blockIdx.x, k = blockIdx.y, p = blockIdx.z & 1;
+    const size_t qi = blockIdx.z >> 1;
+    ecoef += qi * S.ecoef;
+    epart += qi * S.epart;
+    const uint32_t* rgsw = kp[qi].rgsw;
     const PrimeTab& P = T.p[p];
     const int src = k >> 1, j = k & 1;
     const uint32_t* c1 = ecoef + (((size_t)blk * 2 + src) * 2 + 0) * kN;
@@ -434,18 +486,21 @@ __global__ void __launch_bounds__(kNttThreads) k_ext_digit(const uint32_t* __res
 
 // ext[blk][ab][p] = sum_k epart[blk][k][ab][p]  (elementwise, 4*r*N lanes)
 __global__ void __launch_bounds__(256) k_ext_sum(const uint32_t* __restrict__ epart, uint32_t* __restrict__ ext,
-                                                 int r, Tabs T) {
-    const size_t total = (size_t)r * 4 * kN;
-    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (size_t)gridDim.x * blockDim.x) {
+                                                 int r, int nq, QS S, Tabs T) {
+    const size_t per = (size_t)r * 4 * kN;
+    const size_t total = per * nq;
+    for (size_t gidx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; gidx < total;
+         gidx += (size_t)gridDim.x * blockDim.x) {
+        const size_t qi = gidx / per, idx = gidx % per;
+        const uint32_t* epart_q = epart + qi * S.epart;
         const size_t blk = idx / (4 * kN);
         const size_t w = idx % (4 * kN);     // (ab, p, x)
         const int p = (int)((w / kN) & 1);
         const uint32_t q = T.p[p].q;
         uint32_t v = 0;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) v = add_mod(v, epart[(blk * 4 + k) * 4 * kN + w], q);
-        ext[idx] = v;
+        for (int k = 0; k < 4; ++k) v = add_mod(v, epart_q[(blk * 4 + k) * 4 * kN + w], q);
+        ext[qi * S.ext + idx] = v;
     }
 }
 
@@ -455,9 +510,12 @@ __global__ void __launch_bounds__(256) k_ext_sum(const uint32_t* __restrict__ ep
 
 // (ext[j].a - ext[i+j].a * X^i) -> INTT -> pcoef[j][p]: grid (i, 2 p)
 __global__ void __launch_bounds__(kNttThreads) k_pack_coef(const uint32_t* __restrict__ ext,
-        uint32_t* __restrict__ pcoef, int i, Tabs T) {
+        uint32_t* __restrict__ pcoef, int i, QS S, Tabs T) {
     __shared__ uint32_t s[kPadN];
-    const int j = blockIdx.x, p = blockIdx.y;
+    const int j = blockIdx.x, p = blockIdx.y & 1;
+    const size_t qi = blockIdx.y >> 1;
+    ext += qi * S.ext;
+    pcoef += qi * S.pcoef;
     const PrimeTab& P = T.p[p];
     const uint32_t* ej = ext + ((size_t)j * 4 + p) * kN;
     const uint32_t* ei = ext + ((size_t)(i + j) * 4 + p) * kN;
@@ -471,9 +529,13 @@ __global__ void __launch_bounds__(kNttThreads) k_pack_coef(const uint32_t* __res
 // answer.cpp:576-648 evalAuto: digit k of decompose(tau_g(a)) (ell 4, PP 0,
 // Bg 2^14), NTT, times packing key: grid (i, 4, 2 p);  ppart: [i][4][2][2][N]
 __global__ void __launch_bounds__(kNttThreads) k_pack_digit(const uint32_t* __restrict__ pcoef,
-        const uint32_t* __restrict__ pkey, uint32_t* __restrict__ ppart, uint32_t ginv, Tabs T) {
+        const KeyPtrs* __restrict__ kp, int li, uint32_t* __restrict__ ppart, uint32_t ginv, QS S, Tabs T) {
     __shared__ uint32_t s[kPadN];
-    const int j = blockIdx.x, k = blockIdx.y, p = blockIdx.z;
+    const int j = blockIdx.x, k = blockIdx.y, p = blockIdx.z & 1;
+    const size_t qi = blockIdx.z >> 1;
+    pcoef += qi * S.pcoef;
+    ppart += qi * S.ppart;
+    const uint32_t* pkey = kp[qi].packing + (size_t)li * 4 * 2 * 2 * kN;
     const PrimeTab& P = T.p[p];
     const uint32_t* a1 = pcoef + ((size_t)j * 2 + 0) * kN;
     const uint32_t* a2 = a1 + kN;
@@ -495,8 +557,12 @@ __global__ void __launch_bounds__(kNttThreads) k_pack_digit(const uint32_t* __re
 //   c' = ext[i+j]*X^i + ext[j];  new = -sum_k ppart  (+ perm_g(ext[j].b - ext[i+j].b*X^i) for b)
 //   next[j] = new + c'
 __global__ void __launch_bounds__(kNttThreads) k_pack_combine(const uint32_t* __restrict__ ext,
-        const uint32_t* __restrict__ ppart, uint32_t* __restrict__ next, int i, uint32_t g, Tabs T) {
-    const int j = blockIdx.x, ab = blockIdx.y, p = blockIdx.z;
+        const uint32_t* __restrict__ ppart, uint32_t* __restrict__ next, int i, uint32_t g, QS S, Tabs T) {
+    const int j = blockIdx.x, ab = blockIdx.y, p = blockIdx.z & 1;
+    const size_t qi = blockIdx.z >> 1;
+    ext += qi * S.ext;
+    ppart += qi * S.ppart;
+    next += qi * S.ext;
     const PrimeTab& P = T.p[p];
     const uint32_t* ej = ext + ((size_t)j * 4 + ab * 2 + p) * kN;
     const uint32_t* ei = ext + ((size_t)(i + j) * 4 + ab * 2 + p) * kN;
@@ -517,13 +583,13 @@ __global__ void __launch_bounds__(kNttThreads) k_pack_combine(const uint32_t* __
 }
 
 // INTT of both primes and CRT to u64 words in [0, Q): grid (2 ab, cnt)
-// ext: [cnt][2 ab][2 p][N] NTT domain -> resp: [cnt][2][N] coefficient form
+// ext + c*ext_stride: [2 ab][2 p][N] NTT domain -> resp: [cnt][2][N] coefficient form
 __global__ void __launch_bounds__(kNttThreads) k_response(const uint32_t* __restrict__ ext,
-                                                          uint64_t* __restrict__ resp, Tabs T) {
+                                                          uint64_t* __restrict__ resp, size_t ext_stride, Tabs T) {
     __shared__ uint32_t s1[kPadN];
     __shared__ uint32_t s2[kPadN];
     const int ab = blockIdx.x;
-    const uint32_t* e = ext + (size_t)blockIdx.y * 4 * kN;
+    const uint32_t* e = ext + (size_t)blockIdx.y * ext_stride;
     for (int x = threadIdx.x; x < kN; x += kNttThreads) {
         s1[lds_pad(x)] = e[(ab * 2 + 0) * kN + x];
         s2[lds_pad(x)] = e[(ab * 2 + 1) * kN + x];
@@ -550,14 +616,29 @@ __global__ void __launch_bounds__(kNttThreads) k_u64_to_rns(const uint64_t* __re
     for (int x = threadIdx.x; x < kN; x += kNttThreads) dst[x] = s[lds_pad(x)];
 }
 
-// rotation-0 words of the packed rotation buffer -> u64 [2][N] mod Q
+// reorientCipher's packed buffer (twosteps.cpp:1386-1403) from the rotation
+// residues: out[z][k][ab] = (x mod q1) | (x mod q2) << 32
+__global__ void __launch_bounds__(256) k_ctw_pack(const uint32_t* __restrict__ ctw, uint64_t* __restrict__ out,
+                                                  int n1) {
+    const size_t total = (size_t)kN * n1 * 2;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (size_t)gridDim.x * blockDim.x) {
+        const int ab = (int)(idx & 1);
+        const size_t k = (idx >> 1) % n1;
+        const size_t z = (idx >> 1) / n1;
+        const uint32_t* w = ctw + (k * 2 + ab) * 2 * kN + z;
+        out[idx] = (uint64_t)w[0] | ((uint64_t)w[kN] << 32);
+    }
+}
+
+// rotation-0 words of the rotation buffer -> u64 [2][N] mod Q
 __global__ void __launch_bounds__(256) k_ctw0_to_u64(const uint32_t* __restrict__ ctw, uint64_t* __restrict__ out,
                                                      int n1, Tabs T) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= 2 * kN) return;
     const int ab = idx / kN, x = idx % kN;
-    const uint32_t* w = ctw + (((size_t)x * n1 + 0) * 2 + ab) * 2;
-    out[idx] = crt_q(w[0], w[1], T);
+    const uint32_t* w = ctw + (size_t)(0 * 2 + ab) * 2 * kN + x;
+    out[idx] = crt_q(w[0], w[kN], T);
 }
 
 // RLWE words mod Q, RNS planes [cnt][2 ab][2 p][N] -> u64 [cnt][2][N]

@@ -134,3 +134,35 @@ def test_keys_and_db_shape_checks(ctx, oracle):
         ctx.answer(db, keys, inst.query)
     db.close()
     keys.close()
+
+
+def test_batch_matches_single_and_oracle(ctx, oracle):
+    """a batch of queries from different clients (own secrets and keys, seeds
+    11..16, random targets) gives, for every query, the words the query
+    gets alone, and those equal the oracle's"""
+    say("[gpu] batch r=4 n1=128 nq=6")
+    r, n1 = 4, 128
+    db_words = oracle.gen_db(r, n1, seed=99)
+    perms = oracle.perm_matrix(n1)
+    rng = np.random.default_rng(0)
+    clients = []
+    for c in range(6):
+        tcol, tpack = int(rng.integers(0, N // 2)), int(rng.integers(0, r))
+        clients.append(oracle.gen_client(r, n1, 11 + c, tcol, tpack))
+    db = ctx.database(db_words, r, n1)
+    keys = [ctx.keys(r, n1, cl["baby"], cl["giant"], cl["packing"], cl["rgsw"], perms) for cl in clients]
+    queries = np.stack([cl["query"] for cl in clients])
+    got = ctx.answer_batch(db, keys, queries)
+    for c, cl in enumerate(clients):
+        alone = ctx.answer(db, keys[c], cl["query"])
+        assert np.array_equal(got[c], alone), c
+        want = oracle.answer(r=r, n1=n1, db=db_words, query=cl["query"], rgsw=cl["rgsw"], baby=cl["baby"],
+                             giant=cl["giant"], packing=cl["packing"], perms=perms)
+        assert np.array_equal(got[c], want), c
+    # the same client twice in one batch, and a batch of one
+    got2 = ctx.answer_batch(db, [keys[1], keys[1]], np.stack([queries[1], queries[1]]))
+    assert np.array_equal(got2[0], got[1]) and np.array_equal(got2[1], got[1])
+    assert np.array_equal(ctx.answer_batch(db, [keys[3]], queries[3:4])[0], got[3])
+    for k in keys:
+        k.close()
+    db.close()
