@@ -60,7 +60,8 @@ void kspir_context_destroy(kspir_context* ctx);
 /* Preprocessed database, resident in HBM for the lifetime of the handle.
  * Replaces the `data` argument of matrix_vector_mul_bsgs_rns_crt_large
  * (twosteps.h:55, twosteps.cpp:1527); r blocks of N*N/2 words each.
- * `datacrt` is a host pointer, or a device pointer when on_device != 0. */
+ * `datacrt` is a host pointer, or a device pointer when on_device != 0.
+ * r: a power of two <= N; N1: a power of two in [32, N/2] (N2 = N/2/N1). */
 int kspir_database_create(kspir_context* ctx, const uint64_t* datacrt, int32_t r, int32_t n1,
                           int on_device, kspir_database** out);
 void kspir_database_destroy(kspir_database* db);

@@ -242,12 +242,16 @@ int launch_db_product(kspir_context* c, const kspir_database* db, int nq, hipStr
     const dim3 dg(kN / 64, r, nq);
     const QS& S = c->S;
     const Tabs& T = c->T;
-    switch (n2 < 16 ? n2 : 16) {
-        case 1: hipLaunchKernelGGL(k_db_product<1>, dg, dim3(64), 0, st, db->words, c->ctw, c->dbout, n1, n2, bw, obw, S, T); break;
-        case 2: hipLaunchKernelGGL(k_db_product<2>, dg, dim3(64), 0, st, db->words, c->ctw, c->dbout, n1, n2, bw, obw, S, T); break;
-        case 4: hipLaunchKernelGGL(k_db_product<4>, dg, dim3(64), 0, st, db->words, c->ctw, c->dbout, n1, n2, bw, obw, S, T); break;
-        case 8: hipLaunchKernelGGL(k_db_product<8>, dg, dim3(64), 0, st, db->words, c->ctw, c->dbout, n1, n2, bw, obw, S, T); break;
-        default: hipLaunchKernelGGL(k_db_product<16>, dg, dim3(64), 0, st, db->words, c->ctw, c->dbout, n1, n2, bw, obw, S, T); break;
+    // rows per wave: N2 / 4 rounded up (4 waves per workgroup)
+    const int rpw = n2 <= 4 ? 1 : n2 / 4;
+    switch (rpw) {
+        case 1: hipLaunchKernelGGL(k_db_product<1>, dg, dim3(256), 0, st, db->words, c->ctw, c->dbout, n1, n2, bw, obw, S, T); break;
+        case 2: hipLaunchKernelGGL(k_db_product<2>, dg, dim3(256), 0, st, db->words, c->ctw, c->dbout, n1, n2, bw, obw, S, T); break;
+        case 4: hipLaunchKernelGGL(k_db_product<4>, dg, dim3(256), 0, st, db->words, c->ctw, c->dbout, n1, n2, bw, obw, S, T); break;
+        case 8: hipLaunchKernelGGL(k_db_product<8>, dg, dim3(256), 0, st, db->words, c->ctw, c->dbout, n1, n2, bw, obw, S, T); break;
+        case 16: hipLaunchKernelGGL(k_db_product<16>, dg, dim3(256), 0, st, db->words, c->ctw, c->dbout, n1, n2, bw, obw, S, T); break;
+        case 32: hipLaunchKernelGGL(k_db_product<32>, dg, dim3(256), 0, st, db->words, c->ctw, c->dbout, n1, n2, bw, obw, S, T); break;
+        default: return fail(KSPIR_EINVAL, "unsupported N2 for the database pass");
     }
     LAUNCH_OK("k_db_product");
     return KSPIR_OK;
@@ -438,8 +442,8 @@ void* kspir_context_stream(kspir_context* c) { return c ? (void*)c->stream : nul
 int kspir_database_create(kspir_context* c, const uint64_t* datacrt, int32_t r, int32_t n1, int on_device,
                           kspir_database** out) {
     if (!c || !datacrt || !out) return fail(KSPIR_EINVAL, "null argument");
-    if (!pow2(r) || r > kN || !pow2(n1) || n1 < 4 || n1 > kN / 2)
-        return fail(KSPIR_EINVAL, "r must be a power of two <= N, N1 a power of two in [4, N/2]");
+    if (!pow2(r) || r > kN || !pow2(n1) || n1 < 32 || n1 > kN / 2)
+        return fail(KSPIR_EINVAL, "r must be a power of two <= N, N1 a power of two in [32, N/2]");
     HIP_OK(hipSetDevice(c->device));
     const int n2 = kN / 2 / n1;
     const size_t words = (size_t)kN * kN / 2;
@@ -478,8 +482,8 @@ void kspir_database_destroy(kspir_database* db) {
 int kspir_keys_create(kspir_context* c, int32_t r, int32_t n1, const uint64_t* baby, const uint64_t* giant,
                       const uint64_t* packing, const uint64_t* rgsw, const int32_t* permutations, kspir_keys** out) {
     if (!c || !baby || !permutations || !out) return fail(KSPIR_EINVAL, "null argument");
-    if (!pow2(r) || r > kN || !pow2(n1) || n1 < 4 || n1 > kN / 2)
-        return fail(KSPIR_EINVAL, "r must be a power of two <= N, N1 a power of two in [4, N/2]");
+    if (!pow2(r) || r > kN || !pow2(n1) || n1 < 32 || n1 > kN / 2)
+        return fail(KSPIR_EINVAL, "r must be a power of two <= N, N1 a power of two in [32, N/2]");
     const int n2 = kN / 2 / n1;
     if (n2 > 1 && !giant) return fail(KSPIR_EINVAL, "giant-step keys required when N2 > 1");
     if (r > 1 && !packing) return fail(KSPIR_EINVAL, "packing keys required when r > 1");

@@ -325,30 +325,36 @@ __global__ void __launch_bounds__(kNttThreads) k_baby_mid_digits(const uint32_t*
 // ------------------------------------------------------------- database
 
 // crt.cpp:643-768 restated for a slot-minor layout: one lane per NTT slot z.
+// A workgroup is 4 waves on the same 64 slots; wave w takes rows w, w+4, ...
+// so the rotation words of a slot are fetched once per workgroup (L1) and
+// every database word exactly once.  Sums are exact u64 (N1 * (q-1)^2 < 2^64
+// for N1 <= 256) and reduced once per output.
 // db: [N2][N1][N] packed (x mod q1) | (x mod q2) << 32 for one block
 // ctw: [N1][2 ab][2 p][N] rotation residues, slot-minor (k = 0..N1-1)
-// out: [N2][2 ab][2 primes][N] residues
-template <int RG>
-__global__ void __launch_bounds__(64) k_db_product(const uint64_t* __restrict__ db, const uint32_t* __restrict__ ctw,
-                                                   uint32_t* __restrict__ out, int n1, int n2,
-                                                   size_t block_words, size_t out_block_words, QS S, Tabs T) {
-    const int z = blockIdx.x * 64 + threadIdx.x;
+// out: [N2][2 ab][2 primes][N] residues.  grid (N/64, r, nq), 256 threads.
+template <int RPW>
+__global__ void __launch_bounds__(256) k_db_product(const uint64_t* __restrict__ db, const uint32_t* __restrict__ ctw,
+                                                    uint32_t* __restrict__ out, int n1, int n2,
+                                                    size_t block_words, size_t out_block_words, QS S, Tabs T) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int z = blockIdx.x * 64 + lane;
     const int blk = blockIdx.y;
     const size_t qi = blockIdx.z;
-    ctw += qi * S.ctw;
-    const uint64_t* dbb = db + blk * block_words;
-    uint32_t* ob = out + qi * S.dbout + blk * out_block_words;
-    const uint32_t* ct = ctw + z;
-    for (int r0 = 0; r0 < n2; r0 += RG) {
-        uint64_t a1[RG], a2[RG], b1[RG], b2[RG];
+    const uint32_t* ct = ctw + qi * S.ctw + z;
+    const uint64_t* dbb = db + blk * block_words + z;
+    uint32_t* ob = out + qi * S.dbout + blk * out_block_words + z;
+    uint64_t a1[RPW], a2[RPW], b1[RPW], b2[RPW];
 #pragma unroll
-        for (int i = 0; i < RG; ++i) a1[i] = a2[i] = b1[i] = b2[i] = 0;
-        for (int k = 0; k < n1; ++k) {
-            const uint32_t* c = ct + (size_t)k * 4 * kN;
-            const uint32_t ca1 = c[0], ca2 = c[kN], cb1 = c[2 * kN], cb2 = c[3 * kN];
+    for (int i = 0; i < RPW; ++i) a1[i] = a2[i] = b1[i] = b2[i] = 0;
+#pragma unroll 2
+    for (int k = 0; k < n1; ++k) {
+        const uint32_t* c = ct + (size_t)k * 4 * kN;
+        const uint32_t ca1 = c[0], ca2 = c[kN], cb1 = c[2 * kN], cb2 = c[3 * kN];
 #pragma unroll
-            for (int i = 0; i < RG; ++i) {
-                const uint64_t d = __builtin_nontemporal_load(dbb + ((size_t)(r0 + i) * n1 + k) * kN + z);
+        for (int i = 0; i < RPW; ++i) {
+            const int row = wv + 4 * i;
+            if (row < n2) {
+                const uint64_t d = __builtin_nontemporal_load(dbb + ((size_t)row * n1 + k) * kN);
                 const uint32_t d1 = (uint32_t)d, d2 = (uint32_t)(d >> 32);
                 a1[i] += (uint64_t)d1 * ca1;
                 a2[i] += (uint64_t)d2 * ca2;
@@ -356,9 +362,12 @@ __global__ void __launch_bounds__(64) k_db_product(const uint64_t* __restrict__ 
                 b2[i] += (uint64_t)d2 * cb2;
             }
         }
+    }
 #pragma unroll
-        for (int i = 0; i < RG; ++i) {
-            uint32_t* o = ob + (size_t)(r0 + i) * 4 * kN + z;
+    for (int i = 0; i < RPW; ++i) {
+        const int row = wv + 4 * i;
+        if (row < n2) {
+            uint32_t* o = ob + (size_t)row * 4 * kN;
             o[0 * kN] = (uint32_t)(a1[i] % kQ1);
             o[1 * kN] = (uint32_t)(a2[i] % kQ2);
             o[2 * kN] = (uint32_t)(b1[i] % kQ1);
