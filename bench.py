@@ -1,14 +1,16 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X KsPIR online answer.
 
-One step = one online server response (test-pir.cpp:107-116: BSGS first
-dimension over the r database blocks, r external products, packing) for one
-query against a 256 MB database (32768 records x 8 KB: N 4096, r 16, N1 128,
-N2 16), with the database, keys and query resident in HBM before timing.
-Inputs are synthetic (uniform residues of the right moduli; the answer's
-cost does not depend on their values).  Prints one JSON line.
+One step = one batched online server response: B queries (default 64) from
+B clients with their own keys, each answered as test-pir.cpp:107-116 does
+(BSGS first dimension over the r database blocks, r external products,
+packing), against one 256 MB database (32768 records x 8 KB: N 4096, r 16,
+N1 128, N2 16).  Database, keys and queries are resident in HBM before
+timing.  The plain single-query answer is timed in the same run and reported
+beside it.  Inputs are synthetic (uniform residues of the right moduli; the
+answer's cost does not depend on their values).  Prints one JSON line.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--db-mb 256] [--no-cpu-baseline]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch 64] [--db-mb 256] [--no-cpu-baseline]
 """
 import argparse
 import json
@@ -29,15 +31,20 @@ HBM_PEAK_GBPS = 8000.0
 README_SECONDS_256MB = 0.236635     # README.md:62 online response, 1 query, 256 MB (CPU, HEXL)
 
 
-def synthetic_inputs(r, n1, seed=1234):
-    """uniform residues in the reference layouts (see include/kspir_amd.h)"""
+def synthetic_db(r, seed=1234):
+    """uniform residue pairs in the database_tocrt word layout"""
     rng = np.random.default_rng(seed)
-    n2 = N // 2 // n1
     words = r * N * N // 2
     lo = rng.integers(0, Q1, words, dtype=np.uint64)
     hi = rng.integers(0, Q2, words, dtype=np.uint64)
-    db = (lo | (hi << np.uint64(32))).reshape(r, -1)
-    del lo, hi
+    return (lo | (hi << np.uint64(32))).reshape(r, -1)
+
+
+def synthetic_client(r, n1, seed):
+    """one client's query and keys: uniform residues of the right moduli in
+    the reference layouts (see include/kspir_amd.h)"""
+    rng = np.random.default_rng(seed)
+    n2 = N // 2 // n1
     query = np.empty((2, 2, N), dtype=np.uint64)
     query[0] = rng.integers(0, Q, (2, N), dtype=np.uint64)
     query[1] = rng.integers(0, BS, (2, N), dtype=np.uint64)
@@ -47,7 +54,7 @@ def synthetic_inputs(r, n1, seed=1234):
     giant = rng.integers(0, Q, (n2 - 1, 2, 2, N), dtype=np.uint64)
     packing = rng.integers(0, Q, (max(r.bit_length() - 1, 0), 4, 2, N), dtype=np.uint64)
     rgsw = rng.integers(0, Q, (4, 2, N), dtype=np.uint64)
-    return db, query, baby, giant, packing, rgsw
+    return query, baby, giant, packing, rgsw
 
 
 def traffic_from_profile(kernel="k_db_product"):
@@ -84,11 +91,13 @@ def cpu_baseline(r, n1, reps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=50, help="timed batched answers")
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--batch", type=int, default=64, help="queries per step (distinct clients)")
     ap.add_argument("--db-mb", type=int, default=256, help="database size: 16 MB per block (r = MB/16)")
     ap.add_argument("--n1", type=int, default=128)
-    ap.add_argument("--lat-samples", type=int, default=100)
+    ap.add_argument("--plain-steps", type=int, default=100)
+    ap.add_argument("--lat-samples", type=int, default=50)
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -98,33 +107,42 @@ def main():
         sys.exit(2)
 
     import kspir_amd
-    r, n1 = args.db_mb // 16, args.n1
+    r, n1, B = args.db_mb // 16, args.n1, args.batch
     n2 = N // 2 // n1
-    print(f"[bench] generating synthetic inputs r={r} n1={n1}", file=sys.stderr, flush=True)
-    db_h, query, baby, giant, packing, rgsw = synthetic_inputs(r, n1)
-    perms = kspir_amd.compute_permutation_matrix(n1)
+    log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)
+    log(f"synthetic database r={r} n1={n1}")
     ctx = kspir_amd.Context(0)
-    db = ctx.database(db_h, r, n1)
-    del db_h
-    keys = ctx.keys(r, n1, baby, giant, packing, rgsw, perms)
-    qbuf = ctx.alloc(query.nbytes)
-    qbuf.upload(query)
-    rbuf = ctx.alloc(2 * N * 8)
+    db = ctx.database(synthetic_db(r), r, n1)
+    perms = kspir_amd.compute_permutation_matrix(n1)
+    log(f"{B} synthetic clients")
+    keys, queries = [], []
+    for c in range(B):
+        query, baby, giant, packing, rgsw = synthetic_client(r, n1, 1000 + c)
+        keys.append(ctx.keys(r, n1, baby, giant, packing, rgsw, perms))
+        queries.append(query)
+    qbuf = ctx.alloc(B * 4 * N * 8)
+    qbuf.upload(np.stack(queries))
+    rbuf = ctx.alloc(B * 2 * N * 8)
 
-    print("[bench] timing", file=sys.stderr, flush=True)
+    # plain single-query answer (client 0), one query in flight
+    log("timing plain single-query answers")
     ctx.synchronize()
-    total_ms = ctx.time_answer(db, keys, qbuf, rbuf, args.warmup, args.steps)
-    ctx.synchronize()
+    plain_ms = ctx.time_answer(db, keys[0], qbuf, rbuf, 10, args.plain_steps) / args.plain_steps
+    plain_lat = [ctx.time_answer(db, keys[0], qbuf, rbuf, 0, 1) for _ in range(args.lat_samples)]
+
+    # batched answer: B queries of B clients per step
+    log(f"timing batched answers (B={B})")
+    total_ms = ctx.time_answer_batch(db, keys, qbuf, rbuf, args.warmup, args.steps)
     ms_per_step = total_ms / args.steps
-    qps = 1000.0 / ms_per_step
+    qps = B * 1000.0 / ms_per_step
+    batch_lat = [ctx.time_answer_batch(db, keys, qbuf, rbuf, 0, 1) for _ in range(max(5, args.lat_samples // 5))]
 
-    # per-query latency (submit -> response in HBM), one query in flight
-    lats = [ctx.time_answer(db, keys, qbuf, rbuf, 0, 1) for _ in range(args.lat_samples)]
-    # dominant kernel: the database pass, timed alone with HIP events on the
-    # context stream; algorithmic bytes = 8 B per database word, r*N*N/2 words
-    db_ms = ctx.time_db_pass(db, keys, qbuf, 20)
+    # dominant single-query kernel: the database pass, timed alone with HIP
+    # events on the context stream; algorithmic bytes = 8 B per word, r*N*N/2 words
+    db_ms = ctx.time_db_pass(db, keys[0], qbuf, 20)
     db_bytes = 8 * r * N * (N // 2)
     achieved = db_bytes / (db_ms * 1e-3) / 1e9
+    plain_qps = 1000.0 / plain_ms
 
     out = {
         "metric": "PIR queries/sec + DB GB/s at 256 MB, batched vs plain 1-query; p50/p99 latency",
@@ -136,30 +154,39 @@ def main():
         "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": round(qps / (1.0 / README_SECONDS_256MB), 1) if args.db_mb == 256 and n1 == 128 else None,
+        "vs_baseline": round(qps * README_SECONDS_256MB, 1) if args.db_mb == 256 and n1 == 128 else None,
         "dtype": "uint64",
         "data": "synthetic",
         "config": {
-            "workload": f"plain single-query online answer, {args.db_mb} MB database "
-                        f"({r * N // 2} records x 8 KB), N=4096, r={r}, N1={n1}, N2={n2}, 1 query/step",
-            "records": r * N // 2, "record_bytes": 8192, "r": r, "N1": n1, "N2": n2, "queries_per_step": 1,
+            "workload": f"batched online answer: {B} queries/step from {B} clients with distinct keys, "
+                        f"one {args.db_mb} MB database ({r * N // 2} records x 8 KB), N=4096, r={r}, "
+                        f"N1={n1}, N2={n2}",
+            "records": r * N // 2, "record_bytes": 8192, "r": r, "N1": n1, "N2": n2, "queries_per_step": B,
         },
         "db_GBps_raw": round(r * N * (N // 2) * 2 * qps / 1e9, 2),
-        "db_GBps_streamed": round(db_bytes * qps / 1e9, 2),
-        "latency_ms": {"p50": round(float(np.percentile(lats, 50)), 4),
-                       "p99": round(float(np.percentile(lats, 99)), 4)},
-        "roofline": {"kernel": "k_db_product", "bound": "hbm", "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                     "traffic": traffic_from_profile(), "algorithmic_bytes": db_bytes,
-                     "avg_launch_ms": round(db_ms, 4)},
+        "latency_ms": {"p50": round(float(np.percentile(batch_lat, 50)), 4),
+                       "p99": round(float(np.percentile(batch_lat, 99)), 4)},
+        "plain_single_query": {
+            "qps": round(plain_qps, 2), "ms": round(plain_ms, 4),
+            "db_GBps_raw": round(r * N * (N // 2) * 2 * plain_qps / 1e9, 2),
+            "db_GBps_streamed": round(db_bytes * plain_qps / 1e9, 2),
+            "latency_ms": {"p50": round(float(np.percentile(plain_lat, 50)), 4),
+                           "p99": round(float(np.percentile(plain_lat, 99)), 4)},
+        },
+        "batched_over_plain": round(qps / plain_qps, 2),
+        "roofline": {"kernel": "k_db_product (single-query database pass)", "bound": "hbm",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic_from_profile(),
+                     "algorithmic_bytes": db_bytes, "avg_launch_ms": round(db_ms, 4)},
     }
-    keys.close()
+    for k in keys:
+        k.close()
     db.close()
     qbuf.free()
     rbuf.free()
     ctx.close()
     if not args.no_cpu_baseline:
-        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(r, n1, args.cpu_reps)
     print(json.dumps(out), flush=True)
 

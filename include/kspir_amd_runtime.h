@@ -17,6 +17,13 @@ extern "C" {
 #endif
 
 int kspir_device_count(int* count);
+
+/* Tuning options of a context:
+ *   "mfma_min_batch"  batches of at least this many queries take the int8-MFMA
+ *                     database pass (default 2); smaller ones the VALU pass
+ *   "mfma_max_chunk"  most queries per MFMA database sweep (default: as many
+ *                     as the LDS holds) */
+int kspir_context_set(kspir_context* ctx, const char* key, int64_t value);
 int kspir_device_alloc(kspir_context* ctx, size_t bytes, void** out);
 int kspir_device_free(kspir_context* ctx, void* ptr);
 int kspir_memcpy_h2d(kspir_context* ctx, void* dst, const void* src, size_t bytes);

@@ -38,7 +38,7 @@ def _load():
                  "kspir_memcpy_h2d", "kspir_memcpy_d2h", "kspir_synchronize", "kspir_time_answer",
                  "kspir_time_db_pass", "kspir_test_ntt", "kspir_device_count",
                  "kspir_compute_permutation_matrix", "kspir_answer_batch", "kspir_answer_batch_device",
-                 "kspir_time_answer_batch"):
+                 "kspir_time_answer_batch", "kspir_context_set"):
         getattr(L, name).restype = ctypes.c_int
     L.kspir_context_destroy.argtypes = [_vp]
     L.kspir_database_destroy.argtypes = [_vp]
@@ -54,7 +54,7 @@ EXPORTS = (
     "kspir_database_destroy", "kspir_keys_create", "kspir_keys_destroy", "kspir_answer",
     "kspir_answer_device", "kspir_matrix_vector_mul_bsgs_rns_crt_large", "kspir_external_product",
     "kspir_packing_rlwes", "kspir_answer_inspect", "kspir_compute_permutation_matrix",
-    "kspir_answer_batch", "kspir_answer_batch_device", "kspir_time_answer_batch",
+    "kspir_answer_batch", "kspir_answer_batch_device", "kspir_time_answer_batch", "kspir_context_set",
     "kspir_device_count", "kspir_device_alloc", "kspir_device_free", "kspir_memcpy_h2d", "kspir_memcpy_d2h",
     "kspir_synchronize", "kspir_context_stream", "kspir_time_answer", "kspir_time_db_pass", "kspir_test_ntt",
 )
@@ -163,6 +163,9 @@ class Context:
 
     def __exit__(self, *a):
         self.close()
+
+    def set(self, key, value):
+        _check(lib.kspir_context_set(self.h, key.encode(), ctypes.c_int64(value)))
 
     @property
     def stream(self):

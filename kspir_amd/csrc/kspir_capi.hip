@@ -95,12 +95,18 @@ struct kspir_context {
     KeyPtrs* kp_dev = nullptr;
     KeyPtrs* kp_host = nullptr;     // pinned staging
     int kp_cap = 0, kp_count = 0;
+    // batched database pass scratch (query limbs + tile-major products)
+    void* mws = nullptr;
+    size_t mws_bytes = 0;
+    int mfma_min_batch = 2;         // batches of at least this many queries use the MFMA pass
+    int mfma_max_chunk = 1 << 30;   // cap on queries per MFMA database sweep (LDS also caps it)
 };
 
 struct kspir_database {
     kspir_context* ctx;
     int r, n1, n2;
-    uint64_t* words;   // [r][N2][N1][N]
+    uint64_t* words;            // [r][N2][N1][N]
+    int8_t* limbs = nullptr;    // [N][2][4][r*N2][N1] 7-bit limbs, built on the first batched call
 };
 
 struct kspir_keys {
@@ -257,6 +263,69 @@ int launch_db_product(kspir_context* c, const kspir_database* db, int nq, hipStr
     return KSPIR_OK;
 }
 
+bool mfma_shape(const kspir_database* db) {
+    return db->n1 % 64 == 0 && db->n1 <= 256 && (db->r * db->n2) % 16 == 0;
+}
+
+// batched database pass on int8 MFMA (one database read per chunk of up to
+// nqc_max queries); results land in c->dbout exactly as k_db_product's
+int launch_db_batched(kspir_context* c, const kspir_database* dbc, int nq, hipStream_t st) {
+    kspir_database* db = const_cast<kspir_database*>(dbc);
+    const int r = db->r, n1 = db->n1, n2 = db->n2, R = r * n2;
+    const size_t dbwords = (size_t)r * kN * (kN / 2);
+    if (!db->limbs) {
+        HIP_OK(hipMalloc(&db->limbs, dbwords * 8));
+        hipLaunchKernelGGL(k_db_limbs, dim3(kN / 64, R), dim3(256), 0, st, db->words, db->limbs, n1, R);
+        LAUNCH_OK("k_db_limbs");
+    }
+    const int lds_cap = 160 * 1024;
+    int cmax = lds_cap / (4 * (n1 + 16)) / 16 * 16;
+    if (cmax > 256) cmax = 256;
+    const int nqc_max = cmax / 2 < c->mfma_max_chunk ? cmax / 2 : c->mfma_max_chunk;
+    const int C_full = ((2 * (nq < nqc_max ? nq : nqc_max)) + 15) / 16 * 16;
+    const size_t ql_bytes = (size_t)kN * 2 * 4 * C_full * n1;
+    const size_t mf_bytes = (size_t)2 * R * C_full * kN * 4;
+    if (ql_bytes + mf_bytes > c->mws_bytes) {
+        if (c->mws) { HIP_OK(hipStreamSynchronize(st)); HIP_OK(hipFree(c->mws)); c->mws = nullptr; }
+        HIP_OK(hipMalloc(&c->mws, ql_bytes + mf_bytes));
+        c->mws_bytes = ql_bytes + mf_bytes;
+    }
+    int8_t* ql = (int8_t*)c->mws;
+    uint32_t* mf = (uint32_t*)((char*)c->mws + ql_bytes);
+    MfmaConst MC;
+    for (int p = 0; p < 2; ++p) {
+        const u64 q = c->T.p[p].q;
+        for (int w = 0; w < 7; ++w) MC.cw[p][w] = (uint32_t)(((u128)powmod(2, 7 * w, q) << 32) % q);
+    }
+    const QS& S = c->S;
+    for (int q0 = 0; q0 < nq; q0 += nqc_max) {
+        const int nqc = nq - q0 < nqc_max ? nq - q0 : nqc_max;
+        const int C = (2 * nqc + 15) / 16 * 16;
+        hipLaunchKernelGGL(k_ct_limbs, dim3(kN / 64, C / 2, 2), dim3(256), 0, st, c->ctw + q0 * S.ctw, ql, n1, nqc, C, S);
+        LAUNCH_OK("k_ct_limbs");
+        const size_t lds = (size_t)4 * C * (n1 + 16);
+        switch (n1) {
+            case 64:
+                HIP_OK(hipFuncSetAttribute((const void*)k_db_mfma<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                hipLaunchKernelGGL(k_db_mfma<1>, dim3(kN), dim3(256), lds, st, db->limbs, ql, mf, R, C, MC, c->T);
+                break;
+            case 128:
+                HIP_OK(hipFuncSetAttribute((const void*)k_db_mfma<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                hipLaunchKernelGGL(k_db_mfma<2>, dim3(kN), dim3(256), lds, st, db->limbs, ql, mf, R, C, MC, c->T);
+                break;
+            default:
+                HIP_OK(hipFuncSetAttribute((const void*)k_db_mfma<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                hipLaunchKernelGGL(k_db_mfma<4>, dim3(kN), dim3(256), lds, st, db->limbs, ql, mf, R, C, MC, c->T);
+                break;
+        }
+        LAUNCH_OK("k_db_mfma");
+        hipLaunchKernelGGL(k_mfma_scatter, dim3(kN / 32, (R / 16) * (C / 16), 2), dim3(256), 0, st, mf,
+                           c->dbout + q0 * S.dbout, R, C, n2, nqc, S);
+        LAUNCH_OK("k_mfma_scatter");
+    }
+    return KSPIR_OK;
+}
+
 // packing tree on the NTT-domain planes at `cur` for nq queries; returns the
 // buffer that holds the packed RLWE of each query (answer.cpp:800-835)
 int enqueue_packing(kspir_context* c, const KeyPtrs* kp, int r, int nq, hipStream_t st, uint32_t** out) {
@@ -307,8 +376,10 @@ int enqueue_answer(kspir_context* c, const kspir_database* db, const KeyPtrs* kp
     hipLaunchKernelGGL(k_baby_rot, dim3(n1h, 2, nq), dim3(kNttThreads), 0, st, c->D1, c->mid + 3 * kN, kp, c->ctw,
                        (uint32_t*)nullptr, n1, n1h, -1, S.D, S.mid, S, T);
     LAUNCH_OK("k_baby_rot(second)");
-    // database pass (crt.cpp:643-768)
-    int rc = launch_db_product(c, db, nq, st);
+    // database pass (crt.cpp:643-768): int8-MFMA limb GEMM for a batch,
+    // u64-accumulating VALU stream for a single query
+    int rc = (nq >= c->mfma_min_batch && mfma_shape(db)) ? launch_db_batched(c, db, nq, st)
+                                                          : launch_db_product(c, db, nq, st);
     if (rc) return rc;
     // giant steps (twosteps.cpp:599-677)
     if (n2 > 1) {
@@ -433,6 +504,7 @@ void kspir_context_destroy(kspir_context* c) {
     if (c->ws) hipFree(c->ws);
     if (c->kp_dev) hipFree(c->kp_dev);
     if (c->kp_host) hipHostFree(c->kp_host);
+    if (c->mws) hipFree(c->mws);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -476,6 +548,7 @@ void kspir_database_destroy(kspir_database* db) {
     hipSetDevice(db->ctx->device);
     hipStreamSynchronize(db->ctx->stream);
     hipFree(db->words);
+    if (db->limbs) hipFree(db->limbs);
     delete db;
 }
 
@@ -717,6 +790,21 @@ int kspir_packing_rlwes(kspir_context* c, const kspir_keys* k, uint64_t* result,
 }
 
 // ------------------------------------------------------------ runtime plumbing
+
+int kspir_context_set(kspir_context* c, const char* key, int64_t value) {
+    if (!c || !key) return fail(KSPIR_EINVAL, "null argument");
+    const std::string k(key);
+    if (k == "mfma_min_batch") {
+        if (value < 1) return fail(KSPIR_EINVAL, "mfma_min_batch must be >= 1");
+        c->mfma_min_batch = (int)value;
+    } else if (k == "mfma_max_chunk") {
+        if (value < 1) return fail(KSPIR_EINVAL, "mfma_max_chunk must be >= 1");
+        c->mfma_max_chunk = (int)value;
+    } else {
+        return fail(KSPIR_EINVAL, "unknown option " + k);
+    }
+    return KSPIR_OK;
+}
 
 int kspir_device_alloc(kspir_context* c, size_t bytes, void** out) {
     if (!c || !out) return fail(KSPIR_EINVAL, "null argument");

@@ -376,6 +376,166 @@ __global__ void __launch_bounds__(256) k_db_product(const uint64_t* __restrict__
     }
 }
 
+// ------------------------------------------------ batched database pass
+// For a batch the per-slot product out[rho][col] = sum_k D[z][rho][k] *
+// C[z][k][col] (rho = block*N2 + row, col = query*2 + a|b) is a small GEMM
+// per slot z and residue.  Both operands are split into four 7-bit limbs
+// (residues < 2^28), every limb product is an exact int8 MFMA with int32
+// accumulation, and the limb pairs are summed by diagonal
+//   S_w = sum_{u+v=w} A_u B_v,   w = 0..6,   0 <= S_w < 4*127^2*N1 <= 2^24 < q
+// so the residue is V = sum_w S_w * 2^(7w) mod q = mont(sum_w S_w * c_w)
+// with c_w = 2^(7w) * 2^32 mod q.  Requires N1 % 64 == 0, N1 <= 256 and
+// r*N2 % 16 == 0.
+
+typedef int8_t i8x16 __attribute__((ext_vector_type(16)));
+typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+
+// database limbs, once per database: words [rho][k][z] -> dbl [z][p][u][rho][k]
+// grid (N/64, R), 256 threads
+__global__ void __launch_bounds__(256) k_db_limbs(const uint64_t* __restrict__ words, int8_t* __restrict__ dbl,
+                                                  int n1, int R) {
+    __shared__ uint64_t tile[64][65];
+    const int z0 = blockIdx.x * 64, rho = blockIdx.y;
+    for (int kc = 0; kc < n1; kc += 64) {
+        for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+            const int kk = e >> 6, zz = e & 63;
+            tile[kk][zz] = words[((size_t)rho * n1 + kc + kk) * kN + z0 + zz];
+        }
+        __syncthreads();
+        // 64 slots x 8 (p,u) x 4 chunks of 16 bytes
+        for (int e = threadIdx.x; e < 64 * 8 * 4; e += 256) {
+            const int c16 = e & 3, pu = (e >> 2) & 7, zz = e >> 5;
+            const int p = pu >> 2, u = pu & 3;
+            i8x16 v;
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                const uint64_t w = tile[c16 * 16 + t][zz];
+                const uint32_t x = p ? (uint32_t)(w >> 32) : (uint32_t)w;
+                v[t] = (int8_t)((x >> (7 * u)) & 127);
+            }
+            int8_t* dst = dbl + ((((size_t)(z0 + zz) * 2 + p) * 4 + u) * R + rho) * n1 + kc + c16 * 16;
+            *reinterpret_cast<i8x16*>(dst) = v;
+        }
+        __syncthreads();
+    }
+}
+
+// query limbs, per batch: ctw [q][k][ab][p][z] -> ql [z][p][v][col][k],
+// col = q*2 + ab, cols padded with zeros to C; ctw points at the chunk's
+// first query, nq = queries in the chunk.  grid (N/64, C/2, 2 p)
+__global__ void __launch_bounds__(256) k_ct_limbs(const uint32_t* __restrict__ ctw, int8_t* __restrict__ ql,
+                                                  int n1, int nq, int C, QS S) {
+    __shared__ uint32_t tile[64][2][65];
+    const int z0 = blockIdx.x * 64, qs = blockIdx.y, p = blockIdx.z;   // qs: query within the chunk
+    for (int kc = 0; kc < n1; kc += 64) {
+        for (int e = threadIdx.x; e < 64 * 2 * 64; e += 256) {
+            const int zz = e & 63, ab = (e >> 6) & 1, kk = e >> 7;
+            tile[kk][ab][zz] = qs < nq ? ctw[qs * S.ctw + ((size_t)((kc + kk) * 2 + ab) * 2 + p) * kN + z0 + zz] : 0u;
+        }
+        __syncthreads();
+        // 64 slots x 4 limbs x 2 ab x 4 chunks of 16 bytes
+        for (int e = threadIdx.x; e < 64 * 4 * 2 * 4; e += 256) {
+            const int c16 = e & 3, ab = (e >> 2) & 1, v = (e >> 3) & 3, zz = e >> 5;
+            i8x16 b;
+#pragma unroll
+            for (int t = 0; t < 16; ++t) b[t] = (int8_t)((tile[c16 * 16 + t][ab][zz] >> (7 * v)) & 127);
+            int8_t* dst = ql + ((((size_t)(z0 + zz) * 2 + p) * 4 + v) * C + qs * 2 + ab) * n1 + kc + c16 * 16;
+            *reinterpret_cast<i8x16*>(dst) = b;
+        }
+        __syncthreads();
+    }
+}
+
+struct MfmaConst {
+    uint32_t cw[2][7];   // 2^(7w) * 2^32 mod q_p
+};
+
+// one slot z per workgroup (4 waves).  The query limbs of the slot for one
+// residue are staged in LDS (row pitch n1 + 16 bytes: conflict-free 16-byte
+// reads); each wave streams the database limb tiles of its row tiles from
+// HBM once and sweeps all column tiles.  Output tile-major:
+//   mf [p][rt][ct][z][16 col][16 rho] (canonical residues).
+// Lane l of a 16x16x64 MFMA holds A row l%16 / B column l%16 with 16
+// k-bytes at 16*(l/16); D holds rows 4*(l/16)+j of column l%16.
+template <int KS>
+__global__ void __launch_bounds__(256) k_db_mfma(const int8_t* __restrict__ dbl, const int8_t* __restrict__ ql,
+                                                 uint32_t* __restrict__ mf, int R, int C, MfmaConst MC, Tabs T) {
+    extern __shared__ __attribute__((aligned(16))) int8_t lds_b[];
+    constexpr int n1 = KS * 64;
+    constexpr int pitch = n1 + 16;
+    const int z = blockIdx.x;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int RT = R / 16, CT = C / 16;
+    const int lr = lane & 15, lk = (lane >> 4) * 16;
+    for (int p = 0; p < 2; ++p) {
+        const PrimeTab& P = T.p[p];
+        const int8_t* qsrc = ql + ((size_t)z * 2 + p) * 4 * C * n1;
+        for (int e = threadIdx.x; e < 4 * C * (n1 / 16); e += 256) {
+            const int row = e / (n1 / 16), c16 = e % (n1 / 16);   // row = v*C + col
+            *reinterpret_cast<i8x16*>(lds_b + row * pitch + c16 * 16) =
+                *reinterpret_cast<const i8x16*>(qsrc + (size_t)row * n1 + c16 * 16);
+        }
+        __syncthreads();
+        const int8_t* asrc = dbl + ((size_t)z * 2 + p) * 4 * R * n1;
+        for (int rt = wv; rt < RT; rt += 4) {
+            i8x16 a[4][KS];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int ks = 0; ks < KS; ++ks)
+                    a[u][ks] = *reinterpret_cast<const i8x16*>(asrc + ((size_t)u * R + rt * 16 + lr) * n1 + ks * 64 + lk);
+            for (int ct = 0; ct < CT; ++ct) {
+                i32x4 acc[7];
+#pragma unroll
+                for (int w = 0; w < 7; ++w) acc[w] = i32x4{0, 0, 0, 0};
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+#pragma unroll
+                    for (int ks = 0; ks < KS; ++ks) {
+                        const i8x16 b = *reinterpret_cast<const i8x16*>(lds_b + (v * C + ct * 16 + lr) * pitch + ks * 64 + lk);
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            acc[u + v] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[u][ks], b, acc[u + v], 0, 0, 0);
+                    }
+                uint4 o;
+                uint32_t* op = reinterpret_cast<uint32_t*>(&o);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    uint64_t t = 0;
+#pragma unroll
+                    for (int w = 0; w < 7; ++w) t += (uint64_t)(uint32_t)acc[w][j] * MC.cw[p][w];
+                    op[j] = mont_reduce(t, P.q, P.mont_qinv);
+                }
+                uint32_t* dst = mf + ((((size_t)(p * RT + rt) * CT + ct) * kN + z) * 16 + lr) * 16 + (lane >> 4) * 4;
+                *reinterpret_cast<uint4*>(dst) = o;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// tile-major MFMA output -> dbout [q][b][i][ab][p][z] for the nq queries of
+// a chunk (dbout points at the chunk's first query): grid (N/32, RT*CT, 2 p)
+__global__ void __launch_bounds__(256) k_mfma_scatter(const uint32_t* __restrict__ mf, uint32_t* __restrict__ dbout,
+                                                      int R, int C, int n2, int nq, QS S) {
+    __shared__ uint32_t tile[256][33];
+    const int z0 = blockIdx.x * 32, t = blockIdx.y, p = blockIdx.z;
+    const int RT = R / 16, CT = C / 16;
+    const int rt = t / CT, ct = t % CT;
+    const uint32_t* src = mf + (((size_t)(p * RT + rt) * CT + ct) * kN + z0) * 256;
+    for (int e = threadIdx.x; e < 32 * 256; e += 256) tile[e & 255][e >> 8] = src[e];
+    __syncthreads();
+    for (int e = threadIdx.x; e < 256 * 32; e += 256) {
+        const int zz = e & 31, cr = e >> 5;          // cr = col_in * 16 + rho_in
+        const int col = ct * 16 + (cr >> 4), rho = rt * 16 + (cr & 15);
+        const int q = col >> 1, ab = col & 1;
+        if (q < nq) {
+            const int b = rho / n2, i = rho % n2;
+            dbout[q * S.dbout + ((size_t)(b * n2 + i) * 4 + ab * 2 + p) * kN + z0 + zz] = tile[cr][zz];
+        }
+    }
+}
+
 // ------------------------------------------------------------ giant steps
 
 // INTT of row c >= 1 (a part) of every block: grid (N2-1, r, 2 primes)

@@ -166,3 +166,31 @@ def test_batch_matches_single_and_oracle(ctx, oracle):
     for k in keys:
         k.close()
     db.close()
+
+
+@pytest.mark.parametrize("n1,chunk", [(64, 3), (256, 1 << 30), (128, 2)])
+def test_batch_mfma_shapes_and_chunks(ctx, oracle, n1, chunk):
+    """the int8-MFMA database pass at N1 64/128/256, with the batch split into
+    chunks of `chunk` queries, equals the single-query VALU pass"""
+    say(f"[gpu] batch mfma n1={n1} chunk={chunk}")
+    r = 4
+    db_words = oracle.gen_db(r, n1, seed=7)
+    perms = oracle.perm_matrix(n1)
+    clients = [oracle.gen_client(r, n1, 31 + c, (37 * c) % (N // 2), c % r) for c in range(5)]
+    db = ctx.database(db_words, r, n1)
+    keys = [ctx.keys(r, n1, cl["baby"], cl["giant"], cl["packing"], cl["rgsw"], perms) for cl in clients]
+    queries = np.stack([cl["query"] for cl in clients])
+    ctx.set("mfma_max_chunk", chunk)
+    try:
+        got = ctx.answer_batch(db, keys, queries)
+    finally:
+        ctx.set("mfma_max_chunk", 1 << 30)
+    for c, cl in enumerate(clients):
+        assert np.array_equal(got[c], ctx.answer(db, keys[c], cl["query"])), c
+    want0 = oracle.answer(r=r, n1=n1, db=db_words, query=clients[0]["query"], rgsw=clients[0]["rgsw"],
+                          baby=clients[0]["baby"], giant=clients[0]["giant"], packing=clients[0]["packing"],
+                          perms=perms)
+    assert np.array_equal(got[0], want0)
+    for k in keys:
+        k.close()
+    db.close()
