@@ -147,14 +147,20 @@ int build_tables(kspir_context* c) {
             pw[i] = (uint32_t)(((u128)e << 32) % q);   // Montgomery form
             e = (u64)((u128)e * w % q);
         }
+        std::vector<uint32_t> pairs(4 * kN);
+        for (int i = 0; i < kN; ++i) {
+            pairs[2 * i] = fw[i]; pairs[2 * i + 1] = fwsh[i];
+            pairs[2 * kN + 2 * i] = iv[i]; pairs[2 * kN + 2 * i + 1] = ivsh[i];
+        }
         uint32_t* d = nullptr;
-        HIP_OK(hipMalloc(&d, sizeof(uint32_t) * (4 * kN + 2 * kN)));
+        HIP_OK(hipMalloc(&d, sizeof(uint32_t) * (4 * kN + 2 * kN + 4 * kN)));
         c->tables.push_back(d);
         HIP_OK(hipMemcpy(d + 0 * kN, fw.data(), 4 * kN, hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(d + 1 * kN, fwsh.data(), 4 * kN, hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(d + 2 * kN, iv.data(), 4 * kN, hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(d + 3 * kN, ivsh.data(), 4 * kN, hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(d + 4 * kN, pw.data(), 8 * kN, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(d + 6 * kN, pairs.data(), 16 * kN, hipMemcpyHostToDevice));
         PrimeTab& P = c->T.p[p];
         P.q = q;
         uint32_t inv = 1;   // q^-1 mod 2^32 by Newton iteration
@@ -167,6 +173,8 @@ int build_tables(kspir_context* c) {
         P.iv = d + 2 * kN;
         P.iv_sh = d + 3 * kN;
         P.psi_pow = d + 4 * kN;
+        P.fw2 = reinterpret_cast<const uint2*>(d + 6 * kN);
+        P.iv2 = reinterpret_cast<const uint2*>(d + 8 * kN);
     }
     std::vector<uint16_t> se(kN), es(kN);
     for (int s = 0; s < kN; ++s) {

@@ -39,6 +39,8 @@ struct PrimeTab {
     const uint32_t* iv;    // iv[bitrev(i)] = psi^-i
     const uint32_t* iv_sh;
     const uint32_t* psi_pow;   // psi^e, e in [0, 2N)
+    const uint2* fw2;          // (fw[i], fw_sh[i]) pairs
+    const uint2* iv2;          // (iv[i], iv_sh[i]) pairs
 };
 
 struct Tabs {
@@ -171,21 +173,52 @@ __host__ __device__ __forceinline__ uint32_t mul_shoup_hd(uint32_t x, uint32_t w
     return r >= q ? r - q : r;
 }
 
-// forward pass (stages 4P .. 4P+3) on registers v[16] of thread tid
-__host__ __device__ __forceinline__ void ntt_fwd_pass(uint32_t v[16], int pass, int tid, const uint32_t* fw,
-                                                      const uint32_t* fwsh, uint32_t q) {
+// Twiddles of pass P for thread tid: stage s (s = 0..3) of the pass uses the
+// 2^s consecutive table entries starting at 2^(4P+s) + base * 2^s, with
+// base = 0, tid >> 4, tid for P = 0, 1, 2, i.e. the entry for register k is
+// table[2^(4P+s) + base * 2^s + (k >> (4 - s))].  tw2 interleaves (w, w')
+// pairs so each stage's group is one aligned vector load; w[], ws[] hold
+// the 15 pairs as stage s at offset 2^s - 1.
+__host__ __device__ __forceinline__ void load_pass_twiddles(const uint2* tw2, int pass, int tid, uint32_t w[15],
+                                                            uint32_t ws[15]) {
+    const int base = pass == 0 ? 0 : pass == 1 ? (tid >> 4) : tid;
+    const int lg0 = 4 * pass;
+    // s = 0: one pair
+    {
+        const uint2 t = tw2[(1 << lg0) + base];
+        w[0] = t.x; ws[0] = t.y;
+    }
+    // s = 1: two pairs (16 bytes)
+    {
+        const uint4 t = *reinterpret_cast<const uint4*>(tw2 + (1 << (lg0 + 1)) + base * 2);
+        w[1] = t.x; ws[1] = t.y; w[2] = t.z; ws[2] = t.w;
+    }
+    // s = 2: four pairs
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint4 t = *reinterpret_cast<const uint4*>(tw2 + (1 << (lg0 + 2)) + base * 4 + 2 * h);
+        w[3 + 2 * h] = t.x; ws[3 + 2 * h] = t.y; w[4 + 2 * h] = t.z; ws[4 + 2 * h] = t.w;
+    }
+    // s = 3: eight pairs
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+        const uint4 t = *reinterpret_cast<const uint4*>(tw2 + (1 << (lg0 + 3)) + base * 8 + 2 * h);
+        w[7 + 2 * h] = t.x; ws[7 + 2 * h] = t.y; w[8 + 2 * h] = t.z; ws[8 + 2 * h] = t.w;
+    }
+}
+
+// forward pass (stages 4P .. 4P+3, Cooley-Tukey) on registers v[16]
+__host__ __device__ __forceinline__ void ntt_fwd_pass(uint32_t v[16], const uint32_t w[15], const uint32_t ws[15],
+                                                      uint32_t q) {
 #pragma unroll
     for (int st = 0; st < 4; ++st) {
-        const int lg = 4 * pass + st;
-        const int lt = kLogN - 1 - lg;
         const int kb = 8 >> st;   // register-index bit flipped by this stage
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
             if (k & kb) continue;
-            const int e = pass_elem(pass, tid, k);
-            const int w_i = (1 << lg) + (e >> (lt + 1));
+            const int ti = (1 << st) - 1 + (k >> (4 - st));
             const uint32_t u = v[k];
-            const uint32_t t = mul_shoup_hd(v[k | kb], fw[w_i], fwsh[w_i], q);
+            const uint32_t t = mul_shoup_hd(v[k | kb], w[ti], ws[ti], q);
             const uint32_t a = u + t, b = u + q - t;
             v[k] = a >= q ? a - q : a;
             v[k | kb] = b >= q ? b - q : b;
@@ -194,22 +227,19 @@ __host__ __device__ __forceinline__ void ntt_fwd_pass(uint32_t v[16], int pass, 
 }
 
 // inverse pass (stages 4P+3 .. 4P, Gentleman-Sande)
-__host__ __device__ __forceinline__ void ntt_inv_pass(uint32_t v[16], int pass, int tid, const uint32_t* iv,
-                                                      const uint32_t* ivsh, uint32_t q) {
+__host__ __device__ __forceinline__ void ntt_inv_pass(uint32_t v[16], const uint32_t w[15], const uint32_t ws[15],
+                                                      uint32_t q) {
 #pragma unroll
     for (int st = 3; st >= 0; --st) {
-        const int lg = 4 * pass + st;
-        const int lt = kLogN - 1 - lg;
         const int kb = 8 >> st;
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
             if (k & kb) continue;
-            const int e = pass_elem(pass, tid, k);
-            const int w_i = (1 << lg) + (e >> (lt + 1));
-            const uint32_t u = v[k], w = v[k | kb];
-            const uint32_t a = u + w;
+            const int ti = (1 << st) - 1 + (k >> (4 - st));
+            const uint32_t u = v[k], x = v[k | kb];
+            const uint32_t a = u + x;
             v[k] = a >= q ? a - q : a;
-            v[k | kb] = mul_shoup_hd(u + q - w, iv[w_i], ivsh[w_i], q);
+            v[k | kb] = mul_shoup_hd(u + q - x, w[ti], ws[ti], q);
         }
     }
 }
@@ -218,12 +248,13 @@ __host__ __device__ __forceinline__ void ntt_inv_pass(uint32_t v[16], int pass, 
 // synchronises before (data in s) and after (result in s)
 __device__ __forceinline__ void ntt_fwd_r16(uint32_t* s, const PrimeTab& P) {
     const int tid = threadIdx.x;
-    uint32_t v[16];
+    uint32_t v[16], w[15], ws[15];
 #pragma unroll 1
     for (int pass = 0; pass < 3; ++pass) {
+        load_pass_twiddles(P.fw2, pass, tid, w, ws);
 #pragma unroll
         for (int k = 0; k < 16; ++k) v[k] = s[lds_pad(pass_elem(pass, tid, k))];
-        ntt_fwd_pass(v, pass, tid, P.fw, P.fw_sh, P.q);
+        ntt_fwd_pass(v, w, ws, P.q);
 #pragma unroll
         for (int k = 0; k < 16; ++k) s[lds_pad(pass_elem(pass, tid, k))] = v[k];
         __syncthreads();
@@ -232,12 +263,13 @@ __device__ __forceinline__ void ntt_fwd_r16(uint32_t* s, const PrimeTab& P) {
 
 __device__ __forceinline__ void ntt_inv_r16(uint32_t* s, const PrimeTab& P) {
     const int tid = threadIdx.x;
-    uint32_t v[16];
+    uint32_t v[16], w[15], ws[15];
 #pragma unroll 1
     for (int pass = 2; pass >= 0; --pass) {
+        load_pass_twiddles(P.iv2, pass, tid, w, ws);
 #pragma unroll
         for (int k = 0; k < 16; ++k) v[k] = s[lds_pad(pass_elem(pass, tid, k))];
-        ntt_inv_pass(v, pass, tid, P.iv, P.iv_sh, P.q);
+        ntt_inv_pass(v, w, ws, P.q);
         if (pass == 0) {
 #pragma unroll
             for (int k = 0; k < 16; ++k) v[k] = mul_shoup_hd(v[k], P.ninv, P.ninv_sh, P.q);

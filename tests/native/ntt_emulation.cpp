@@ -31,6 +31,9 @@ extern "C" int emul_ntt(uint32_t* poly, uint32_t q, uint32_t root, int inverse) 
         ivsh[i] = (uint32_t)(((uint64_t)iv[i] << 32) / q);
     }
     const uint32_t ninv = (uint32_t)powmod(kN, q - 2, q), ninvsh = (uint32_t)(((uint64_t)ninv << 32) / q);
+    std::vector<uint2> fw2(kN), iv2(kN);
+    for (int i = 0; i < kN; ++i) { fw2[i] = make_uint2(fw[i], fwsh[i]); iv2[i] = make_uint2(iv[i], ivsh[i]); }
+    uint32_t w[15], ws[15];
     std::vector<uint32_t> s(kPadN, 0);
     for (int e = 0; e < kN; ++e) s[lds_pad(e)] = poly[e];
     uint32_t v[16];
@@ -39,10 +42,12 @@ extern "C" int emul_ntt(uint32_t* poly, uint32_t q, uint32_t root, int inverse) 
         for (int tid = 0; tid < 256; ++tid) {
             for (int k = 0; k < 16; ++k) v[k] = s[lds_pad(pass_elem(pass, tid, k))];
             if (inverse) {
-                ntt_inv_pass(v, pass, tid, iv.data(), ivsh.data(), q);
+                load_pass_twiddles(iv2.data(), pass, tid, w, ws);
+                ntt_inv_pass(v, w, ws, q);
                 if (pass == 0) for (int k = 0; k < 16; ++k) v[k] = mul_shoup_hd(v[k], ninv, ninvsh, q);
             } else {
-                ntt_fwd_pass(v, pass, tid, fw.data(), fwsh.data(), q);
+                load_pass_twiddles(fw2.data(), pass, tid, w, ws);
+                ntt_fwd_pass(v, w, ws, q);
             }
             for (int k = 0; k < 16; ++k) s[lds_pad(pass_elem(pass, tid, k))] = v[k];
         }
