@@ -70,8 +70,9 @@ void kspir_database_destroy(kspir_database* db);
  * permutations arguments of matrix_vector_mul_bsgs_rns_crt_large
  * (twosteps.cpp:1527), the query argument of externalProduct (lwe.cpp:761)
  * and the autokey argument of packingRLWEs (answer.cpp:800).  Host pointers.
- * packing may be NULL when r == 1; rgsw may be NULL when only the first
- * dimension is evaluated. */
+ * Every part is optional (NULL) and checked by the call that needs it:
+ * baby + permutations (+ giant when N2 > 1) for the first dimension, rgsw
+ * for the external product, packing (r > 1) for packing. */
 int kspir_keys_create(kspir_context* ctx, int32_t r, int32_t n1, const uint64_t* baby,
                       const uint64_t* giant, const uint64_t* packing, const uint64_t* rgsw,
                       const int32_t* permutations, kspir_keys** out);

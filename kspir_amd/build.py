@@ -7,8 +7,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 OUT = os.path.join(HERE, "libkspir_amd.so")
-SOURCES = [os.path.join(HERE, "csrc", "kspir_capi.hip")]
-DEPS = SOURCES + [os.path.join(HERE, "csrc", f) for f in ("kspir_device.h", "kspir_kernels.h")] + [
+SOURCES = [os.path.join(HERE, "csrc", "kspir_capi.hip"), os.path.join(HERE, "cpp", "kspir_pir.cpp")]
+DEPS = SOURCES + [os.path.join(HERE, "csrc", f) for f in ("kspir_device.h", "kspir_kernels.h")] + [os.path.join(HERE, "cpp", "kspir_pir.h")] + [
     os.path.join(ROOT, "include", f) for f in ("kspir_amd.h", "kspir_amd_runtime.h")]
 
 
@@ -17,6 +17,20 @@ def hipcc():
         if c and os.path.exists(c):
             return c
     raise RuntimeError("hipcc not found")
+
+
+def build_dropin_test(verbose=False):
+    """tests/native/dropin_test: the test-pir online calls against kspir_pir.h"""
+    outdir = os.path.join(ROOT, "tests", "native", "_build")
+    os.makedirs(outdir, exist_ok=True)
+    out = os.path.join(outdir, "dropin_test")
+    cmd = ["g++", "-O2", "-std=c++17", os.path.join(ROOT, "tests", "native", "dropin_test.cpp"),
+           "-I", os.path.join(HERE, "cpp"), "-L", HERE, "-lkspir_amd",
+           "-Wl,-rpath," + os.path.relpath(HERE, outdir).join(["$ORIGIN/", ""]), "-o", out]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    return out
 
 
 def build(verbose=False):
@@ -28,6 +42,7 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
     os.replace(OUT + ".tmp", OUT)
+    build_dropin_test(verbose)
     return OUT
 
 

@@ -1,0 +1,274 @@
+// kspir_pir.cpp — the drop-in C++ surface (kspir_pir.h) over the C ABI.
+// Reference objects are flattened into the C ABI's layouts; databases and
+// keys are uploaded on first use and cached per object, keyed by address and
+// a fingerprint of sampled words so that a changed or reallocated object is
+// re-uploaded.
+#include "kspir_pir.h"
+
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/kspir_amd.h"
+
+using kspir_params::N;
+
+RlweCiphertext::RlweCiphertext() : a(N, 0), b(N, 0) {}
+RlweCiphertext::RlweCiphertext(uint64_t len, uint64_t module) : length(len), modulus(module), a(len, 0), b(len, 0) {}
+
+RGSWCiphertext::RGSWCiphertext(uint64_t len, uint64_t module, uint64_t elln, uint64_t base, uint64_t bg)
+    : length(len), modulus(module), ellnum(elln), PP(base), BBg(bg) {
+    a.assign(2 * ellnum, std::vector<uint64_t>(len, 0));
+    b.assign(2 * ellnum, std::vector<uint64_t>(len, 0));
+}
+
+AutoKey::AutoKey(int32_t len, uint64_t module, int32_t elln, uint64_t base, uint64_t bg)
+    : length((uint64_t)len), modulus(module), ellnum(elln), PP(base), BBg(bg) {}
+
+AutoKeyBSGSRNS::AutoKeyBSGSRNS(int32_t len, uint64_t module, uint64_t bsModule)
+    : length((uint64_t)len), modulus(module), bsModulus(bsModule) {}
+
+namespace {
+
+std::mutex g_mu;
+kspir_context* g_ctx = nullptr;
+
+void check(int rc, const char* what) {
+    if (rc != 0) throw std::runtime_error(std::string(what) + ": " + kspir_last_error());
+}
+
+kspir_context* ctx() {
+    if (!g_ctx) {
+        const char* dev = std::getenv("KSPIR_DEVICE");
+        check(kspir_context_create(dev ? std::atoi(dev) : 0, &g_ctx), "kspir_context_create");
+    }
+    return g_ctx;
+}
+
+bool pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
+
+uint32_t pow5(int64_t e) {
+    uint32_t r = 1;
+    for (int64_t i = 0; i < e; ++i) r = (r * 5u) & (2 * N - 1);
+    return r;
+}
+
+struct Fp {
+    uint64_t h = 1469598103934665603ULL;
+    void add(uint64_t v) { h = (h ^ v) * 1099511628211ULL; }
+    void poly(const std::vector<uint64_t>& p) {
+        add(p.size());
+        for (size_t i = 0; i < p.size(); i += 509) add(p[i]);
+        if (!p.empty()) add(p.back());
+    }
+};
+
+const std::vector<RlweCiphertext>& at(const std::map<int32_t, std::vector<RlweCiphertext> >& m, int32_t idx,
+                                      size_t need) {
+    const std::vector<RlweCiphertext>& v = m.at(idx);   // std::out_of_range as in the reference
+    if (v.size() < need) throw std::runtime_error("key entry " + std::to_string(idx) + " has too few ciphertexts");
+    return v;
+}
+
+void put_rlwe(uint64_t* dst, const RlweCiphertext& c) {
+    if (c.a.size() != (size_t)N || c.b.size() != (size_t)N) throw std::runtime_error("RLWE of wrong length");
+    std::memcpy(dst, c.a.data(), N * 8);
+    std::memcpy(dst + N, c.b.data(), N * 8);
+}
+
+struct DbEntry { int32_t r, n1; uint64_t fp; kspir_database* h; };
+struct KeyEntry { uint64_t fp; kspir_keys* h; };
+std::map<const void*, DbEntry> g_dbs;
+std::map<std::pair<const void*, uint64_t>, KeyEntry> g_keys;   // (object, shape tag)
+
+uint64_t db_fingerprint(const uint64_t* data, int32_t r) {
+    Fp f;
+    const size_t words = (size_t)r * N * (N / 2);
+    f.add(words);
+    for (size_t i = 0; i < words; i += words / 97 + 1) f.add(data[i]);
+    f.add(data[words - 1]);
+    return f.h;
+}
+
+kspir_database* database(uint64_t* data, int32_t r, int32_t n1) {
+    const uint64_t fp = db_fingerprint(data, r);
+    auto it = g_dbs.find(data);
+    if (it != g_dbs.end()) {
+        if (it->second.r == r && it->second.n1 == n1 && it->second.fp == fp) return it->second.h;
+        kspir_database_destroy(it->second.h);
+        g_dbs.erase(it);
+    }
+    kspir_database* h = nullptr;
+    check(kspir_database_create(ctx(), data, r, n1, 0, &h), "kspir_database_create");
+    g_dbs[data] = DbEntry{r, n1, fp, h};
+    return h;
+}
+
+kspir_keys* cached_keys(const void* obj, uint64_t tag, uint64_t fp, int32_t r, int32_t n1, const uint64_t* baby,
+                        const uint64_t* giant, const uint64_t* packing, const uint64_t* rgsw, const int32_t* perms) {
+    auto key = std::make_pair(obj, tag);
+    auto it = g_keys.find(key);
+    if (it != g_keys.end()) {
+        if (it->second.fp == fp) return it->second.h;
+        kspir_keys_destroy(it->second.h);
+        g_keys.erase(it);
+    }
+    kspir_keys* h = nullptr;
+    check(kspir_keys_create(ctx(), r, n1, baby, giant, packing, rgsw, perms, &h), "kspir_keys_create");
+    g_keys[key] = KeyEntry{fp, h};
+    return h;
+}
+
+// BSGS keys + permutation table of one AutoKeyBSGSRNS for (r, N1)
+kspir_keys* bsgs_keys(const AutoKeyBSGSRNS& k, int32_t r, int32_t n1,
+                      const std::vector<std::vector<int32_t> >& perms) {
+    const int32_t n2 = N / 2 / n1;
+    Fp f;
+    f.add((uint64_t)r);
+    f.add((uint64_t)n1);
+    for (int32_t i = 1; i <= n1 / 2; ++i)
+        for (const RlweCiphertext& c : at(k.keyMap, (int32_t)pow5(i), 6)) { f.poly(c.a); f.poly(c.b); }
+    for (int32_t i = 1; i < n2; ++i)
+        for (const RlweCiphertext& c : at(k.keyMap, (int32_t)pow5((int64_t)n1 * i), 2)) { f.poly(c.a); f.poly(c.b); }
+    if ((int32_t)perms.size() < n1) throw std::runtime_error("permutations has fewer than N1 rows");
+    for (int32_t i = 0; i < n1; ++i) {
+        if (perms[i].size() != (size_t)N) throw std::runtime_error("permutation row of wrong length");
+        for (size_t j = 0; j < perms[i].size(); j += 251) f.add((uint64_t)perms[i][j]);
+    }
+    const uint64_t tag = ((uint64_t)r << 32) | (uint32_t)n1;
+    auto it = g_keys.find(std::make_pair((const void*)&k, tag));
+    if (it != g_keys.end() && it->second.fp == f.h) return it->second.h;
+    std::vector<uint64_t> baby((size_t)(n1 / 2) * 6 * 2 * N), giant((size_t)(n2 > 1 ? n2 - 1 : 0) * 2 * 2 * N);
+    std::vector<int32_t> pm((size_t)n1 * N);
+    for (int32_t i = 1; i <= n1 / 2; ++i) {
+        const std::vector<RlweCiphertext>& v = at(k.keyMap, (int32_t)pow5(i), 6);
+        for (int j = 0; j < 6; ++j) put_rlwe(&baby[(((size_t)(i - 1) * 6 + j) * 2) * N], v[j]);
+    }
+    for (int32_t i = 1; i < n2; ++i) {
+        const std::vector<RlweCiphertext>& v = at(k.keyMap, (int32_t)pow5((int64_t)n1 * i), 2);
+        for (int j = 0; j < 2; ++j) put_rlwe(&giant[(((size_t)(i - 1) * 2 + j) * 2) * N], v[j]);
+    }
+    for (int32_t i = 0; i < n1; ++i) std::memcpy(&pm[(size_t)i * N], perms[i].data(), N * 4);
+    return cached_keys(&k, tag, f.h, r, n1, baby.data(), n2 > 1 ? giant.data() : nullptr, nullptr, nullptr, pm.data());
+}
+
+kspir_keys* rgsw_keys(const RGSWCiphertext& g) {
+    if (g.getEllnum() != 2 || g.getBase() != (1ULL << 20) || g.getBg() != (1ULL << 18))
+        throw std::runtime_error("externalProduct: RGSW gadget must be (ell 2, PP 2^20, Bg 2^18)");
+    if (g.a.size() != 4 || g.b.size() != 4) throw std::runtime_error("RGSW must have 4 rows");
+    Fp f;
+    for (int i = 0; i < 4; ++i) { f.poly(g.a[i]); f.poly(g.b[i]); }
+    auto it = g_keys.find(std::make_pair((const void*)&g, (uint64_t)0));
+    if (it != g_keys.end() && it->second.fp == f.h) return it->second.h;
+    std::vector<uint64_t> rg(4 * 2 * (size_t)N);
+    for (int i = 0; i < 4; ++i) {
+        if (g.a[i].size() != (size_t)N || g.b[i].size() != (size_t)N) throw std::runtime_error("RGSW row length");
+        std::memcpy(&rg[(size_t)i * 2 * N], g.a[i].data(), N * 8);
+        std::memcpy(&rg[(size_t)i * 2 * N + N], g.b[i].data(), N * 8);
+    }
+    return cached_keys(&g, 0, f.h, 1, 128, nullptr, nullptr, nullptr, rg.data(), nullptr);
+}
+
+kspir_keys* packing_keys(const AutoKey& k, int32_t r) {
+    if (k.getEllnum() != 4 || k.getBase() != 0 || k.getBg() != (1ULL << 14))
+        throw std::runtime_error("packingRLWEs: packing-key gadget must be (ell 4, PP 0, Bg 2^14)");
+    Fp f;
+    f.add((uint64_t)r);
+    int lr = 0;
+    for (int32_t i = 1; i < r; i <<= 1, ++lr)
+        for (const RlweCiphertext& c : at(k.keyMap, N / i + 1, 4)) { f.poly(c.a); f.poly(c.b); }
+    auto it = g_keys.find(std::make_pair((const void*)&k, (uint64_t)r));
+    if (it != g_keys.end() && it->second.fp == f.h) return it->second.h;
+    std::vector<uint64_t> pk((size_t)lr * 4 * 2 * N);
+    lr = 0;
+    for (int32_t i = 1; i < r; i <<= 1, ++lr) {
+        const std::vector<RlweCiphertext>& v = at(k.keyMap, N / i + 1, 4);
+        for (int j = 0; j < 4; ++j) put_rlwe(&pk[(((size_t)lr * 4 + j) * 2) * N], v[j]);
+    }
+    return cached_keys(&k, (uint64_t)r, f.h, r, 128, nullptr, nullptr, lr ? pk.data() : nullptr, nullptr, nullptr);
+}
+
+}  // namespace
+
+void matrix_vector_mul_bsgs_rns_crt_large(std::vector<RlweCiphertext>& result, std::vector<RlweCiphertext>& input,
+                                          uint64_t* data, const AutoKeyBSGSRNS& autoKey, const int32_t N1,
+                                          std::vector<std::vector<int32_t> >& permutations, int32_t r) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (!data) throw std::runtime_error("matrix_vector_mul_bsgs_rns_crt_large: null database");
+    if (!pow2(r) || !pow2(N1) || N1 < 32 || N1 > N / 2)
+        throw std::runtime_error("matrix_vector_mul_bsgs_rns_crt_large: r and N1 must be powers of two, N1 in [32, N/2]");
+    if (input.size() < 2) throw std::runtime_error("input needs the crtMod and bsMod parts");
+    if (input[0].getIsNtt() || input[1].getIsNtt())
+        throw std::runtime_error("input must be in coefficient form (query_bsgs_rns output)");
+    if (result.size() < (size_t)r) throw std::runtime_error("result must hold r ciphertexts");
+    kspir_database* db = database(data, r, N1);
+    kspir_keys* keys = bsgs_keys(autoKey, r, N1, permutations);
+    std::vector<uint64_t> q(4 * (size_t)N), res((size_t)r * 2 * N);
+    put_rlwe(q.data(), input[0]);
+    put_rlwe(q.data() + 2 * N, input[1]);
+    check(kspir_matrix_vector_mul_bsgs_rns_crt_large(ctx(), db, keys, q.data(), res.data()),
+          "matrix_vector_mul_bsgs_rns_crt_large");
+    for (int32_t i = 0; i < r; ++i) {
+        result[i].a.assign(res.begin() + (size_t)i * 2 * N, res.begin() + (size_t)i * 2 * N + N);
+        result[i].b.assign(res.begin() + (size_t)i * 2 * N + N, res.begin() + (size_t)(i + 1) * 2 * N);
+        result[i].setIsNtt(true);
+    }
+    // twosteps.cpp:1074-1086: input[0] becomes the modulus-switched query, NTT form
+    input[0].a.assign(q.begin(), q.begin() + N);
+    input[0].b.assign(q.begin() + N, q.begin() + 2 * N);
+    input[0].setIsNtt(true);
+}
+
+void externalProduct(RlweCiphertext& result, RlweCiphertext& firstDim, const RGSWCiphertext& query) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (!query.getIsNtt()) throw std::runtime_error("externalProduct: RGSW must be in NTT form");   // lwe.cpp:763
+    kspir_keys* keys = rgsw_keys(query);
+    std::vector<uint64_t> res(2 * (size_t)N), fd(2 * (size_t)N);
+    put_rlwe(res.data(), result);
+    put_rlwe(fd.data(), firstDim);
+    const bool was_ntt = firstDim.getIsNtt();
+    check(kspir_external_product(ctx(), keys, res.data(), fd.data(), was_ntt ? 1 : 0), "externalProduct");
+    result.a.assign(res.begin(), res.begin() + N);
+    result.b.assign(res.begin() + N, res.end());
+    result.setIsNtt(false);
+    if (was_ntt) {
+        firstDim.a.assign(fd.begin(), fd.begin() + N);
+        firstDim.b.assign(fd.begin() + N, fd.end());
+        firstDim.setIsNtt(false);
+    }
+}
+
+void packingRLWEs(RlweCiphertext& result, std::vector<RlweCiphertext>& rlwes, const AutoKey& autokey) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    const int32_t r = (int32_t)rlwes.size();
+    if (!pow2(r)) throw std::runtime_error("packingRLWEs: the number of RLWEs must be a power of two");
+    const bool ntt = rlwes[0].getIsNtt();
+    for (const RlweCiphertext& c : rlwes)
+        if (c.getIsNtt() != ntt) throw std::runtime_error("packingRLWEs: RLWEs must all be in the same form");
+    kspir_keys* keys = packing_keys(autokey, r);
+    std::vector<uint64_t> rl((size_t)r * 2 * N), res(2 * (size_t)N);
+    for (int32_t i = 0; i < r; ++i) put_rlwe(&rl[(size_t)i * 2 * N], rlwes[i]);
+    check(kspir_packing_rlwes(ctx(), keys, res.data(), rl.data(), ntt ? 1 : 0), "packingRLWEs");
+    result.a.assign(res.begin(), res.begin() + N);
+    result.b.assign(res.begin() + N, res.end());
+    result.setIsNtt(false);
+}
+
+void kspir_pir_forget(const void* p) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    auto it = g_dbs.find(p);
+    if (it != g_dbs.end()) {
+        kspir_database_destroy(it->second.h);
+        g_dbs.erase(it);
+    }
+    for (auto k = g_keys.begin(); k != g_keys.end();) {
+        if (k->first.first == p) {
+            kspir_keys_destroy(k->second.h);
+            k = g_keys.erase(k);
+        } else {
+            ++k;
+        }
+    }
+}

@@ -1,0 +1,137 @@
+// kspir_pir.h — drop-in C++ surface of the KsPIR online server answer on
+// MI355X.  It keeps the reference's class and function names, argument
+// meanings and visible side effects for the three calls of the online answer
+// (tests/test-pir.cpp:111-115):
+//
+//   matrix_vector_mul_bsgs_rns_crt_large   src/twosteps.h:55
+//   externalProduct                        src/lwe.h:292
+//   packingRLWEs                           src/answer.h:79
+//
+// and the data types they take (src/lwe.h:48-110 RlweCiphertext, :113-169
+// RGSWCiphertext, :175-234 AutoKey, :381-466 AutoKeyBSGSRNS), whose public
+// members (a, b, keyMap) hold the reference's words.  The work runs on the
+// GPU through the C ABI of include/kspir_amd.h; databases and keys are
+// uploaded to HBM on first use and kept there (see kspir_pir_forget).
+//
+// Errors: a missing key index throws std::out_of_range (as keyMap.at does in
+// the reference); shapes the reference asserts on, and device failures,
+// throw std::runtime_error.
+#ifndef KSPIR_PIR_H
+#define KSPIR_PIR_H
+
+#include <stdint.h>
+#include <map>
+#include <vector>
+
+namespace kspir_params {
+constexpr int32_t N = 4096;                                    // params.h:16
+constexpr uint64_t crtq1 = 268369921ULL, crtq2 = 249561089ULL;  // crt.h:16-17
+constexpr uint64_t crtMod = crtq1 * crtq2;                       // crt.h:18
+constexpr uint64_t bsMod = 16760833ULL;                          // crt.h:32
+}  // namespace kspir_params
+
+// src/lwe.h:48-110
+class RlweCiphertext {
+  private:
+    uint64_t length = kspir_params::N;
+    uint64_t modulus = kspir_params::crtMod;
+    bool isntt = false;
+
+  public:
+    std::vector<uint64_t> a;
+    std::vector<uint64_t> b;
+
+    RlweCiphertext();
+    RlweCiphertext(uint64_t len, uint64_t module);
+    uint64_t getModulus() const { return modulus; }
+    void setModulus(uint64_t module) { modulus = module; }
+    int32_t getLength() const { return (int32_t)length; }
+    bool getIsNtt() const { return isntt; }
+    void setIsNtt(bool is) { isntt = is; }
+};
+
+// src/lwe.h:113-169 (rows a[0..2ell), b[0..2ell), NTT form)
+class RGSWCiphertext {
+  private:
+    uint64_t length = kspir_params::N;
+    uint64_t modulus = kspir_params::crtMod;
+    uint64_t ellnum = 2, PP = 1ULL << 20, BBg = 1ULL << 18;
+    bool isntt = true;
+
+  public:
+    std::vector<std::vector<uint64_t> > a;
+    std::vector<std::vector<uint64_t> > b;
+
+    RGSWCiphertext(uint64_t len, uint64_t module, uint64_t elln, uint64_t base, uint64_t bg);
+    uint64_t getModulus() const { return modulus; }
+    uint64_t getLength() const { return length; }
+    uint64_t getEllnum() const { return ellnum; }
+    uint64_t getBg() const { return BBg; }
+    uint64_t getBase() const { return PP; }
+    bool getIsNtt() const { return isntt; }
+    void setIsNtt(bool is) { isntt = is; }
+};
+
+// src/lwe.h:175-234 (packing keys; keyMap[N/i + 1] = ell RLWE, NTT form)
+class AutoKey {
+  private:
+    uint64_t length = kspir_params::N;
+    uint64_t modulus = kspir_params::crtMod;
+    int32_t ellnum = 4;
+    uint64_t PP = 0, BBg = 1ULL << 14;
+
+  public:
+    std::map<int32_t, std::vector<RlweCiphertext> > keyMap;
+
+    AutoKey(int32_t len, uint64_t module, int32_t elln, uint64_t base, uint64_t bg);
+    uint64_t getModulus() const { return modulus; }
+    uint64_t getLength() const { return length; }
+    uint64_t getEllnum() const { return (uint64_t)ellnum; }
+    uint64_t getBg() const { return BBg; }
+    uint64_t getBase() const { return PP; }
+};
+
+// src/lwe.h:381-466 (BSGS keys; keyMap[5^i], i <= N1/2: 3 RLWE mod crtMod
+// then 3 mod bsMod; keyMap[5^(N1*i)], i < N2: 2 RLWE mod crtMod)
+class AutoKeyBSGSRNS {
+  private:
+    uint64_t length = kspir_params::N;
+    uint64_t modulus = kspir_params::crtMod;
+    uint64_t bsModulus = kspir_params::bsMod;
+
+  public:
+    std::map<int32_t, std::vector<RlweCiphertext> > keyMap;
+
+    AutoKeyBSGSRNS(int32_t len, uint64_t module, uint64_t bsModule);
+    uint64_t getModulus() const { return modulus; }
+    uint64_t getBSModulus() const { return bsModulus; }
+    uint64_t getLength() const { return length; }
+    uint64_t getEllnumBS() const { return 3; }
+    uint64_t getBaseBS() const { return 1ULL << 20; }
+    uint64_t getBgBS() const { return 1ULL << 20; }
+    uint64_t getEllnumGS() const { return 2; }
+    uint64_t getBaseGS() const { return 1ULL << 20; }
+    uint64_t getBgGS() const { return 1ULL << 18; }
+};
+
+// twosteps.h:55.  input: [0] mod crtMod, [1] mod bsMod, coefficient form;
+// as in the reference input[0] becomes the modulus-switched query in NTT
+// form.  result: r entries, NTT form.  data: database_tocrt words, r blocks.
+void matrix_vector_mul_bsgs_rns_crt_large(std::vector<RlweCiphertext>& result, std::vector<RlweCiphertext>& input,
+                                          uint64_t* data, const AutoKeyBSGSRNS& autoKey, const int32_t N1,
+                                          std::vector<std::vector<int32_t> >& permutations, int32_t r = 1);
+
+// lwe.h:292.  Accumulates into result (NTT-domain words) and leaves it in
+// coefficient form; firstDim is turned into coefficient form if it was NTT.
+void externalProduct(RlweCiphertext& result, RlweCiphertext& firstDim, const RGSWCiphertext& query);
+
+// answer.h:79.  rlwes (a power of two of them, coefficient or NTT form) are
+// consumed; result is the packed RLWE in coefficient form.
+void packingRLWEs(RlweCiphertext& result, std::vector<RlweCiphertext>& rlwes, const AutoKey& autokey);
+
+// Drop the device copies made for a database pointer or key object (call
+// after changing their contents in place; destroyed objects are recognised
+// by a fingerprint of their words and re-uploaded automatically).
+void kspir_pir_forget(const void* object_or_data);
+
+#endif

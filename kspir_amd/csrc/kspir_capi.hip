@@ -425,12 +425,15 @@ int enqueue_answer(kspir_context* c, const kspir_database* db, const KeyPtrs* kp
     return KSPIR_OK;
 }
 
-int check_pair(const kspir_context* c, const kspir_database* db, const kspir_keys* k) {
+// the database and keys fit together; the keys hold what the call needs
+int check_pair(const kspir_context* c, const kspir_database* db, const kspir_keys* k, bool first_dim_only = false) {
     if (!c || !db || !k) return fail(KSPIR_EINVAL, "null handle");
     if (db->ctx != c || k->ctx != c) return fail(KSPIR_EINVAL, "handle belongs to another context");
     if (db->r != k->r || db->n1 != k->n1)
         return fail(KSPIR_EINVAL, "database and keys disagree on (r, N1)");
-    if (!k->rgsw || (k->r > 1 && !k->packing)) return fail(KSPIR_EINVAL, "keys lack RGSW or packing keys");
+    if (!k->baby || (db->n2 > 1 && !k->giant)) return fail(KSPIR_EINVAL, "keys lack baby- or giant-step keys");
+    if (!first_dim_only && (!k->rgsw || (k->r > 1 && !k->packing)))
+        return fail(KSPIR_EINVAL, "keys lack the RGSW ciphertext or packing keys");
     return KSPIR_OK;
 }
 
@@ -562,13 +565,13 @@ void kspir_database_destroy(kspir_database* db) {
 
 int kspir_keys_create(kspir_context* c, int32_t r, int32_t n1, const uint64_t* baby, const uint64_t* giant,
                       const uint64_t* packing, const uint64_t* rgsw, const int32_t* permutations, kspir_keys** out) {
-    if (!c || !baby || !permutations || !out) return fail(KSPIR_EINVAL, "null argument");
+    if (!c || !out) return fail(KSPIR_EINVAL, "null argument");
     if (!pow2(r) || r > kN || !pow2(n1) || n1 < 32 || n1 > kN / 2)
         return fail(KSPIR_EINVAL, "r must be a power of two <= N, N1 a power of two in [32, N/2]");
     const int n2 = kN / 2 / n1;
-    if (n2 > 1 && !giant) return fail(KSPIR_EINVAL, "giant-step keys required when N2 > 1");
-    if (r > 1 && !packing) return fail(KSPIR_EINVAL, "packing keys required when r > 1");
-    std::vector<uint16_t> p16((size_t)n1 * kN);
+    if (!baby != !permutations) return fail(KSPIR_EINVAL, "baby-step keys and permutations go together");
+    if (baby && n2 > 1 && !giant) return fail(KSPIR_EINVAL, "giant-step keys required when N2 > 1");
+    std::vector<uint16_t> p16(permutations ? (size_t)n1 * kN : 0);
     for (size_t i = 0; i < p16.size(); ++i) {
         if (permutations[i] < 0 || permutations[i] >= kN) return fail(KSPIR_EINVAL, "permutation entry out of range");
         p16[i] = (uint16_t)permutations[i];
@@ -593,10 +596,13 @@ int kspir_keys_create(kspir_context* c, int32_t r, int32_t n1, const uint64_t* b
     hipError_t e = hipMalloc(&raw, raw_words * 8);
     if (e != hipSuccess) return cleanup(fail_hip("hipMalloc(raw keys)", e));
 #define K_OK(x) do { hipError_t e2 = (x); if (e2 != hipSuccess) return cleanup(fail_hip(#x, e2)); } while (0)
-    K_OK(hipMalloc(&k->baby, (size_t)nb * 3 * 2 * 3 * kN * 4));
-    K_OK(hipMemcpy(raw, baby, (size_t)nb * 6 * 2 * kN * 8, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_keys_baby, dim3(grid1d((size_t)nb * 6 * kN)), dim3(256), 0, c->stream, k->baby, raw, nb, c->T);
-    K_OK(hipStreamSynchronize(c->stream));
+    if (baby) {
+        K_OK(hipMalloc(&k->baby, (size_t)nb * 3 * 2 * 3 * kN * 4));
+        K_OK(hipMemcpy(raw, baby, (size_t)nb * 6 * 2 * kN * 8, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_keys_baby, dim3(grid1d((size_t)nb * 6 * kN)), dim3(256), 0, c->stream, k->baby, raw, nb,
+                           c->T);
+        K_OK(hipStreamSynchronize(c->stream));
+    }
     auto upload_q = [&](const uint64_t* src, int rows, uint32_t** dst) -> hipError_t {
         hipError_t ee = hipMalloc(dst, (size_t)rows * 2 * 2 * kN * 4);
         if (ee != hipSuccess) return ee;
@@ -605,11 +611,13 @@ int kspir_keys_create(kspir_context* c, int32_t r, int32_t n1, const uint64_t* b
         hipLaunchKernelGGL(k_keys_q, dim3(grid1d((size_t)rows * 2 * kN)), dim3(256), 0, c->stream, *dst, raw, rows, c->T);
         return hipStreamSynchronize(c->stream);
     };
-    if (ng > 0) K_OK(upload_q(giant, ng * 2, &k->giant));
-    if (np > 0) K_OK(upload_q(packing, np * 4, &k->packing));
+    if (giant && ng > 0) K_OK(upload_q(giant, ng * 2, &k->giant));
+    if (packing && np > 0) K_OK(upload_q(packing, np * 4, &k->packing));
     if (rgsw) K_OK(upload_q(rgsw, 4, &k->rgsw));
-    K_OK(hipMalloc(&k->perms, p16.size() * 2));
-    K_OK(hipMemcpy(k->perms, p16.data(), p16.size() * 2, hipMemcpyHostToDevice));
+    if (!p16.empty()) {
+        K_OK(hipMalloc(&k->perms, p16.size() * 2));
+        K_OK(hipMemcpy(k->perms, p16.data(), p16.size() * 2, hipMemcpyHostToDevice));
+    }
     k->ptrs = KeyPtrs{k->baby, k->giant, k->packing, k->rgsw, k->perms};
     K_OK(hipMalloc(&k->dev_ptrs, sizeof(KeyPtrs)));
     K_OK(hipMemcpy(k->dev_ptrs, &k->ptrs, sizeof(KeyPtrs), hipMemcpyHostToDevice));
@@ -717,9 +725,8 @@ int kspir_answer(kspir_context* c, const kspir_database* db, const kspir_keys* k
 
 int kspir_matrix_vector_mul_bsgs_rns_crt_large(kspir_context* c, const kspir_database* db, const kspir_keys* k,
                                                uint64_t* query, uint64_t* result) {
-    if (!c || !db || !k) return fail(KSPIR_EINVAL, "null handle");
-    if (db->ctx != c || k->ctx != c) return fail(KSPIR_EINVAL, "handle belongs to another context");
-    if (db->r != k->r || db->n1 != k->n1) return fail(KSPIR_EINVAL, "database and keys disagree on (r, N1)");
+    int rc0 = check_pair(c, db, k, true);
+    if (rc0) return rc0;
     if (!query || !result) return fail(KSPIR_EINVAL, "null buffer");
     HIP_OK(hipSetDevice(c->device));
     int rc = carve_workspace(c, db->r, db->n1, 1);
@@ -874,7 +881,7 @@ int kspir_time_answer(kspir_context* c, const kspir_database* db, const kspir_ke
 int kspir_time_db_pass(kspir_context* c, const kspir_database* db, const kspir_keys* k, const uint64_t* query_dev,
                        int iters, float* ms_per_launch) {
     if (!ms_per_launch || iters < 1) return fail(KSPIR_EINVAL, "bad timing arguments");
-    int rc = check_pair(c, db, k);
+    int rc = check_pair(c, db, k, true);
     if (rc) return rc;
     HIP_OK(hipSetDevice(c->device));
     rc = carve_workspace(c, db->r, db->n1, 1);

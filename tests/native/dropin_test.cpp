@@ -1,0 +1,117 @@
+// dropin_test.cpp — the online answer of tests/test-pir.cpp:107-116, written
+// against the drop-in C++ surface (kspir_amd/cpp/kspir_pir.h) exactly as the
+// reference driver calls it.  TEST INFRASTRUCTURE: inputs come from raw files
+// written by the test (oracle-generated, reference layouts); outputs go back
+// as raw files for word-for-word comparison.
+//   dropin_test <dir> <r> <N1>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "kspir_pir.h"
+
+using namespace kspir_params;
+
+static std::vector<uint64_t> read_u64(const std::string& path, size_t n) {
+    std::vector<uint64_t> v(n);
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f || std::fread(v.data(), 8, n, f) != n) { std::fprintf(stderr, "cannot read %s\n", path.c_str()); std::exit(2); }
+    std::fclose(f);
+    return v;
+}
+static std::vector<int32_t> read_i32(const std::string& path, size_t n) {
+    std::vector<int32_t> v(n);
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f || std::fread(v.data(), 4, n, f) != n) { std::fprintf(stderr, "cannot read %s\n", path.c_str()); std::exit(2); }
+    std::fclose(f);
+    return v;
+}
+static void write_u64(const std::string& path, const std::vector<uint64_t>& v) {
+    FILE* f = std::fopen(path.c_str(), "wb");
+    std::fwrite(v.data(), 8, v.size(), f);
+    std::fclose(f);
+}
+static RlweCiphertext rlwe(const uint64_t* p, uint64_t q, bool ntt) {
+    RlweCiphertext c(N, q);
+    c.a.assign(p, p + N);
+    c.b.assign(p + N, p + 2 * N);
+    c.setIsNtt(ntt);
+    return c;
+}
+static uint32_t pow5(int e) { uint32_t r = 1; for (int i = 0; i < e; ++i) r = (r * 5u) & (2 * N - 1); return r; }
+
+int main(int argc, char** argv) {
+    if (argc != 4) { std::fprintf(stderr, "usage: dropin_test <dir> <r> <N1>\n"); return 2; }
+    const std::string dir = argv[1];
+    const int32_t r = std::atoi(argv[2]), N1 = std::atoi(argv[3]), N2 = N / 2 / N1;
+    int lr = 0;
+    while ((1 << lr) < r) ++lr;
+    std::vector<uint64_t> datacrt = read_u64(dir + "/db.bin", (size_t)r * N * N / 2);
+    std::vector<uint64_t> qw = read_u64(dir + "/query.bin", 4 * (size_t)N);
+    std::vector<uint64_t> rg = read_u64(dir + "/rgsw.bin", 8 * (size_t)N);
+    std::vector<uint64_t> baby = read_u64(dir + "/baby.bin", (size_t)(N1 / 2) * 12 * N);
+    std::vector<uint64_t> giant = read_u64(dir + "/giant.bin", (size_t)(N2 - 1) * 4 * N);
+    std::vector<uint64_t> pack = read_u64(dir + "/packing.bin", (size_t)lr * 8 * N);
+    std::vector<int32_t> pm = read_i32(dir + "/perms.bin", (size_t)N1 * N);
+
+    // the objects test-pir.cpp:68-98 builds
+    std::vector<RlweCiphertext> query1(1, rlwe(&qw[0], crtMod, false));
+    query1.push_back(rlwe(&qw[2 * N], bsMod, false));
+    RGSWCiphertext queryGsw(N, crtMod, 2, 1 << 20, 1 << 18);
+    for (int k = 0; k < 4; ++k) {
+        queryGsw.a[k].assign(&rg[(size_t)k * 2 * N], &rg[(size_t)k * 2 * N + N]);
+        queryGsw.b[k].assign(&rg[(size_t)k * 2 * N + N], &rg[(size_t)(k + 1) * 2 * N]);
+    }
+    AutoKeyBSGSRNS autoKey(N, crtMod, bsMod);
+    for (int i = 1; i <= N1 / 2; ++i) {
+        std::vector<RlweCiphertext> v;
+        for (int j = 0; j < 6; ++j) v.push_back(rlwe(&baby[(((size_t)(i - 1) * 6 + j) * 2) * N], j < 3 ? crtMod : bsMod, true));
+        autoKey.keyMap[pow5(i)] = v;
+    }
+    for (int i = 1; i < N2; ++i) {
+        std::vector<RlweCiphertext> v;
+        for (int j = 0; j < 2; ++j) v.push_back(rlwe(&giant[(((size_t)(i - 1) * 2 + j) * 2) * N], crtMod, true));
+        autoKey.keyMap[pow5(N1 * i)] = v;
+    }
+    AutoKey packingKey(N, crtMod, 4, 0, 1 << 14);
+    for (int i = 1, li = 0; i < r; i <<= 1, ++li) {
+        std::vector<RlweCiphertext> v;
+        for (int j = 0; j < 4; ++j) v.push_back(rlwe(&pack[(((size_t)li * 4 + j) * 2) * N], crtMod, true));
+        packingKey.keyMap[N / i + 1] = v;
+    }
+    std::vector<std::vector<int32_t> > permutations(N1, std::vector<int32_t>(N));
+    for (int i = 0; i < N1; ++i) permutations[i].assign(&pm[(size_t)i * N], &pm[(size_t)(i + 1) * N]);
+
+    std::vector<RlweCiphertext> result(r, RlweCiphertext(N, crtMod));
+    RlweCiphertext result_output(N, crtMod);
+    std::vector<RlweCiphertext> ext_rlwes;
+    for (int i = 0; i < r; ++i) ext_rlwes.push_back(RlweCiphertext(N, crtMod));
+
+    // test-pir.cpp:107-116
+    auto start = std::chrono::high_resolution_clock::now();
+    matrix_vector_mul_bsgs_rns_crt_large(result, query1, datacrt.data(), autoKey, N1, permutations, r);
+    std::vector<uint64_t> fd((size_t)r * 2 * N);
+    for (int j = 0; j < r; ++j) {
+        std::copy(result[j].a.begin(), result[j].a.end(), fd.begin() + (size_t)j * 2 * N);
+        std::copy(result[j].b.begin(), result[j].b.end(), fd.begin() + (size_t)j * 2 * N + N);
+    }
+    for (int j = 0; j < r; ++j) externalProduct(ext_rlwes[j], result[j], queryGsw);
+    packingRLWEs(result_output, ext_rlwes, packingKey);
+    auto stop = std::chrono::high_resolution_clock::now();
+
+    std::vector<uint64_t> resp(2 * (size_t)N);
+    std::copy(result_output.a.begin(), result_output.a.end(), resp.begin());
+    std::copy(result_output.b.begin(), result_output.b.end(), resp.begin() + N);
+    write_u64(dir + "/response.bin", resp);
+    write_u64(dir + "/firstdim.bin", fd);
+    std::vector<uint64_t> q0(2 * (size_t)N);
+    std::copy(query1[0].a.begin(), query1[0].a.end(), q0.begin());
+    std::copy(query1[0].b.begin(), query1[0].b.end(), q0.begin() + N);
+    write_u64(dir + "/query0_after.bin", q0);
+    std::printf("online server response costs %lld us (first call, includes uploads); input[0] ntt=%d result ntt=%d\n",
+                (long long)std::chrono::duration_cast<std::chrono::microseconds>(stop - start).count(),
+                (int)query1[0].getIsNtt(), (int)result_output.getIsNtt());
+    return 0;
+}

@@ -194,3 +194,36 @@ def test_batch_mfma_shapes_and_chunks(ctx, oracle, n1, chunk):
     for k in keys:
         k.close()
     db.close()
+
+
+def test_cpp_dropin_matches_oracle(oracle, tmp_path):
+    """tests/native/dropin_test calls matrix_vector_mul_bsgs_rns_crt_large,
+    externalProduct and packingRLWEs through kspir_pir.h exactly as
+    test-pir.cpp:111-115 does; its response words, first-dimension results
+    and mutated query equal the oracle's"""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(__file__), "native", "_build", "dropin_test")
+    if not os.path.exists(exe):
+        pytest.fail("tests/native/_build/dropin_test not built (python kspir_amd/build.py)")
+    r, n1 = 4, 128
+    say(f"[gpu] C++ drop-in r={r} n1={n1}")
+    inst, resp, st = instance(oracle, r, n1, 21)
+    for name in ("db", "query", "rgsw", "baby", "giant", "packing", "perms"):
+        getattr(inst, name).tofile(str(tmp_path / f"{name}.bin"))
+    out = subprocess.run([exe, str(tmp_path), str(r), str(n1)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    say(out.stdout.strip())
+    got = np.fromfile(str(tmp_path / "response.bin"), dtype=np.uint64).reshape(2, N)
+    fd = np.fromfile(str(tmp_path / "firstdim.bin"), dtype=np.uint64).reshape(r, 2, N)
+    q0 = np.fromfile(str(tmp_path / "query0_after.bin"), dtype=np.uint64).reshape(2, N)
+    assert np.array_equal(fd, st["firstdim"])
+    assert np.array_equal(got, resp)
+    cb = st["cipherbuf"][:, 0, :]
+    lo, hi = (cb & np.uint64(0xffffffff)), (cb >> np.uint64(32))
+    want_q0 = np.array([[int(oracle.lib().ko_crt_compose(int(lo[z, ab]), int(hi[z, ab]))) for z in range(N)]
+                        for ab in range(2)], dtype=np.uint64)
+    assert np.array_equal(q0, want_q0)
+    assert "input[0] ntt=1 result ntt=0" in out.stdout
+    msg = oracle.decrypt(r, inst.secret, got)
+    assert np.array_equal(msg, np.arange(1, N + 1, dtype=np.uint64))
