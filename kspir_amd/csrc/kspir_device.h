@@ -244,40 +244,101 @@ __host__ __device__ __forceinline__ void ntt_inv_pass(uint32_t v[16], const uint
     }
 }
 
+// one pass with the pass index known at compile time (constant LDS offsets;
+// pass-0 twiddle addresses are uniform, so they become scalar loads)
+template <int PASS, bool INV>
+__device__ __forceinline__ void ntt_pass_lds(uint32_t* s, const PrimeTab& P, int tid) {
+    uint32_t v[16], w[15], ws[15];
+    load_pass_twiddles(INV ? P.iv2 : P.fw2, PASS, tid, w, ws);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = s[lds_pad(pass_elem(PASS, tid, k))];
+    if (INV) {
+        ntt_inv_pass(v, w, ws, P.q);
+        if (PASS == 0) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) v[k] = mul_shoup_hd(v[k], P.ninv, P.ninv_sh, P.q);
+        }
+    } else {
+        ntt_fwd_pass(v, w, ws, P.q);
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) s[lds_pad(pass_elem(PASS, tid, k))] = v[k];
+}
+
 // whole transforms on a padded LDS buffer s[kPadN]; 256 threads; the caller
 // synchronises before (data in s) and after (result in s)
 __device__ __forceinline__ void ntt_fwd_r16(uint32_t* s, const PrimeTab& P) {
     const int tid = threadIdx.x;
-    uint32_t v[16], w[15], ws[15];
-#pragma unroll 1
-    for (int pass = 0; pass < 3; ++pass) {
-        load_pass_twiddles(P.fw2, pass, tid, w, ws);
-#pragma unroll
-        for (int k = 0; k < 16; ++k) v[k] = s[lds_pad(pass_elem(pass, tid, k))];
-        ntt_fwd_pass(v, w, ws, P.q);
-#pragma unroll
-        for (int k = 0; k < 16; ++k) s[lds_pad(pass_elem(pass, tid, k))] = v[k];
-        __syncthreads();
-    }
+    ntt_pass_lds<0, false>(s, P, tid);
+    __syncthreads();
+    ntt_pass_lds<1, false>(s, P, tid);
+    __syncthreads();
+    ntt_pass_lds<2, false>(s, P, tid);
+    __syncthreads();
 }
 
 __device__ __forceinline__ void ntt_inv_r16(uint32_t* s, const PrimeTab& P) {
     const int tid = threadIdx.x;
-    uint32_t v[16], w[15], ws[15];
-#pragma unroll 1
-    for (int pass = 2; pass >= 0; --pass) {
-        load_pass_twiddles(P.iv2, pass, tid, w, ws);
+    ntt_pass_lds<2, true>(s, P, tid);
+    __syncthreads();
+    ntt_pass_lds<1, true>(s, P, tid);
+    __syncthreads();
+    ntt_pass_lds<0, true>(s, P, tid);
+    __syncthreads();
+}
+
+// two independent transforms (primes PA, PB may differ) interleaved pass by
+// pass: twice the independent work between barriers
+template <int PASS, bool INV>
+__device__ __forceinline__ void ntt_pass_lds2(uint32_t* sa, const PrimeTab& PA, uint32_t* sb, const PrimeTab& PB,
+                                              int tid) {
+    uint32_t va[16], vb[16], wa[15], wsa[15], wb[15], wsb[15];
+    load_pass_twiddles(INV ? PA.iv2 : PA.fw2, PASS, tid, wa, wsa);
+    load_pass_twiddles(INV ? PB.iv2 : PB.fw2, PASS, tid, wb, wsb);
 #pragma unroll
-        for (int k = 0; k < 16; ++k) v[k] = s[lds_pad(pass_elem(pass, tid, k))];
-        ntt_inv_pass(v, w, ws, P.q);
-        if (pass == 0) {
-#pragma unroll
-            for (int k = 0; k < 16; ++k) v[k] = mul_shoup_hd(v[k], P.ninv, P.ninv_sh, P.q);
-        }
-#pragma unroll
-        for (int k = 0; k < 16; ++k) s[lds_pad(pass_elem(pass, tid, k))] = v[k];
-        __syncthreads();
+    for (int k = 0; k < 16; ++k) {
+        va[k] = sa[lds_pad(pass_elem(PASS, tid, k))];
+        vb[k] = sb[lds_pad(pass_elem(PASS, tid, k))];
     }
+    if (INV) {
+        ntt_inv_pass(va, wa, wsa, PA.q);
+        ntt_inv_pass(vb, wb, wsb, PB.q);
+        if (PASS == 0) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                va[k] = mul_shoup_hd(va[k], PA.ninv, PA.ninv_sh, PA.q);
+                vb[k] = mul_shoup_hd(vb[k], PB.ninv, PB.ninv_sh, PB.q);
+            }
+        }
+    } else {
+        ntt_fwd_pass(va, wa, wsa, PA.q);
+        ntt_fwd_pass(vb, wb, wsb, PB.q);
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        sa[lds_pad(pass_elem(PASS, tid, k))] = va[k];
+        sb[lds_pad(pass_elem(PASS, tid, k))] = vb[k];
+    }
+}
+
+__device__ __forceinline__ void ntt_fwd_r16x2(uint32_t* sa, const PrimeTab& PA, uint32_t* sb, const PrimeTab& PB) {
+    const int tid = threadIdx.x;
+    ntt_pass_lds2<0, false>(sa, PA, sb, PB, tid);
+    __syncthreads();
+    ntt_pass_lds2<1, false>(sa, PA, sb, PB, tid);
+    __syncthreads();
+    ntt_pass_lds2<2, false>(sa, PA, sb, PB, tid);
+    __syncthreads();
+}
+
+__device__ __forceinline__ void ntt_inv_r16x2(uint32_t* sa, const PrimeTab& PA, uint32_t* sb, const PrimeTab& PB) {
+    const int tid = threadIdx.x;
+    ntt_pass_lds2<2, true>(sa, PA, sb, PB, tid);
+    __syncthreads();
+    ntt_pass_lds2<1, true>(sa, PA, sb, PB, tid);
+    __syncthreads();
+    ntt_pass_lds2<0, true>(sa, PA, sb, PB, tid);
+    __syncthreads();
 }
 
 // slot permutation realising the automorphism X -> X^g in the NTT domain:

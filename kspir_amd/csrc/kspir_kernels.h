@@ -278,8 +278,7 @@ __global__ void __launch_bounds__(kNttThreads) k_baby_rot(const uint32_t* __rest
     ntt_inv_r16(sb, T.p[2]);
     for (int x = threadIdx.x; x < kN; x += kNttThreads) { s1[lds_pad(x)] = sb[lds_pad(x)]; s2[lds_pad(x)] = sb[lds_pad(x)]; }
     __syncthreads();
-    ntt_fwd_r16(s1, T.p[0]);
-    ntt_fwd_r16(s2, T.p[1]);
+    ntt_fwd_r16x2(s1, T.p[0], s2, T.p[1]);
     const int k = k_base + rot;
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
@@ -583,8 +582,7 @@ __global__ void __launch_bounds__(kNttThreads) k_giant_ks(const uint32_t* __rest
         s1[lds_pad(x)] = digit_residue(tfhe_digit<18, 20, 2>(v, 1), P.q);
     }
     __syncthreads();
-    ntt_fwd_r16(s0, P);
-    ntt_fwd_r16(s1, P);
+    ntt_fwd_r16x2(s0, P, s1, P);
     const uint32_t* key = gkeys + (size_t)(c - 1) * 2 * 2 * 2 * kN;
     const uint32_t* bh = dbout + (((size_t)blk * n2 + c) * 4 + 2 + p) * kN;
     uint32_t* o = gks + (((size_t)blk * n2 + c) * 4) * kN;
@@ -764,8 +762,7 @@ __global__ void __launch_bounds__(kNttThreads) k_response(const uint32_t* __rest
         s2[lds_pad(x)] = e[(ab * 2 + 1) * kN + x];
     }
     __syncthreads();
-    ntt_inv_r16(s1, T.p[0]);
-    ntt_inv_r16(s2, T.p[1]);
+    ntt_inv_r16x2(s1, T.p[0], s2, T.p[1]);
     uint64_t* o = resp + (size_t)blockIdx.y * 2 * kN + ab * kN;
     for (int x = threadIdx.x; x < kN; x += kNttThreads) o[x] = crt_q(s1[lds_pad(x)], s2[lds_pad(x)], T);
 }
