@@ -421,27 +421,30 @@ __global__ void __launch_bounds__(256) k_db_limbs(const uint64_t* __restrict__ w
 
 // query limbs, per batch: ctw [q][k][ab][p][z] -> ql [z][p][v][col][k],
 // col = q*2 + ab, cols padded with zeros to C; ctw points at the chunk's
-// first query, nq = queries in the chunk.  grid (N/64, C/2, 2 p)
+// first query, nq = queries in the chunk.  A workgroup takes 32 slots and two
+// queries with all N1 <= 128 rotations, so it writes 512-byte runs (4 cols x
+// N1 bytes) per (slot, limb).  grid (N/32, C/4, 2 p)
+template <int N1>
 __global__ void __launch_bounds__(256) k_ct_limbs(const uint32_t* __restrict__ ctw, int8_t* __restrict__ ql,
-                                                  int n1, int nq, int C, QS S) {
-    __shared__ uint32_t tile[64][2][65];
-    const int z0 = blockIdx.x * 64, qs = blockIdx.y, p = blockIdx.z;   // qs: query within the chunk
-    for (int kc = 0; kc < n1; kc += 64) {
-        for (int e = threadIdx.x; e < 64 * 2 * 64; e += 256) {
-            const int zz = e & 63, ab = (e >> 6) & 1, kk = e >> 7;
-            tile[kk][ab][zz] = qs < nq ? ctw[qs * S.ctw + ((size_t)((kc + kk) * 2 + ab) * 2 + p) * kN + z0 + zz] : 0u;
-        }
-        __syncthreads();
-        // 64 slots x 4 limbs x 2 ab x 4 chunks of 16 bytes
-        for (int e = threadIdx.x; e < 64 * 4 * 2 * 4; e += 256) {
-            const int c16 = e & 3, ab = (e >> 2) & 1, v = (e >> 3) & 3, zz = e >> 5;
-            i8x16 b;
+                                                  int nq, int C, QS S) {
+    __shared__ uint32_t tile[2][2][N1][32];    // [query][ab][k][slot]
+    const int z0 = blockIdx.x * 32, q0 = blockIdx.y * 2, p = blockIdx.z;
+    for (int e = threadIdx.x; e < 2 * 2 * N1 * 32; e += 256) {
+        const int zz = e & 31, k = (e >> 5) % N1, ab = (e / (32 * N1)) & 1, qq = e / (64 * N1);
+        const int q = q0 + qq;
+        tile[qq][ab][k][zz] = q < nq ? ctw[q * S.ctw + ((size_t)(k * 2 + ab) * 2 + p) * kN + z0 + zz] : 0u;
+    }
+    __syncthreads();
+    // per (slot, limb): 4 cols x N1 bytes, as 16-byte chunks
+    constexpr int CH = N1 / 16;
+    for (int e = threadIdx.x; e < 32 * 4 * 4 * CH; e += 256) {
+        const int c16 = e % CH, col = (e / CH) & 3, v = (e / (CH * 4)) & 3, zz = e / (CH * 16);
+        const int qq = col >> 1, ab = col & 1;
+        i8x16 b;
 #pragma unroll
-            for (int t = 0; t < 16; ++t) b[t] = (int8_t)((tile[c16 * 16 + t][ab][zz] >> (7 * v)) & 127);
-            int8_t* dst = ql + ((((size_t)(z0 + zz) * 2 + p) * 4 + v) * C + qs * 2 + ab) * n1 + kc + c16 * 16;
-            *reinterpret_cast<i8x16*>(dst) = b;
-        }
-        __syncthreads();
+        for (int t = 0; t < 16; ++t) b[t] = (int8_t)((tile[qq][ab][c16 * 16 + t][zz] >> (7 * v)) & 127);
+        int8_t* dst = ql + ((((size_t)(z0 + zz) * 2 + p) * 4 + v) * C + q0 * 2 + col) * N1 + c16 * 16;
+        *reinterpret_cast<i8x16*>(dst) = b;
     }
 }
 
@@ -514,23 +517,29 @@ __global__ void __launch_bounds__(256) k_db_mfma(const int8_t* __restrict__ dbl,
 }
 
 // tile-major MFMA output -> dbout [q][b][i][ab][p][z] for the nq queries of
-// a chunk (dbout points at the chunk's first query): grid (N/32, RT*CT, 2 p)
+// a chunk (dbout points at the chunk's first query).  64 slots per workgroup:
+// 64 KiB contiguous reads, 256-byte runs per written plane; the LDS tile is
+// XOR-swizzled (slot ^ (row & 31)) so both phases are conflict-free.
+// grid (N/64, RT*CT, 2 p)
 __global__ void __launch_bounds__(256) k_mfma_scatter(const uint32_t* __restrict__ mf, uint32_t* __restrict__ dbout,
                                                       int R, int C, int n2, int nq, QS S) {
-    __shared__ uint32_t tile[256][33];
-    const int z0 = blockIdx.x * 32, t = blockIdx.y, p = blockIdx.z;
+    __shared__ uint32_t tile[256][64];
+    const int z0 = blockIdx.x * 64, t = blockIdx.y, p = blockIdx.z;
     const int RT = R / 16, CT = C / 16;
     const int rt = t / CT, ct = t % CT;
     const uint32_t* src = mf + (((size_t)(p * RT + rt) * CT + ct) * kN + z0) * 256;
-    for (int e = threadIdx.x; e < 32 * 256; e += 256) tile[e & 255][e >> 8] = src[e];
+    for (int e = threadIdx.x; e < 64 * 256; e += 256) {
+        const int cr = e & 255, zz = e >> 8;
+        tile[cr][zz ^ (cr & 31)] = src[e];
+    }
     __syncthreads();
-    for (int e = threadIdx.x; e < 256 * 32; e += 256) {
-        const int zz = e & 31, cr = e >> 5;          // cr = col_in * 16 + rho_in
+    for (int e = threadIdx.x; e < 256 * 64; e += 256) {
+        const int zz = e & 63, cr = e >> 6;          // cr = col_in * 16 + rho_in
         const int col = ct * 16 + (cr >> 4), rho = rt * 16 + (cr & 15);
         const int q = col >> 1, ab = col & 1;
         if (q < nq) {
             const int b = rho / n2, i = rho % n2;
-            dbout[q * S.dbout + ((size_t)(b * n2 + i) * 4 + ab * 2 + p) * kN + z0 + zz] = tile[cr][zz];
+            dbout[q * S.dbout + ((size_t)(b * n2 + i) * 4 + ab * 2 + p) * kN + z0 + zz] = tile[cr][zz ^ (cr & 31)];
         }
     }
 }
