@@ -310,13 +310,9 @@ int launch_db_batched(kspir_context* c, const kspir_database* dbc, int nq, hipSt
         const int nqc = nq - q0 < nqc_max ? nq - q0 : nqc_max;
         const int C = (2 * nqc + 15) / 16 * 16;
         switch (n1) {
-            case 64: hipLaunchKernelGGL(k_ct_limbs<64>, dim3(kN / 32, C / 4, 2), dim3(256), 0, st, c->ctw + q0 * S.ctw, ql, nqc, C, S); break;
-            case 128: hipLaunchKernelGGL(k_ct_limbs<128>, dim3(kN / 32, C / 4, 2), dim3(256), 0, st, c->ctw + q0 * S.ctw, ql, nqc, C, S); break;
-            default: {
-                // N1 = 256: two sweeps of 128 rotations would need a k offset; use the 64 KiB tile per 128
-                hipLaunchKernelGGL(k_ct_limbs<256>, dim3(kN / 32, C / 4, 2), dim3(256), 0, st, c->ctw + q0 * S.ctw, ql, nqc, C, S);
-                break;
-            }
+            case 64: hipLaunchKernelGGL(k_ct_limbs<64>, dim3(kN / 32, C / 2, 2), dim3(256), 0, st, c->ctw + q0 * S.ctw, ql, nqc, C, S); break;
+            case 128: hipLaunchKernelGGL(k_ct_limbs<128>, dim3(kN / 32, C / 2, 2), dim3(256), 0, st, c->ctw + q0 * S.ctw, ql, nqc, C, S); break;
+            default: hipLaunchKernelGGL(k_ct_limbs<256>, dim3(kN / 32, C / 2, 2), dim3(256), 0, st, c->ctw + q0 * S.ctw, ql, nqc, C, S); break;
         }
         LAUNCH_OK("k_ct_limbs");
         const size_t lds = (size_t)4 * C * (n1 + 16);
@@ -335,7 +331,7 @@ int launch_db_batched(kspir_context* c, const kspir_database* dbc, int nq, hipSt
                 break;
         }
         LAUNCH_OK("k_db_mfma");
-        hipLaunchKernelGGL(k_mfma_scatter, dim3(kN / 64, (R / 16) * (C / 16), 2), dim3(256), 0, st, mf,
+        hipLaunchKernelGGL(k_mfma_scatter, dim3(kN / 32, (R / 16) * (C / 16), 2), dim3(256), 0, st, mf,
                            c->dbout + q0 * S.dbout, R, C, n2, nqc, S);
         LAUNCH_OK("k_mfma_scatter");
     }

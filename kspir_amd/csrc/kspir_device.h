@@ -207,9 +207,22 @@ __host__ __device__ __forceinline__ void load_pass_twiddles(const uint2* tw2, in
     }
 }
 
-// forward pass (stages 4P .. 4P+3, Cooley-Tukey) on registers v[16]
+__host__ __device__ __forceinline__ uint32_t umin_hd(uint32_t a, uint32_t b) { return a < b ? a : b; }
+
+// x * w mod q up to one extra q: result in [0, 2q) for any x < 2^32 (Shoup,
+// without the final correction), w < q, wp = floor(w * 2^32 / q)
+__host__ __device__ __forceinline__ uint32_t mul_shoup_lazy(uint32_t x, uint32_t w, uint32_t wp, uint32_t q) {
+    return x * w - mulhi32_hd(x, wp) * q;
+}
+
+// Butterflies with lazy reduction (Harvey, "Faster arithmetic for
+// number-theoretic transforms", 2014): every q here is below 2^28, so words
+// up to 4q fit in 32 bits and the butterflies skip most corrections.
+// Forward pass (stages 4P .. 4P+3, Cooley-Tukey) on registers v[16]:
+// inputs in [0, 4q), outputs in [0, 4q).
 __host__ __device__ __forceinline__ void ntt_fwd_pass(uint32_t v[16], const uint32_t w[15], const uint32_t ws[15],
                                                       uint32_t q) {
+    const uint32_t q2 = 2 * q;
 #pragma unroll
     for (int st = 0; st < 4; ++st) {
         const int kb = 8 >> st;   // register-index bit flipped by this stage
@@ -217,18 +230,31 @@ __host__ __device__ __forceinline__ void ntt_fwd_pass(uint32_t v[16], const uint
         for (int k = 0; k < 16; ++k) {
             if (k & kb) continue;
             const int ti = (1 << st) - 1 + (k >> (4 - st));
-            const uint32_t u = v[k];
-            const uint32_t t = mul_shoup_hd(v[k | kb], w[ti], ws[ti], q);
-            const uint32_t a = u + t, b = u + q - t;
-            v[k] = a >= q ? a - q : a;
-            v[k | kb] = b >= q ? b - q : b;
+            const uint32_t u = umin_hd(v[k], v[k] - q2);                    // [0, 2q)
+            // nt = -t mod 2^32 for t = w * v[k|kb] mod q in [0, 2q), so the
+            // two outputs are one subtract and one three-way add
+            const uint32_t x = v[k | kb];
+            const uint32_t nt = mulhi32_hd(x, ws[ti]) * q - x * w[ti];
+            v[k] = u - nt;                 // u + t
+            v[k | kb] = u + nt + q2;       // u - t + 2q
         }
     }
 }
 
-// inverse pass (stages 4P+3 .. 4P, Gentleman-Sande)
+// last forward pass: [0, 4q) -> [0, q)
+__host__ __device__ __forceinline__ void ntt_fwd_finish(uint32_t v[16], uint32_t q) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const uint32_t x = umin_hd(v[k], v[k] - 2 * q);
+        v[k] = umin_hd(x, x - q);
+    }
+}
+
+// inverse pass (stages 4P+3 .. 4P, Gentleman-Sande): inputs and outputs in
+// [0, 2q)
 __host__ __device__ __forceinline__ void ntt_inv_pass(uint32_t v[16], const uint32_t w[15], const uint32_t ws[15],
                                                       uint32_t q) {
+    const uint32_t q2 = 2 * q;
 #pragma unroll
     for (int st = 3; st >= 0; --st) {
         const int kb = 8 >> st;
@@ -238,9 +264,18 @@ __host__ __device__ __forceinline__ void ntt_inv_pass(uint32_t v[16], const uint
             const int ti = (1 << st) - 1 + (k >> (4 - st));
             const uint32_t u = v[k], x = v[k | kb];
             const uint32_t a = u + x;
-            v[k] = a >= q ? a - q : a;
-            v[k | kb] = mul_shoup_hd(u + q - x, w[ti], ws[ti], q);
+            v[k] = umin_hd(a, a - q2);
+            v[k | kb] = mul_shoup_lazy(u - x + q2, w[ti], ws[ti], q);
         }
+    }
+}
+
+// last inverse pass: scale by N^-1 and reduce [0, 2q) -> [0, q)
+__host__ __device__ __forceinline__ void ntt_inv_finish(uint32_t v[16], uint32_t ninv, uint32_t ninv_sh, uint32_t q) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const uint32_t x = mul_shoup_lazy(v[k], ninv, ninv_sh, q);
+        v[k] = umin_hd(x, x - q);
     }
 }
 
@@ -254,12 +289,10 @@ __device__ __forceinline__ void ntt_pass_lds(uint32_t* s, const PrimeTab& P, int
     for (int k = 0; k < 16; ++k) v[k] = s[lds_pad(pass_elem(PASS, tid, k))];
     if (INV) {
         ntt_inv_pass(v, w, ws, P.q);
-        if (PASS == 0) {
-#pragma unroll
-            for (int k = 0; k < 16; ++k) v[k] = mul_shoup_hd(v[k], P.ninv, P.ninv_sh, P.q);
-        }
+        if (PASS == 0) ntt_inv_finish(v, P.ninv, P.ninv_sh, P.q);
     } else {
         ntt_fwd_pass(v, w, ws, P.q);
+        if (PASS == 2) ntt_fwd_finish(v, P.q);
     }
 #pragma unroll
     for (int k = 0; k < 16; ++k) s[lds_pad(pass_elem(PASS, tid, k))] = v[k];
@@ -304,15 +337,16 @@ __device__ __forceinline__ void ntt_pass_lds2(uint32_t* sa, const PrimeTab& PA, 
         ntt_inv_pass(va, wa, wsa, PA.q);
         ntt_inv_pass(vb, wb, wsb, PB.q);
         if (PASS == 0) {
-#pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                va[k] = mul_shoup_hd(va[k], PA.ninv, PA.ninv_sh, PA.q);
-                vb[k] = mul_shoup_hd(vb[k], PB.ninv, PB.ninv_sh, PB.q);
-            }
+            ntt_inv_finish(va, PA.ninv, PA.ninv_sh, PA.q);
+            ntt_inv_finish(vb, PB.ninv, PB.ninv_sh, PB.q);
         }
     } else {
         ntt_fwd_pass(va, wa, wsa, PA.q);
         ntt_fwd_pass(vb, wb, wsb, PB.q);
+        if (PASS == 2) {
+            ntt_fwd_finish(va, PA.q);
+            ntt_fwd_finish(vb, PB.q);
+        }
     }
 #pragma unroll
     for (int k = 0; k < 16; ++k) {

@@ -419,32 +419,57 @@ __global__ void __launch_bounds__(256) k_db_limbs(const uint64_t* __restrict__ w
     }
 }
 
-// query limbs, per batch: ctw [q][k][ab][p][z] -> ql [z][p][v][col][k],
-// col = q*2 + ab, cols padded with zeros to C; ctw points at the chunk's
-// first query, nq = queries in the chunk.  A workgroup takes 32 slots and two
-// queries with all N1 <= 128 rotations, so it writes 512-byte runs (4 cols x
-// N1 bytes) per (slot, limb).  grid (N/32, C/4, 2 p)
+// query limbs, per batch: ctw [q][k][ab][p][z] -> ql [z][p][v][col][k]
+// (7-bit limbs v = 0..3), col = q*2 + ab, cols padded with zeros to C; ctw
+// points at the chunk's first query, nq = queries in the chunk.  A workgroup
+// takes 32 slots of one query for one residue.  It loads 128-byte row pieces
+// with 16-byte loads (all issued before the first LDS write) into the tile
+// [ab*N1 + k][slot] of pitch 33 words, conflict-free both ways.  Then each wave
+// reads 64 consecutive rotations of one slot (one word per lane), packs the
+// four limbs of its word into bytes, and transposes 4x4 bytes across each lane
+// quad with DPP.  Lane 4j+v then holds limb v of rotations 4j..4j+3, so each
+// store instruction writes four 64-byte runs.  grid (N/32, C/2, 2 p)
 template <int N1>
 __global__ void __launch_bounds__(256) k_ct_limbs(const uint32_t* __restrict__ ctw, int8_t* __restrict__ ql,
                                                   int nq, int C, QS S) {
-    __shared__ uint32_t tile[2][2][N1][32];    // [query][ab][k][slot]
-    const int z0 = blockIdx.x * 32, q0 = blockIdx.y * 2, p = blockIdx.z;
-    for (int e = threadIdx.x; e < 2 * 2 * N1 * 32; e += 256) {
-        const int zz = e & 31, k = (e >> 5) % N1, ab = (e / (32 * N1)) & 1, qq = e / (64 * N1);
-        const int q = q0 + qq;
-        tile[qq][ab][k][zz] = q < nq ? ctw[q * S.ctw + ((size_t)(k * 2 + ab) * 2 + p) * kN + z0 + zz] : 0u;
+    constexpr int ROWS = 2 * N1, PITCH = 33, PER = ROWS * 8 / 256;
+    __shared__ uint32_t tile[ROWS * PITCH];
+    const int z0 = blockIdx.x * 32, q = blockIdx.y, p = blockIdx.z;
+    const int tid = threadIdx.x;
+    uint4 buf[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int e = tid + 256 * i, zq = e & 7, row = e >> 3;
+        const int ab = row / N1, k = row % N1;
+        buf[i] = q < nq ? *reinterpret_cast<const uint4*>(ctw + (size_t)q * S.ctw +
+                                                          ((size_t)(k * 2 + ab) * 2 + p) * kN + z0 + 4 * zq)
+                        : make_uint4(0u, 0u, 0u, 0u);
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int e = tid + 256 * i, zq = e & 7, row = e >> 3;
+        uint32_t* t = tile + row * PITCH + 4 * zq;
+        t[0] = buf[i].x;
+        t[1] = buf[i].y;
+        t[2] = buf[i].z;
+        t[3] = buf[i].w;
     }
     __syncthreads();
-    // per (slot, limb): 4 cols x N1 bytes, as 16-byte chunks
-    constexpr int CH = N1 / 16;
-    for (int e = threadIdx.x; e < 32 * 4 * 4 * CH; e += 256) {
-        const int c16 = e % CH, col = (e / CH) & 3, v = (e / (CH * 4)) & 3, zz = e / (CH * 16);
-        const int qq = col >> 1, ab = col & 1;
-        i8x16 b;
-#pragma unroll
-        for (int t = 0; t < 16; ++t) b[t] = (int8_t)((tile[qq][ab][c16 * 16 + t][zz] >> (7 * v)) & 127);
-        int8_t* dst = ql + ((((size_t)(z0 + zz) * 2 + p) * 4 + v) * C + q0 * 2 + col) * N1 + c16 * 16;
-        *reinterpret_cast<i8x16*>(dst) = b;
+    const int lane = tid & 63, wv = tid >> 6;
+    const bool odd = lane & 1, hi = lane & 2;
+    for (int it = wv; it < 32 * 2 * (N1 / 64); it += 4) {
+        const int zz = it & 31, ab = (it >> 5) & 1, kr = it >> 6;
+        const uint32_t x = tile[(ab * N1 + kr * 64 + lane) * PITCH + zz];
+        // byte v = limb v of this lane's word (x < 2^28)
+        const uint32_t w = (x & 0x7Fu) | ((x << 1) & 0x7F00u) | ((x << 2) & 0x7F0000u) | ((x << 3) & 0x7F000000u);
+        // 4x4 byte transpose in each quad: swap bytes with lane^1, then half-words with lane^2
+        const uint32_t pw = (uint32_t)__builtin_amdgcn_mov_dpp((int)w, 0xB1, 0xF, 0xF, false);   // quad_perm 1,0,3,2
+        const uint32_t xx = odd ? (((pw >> 8) & 0x00FF00FFu) | (w & 0xFF00FF00u))
+                                : ((w & 0x00FF00FFu) | ((pw & 0x00FF00FFu) << 8));
+        const uint32_t px = (uint32_t)__builtin_amdgcn_mov_dpp((int)xx, 0x4E, 0xF, 0xF, false);  // quad_perm 2,3,0,1
+        const uint32_t o = hi ? ((px >> 16) | (xx & 0xFFFF0000u)) : ((xx & 0xFFFFu) | (px << 16));
+        const int v = lane & 3, k4 = kr * 16 + (lane >> 2);
+        *reinterpret_cast<uint32_t*>(ql + ((((size_t)(z0 + zz) * 2 + p) * 4 + v) * C + q * 2 + ab) * N1 + 4 * k4) = o;
     }
 }
 
@@ -517,29 +542,40 @@ __global__ void __launch_bounds__(256) k_db_mfma(const int8_t* __restrict__ dbl,
 }
 
 // tile-major MFMA output -> dbout [q][b][i][ab][p][z] for the nq queries of
-// a chunk (dbout points at the chunk's first query).  64 slots per workgroup:
-// 64 KiB contiguous reads, 256-byte runs per written plane; the LDS tile is
-// XOR-swizzled (slot ^ (row & 31)) so both phases are conflict-free.
-// grid (N/64, RT*CT, 2 p)
+// a chunk (dbout points at the chunk's first query).  32 slots of one tile
+// per workgroup: one 32 KiB contiguous read (16-byte loads, all issued before
+// the first LDS write), 128-byte runs per written plane.  The LDS tile
+// [cr][slot ^ ((cr >> 2) & 31)] is conflict-free for both phases.
+// grid (N/32, RT*CT, 2 p)
 __global__ void __launch_bounds__(256) k_mfma_scatter(const uint32_t* __restrict__ mf, uint32_t* __restrict__ dbout,
                                                       int R, int C, int n2, int nq, QS S) {
-    __shared__ uint32_t tile[256][64];
-    const int z0 = blockIdx.x * 64, t = blockIdx.y, p = blockIdx.z;
+    __shared__ uint32_t tile[256 * 32];
+    const int z0 = blockIdx.x * 32, t = blockIdx.y, p = blockIdx.z;
     const int RT = R / 16, CT = C / 16;
     const int rt = t / CT, ct = t % CT;
-    const uint32_t* src = mf + (((size_t)(p * RT + rt) * CT + ct) * kN + z0) * 256;
-    for (int e = threadIdx.x; e < 64 * 256; e += 256) {
-        const int cr = e & 255, zz = e >> 8;
-        tile[cr][zz ^ (cr & 31)] = src[e];
+    const uint4* src = reinterpret_cast<const uint4*>(mf + (((size_t)(p * RT + rt) * CT + ct) * kN + z0) * 256);
+    uint4 buf[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) buf[i] = src[threadIdx.x + 256 * i];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int e = threadIdx.x + 256 * i, zz = e >> 6, c4 = e & 63;   // words cr = 4*c4 .. 4*c4+3 of slot zz
+        uint32_t* d = tile + 4 * c4 * 32 + (zz ^ (c4 & 31));
+        d[0] = buf[i].x;
+        d[32] = buf[i].y;
+        d[64] = buf[i].z;
+        d[96] = buf[i].w;
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < 256 * 64; e += 256) {
-        const int zz = e & 63, cr = e >> 6;          // cr = col_in * 16 + rho_in
+    const int zz = threadIdx.x & 31;
+#pragma unroll 4
+    for (int cr = threadIdx.x >> 5; cr < 256; cr += 8) {   // cr = col_in * 16 + rho_in
         const int col = ct * 16 + (cr >> 4), rho = rt * 16 + (cr & 15);
         const int q = col >> 1, ab = col & 1;
+        const uint32_t val = tile[cr * 32 + (zz ^ ((cr >> 2) & 31))];
         if (q < nq) {
             const int b = rho / n2, i = rho % n2;
-            dbout[q * S.dbout + ((size_t)(b * n2 + i) * 4 + ab * 2 + p) * kN + z0 + zz] = tile[cr][zz ^ (cr & 31)];
+            dbout[q * S.dbout + ((size_t)(b * n2 + i) * 4 + ab * 2 + p) * kN + z0 + zz] = val;
         }
     }
 }

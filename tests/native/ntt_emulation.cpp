@@ -44,10 +44,15 @@ extern "C" int emul_ntt(uint32_t* poly, uint32_t q, uint32_t root, int inverse) 
             if (inverse) {
                 load_pass_twiddles(iv2.data(), pass, tid, w, ws);
                 ntt_inv_pass(v, w, ws, q);
-                if (pass == 0) for (int k = 0; k < 16; ++k) v[k] = mul_shoup_hd(v[k], ninv, ninvsh, q);
+                if (pass == 0) ntt_inv_finish(v, ninv, ninvsh, q);
+                for (int k = 0; k < 16; ++k)      // lazy words between passes stay below 2q
+                    if (v[k] >= (pass == 0 ? q : 2 * q)) return -1;
             } else {
                 load_pass_twiddles(fw2.data(), pass, tid, w, ws);
                 ntt_fwd_pass(v, w, ws, q);
+                if (pass == 2) ntt_fwd_finish(v, q);
+                for (int k = 0; k < 16; ++k)      // and below 4q in the forward direction
+                    if (v[k] >= (pass == 2 ? q : 4 * q)) return -1;
             }
             for (int k = 0; k < 16; ++k) s[lds_pad(pass_elem(pass, tid, k))] = v[k];
         }

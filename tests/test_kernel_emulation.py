@@ -27,15 +27,20 @@ def emul(tmp_path_factory):
 @pytest.mark.parametrize("q,root,modsel", [(268369921, 62736, 3), (249561089, 32309, 4), (16760833, 3689, 1)])
 def test_register_blocked_ntt_matches_oracle(emul, oracle, q, root, modsel):
     rng = np.random.default_rng(q)
-    for trial in range(3):
+    for trial in range(4):
         a = rng.integers(0, q, N, dtype=np.uint32)
         if trial == 1:
             a[:] = q - 1
+        if trial == 2:
+            a[::2] = q - 1
+            a[1::2] = 0
         x = a.copy()
-        emul.emul_ntt(x.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), ctypes.c_uint32(q),
-                      ctypes.c_uint32(root), 0)
+        rc = emul.emul_ntt(x.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), ctypes.c_uint32(q),
+                           ctypes.c_uint32(root), 0)
+        assert rc == 0, "a lazy word left its bound between passes"
         want = oracle.ntt_forward(modsel, a.astype(np.uint64))
         assert np.array_equal(x.astype(np.uint64), want)
-        emul.emul_ntt(x.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), ctypes.c_uint32(q),
-                      ctypes.c_uint32(root), 1)
+        rc = emul.emul_ntt(x.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), ctypes.c_uint32(q),
+                           ctypes.c_uint32(root), 1)
+        assert rc == 0, "a lazy word left its bound between passes"
         assert np.array_equal(x, a)
