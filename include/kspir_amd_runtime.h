@@ -22,7 +22,11 @@ int kspir_device_count(int* count);
  *   "mfma_min_batch"  batches of at least this many queries take the int8-MFMA
  *                     database pass (default 2); smaller ones the VALU pass
  *   "mfma_max_chunk"  most queries per MFMA database sweep (default: as many
- *                     as the LDS holds) */
+ *                     as the LDS holds)
+ *   "fused_min_batch" batches of at least this many queries run the giant
+ *                     steps and first-dimension sum as one workgroup per
+ *                     (query, block) (default 16); smaller ones spread them
+ *                     over one workgroup per (block, row, prime) */
 int kspir_context_set(kspir_context* ctx, const char* key, int64_t value);
 int kspir_device_alloc(kspir_context* ctx, size_t bytes, void** out);
 int kspir_device_free(kspir_context* ctx, void* ptr);

@@ -100,6 +100,7 @@ struct kspir_context {
     size_t mws_bytes = 0;
     int mfma_min_batch = 2;         // batches of at least this many queries use the MFMA pass
     int mfma_max_chunk = 1 << 30;   // cap on queries per MFMA database sweep (LDS also caps it)
+    int fused_min_batch = 16;       // batches of at least this many queries use k_giant_fused
 };
 
 struct kspir_database {
@@ -393,18 +394,26 @@ int enqueue_answer(kspir_context* c, const kspir_database* db, const KeyPtrs* kp
     int rc = (nq >= c->mfma_min_batch && mfma_shape(db)) ? launch_db_batched(c, db, nq, st)
                                                           : launch_db_product(c, db, nq, st);
     if (rc) return rc;
-    // giant steps (twosteps.cpp:599-677)
-    if (n2 > 1) {
-        hipLaunchKernelGGL(k_giant_coef, dim3(n2 - 1, r, 2 * nq), dim3(kNttThreads), 0, st, c->dbout, c->gcoef, n2,
-                           S, T);
-        LAUNCH_OK("k_giant_coef");
-        hipLaunchKernelGGL(k_giant_ks, dim3(n2 - 1, r, 2 * nq), dim3(kNttThreads), 0, st, c->dbout, c->gcoef, kp,
-                           c->gks, n1, n2, S, T);
-        LAUNCH_OK("k_giant_ks");
+    // giant steps (twosteps.cpp:599-677) and the first-dimension sum: one
+    // fused workgroup per (query, block) for a batch; for few queries, one
+    // workgroup per (block, row, prime) keeps the GPU full
+    if (nq >= c->fused_min_batch) {
+        hipLaunchKernelGGL(k_giant_fused, dim3(r * nq), dim3(kFusedThreads), 0, st, c->dbout, kp, c->fd, c->ecoef, r, nq,
+                           n1, n2, S, T);
+        LAUNCH_OK("k_giant_fused");
+    } else {
+        if (n2 > 1) {
+            hipLaunchKernelGGL(k_giant_coef, dim3(n2 - 1, r, 2 * nq), dim3(kNttThreads), 0, st, c->dbout, c->gcoef,
+                               n2, S, T);
+            LAUNCH_OK("k_giant_coef");
+            hipLaunchKernelGGL(k_giant_ks, dim3(n2 - 1, r, 2 * nq), dim3(kNttThreads), 0, st, c->dbout, c->gcoef, kp,
+                               c->gks, n1, n2, S, T);
+            LAUNCH_OK("k_giant_ks");
+        }
+        hipLaunchKernelGGL(k_ext_coef, dim3(r, 2, 2 * nq), dim3(kNttThreads), 0, st, c->dbout, c->gks, c->fd,
+                           c->ecoef, n2, S, T);
+        LAUNCH_OK("k_ext_coef");
     }
-    hipLaunchKernelGGL(k_ext_coef, dim3(r, 2, 2 * nq), dim3(kNttThreads), 0, st, c->dbout, c->gks, c->fd, c->ecoef,
-                       n2, S, T);
-    LAUNCH_OK("k_ext_coef");
     if (fd_u64) {
         hipLaunchKernelGGL(k_rns_to_u64, dim3(grid1d((size_t)r * 2 * kN)), dim3(256), 0, st, c->fd, fd_u64, r, T);
         LAUNCH_OK("k_rns_to_u64");
@@ -819,6 +828,9 @@ int kspir_context_set(kspir_context* c, const char* key, int64_t value) {
     } else if (k == "mfma_max_chunk") {
         if (value < 1) return fail(KSPIR_EINVAL, "mfma_max_chunk must be >= 1");
         c->mfma_max_chunk = (int)value;
+    } else if (k == "fused_min_batch") {
+        if (value < 1) return fail(KSPIR_EINVAL, "fused_min_batch must be >= 1");
+        c->fused_min_batch = (int)value;
     } else {
         return fail(KSPIR_EINVAL, "unknown option " + k);
     }

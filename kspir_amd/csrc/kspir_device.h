@@ -298,10 +298,11 @@ __device__ __forceinline__ void ntt_pass_lds(uint32_t* s, const PrimeTab& P, int
     for (int k = 0; k < 16; ++k) s[lds_pad(pass_elem(PASS, tid, k))] = v[k];
 }
 
-// whole transforms on a padded LDS buffer s[kPadN]; 256 threads; the caller
-// synchronises before (data in s) and after (result in s)
-__device__ __forceinline__ void ntt_fwd_r16(uint32_t* s, const PrimeTab& P) {
-    const int tid = threadIdx.x;
+// whole transforms on a padded LDS buffer s[kPadN] by 256 threads (tid 0..255;
+// a larger workgroup may run one transform per 256-thread group, every
+// thread reaching the same barriers); the caller synchronises before (data in
+// s) and after (result in s)
+__device__ __forceinline__ void ntt_fwd_r16(uint32_t* s, const PrimeTab& P, int tid) {
     ntt_pass_lds<0, false>(s, P, tid);
     __syncthreads();
     ntt_pass_lds<1, false>(s, P, tid);
@@ -309,9 +310,9 @@ __device__ __forceinline__ void ntt_fwd_r16(uint32_t* s, const PrimeTab& P) {
     ntt_pass_lds<2, false>(s, P, tid);
     __syncthreads();
 }
+__device__ __forceinline__ void ntt_fwd_r16(uint32_t* s, const PrimeTab& P) { ntt_fwd_r16(s, P, (int)threadIdx.x); }
 
-__device__ __forceinline__ void ntt_inv_r16(uint32_t* s, const PrimeTab& P) {
-    const int tid = threadIdx.x;
+__device__ __forceinline__ void ntt_inv_r16(uint32_t* s, const PrimeTab& P, int tid) {
     ntt_pass_lds<2, true>(s, P, tid);
     __syncthreads();
     ntt_pass_lds<1, true>(s, P, tid);
@@ -319,15 +320,21 @@ __device__ __forceinline__ void ntt_inv_r16(uint32_t* s, const PrimeTab& P) {
     ntt_pass_lds<0, true>(s, P, tid);
     __syncthreads();
 }
+__device__ __forceinline__ void ntt_inv_r16(uint32_t* s, const PrimeTab& P) { ntt_inv_r16(s, P, (int)threadIdx.x); }
 
 // two independent transforms (primes PA, PB may differ) interleaved pass by
 // pass: twice the independent work between barriers
-template <int PASS, bool INV>
+template <int PASS, bool INV, bool SAME = false>
 __device__ __forceinline__ void ntt_pass_lds2(uint32_t* sa, const PrimeTab& PA, uint32_t* sb, const PrimeTab& PB,
                                               int tid) {
     uint32_t va[16], vb[16], wa[15], wsa[15], wb[15], wsb[15];
     load_pass_twiddles(INV ? PA.iv2 : PA.fw2, PASS, tid, wa, wsa);
-    load_pass_twiddles(INV ? PB.iv2 : PB.fw2, PASS, tid, wb, wsb);
+    if (SAME) {     // one prime for both: share the twiddle registers
+#pragma unroll
+        for (int t = 0; t < 15; ++t) { wb[t] = wa[t]; wsb[t] = wsa[t]; }
+    } else {
+        load_pass_twiddles(INV ? PB.iv2 : PB.fw2, PASS, tid, wb, wsb);
+    }
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
         va[k] = sa[lds_pad(pass_elem(PASS, tid, k))];
@@ -355,8 +362,7 @@ __device__ __forceinline__ void ntt_pass_lds2(uint32_t* sa, const PrimeTab& PA, 
     }
 }
 
-__device__ __forceinline__ void ntt_fwd_r16x2(uint32_t* sa, const PrimeTab& PA, uint32_t* sb, const PrimeTab& PB) {
-    const int tid = threadIdx.x;
+__device__ __forceinline__ void ntt_fwd_r16x2(uint32_t* sa, const PrimeTab& PA, uint32_t* sb, const PrimeTab& PB, int tid) {
     ntt_pass_lds2<0, false>(sa, PA, sb, PB, tid);
     __syncthreads();
     ntt_pass_lds2<1, false>(sa, PA, sb, PB, tid);
@@ -364,9 +370,19 @@ __device__ __forceinline__ void ntt_fwd_r16x2(uint32_t* sa, const PrimeTab& PA, 
     ntt_pass_lds2<2, false>(sa, PA, sb, PB, tid);
     __syncthreads();
 }
+__device__ __forceinline__ void ntt_fwd_r16x2(uint32_t* sa, const PrimeTab& PA, uint32_t* sb, const PrimeTab& PB) { ntt_fwd_r16x2(sa, PA, sb, PB, (int)threadIdx.x); }
 
-__device__ __forceinline__ void ntt_inv_r16x2(uint32_t* sa, const PrimeTab& PA, uint32_t* sb, const PrimeTab& PB) {
-    const int tid = threadIdx.x;
+// two forward transforms in the same prime
+__device__ __forceinline__ void ntt_fwd_r16x2_same(uint32_t* sa, uint32_t* sb, const PrimeTab& P, int tid) {
+    ntt_pass_lds2<0, false, true>(sa, P, sb, P, tid);
+    __syncthreads();
+    ntt_pass_lds2<1, false, true>(sa, P, sb, P, tid);
+    __syncthreads();
+    ntt_pass_lds2<2, false, true>(sa, P, sb, P, tid);
+    __syncthreads();
+}
+
+__device__ __forceinline__ void ntt_inv_r16x2(uint32_t* sa, const PrimeTab& PA, uint32_t* sb, const PrimeTab& PB, int tid) {
     ntt_pass_lds2<2, true>(sa, PA, sb, PB, tid);
     __syncthreads();
     ntt_pass_lds2<1, true>(sa, PA, sb, PB, tid);
@@ -374,6 +390,7 @@ __device__ __forceinline__ void ntt_inv_r16x2(uint32_t* sa, const PrimeTab& PA, 
     ntt_pass_lds2<0, true>(sa, PA, sb, PB, tid);
     __syncthreads();
 }
+__device__ __forceinline__ void ntt_inv_r16x2(uint32_t* sa, const PrimeTab& PA, uint32_t* sb, const PrimeTab& PB) { ntt_inv_r16x2(sa, PA, sb, PB, (int)threadIdx.x); }
 
 // slot permutation realising the automorphism X -> X^g in the NTT domain:
 // NTT(tau_g(a))[s] = NTT(a)[perm_g(s)]

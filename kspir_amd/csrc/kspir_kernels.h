@@ -667,6 +667,119 @@ __global__ void __launch_bounds__(kNttThreads) k_ext_coef(const uint32_t* __rest
     for (int x = threadIdx.x; x < kN; x += kNttThreads) ecoef[(((size_t)blk * 2 + ab) * 2 + p) * kN + x] = s[lds_pad(x)];
 }
 
+// Batched giant steps and first-dimension sum, one block of one query per
+// workgroup, both primes: k_giant_coef + k_giant_ks + k_ext_coef in one pass
+// that keeps every intermediate in LDS and registers.  For each c >= 1
+// (twosteps.cpp:599-677) the a-row is INTT'd in LDS, tau_g (g = 5^(N1*c)) is
+// read back from LDS, decomposed (ell 2, PP 2^20, Bg 2^18) and NTT'd, and the
+// key-switched row is accumulated in registers; row 0 is added at the end.
+// Writes fd (optional, NTT form) and ecoef (coefficients) as k_ext_coef does.
+// 512 threads: the two 256-thread halves run the transforms of the two
+// primes side by side, and every thread holds 8 slots of the accumulators.
+// One workgroup per (query, block), r*nq of them; when that count is a
+// multiple of 8 the ids are dealt so that the r blocks of a query run on one
+// XCD (the hardware hands consecutive workgroups to the 8 XCDs in turn), where
+// they share the query's giant-step keys in L2.
+constexpr int kFusedThreads = 512;
+__global__ void __launch_bounds__(kFusedThreads) k_giant_fused(const uint32_t* __restrict__ dbout,
+        const KeyPtrs* __restrict__ kp, uint32_t* __restrict__ fd, uint32_t* __restrict__ ecoef, int r, int nq,
+        int n1, int n2, QS S, Tabs T) {
+    __shared__ uint32_t s[4][kPadN];
+    const int total = r * nq, bid = blockIdx.x;
+    const int L = (total & 7) ? bid : (bid & 7) * (total >> 3) + (bid >> 3);
+    const int blk = L % r;
+    const size_t qi = L / r;
+    dbout += qi * S.dbout;
+    if (fd) fd += qi * S.fd;
+    ecoef += qi * S.ecoef;
+    const uint32_t* gkeys = kp[qi].giant;
+    const int tid = threadIdx.x;
+    const int half = __builtin_amdgcn_readfirstlane(tid >> 8), ht = tid & 255;
+    constexpr int PER = kN / kFusedThreads;
+    const uint32_t q1 = T.p[0].q, q2 = T.p[1].q;
+    uint32_t acc[2][2][PER];    // [ab][p][e], slot x = tid + 512 e
+#pragma unroll
+    for (int e = 0; e < PER; ++e) acc[0][0][e] = acc[0][1][e] = acc[1][0][e] = acc[1][1][e] = 0;
+    for (int c = 1; c < n2; ++c) {
+        const uint32_t* row = dbout + ((size_t)blk * n2 + c) * 4 * kN;   // [ab][p][N]
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int x = tid + e * kFusedThreads;
+            s[0][lds_pad(x)] = row[x];
+            s[1][lds_pad(x)] = row[kN + x];
+        }
+        __syncthreads();
+        ntt_inv_r16(s[half], T.p[half], ht);
+        const uint32_t ex = (uint32_t)(n1 * c) % (kN / 2);
+        const uint32_t g = pow5_mod2n(ex);
+        const uint32_t ginv = pow5_mod2n((kN / 2 - ex) % (kN / 2));
+        // digits of tau_g(a): mod q2 straight into planes 2, 3 (not read
+        // here); mod q1 held until the gathers from planes 0, 1 are done
+        uint32_t dg[2][PER];
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int x = tid + e * kFusedThreads;
+            const uint32_t i0 = ((uint32_t)x * ginv) & (2 * kN - 1);
+            const bool neg = i0 >= (uint32_t)kN;
+            const int i = neg ? (int)i0 - kN : (int)i0;
+            uint64_t v = crt_q(s[0][lds_pad(i)], s[1][lds_pad(i)], T);
+            if (neg) v = v ? kQ - v : 0;
+            const int32_t d0 = tfhe_digit<18, 20, 2>(v, 0), d1 = tfhe_digit<18, 20, 2>(v, 1);
+            dg[0][e] = digit_residue(d0, q1);
+            dg[1][e] = digit_residue(d1, q1);
+            s[2][lds_pad(x)] = digit_residue(d0, q2);
+            s[3][lds_pad(x)] = digit_residue(d1, q2);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int x = tid + e * kFusedThreads;
+            s[0][lds_pad(x)] = dg[0][e];
+            s[1][lds_pad(x)] = dg[1][e];
+        }
+        __syncthreads();
+        ntt_fwd_r16x2_same(s[2 * half], s[2 * half + 1], T.p[half], ht);
+        const uint32_t* key = gkeys + (size_t)(c - 1) * 2 * 2 * 2 * kN;   // [j][ab][p][N]
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int x = tid + e * kFusedThreads;
+            const int px = perm_slot(x, g, T);
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const PrimeTab& P = T.p[p];
+                const uint64_t d0 = s[2 * p][lds_pad(x)], d1 = s[2 * p + 1][lds_pad(x)];
+                const uint64_t ta = d0 * key[((0 * 2 + 0) * 2 + p) * kN + x] + d1 * key[((1 * 2 + 0) * 2 + p) * kN + x];
+                const uint64_t tb = d0 * key[((0 * 2 + 1) * 2 + p) * kN + x] + d1 * key[((1 * 2 + 1) * 2 + p) * kN + x];
+                acc[0][p][e] = sub_mod(acc[0][p][e], mont_reduce(ta, P.q, P.mont_qinv), P.q);
+                const uint32_t bb = add_mod(acc[1][p][e], row[(2 + p) * kN + px], P.q);
+                acc[1][p][e] = sub_mod(bb, mont_reduce(tb, P.q, P.mont_qinv), P.q);
+            }
+        }
+        __syncthreads();
+    }
+    const uint32_t* row0 = dbout + (size_t)blk * n2 * 4 * kN;
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int x = tid + e * kFusedThreads;
+#pragma unroll
+        for (int ab = 0; ab < 2; ++ab)
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const uint32_t v = add_mod(acc[ab][p][e], row0[(ab * 2 + p) * kN + x], T.p[p].q);
+                s[ab * 2 + p][lds_pad(x)] = v;
+                if (fd) fd[(((size_t)blk * 2 + ab) * 2 + p) * kN + x] = v;
+            }
+    }
+    __syncthreads();
+    ntt_inv_r16x2(s[2 * half], T.p[0], s[2 * half + 1], T.p[1], ht);    // half = ab
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int x = tid + e * kFusedThreads;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) ecoef[((size_t)blk * 4 + t) * kN + x] = s[t][lds_pad(x)];
+    }
+}
+
 // lwe.cpp:761-808: digit k (0,1 from a; 2,3 from b) of decompose_rlwe
 // (ell 2, PP 2^20, Bg 2^18), NTT, times RGSW row k: grid (r, 4, 2 p)
 // epart: [r][4][2 ab][2 p][N]

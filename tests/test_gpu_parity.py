@@ -196,6 +196,35 @@ def test_batch_mfma_shapes_and_chunks(ctx, oracle, n1, chunk):
     db.close()
 
 
+@pytest.mark.parametrize("r,n1,nq", [(4, 128, 2), (4, 64, 5), (2, 512, 3), (8, 256, 1)])
+def test_fused_giant_steps(ctx, oracle, r, n1, nq):
+    """the fused per-(query, block) giant-step kernel (forced on with
+    fused_min_batch = 1) gives the words of the per-row kernels for every
+    query, and query 0's equal the oracle's; r*nq = 8 takes the XCD-dealt
+    workgroup order, the other shapes the plain one"""
+    say(f"[gpu] fused giant steps r={r} n1={n1} nq={nq}")
+    db_words = oracle.gen_db(r, n1, seed=5)
+    perms = oracle.perm_matrix(n1)
+    clients = [oracle.gen_client(r, n1, 51 + c, (101 * c + 3) % (N // 2), c % r) for c in range(nq)]
+    db = ctx.database(db_words, r, n1)
+    keys = [ctx.keys(r, n1, cl["baby"], cl["giant"], cl["packing"], cl["rgsw"], perms) for cl in clients]
+    queries = np.stack([cl["query"] for cl in clients])
+    plain = ctx.answer_batch(db, keys, queries)
+    ctx.set("fused_min_batch", 1)
+    try:
+        fused = ctx.answer_batch(db, keys, queries)
+    finally:
+        ctx.set("fused_min_batch", 16)
+    assert np.array_equal(fused, plain)
+    want0 = oracle.answer(r=r, n1=n1, db=db_words, query=clients[0]["query"], rgsw=clients[0]["rgsw"],
+                          baby=clients[0]["baby"], giant=clients[0]["giant"], packing=clients[0]["packing"],
+                          perms=perms)
+    assert np.array_equal(fused[0], want0)
+    for k in keys:
+        k.close()
+    db.close()
+
+
 def test_cpp_dropin_matches_oracle(oracle, tmp_path):
     """tests/native/dropin_test calls matrix_vector_mul_bsgs_rns_crt_large,
     externalProduct and packingRLWEs through kspir_pir.h exactly as
