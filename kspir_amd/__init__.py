@@ -38,7 +38,7 @@ def _load():
                  "kspir_memcpy_h2d", "kspir_memcpy_d2h", "kspir_synchronize", "kspir_time_answer",
                  "kspir_time_db_pass", "kspir_test_ntt", "kspir_device_count",
                  "kspir_compute_permutation_matrix", "kspir_answer_batch", "kspir_answer_batch_device",
-                 "kspir_time_answer_batch", "kspir_context_set"):
+                 "kspir_time_answer_batch", "kspir_context_set", "kspir_time_copy"):
         getattr(L, name).restype = ctypes.c_int
     L.kspir_context_destroy.argtypes = [_vp]
     L.kspir_database_destroy.argtypes = [_vp]
@@ -57,6 +57,7 @@ EXPORTS = (
     "kspir_answer_batch", "kspir_answer_batch_device", "kspir_time_answer_batch", "kspir_context_set",
     "kspir_device_count", "kspir_device_alloc", "kspir_device_free", "kspir_memcpy_h2d", "kspir_memcpy_d2h",
     "kspir_synchronize", "kspir_context_stream", "kspir_time_answer", "kspir_time_db_pass", "kspir_test_ntt",
+    "kspir_time_copy",
 )
 
 
@@ -249,6 +250,13 @@ class Context:
         ms = ctypes.c_float(0)
         _check(lib.kspir_time_db_pass(self.h, db.h, keys.h, ctypes.cast(_vp(query_buf.ptr), _u64p), iters,
                                       ctypes.byref(ms)))
+        return ms.value
+
+    def time_copy(self, dst_buf, src_buf, nbytes, iters):
+        """average ms of a plain 16-byte-per-lane copy of nbytes (device buffers)"""
+        ms = ctypes.c_float(0)
+        _check(lib.kspir_time_copy(self.h, _vp(dst_buf.ptr), _vp(src_buf.ptr), ctypes.c_size_t(nbytes), iters,
+                                   ctypes.byref(ms)))
         return ms.value
 
     def test_ntt(self, polys, prime, inverse):

@@ -378,16 +378,16 @@ int enqueue_answer(kspir_context* c, const kspir_database* db, const KeyPtrs* kp
     hipLaunchKernelGGL(k_baby_prep, dim3(16, nq), dim3(kNttThreads), 0, st, q_dev, c->D0, c->bhat0, c->ctw, n1, S, T);
     LAUNCH_OK("k_baby_prep");
     if (n1h - 1 > 0) {
-        hipLaunchKernelGGL(k_baby_rot, dim3(n1h - 1, 2, nq), dim3(kNttThreads), 0, st, c->D0, c->bhat0, kp, c->ctw,
-                           c->mid, n1, 1, n1h - 2, S.D, S.bhat, S, T);
+        hipLaunchKernelGGL(k_baby_rot, dim3((n1h - 1) * 2 * nq), dim3(kNttThreads), 0, st, c->D0, c->bhat0, kp,
+                           c->ctw, c->mid, n1h - 1, nq, 1, n1h - 2, S.D, S.bhat, S, T);
         LAUNCH_OK("k_baby_rot(first)");
     }
     hipLaunchKernelGGL(k_baby_mid_coef, dim3(3, nq), dim3(kNttThreads), 0, st, c->mid, c->icoef, S, T);
     LAUNCH_OK("k_baby_mid_coef");
     hipLaunchKernelGGL(k_baby_mid_digits, dim3(9, nq), dim3(kNttThreads), 0, st, c->icoef, c->D1, S, T);
     LAUNCH_OK("k_baby_mid_digits");
-    hipLaunchKernelGGL(k_baby_rot, dim3(n1h, 2, nq), dim3(kNttThreads), 0, st, c->D1, c->mid + 3 * kN, kp, c->ctw,
-                       (uint32_t*)nullptr, n1, n1h, -1, S.D, S.mid, S, T);
+    hipLaunchKernelGGL(k_baby_rot, dim3(n1h * 2 * nq), dim3(kNttThreads), 0, st, c->D1, c->mid + 3 * kN, kp, c->ctw,
+                       (uint32_t*)nullptr, n1h, nq, n1h, -1, S.D, S.mid, S, T);
     LAUNCH_OK("k_baby_rot(second)");
     // database pass (crt.cpp:643-768): int8-MFMA limb GEMM for a batch,
     // u64-accumulating VALU stream for a single query
@@ -919,6 +919,30 @@ int kspir_time_db_pass(kspir_context* c, const kspir_database* db, const kspir_k
     *ms_per_launch = ms / iters;
     hipEventDestroy(e0);
     hipEventDestroy(e1);
+    return KSPIR_OK;
+}
+
+int kspir_time_copy(kspir_context* c, void* dst, const void* src, size_t bytes, int iters, float* ms_per_launch) {
+    if (!c || !dst || !src || !ms_per_launch || iters < 1 || bytes == 0 || bytes % 16)
+        return fail(KSPIR_EINVAL, "bad copy arguments");
+    HIP_OK(hipSetDevice(c->device));
+    const size_t n = bytes / 16;
+    const int grid = 256 * 8;
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    HIP_OK(hipEventRecord(e0, c->stream));
+    for (int i = 0; i < iters; ++i) {
+        hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, c->stream, (const uint4*)src, (uint4*)dst, n);
+        LAUNCH_OK("k_copy");
+    }
+    HIP_OK(hipEventRecord(e1, c->stream));
+    HIP_OK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+    *ms_per_launch = ms / iters;
+    HIP_OK(hipEventDestroy(e0));
+    HIP_OK(hipEventDestroy(e1));
     return KSPIR_OK;
 }
 

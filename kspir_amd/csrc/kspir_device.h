@@ -392,11 +392,18 @@ __device__ __forceinline__ void ntt_inv_r16x2(uint32_t* sa, const PrimeTab& PA, 
 }
 __device__ __forceinline__ void ntt_inv_r16x2(uint32_t* sa, const PrimeTab& PA, uint32_t* sb, const PrimeTab& PB) { ntt_inv_r16x2(sa, PA, sb, PB, (int)threadIdx.x); }
 
+// 12-bit bit reversal
+__device__ __forceinline__ uint32_t brev12(uint32_t x) { return __builtin_bitreverse32(x) >> (32 - kLogN); }
+
+// NTT slot s holds the evaluation at psi^(2 brev(s) + 1) (the tables
+// T.slot_exp / T.exp_slot hold the same map)
+__device__ __forceinline__ uint32_t slot_exponent(int s) { return 2 * brev12((uint32_t)s) + 1; }
+
 // slot permutation realising the automorphism X -> X^g in the NTT domain:
-// NTT(tau_g(a))[s] = NTT(a)[perm_g(s)]
-__device__ __forceinline__ int perm_slot(int s, uint32_t g, const Tabs& T) {
-    uint32_t e = ((uint32_t)T.slot_exp[s] * g) & (2 * kN - 1);
-    return T.exp_slot[e >> 1];
+// NTT(tau_g(a))[s] = NTT(a)[perm_g(s)], computed in registers
+__device__ __forceinline__ int perm_slot(int s, uint32_t g, const Tabs&) {
+    const uint32_t e = (slot_exponent(s) * g) & (2 * kN - 1);
+    return (int)brev12(e >> 1);
 }
 
 }  // namespace kspir

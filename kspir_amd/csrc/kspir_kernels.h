@@ -112,7 +112,7 @@ __device__ __forceinline__ uint32_t pow5_mod2n(uint32_t e) {
 
 // X^x at NTT slot s, in Montgomery form (psi_pow holds psi^e * 2^32 mod q)
 __device__ __forceinline__ uint32_t xpow_m(int s, uint32_t x, const PrimeTab& P, const Tabs& T) {
-    return P.psi_pow[((uint32_t)T.slot_exp[s] * x) & (2 * kN - 1)];
+    return P.psi_pow[(slot_exponent(s) * x) & (2 * kN - 1)];
 }
 
 __device__ __forceinline__ uint32_t mont_mul(uint32_t a, uint32_t bm, const PrimeTab& P) {
@@ -226,65 +226,84 @@ __global__ void __launch_bounds__(kNttThreads) k_baby_prep(const uint64_t* __res
     }
 }
 
-// One baby-step rotation per workgroup (blockIdx.x = rot, blockIdx.y = a|b).
-// rotation rot applies tau_{5^(rot+1)}: key index rot, permutation row rot+1.
+// workgroup id -> linear job id; when the job count is a multiple of 8 the
+// ids are dealt so that consecutive jobs run on one XCD (the hardware hands
+// consecutive workgroups to the 8 XCDs in turn), where jobs of one query
+// share that query's data in L2
+__device__ __forceinline__ int xcd_job(int bid, int total) {
+    return (total & 7) ? bid : (bid & 7) * (total >> 3) + (bid >> 3);
+}
+
+// One baby-step rotation of one query per workgroup, a or b part.
+// Rotation rot applies tau_{5^(rot+1)}: key index rot, permutation row rot+1.
 //   acc_p = -sum_j D[j][p][perm(s)] * key[rot][j][ab][p][s]       (p = q1,q2,bs)
 //   b-part adds bhat[p][perm(s)]  (tau(b) in the NTT domain)
 //   c = INTT_bs(acc_bs) in [0,bsMod);  out_p = (acc_p - NTT_p(c)) * bsMod^-1
 // out goes to ctw[k_base + rot][ab][p][z].  Rotation `mid` also stores its
 // un-switched NTT-domain accumulators in mid_out[ab][p] (evalAutoSRNSFirst's
 // intermediate).  twosteps.cpp:937-1204.
+// The three digit planes D[0..2][p] of a prime are staged in LDS with
+// coalesced loads and permuted from there.  Jobs: nrot * 2 * nq, rotation
+// fastest, then a|b, then query (one workgroup each, 1-D grid).
 __global__ void __launch_bounds__(kNttThreads) k_baby_rot(const uint32_t* __restrict__ D,
         const uint32_t* __restrict__ bhat, const KeyPtrs* __restrict__ kp,
-        uint32_t* __restrict__ ctw, uint32_t* __restrict__ mid_out, int n1, int k_base, int mid,
+        uint32_t* __restrict__ ctw, uint32_t* __restrict__ mid_out, int nrot, int nq, int k_base, int mid,
         size_t D_stride, size_t bhat_stride, QS S, Tabs T) {
-    __shared__ uint32_t sb[kPadN];
-    __shared__ uint32_t s1[kPadN];
-    __shared__ uint32_t s2[kPadN];
-    const int rot = blockIdx.x, ab = blockIdx.y;
-    const size_t qi = blockIdx.z;
+    __shared__ uint32_t s[3][kPadN];
+    const int L = xcd_job(blockIdx.x, nrot * 2 * nq);
+    const int rot = L % nrot, ab = (L / nrot) & 1;
+    const size_t qi = L / (2 * nrot);
     D += qi * D_stride;
     bhat += qi * bhat_stride;
     ctw += qi * S.ctw;
     if (mid_out) mid_out += qi * S.mid;
     const uint16_t* pm = kp[qi].perms + (size_t)(rot + 1) * kN;
     const uint32_t* key = kp[qi].baby + (size_t)rot * 3 * 2 * 3 * kN;
+    const int tid = threadIdx.x;
     constexpr int PER = kN / kNttThreads;
-    uint32_t acc1[PER], acc2[PER];
+    int ps[PER];
+    uint32_t acc[3][PER];
 #pragma unroll
-    for (int e = 0; e < PER; ++e) {
-        const int x = threadIdx.x + e * kNttThreads;
-        const int ps = pm[x];
-        uint32_t res[3];
+    for (int e = 0; e < PER; ++e) ps[e] = pm[tid + e * kNttThreads];
 #pragma unroll
-        for (int p = 0; p < 3; ++p) {
+    for (int p = 0; p < 3; ++p) {
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int x = tid + e * kNttThreads;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) s[j][lds_pad(x)] = D[(j * 3 + p) * kN + x];
+        }
+        __syncthreads();
+        const uint32_t q = T.p[p].q;
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int x = tid + e * kNttThreads;
             uint64_t t = 0;
 #pragma unroll
             for (int j = 0; j < 3; ++j)
-                t += (uint64_t)D[(j * 3 + p) * kN + ps] * key[((j * 2 + ab) * 3 + p) * kN + x];
-            uint32_t r = neg_mod(mont_reduce(t, T.p[p].q, T.p[p].mont_qinv), T.p[p].q);
-            if (ab) r = add_mod(r, bhat[p * kN + ps], T.p[p].q);
-            res[p] = r;
+                t += (uint64_t)s[j][lds_pad(ps[e])] * key[((j * 2 + ab) * 3 + p) * kN + x];
+            uint32_t r = neg_mod(mont_reduce(t, q, T.p[p].mont_qinv), q);
+            if (ab) r = add_mod(r, bhat[p * kN + ps[e]], q);
+            if (rot == mid) mid_out[(ab * 3 + p) * kN + x] = r;
+            acc[p][e] = r;
         }
-        if (rot == mid) {
-#pragma unroll
-            for (int p = 0; p < 3; ++p) mid_out[(ab * 3 + p) * kN + x] = res[p];
-        }
-        acc1[e] = res[0];
-        acc2[e] = res[1];
-        sb[lds_pad(x)] = res[2];
+        __syncthreads();
     }
+#pragma unroll
+    for (int e = 0; e < PER; ++e) s[0][lds_pad(tid + e * kNttThreads)] = acc[2][e];
     __syncthreads();
-    ntt_inv_r16(sb, T.p[2]);
-    for (int x = threadIdx.x; x < kN; x += kNttThreads) { s1[lds_pad(x)] = sb[lds_pad(x)]; s2[lds_pad(x)] = sb[lds_pad(x)]; }
+    // c in [0, bsMod) is below q1 and q2: the same words are c mod q1, q2
+    ntt_inv_r16(s[0], T.p[2]);
+#pragma unroll
+    for (int e = 0; e < PER; ++e) s[1][lds_pad(tid + e * kNttThreads)] = s[0][lds_pad(tid + e * kNttThreads)];
     __syncthreads();
-    ntt_fwd_r16x2(s1, T.p[0], s2, T.p[1]);
+    ntt_fwd_r16x2(s[0], T.p[0], s[1], T.p[1]);
     const int k = k_base + rot;
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
-        const int x = threadIdx.x + e * kNttThreads;
-        uint32_t o1 = mul_shoup(sub_mod(acc1[e], s1[lds_pad(x)], kQ1), T.bsinv_q[0], T.bsinv_q_sh[0], kQ1);
-        uint32_t o2 = mul_shoup(sub_mod(acc2[e], s2[lds_pad(x)], kQ2), T.bsinv_q[1], T.bsinv_q_sh[1], kQ2);
+        const int x = tid + e * kNttThreads;
+        uint32_t o1 = mul_shoup(sub_mod(acc[0][e], s[0][lds_pad(x)], kQ1), T.bsinv_q[0], T.bsinv_q_sh[0], kQ1);
+        uint32_t o2 = mul_shoup(sub_mod(acc[1][e], s[1][lds_pad(x)], kQ2), T.bsinv_q[1], T.bsinv_q_sh[1], kQ2);
         ctw[((size_t)(k * 2 + ab) * 2 + 0) * kN + x] = o1;
         ctw[((size_t)(k * 2 + ab) * 2 + 1) * kN + x] = o2;
     }
@@ -684,9 +703,8 @@ constexpr int kFusedThreads = 512;
 __global__ void __launch_bounds__(kFusedThreads) k_giant_fused(const uint32_t* __restrict__ dbout,
         const KeyPtrs* __restrict__ kp, uint32_t* __restrict__ fd, uint32_t* __restrict__ ecoef, int r, int nq,
         int n1, int n2, QS S, Tabs T) {
-    __shared__ uint32_t s[4][kPadN];
-    const int total = r * nq, bid = blockIdx.x;
-    const int L = (total & 7) ? bid : (bid & 7) * (total >> 3) + (bid >> 3);
+    __shared__ uint32_t s[6][kPadN];     // 0..3 transforms, 4..5 the b-row of step c
+    const int L = xcd_job(blockIdx.x, r * nq);
     const int blk = L % r;
     const size_t qi = L / r;
     dbout += qi * S.dbout;
@@ -707,6 +725,8 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused(const uint32_t* _
             const int x = tid + e * kFusedThreads;
             s[0][lds_pad(x)] = row[x];
             s[1][lds_pad(x)] = row[kN + x];
+            s[4][lds_pad(x)] = row[2 * kN + x];
+            s[5][lds_pad(x)] = row[3 * kN + x];
         }
         __syncthreads();
         ntt_inv_r16(s[half], T.p[half], ht);
@@ -751,7 +771,7 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused(const uint32_t* _
                 const uint64_t ta = d0 * key[((0 * 2 + 0) * 2 + p) * kN + x] + d1 * key[((1 * 2 + 0) * 2 + p) * kN + x];
                 const uint64_t tb = d0 * key[((0 * 2 + 1) * 2 + p) * kN + x] + d1 * key[((1 * 2 + 1) * 2 + p) * kN + x];
                 acc[0][p][e] = sub_mod(acc[0][p][e], mont_reduce(ta, P.q, P.mont_qinv), P.q);
-                const uint32_t bb = add_mod(acc[1][p][e], row[(2 + p) * kN + px], P.q);
+                const uint32_t bb = add_mod(acc[1][p][e], s[4 + p][lds_pad(px)], P.q);
                 acc[1][p][e] = sub_mod(bb, mont_reduce(tb, P.q, P.mont_qinv), P.q);
             }
         }
@@ -986,6 +1006,12 @@ __global__ void __launch_bounds__(256) k_add_planes(uint32_t* __restrict__ a, co
         const int p = (int)((idx / kN) & 1);
         a[idx] = add_mod(a[idx], b[idx], T.p[p].q);
     }
+}
+
+// plain copy, 16 bytes per lane per step (reference kernel for HBM counters)
+__global__ void __launch_bounds__(256) k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
 }
 
 // test hooks: batched forward / inverse NTT of polynomials in prime p
