@@ -143,6 +143,15 @@ def main():
     db_bytes = 8 * r * N * (N // 2)
     achieved = db_bytes / (db_ms * 1e-3) / 1e9
     plain_qps = 1000.0 / plain_ms
+    # what a plain 16-byte copy kernel reaches on this GPU (read + write bytes
+    # over time); its HBM counters also calibrate the traffic figure
+    log("timing a 256 MiB device copy")
+    cbytes = 256 << 20
+    csrc, cdst = ctx.alloc(cbytes), ctx.alloc(cbytes)
+    copy_ms = ctx.time_copy(cdst, csrc, cbytes, 10)
+    copy_gbps = 2 * cbytes / (copy_ms * 1e-3) / 1e9
+    csrc.free()
+    cdst.free()
 
     out = {
         "metric": "PIR queries/sec + DB GB/s at 256 MB, batched vs plain 1-query; p50/p99 latency",
@@ -177,7 +186,8 @@ def main():
         "roofline": {"kernel": "k_db_product (single-query database pass)", "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic_from_profile(),
-                     "algorithmic_bytes": db_bytes, "avg_launch_ms": round(db_ms, 4)},
+                     "algorithmic_bytes": db_bytes, "avg_launch_ms": round(db_ms, 4),
+                     "copy_kernel_GBps": round(copy_gbps, 1)},
     }
     for k in keys:
         k.close()

@@ -53,6 +53,13 @@ int kspir_time_answer_batch(kspir_context* ctx, const kspir_database* db, const 
 int kspir_time_db_pass(kspir_context* ctx, const kspir_database* db, const kspir_keys* keys,
                        const uint64_t* query_dev, int iters, float* ms_per_launch);
 
+/* Plain device-to-device copy kernel (16-byte loads and stores, `bytes` a
+ * multiple of 16) timed with HIP events: *ms_per_launch = average
+ * milliseconds.  It is the reference kernel for HBM counters (FETCH_SIZE /
+ * WRITE_SIZE of a copy of known size) and for achievable copy bandwidth. */
+int kspir_time_copy(kspir_context* ctx, void* dst_dev, const void* src_dev, size_t bytes, int iters,
+                    float* ms_per_launch);
+
 /* Test hook: `count` polynomials of N u32 residues in prime p (0: q1,
  * 1: q2, 2: bsMod), forward (inverse != 0: inverse) NTT in place, host memory. */
 int kspir_test_ntt(kspir_context* ctx, uint32_t* polys, int count, int prime, int inverse);
