@@ -150,7 +150,7 @@ int build_tables(kspir_context* c) {
         }
         std::vector<uint32_t> pairs(4 * kN);
         for (int i = 0; i < kN; ++i) {
-            pairs[2 * i] = fw[i]; pairs[2 * i + 1] = fwsh[i];
+            pairs[2 * i] = 0u - fw[i]; pairs[2 * i + 1] = fwsh[i];   // forward: 2^32 - w (see ntt_fwd_pass)
             pairs[2 * kN + 2 * i] = iv[i]; pairs[2 * kN + 2 * i + 1] = ivsh[i];
         }
         uint32_t* d = nullptr;

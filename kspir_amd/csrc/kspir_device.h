@@ -39,7 +39,7 @@ struct PrimeTab {
     const uint32_t* iv;    // iv[bitrev(i)] = psi^-i
     const uint32_t* iv_sh;
     const uint32_t* psi_pow;   // psi^e, e in [0, 2N)
-    const uint2* fw2;          // (fw[i], fw_sh[i]) pairs
+    const uint2* fw2;          // (2^32 - fw[i], fw_sh[i]) pairs (forward butterflies use -w)
     const uint2* iv2;          // (iv[i], iv_sh[i]) pairs
 };
 
@@ -173,7 +173,7 @@ __host__ __device__ __forceinline__ uint32_t mul_shoup_hd(uint32_t x, uint32_t w
     return r >= q ? r - q : r;
 }
 
-// Twiddles of pass P for thread tid: stage s (s = 0..3) of the pass uses the
+// Twiddles of pass P for thread tid (forward tables hold 2^32 - w): stage s (s = 0..3) of the pass uses the
 // 2^s consecutive table entries starting at 2^(4P+s) + base * 2^s, with
 // base = 0, tid >> 4, tid for P = 0, 1, 2, i.e. the entry for register k is
 // table[2^(4P+s) + base * 2^s + (k >> (4 - s))].  tw2 interleaves (w, w')
@@ -209,10 +209,33 @@ __host__ __device__ __forceinline__ void load_pass_twiddles(const uint2* tw2, in
 
 __host__ __device__ __forceinline__ uint32_t umin_hd(uint32_t a, uint32_t b) { return a < b ? a : b; }
 
+// 64-bit a * b and a * b + c.  On the GPU each is one v_mad_u64_u32 (full
+// rate on gfx950); only the low words are used by the callers, so the high
+// word of c is whatever the previous product left there (no zeroing moves).
+__host__ __device__ __forceinline__ uint64_t mul_wide(uint32_t a, uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint64_t r;
+    asm("v_mad_u64_u32 %0, vcc, %1, %2, 0" : "=v"(r) : "v"(a), "v"(b) : "vcc");
+    return r;
+#else
+    return (uint64_t)a * b;
+#endif
+}
+__host__ __device__ __forceinline__ uint64_t mad_wide(uint32_t a, uint32_t b, uint64_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint64_t r;
+    asm("v_mad_u64_u32 %0, vcc, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c) : "vcc");
+    return r;
+#else
+    return (uint64_t)a * b + c;
+#endif
+}
+
 // x * w mod q up to one extra q: result in [0, 2q) for any x < 2^32 (Shoup,
-// without the final correction), w < q, wp = floor(w * 2^32 / q)
+// without the final correction), w < q, wp = floor(w * 2^32 / q):
+// x*w - floor(x*wp / 2^32)*q, formed as mulhi * (2^32 - q) + x*w
 __host__ __device__ __forceinline__ uint32_t mul_shoup_lazy(uint32_t x, uint32_t w, uint32_t wp, uint32_t q) {
-    return x * w - mulhi32_hd(x, wp) * q;
+    return (uint32_t)mad_wide(mulhi32_hd(x, wp), 0u - q, mul_wide(x, w));
 }
 
 // Butterflies with lazy reduction (Harvey, "Faster arithmetic for
@@ -231,10 +254,12 @@ __host__ __device__ __forceinline__ void ntt_fwd_pass(uint32_t v[16], const uint
             if (k & kb) continue;
             const int ti = (1 << st) - 1 + (k >> (4 - st));
             const uint32_t u = umin_hd(v[k], v[k] - q2);                    // [0, 2q)
-            // nt = -t mod 2^32 for t = w * v[k|kb] mod q in [0, 2q), so the
-            // two outputs are one subtract and one three-way add
+            // nt = -t mod 2^32 for t = w * v[k|kb] mod q in [0, 2q) (Shoup);
+            // w[] holds 2^32 - w, so nt = mulhi * q + x * (2^32 - w) is one
+            // multiply-add, and the two outputs are one subtract and one
+            // three-way add
             const uint32_t x = v[k | kb];
-            const uint32_t nt = mulhi32_hd(x, ws[ti]) * q - x * w[ti];
+            const uint32_t nt = (uint32_t)mad_wide(mulhi32_hd(x, ws[ti]), q, mul_wide(x, w[ti]));
             v[k] = u - nt;                 // u + t
             v[k | kb] = u + nt + q2;       // u - t + 2q
         }

@@ -698,7 +698,9 @@ __global__ void __launch_bounds__(kNttThreads) k_ext_coef(const uint32_t* __rest
 // One workgroup per (query, block), r*nq of them; when that count is a
 // multiple of 8 the ids are dealt so that the r blocks of a query run on one
 // XCD (the hardware hands consecutive workgroups to the 8 XCDs in turn), where
-// they share the query's giant-step keys in L2.
+// they share the query's giant-step keys in L2.  Scheduling barriers after
+// each phase keep the compiler from hoisting one phase's loads into the
+// previous one, which would raise register pressure past the 256-VGPR budget.
 constexpr int kFusedThreads = 512;
 __global__ void __launch_bounds__(kFusedThreads) k_giant_fused(const uint32_t* __restrict__ dbout,
         const KeyPtrs* __restrict__ kp, uint32_t* __restrict__ fd, uint32_t* __restrict__ ecoef, int r, int nq,
@@ -729,7 +731,9 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused(const uint32_t* _
             s[5][lds_pad(x)] = row[3 * kN + x];
         }
         __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
         ntt_inv_r16(s[half], T.p[half], ht);
+        __builtin_amdgcn_sched_barrier(0);
         const uint32_t ex = (uint32_t)(n1 * c) % (kN / 2);
         const uint32_t g = pow5_mod2n(ex);
         const uint32_t ginv = pow5_mod2n((kN / 2 - ex) % (kN / 2));
@@ -751,6 +755,7 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused(const uint32_t* _
             s[3][lds_pad(x)] = digit_residue(d1, q2);
         }
         __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int e = 0; e < PER; ++e) {
             const int x = tid + e * kFusedThreads;
@@ -758,7 +763,9 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused(const uint32_t* _
             s[1][lds_pad(x)] = dg[1][e];
         }
         __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
         ntt_fwd_r16x2_same(s[2 * half], s[2 * half + 1], T.p[half], ht);
+        __builtin_amdgcn_sched_barrier(0);
         const uint32_t* key = gkeys + (size_t)(c - 1) * 2 * 2 * 2 * kN;   // [j][ab][p][N]
 #pragma unroll
         for (int e = 0; e < PER; ++e) {
@@ -776,6 +783,7 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused(const uint32_t* _
             }
         }
         __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
     }
     const uint32_t* row0 = dbout + (size_t)blk * n2 * 4 * kN;
 #pragma unroll
@@ -791,6 +799,7 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused(const uint32_t* _
             }
     }
     __syncthreads();
+    __builtin_amdgcn_sched_barrier(0);
     ntt_inv_r16x2(s[2 * half], T.p[0], s[2 * half + 1], T.p[1], ht);    // half = ab
 #pragma unroll
     for (int e = 0; e < PER; ++e) {

@@ -32,7 +32,7 @@ extern "C" int emul_ntt(uint32_t* poly, uint32_t q, uint32_t root, int inverse) 
     }
     const uint32_t ninv = (uint32_t)powmod(kN, q - 2, q), ninvsh = (uint32_t)(((uint64_t)ninv << 32) / q);
     std::vector<uint2> fw2(kN), iv2(kN);
-    for (int i = 0; i < kN; ++i) { fw2[i] = make_uint2(fw[i], fwsh[i]); iv2[i] = make_uint2(iv[i], ivsh[i]); }
+    for (int i = 0; i < kN; ++i) { fw2[i] = make_uint2(0u - fw[i], fwsh[i]); iv2[i] = make_uint2(iv[i], ivsh[i]); }
     uint32_t w[15], ws[15];
     std::vector<uint32_t> s(kPadN, 0);
     for (int e = 0; e < kN; ++e) s[lds_pad(e)] = poly[e];
