@@ -257,8 +257,9 @@ int launch_db_product(kspir_context* c, const kspir_database* db, int nq, hipStr
     const dim3 dg(kN / 64, r, nq);
     const QS& S = c->S;
     const Tabs& T = c->T;
-    // rows per wave: N2 / 4 rounded up (4 waves per workgroup)
+    // rows per wave: N2 / 4 (N2 a power of two), 1 for N2 < 4 (4 waves per workgroup)
     const int rpw = n2 <= 4 ? 1 : n2 / 4;
+    if (n2 > 4 && n2 % 4) return fail(KSPIR_EINVAL, "N2 must be a power of two");
     switch (rpw) {
         case 1: hipLaunchKernelGGL(k_db_product<1>, dg, dim3(256), 0, st, db->words, c->ctw, c->dbout, n1, n2, bw, obw, S, T); break;
         case 2: hipLaunchKernelGGL(k_db_product<2>, dg, dim3(256), 0, st, db->words, c->ctw, c->dbout, n1, n2, bw, obw, S, T); break;

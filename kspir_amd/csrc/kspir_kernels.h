@@ -361,9 +361,13 @@ __global__ void __launch_bounds__(256) k_db_product(const uint64_t* __restrict__
     const uint32_t* ct = ctw + qi * S.ctw + z;
     const uint64_t* dbb = db + blk * block_words + z;
     uint32_t* ob = out + qi * S.dbout + blk * out_block_words + z;
+    // RPW = N2 / 4 for N2 >= 4, so every row of the wave exists; for N2 < 4
+    // (RPW = 1) the waves past the last row have nothing to do
+    if (wv >= n2) return;
     uint64_t a1[RPW], a2[RPW], b1[RPW], b2[RPW];
 #pragma unroll
     for (int i = 0; i < RPW; ++i) a1[i] = a2[i] = b1[i] = b2[i] = 0;
+    // the row loads of one k carry no condition, so they issue together
 #pragma unroll 2
     for (int k = 0; k < n1; ++k) {
         const uint32_t* c = ct + (size_t)k * 4 * kN;
@@ -371,26 +375,22 @@ __global__ void __launch_bounds__(256) k_db_product(const uint64_t* __restrict__
 #pragma unroll
         for (int i = 0; i < RPW; ++i) {
             const int row = wv + 4 * i;
-            if (row < n2) {
-                const uint64_t d = __builtin_nontemporal_load(dbb + ((size_t)row * n1 + k) * kN);
-                const uint32_t d1 = (uint32_t)d, d2 = (uint32_t)(d >> 32);
-                a1[i] += (uint64_t)d1 * ca1;
-                a2[i] += (uint64_t)d2 * ca2;
-                b1[i] += (uint64_t)d1 * cb1;
-                b2[i] += (uint64_t)d2 * cb2;
-            }
+            const uint64_t d = __builtin_nontemporal_load(dbb + ((size_t)row * n1 + k) * kN);
+            const uint32_t d1 = (uint32_t)d, d2 = (uint32_t)(d >> 32);
+            a1[i] += (uint64_t)d1 * ca1;
+            a2[i] += (uint64_t)d2 * ca2;
+            b1[i] += (uint64_t)d1 * cb1;
+            b2[i] += (uint64_t)d2 * cb2;
         }
     }
 #pragma unroll
     for (int i = 0; i < RPW; ++i) {
         const int row = wv + 4 * i;
-        if (row < n2) {
-            uint32_t* o = ob + (size_t)row * 4 * kN;
-            o[0 * kN] = (uint32_t)(a1[i] % kQ1);
-            o[1 * kN] = (uint32_t)(a2[i] % kQ2);
-            o[2 * kN] = (uint32_t)(b1[i] % kQ1);
-            o[3 * kN] = (uint32_t)(b2[i] % kQ2);
-        }
+        uint32_t* o = ob + (size_t)row * 4 * kN;
+        o[0 * kN] = (uint32_t)(a1[i] % kQ1);
+        o[1 * kN] = (uint32_t)(a2[i] % kQ2);
+        o[2 * kN] = (uint32_t)(b1[i] % kQ1);
+        o[3 * kN] = (uint32_t)(b2[i] % kQ2);
     }
 }
 
