@@ -19,6 +19,8 @@ namespace kspir {
 
 constexpr int kN = 4096;       // ring dimension (params.h:16)
 constexpr int kLogN = 12;
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 constexpr int kNttThreads = 256;   // threads per workgroup of every NTT kernel
 
 // the three primes of the residue system (crt.h:16-17, :32)

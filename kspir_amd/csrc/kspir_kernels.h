@@ -261,12 +261,33 @@ __global__ void __launch_bounds__(kNttThreads) k_baby_rot(const uint32_t* __rest
     const uint32_t* key = kp[qi].baby + (size_t)rot * 3 * 2 * 3 * kN;
     const int tid = threadIdx.x;
     constexpr int PER = kN / kNttThreads;
+    // a thread owns 4 consecutive slots in each 1024-slot group g (e = 4g + i),
+    // so keys, permutation rows and outputs move as 16-byte (8-byte) vectors
+    const int x0 = 4 * tid;
     int ps[PER];
     uint32_t acc[3][PER];
 #pragma unroll
-    for (int e = 0; e < PER; ++e) ps[e] = pm[tid + e * kNttThreads];
+    for (int g = 0; g < 4; ++g) {
+        const uint2 v = *reinterpret_cast<const uint2*>(pm + g * 1024 + x0);
+        ps[4 * g + 0] = (int)(v.x & 0xFFFFu);
+        ps[4 * g + 1] = (int)(v.x >> 16);
+        ps[4 * g + 2] = (int)(v.y & 0xFFFFu);
+        ps[4 * g + 3] = (int)(v.y >> 16);
+    }
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
+        // this prime's key words first, so they are in flight with the staging
+        u32x4 kv[3][4];     // read once: streamed past the caches that hold D
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+                kv[j][g] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(key + ((j * 2 + ab) * 3 + p) * kN + g * 1024 + x0));
+        uint32_t bh[PER];
+        if (ab) {
+#pragma unroll
+            for (int e = 0; e < PER; ++e) bh[e] = bhat[p * kN + ps[e]];
+        }
 #pragma unroll
         for (int e = 0; e < PER; ++e) {
             const int x = tid + e * kNttThreads;
@@ -277,20 +298,22 @@ __global__ void __launch_bounds__(kNttThreads) k_baby_rot(const uint32_t* __rest
         const uint32_t q = T.p[p].q;
 #pragma unroll
         for (int e = 0; e < PER; ++e) {
-            const int x = tid + e * kNttThreads;
+            const int g = e >> 2, i = e & 3;
             uint64_t t = 0;
 #pragma unroll
-            for (int j = 0; j < 3; ++j)
-                t += (uint64_t)s[j][lds_pad(ps[e])] * key[((j * 2 + ab) * 3 + p) * kN + x];
+            for (int j = 0; j < 3; ++j) {
+                const uint32_t kw = kv[j][g][i];
+                t += (uint64_t)s[j][lds_pad(ps[e])] * kw;
+            }
             uint32_t r = neg_mod(mont_reduce(t, q, T.p[p].mont_qinv), q);
-            if (ab) r = add_mod(r, bhat[p * kN + ps[e]], q);
-            if (rot == mid) mid_out[(ab * 3 + p) * kN + x] = r;
+            if (ab) r = add_mod(r, bh[e], q);
+            if (rot == mid) mid_out[(ab * 3 + p) * kN + g * 1024 + x0 + i] = r;
             acc[p][e] = r;
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int e = 0; e < PER; ++e) s[0][lds_pad(tid + e * kNttThreads)] = acc[2][e];
+    for (int e = 0; e < PER; ++e) s[0][lds_pad((e >> 2) * 1024 + x0 + (e & 3))] = acc[2][e];
     __syncthreads();
     // c in [0, bsMod) is below q1 and q2: the same words are c mod q1, q2
     ntt_inv_r16(s[0], T.p[2]);
@@ -300,12 +323,16 @@ __global__ void __launch_bounds__(kNttThreads) k_baby_rot(const uint32_t* __rest
     ntt_fwd_r16x2(s[0], T.p[0], s[1], T.p[1]);
     const int k = k_base + rot;
 #pragma unroll
-    for (int e = 0; e < PER; ++e) {
-        const int x = tid + e * kNttThreads;
-        uint32_t o1 = mul_shoup(sub_mod(acc[0][e], s[0][lds_pad(x)], kQ1), T.bsinv_q[0], T.bsinv_q_sh[0], kQ1);
-        uint32_t o2 = mul_shoup(sub_mod(acc[1][e], s[1][lds_pad(x)], kQ2), T.bsinv_q[1], T.bsinv_q_sh[1], kQ2);
-        ctw[((size_t)(k * 2 + ab) * 2 + 0) * kN + x] = o1;
-        ctw[((size_t)(k * 2 + ab) * 2 + 1) * kN + x] = o2;
+    for (int g = 0; g < 4; ++g) {
+        uint32_t o1[4], o2[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int e = 4 * g + i, x = g * 1024 + x0 + i;
+            o1[i] = mul_shoup(sub_mod(acc[0][e], s[0][lds_pad(x)], kQ1), T.bsinv_q[0], T.bsinv_q_sh[0], kQ1);
+            o2[i] = mul_shoup(sub_mod(acc[1][e], s[1][lds_pad(x)], kQ2), T.bsinv_q[1], T.bsinv_q_sh[1], kQ2);
+        }
+        *reinterpret_cast<uint4*>(ctw + ((size_t)(k * 2 + ab) * 2 + 0) * kN + g * 1024 + x0) = make_uint4(o1[0], o1[1], o1[2], o1[3]);
+        *reinterpret_cast<uint4*>(ctw + ((size_t)(k * 2 + ab) * 2 + 1) * kN + g * 1024 + x0) = make_uint4(o2[0], o2[1], o2[2], o2[3]);
     }
 }
 
