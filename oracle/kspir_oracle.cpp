@@ -61,7 +61,7 @@ constexpr u128 B_INV_PA = (u128)163640210ULL * Q2;
 constexpr u128 PA_INV_B = (u128)97389680ULL * Q1;
 
 int g_threads = 1;
-double g_stage_us[5] = {0, 0, 0, 0, 0};
+thread_local double g_stage_us[5] = {0, 0, 0, 0, 0};   // per calling thread
 
 // ---------------------------------------------------------------- arithmetic
 
@@ -1266,6 +1266,27 @@ int ko_answer(int r, int n1, const uint64_t* db, const uint64_t* query, const ui
     std::memcpy(response, res.a.data(), N * 8);
     std::memcpy(response + N, res.b.data(), N * 8);
     g_stage_us[4] = now_us() - t3;
+    return 0;
+}
+
+// `count` independent online answers of the same inputs on `threads` OpenMP
+// threads: the all-cores CPU baseline of a batch.  Returns 0 when every
+// response equals the first (written to `response`), -1 on a shape error,
+// -2 on a mismatch.
+int ko_answer_parallel(int count, int threads, int r, int n1, const uint64_t* db, const uint64_t* query,
+                       const uint64_t* rgsw, const uint64_t* baby, const uint64_t* giant,
+                       const uint64_t* packing, const int32_t* perms, uint64_t* response) {
+    if (count < 1 || threads < 1 || !valid_shape(r, n1) || n1 < 4) return -1;
+    std::vector<std::vector<u64>> outs(count, std::vector<u64>(2 * N));
+    int bad = 0;
+#pragma omp parallel for num_threads(threads) schedule(dynamic) reduction(+ : bad)
+    for (int i = 0; i < count; ++i)
+        bad += ko_answer(r, n1, db, query, rgsw, baby, giant, packing, perms, outs[i].data(), nullptr, nullptr,
+                         nullptr) != 0;
+    if (bad) return -1;
+    for (int i = 1; i < count; ++i)
+        if (outs[i] != outs[0]) return -2;
+    std::memcpy(response, outs[0].data(), 2 * N * 8);
     return 0;
 }
 

@@ -73,6 +73,9 @@ int ko_answer(int r, int n1, const uint64_t* db, const uint64_t* query, const ui
               const uint64_t* baby, const uint64_t* giant, const uint64_t* packing,
               const int32_t* perms, uint64_t* response, uint64_t* cipherbuf,
               uint64_t* firstdim, uint64_t* ext);
+int ko_answer_parallel(int count, int threads, int r, int n1, const uint64_t* db, const uint64_t* query,
+                       const uint64_t* rgsw, const uint64_t* baby, const uint64_t* giant,
+                       const uint64_t* packing, const int32_t* perms, uint64_t* response);
 /* timing split of the last ko_answer call in microseconds:
    [0] baby steps [1] DB pass [2] giant steps [3] external products [4] packing */
 void ko_last_stage_us(double* out5);

@@ -160,6 +160,16 @@ def answer(inst=None, *, r=None, n1=None, db=None, query=None, rgsw=None, baby=N
     return resp
 
 
+def answer_parallel(inst, count, threads):
+    """`count` independent answers of inst's query on `threads` threads
+    (the all-cores CPU baseline); returns the response of the first"""
+    resp = np.zeros((2, N), dtype=np.uint64)
+    rc = lib().ko_answer_parallel(count, threads, inst.r, inst.n1, p64(inst.db), p64(inst.query), p64(inst.rgsw),
+                                  p64(inst.baby), p64(inst.giant), p64(inst.packing), p32(inst.perms), p64(resp))
+    assert rc == 0, rc
+    return resp
+
+
 def last_stage_us():
     out = (ctypes.c_double * 5)()
     lib().ko_last_stage_us(out)
