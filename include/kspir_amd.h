@@ -66,6 +66,27 @@ int kspir_database_create(kspir_context* ctx, const uint64_t* datacrt, int32_t r
                           int on_device, kspir_database** out);
 void kspir_database_destroy(kspir_database* db);
 
+/* Server preprocessing on the GPU: database_tobsgsntt (twosteps.h:22,
+ * twosteps.cpp:42-179) followed by database_tocrt (crt.h:133,
+ * crt.cpp:859-885), straight into a resident database handle.  records: r
+ * blocks in the layout sample_database_bsgs fills (twosteps.cpp:23-39), N
+ * rows of N/2 values < p per block; block b starts at records + b *
+ * block_stride (block_stride = N*N/2, or 0 to repeat one block).  Host
+ * pointer, or device pointer when on_device != 0.  The words equal those of
+ * database_tobsgsntt + database_tocrt on the CPU. */
+int kspir_database_create_from_records(kspir_context* ctx, const uint64_t* records, int64_t block_stride,
+                                       int32_t r, int32_t n1, int on_device, kspir_database** out);
+
+/* database_tobsgsntt (twosteps.cpp:42-179) for one block on the GPU: data
+ * [N][N/2] record values < p (host), result [N/2][N] values mod Q in NTT
+ * form (host), as the reference returns them. */
+int kspir_database_tobsgsntt(kspir_context* ctx, const uint64_t* data, int32_t n1, uint64_t* result);
+
+/* Copy a resident database back to the host in the reference's datacrt
+ * layout (crt.cpp:859-885: per block [N][N2][N1] words): the persisted form
+ * that kspir_database_create loads again. */
+int kspir_database_export(const kspir_database* db, uint64_t* datacrt);
+
 /* Per-client evaluation keys, resident in HBM.  Replaces the autoKey and
  * permutations arguments of matrix_vector_mul_bsgs_rns_crt_large
  * (twosteps.cpp:1527), the query argument of externalProduct (lwe.cpp:761)

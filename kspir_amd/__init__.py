@@ -38,7 +38,8 @@ def _load():
                  "kspir_memcpy_h2d", "kspir_memcpy_d2h", "kspir_synchronize", "kspir_time_answer",
                  "kspir_time_db_pass", "kspir_test_ntt", "kspir_device_count",
                  "kspir_compute_permutation_matrix", "kspir_answer_batch", "kspir_answer_batch_device",
-                 "kspir_time_answer_batch", "kspir_context_set", "kspir_time_copy"):
+                 "kspir_time_answer_batch", "kspir_context_set", "kspir_time_copy",
+                 "kspir_database_create_from_records", "kspir_database_tobsgsntt", "kspir_database_export"):
         getattr(L, name).restype = ctypes.c_int
     L.kspir_context_destroy.argtypes = [_vp]
     L.kspir_database_destroy.argtypes = [_vp]
@@ -57,7 +58,7 @@ EXPORTS = (
     "kspir_answer_batch", "kspir_answer_batch_device", "kspir_time_answer_batch", "kspir_context_set",
     "kspir_device_count", "kspir_device_alloc", "kspir_device_free", "kspir_memcpy_h2d", "kspir_memcpy_d2h",
     "kspir_synchronize", "kspir_context_stream", "kspir_time_answer", "kspir_time_db_pass", "kspir_test_ntt",
-    "kspir_time_copy",
+    "kspir_time_copy", "kspir_database_create_from_records", "kspir_database_tobsgsntt", "kspir_database_export",
 )
 
 
@@ -117,12 +118,23 @@ class DeviceBuffer:
 class Database:
     """Preprocessed database resident in HBM (database_tocrt words, r blocks)."""
 
-    def __init__(self, ctx, datacrt, r, n1, on_device=False):
+    def __init__(self, ctx, datacrt, r, n1, on_device=False, records=False, block_stride=None):
         self.ctx, self.r, self.n1 = ctx, r, n1
         h = _vp()
         ptr = ctypes.cast(_vp(datacrt), _u64p) if on_device else _p64(datacrt)
-        _check(lib.kspir_database_create(ctx.h, ptr, r, n1, int(on_device), ctypes.byref(h)))
+        if records:
+            stride = N * N // 2 if block_stride is None else block_stride
+            _check(lib.kspir_database_create_from_records(ctx.h, ptr, ctypes.c_int64(stride), r, n1, int(on_device),
+                                                          ctypes.byref(h)))
+        else:
+            _check(lib.kspir_database_create(ctx.h, ptr, r, n1, int(on_device), ctypes.byref(h)))
         self.h = h
+
+    def export(self):
+        """the words in the reference's datacrt layout, [r][N*N/2]"""
+        out = np.zeros((self.r, N * N // 2), dtype=np.uint64)
+        _check(lib.kspir_database_export(self.h, _p64(out)))
+        return out
 
     def close(self):
         if self.h:
@@ -174,6 +186,21 @@ class Context:
 
     def database(self, datacrt, r, n1, on_device=False):
         return Database(self, datacrt, r, n1, on_device)
+
+    def database_from_records(self, records, r, n1, block_stride=None, on_device=False):
+        """preprocess records ([r][N][N/2] values < p, the sample_database_bsgs
+        layout) on the GPU into a resident database; block_stride 0 repeats
+        the first block"""
+        if not on_device:
+            records = np.ascontiguousarray(records, dtype=np.uint64)
+        return Database(self, records, r, n1, on_device, records=True, block_stride=block_stride)
+
+    def database_tobsgsntt(self, data, n1):
+        """database_tobsgsntt of one block on the GPU: [N][N/2] -> [N/2][N] mod Q"""
+        out = np.zeros((N // 2, N), dtype=np.uint64)
+        _check(lib.kspir_database_tobsgsntt(self.h, _p64(np.ascontiguousarray(data, dtype=np.uint64)), n1,
+                                            _p64(out)))
+        return out
 
     def keys(self, r, n1, baby, giant, packing, rgsw, perms):
         return Keys(self, r, n1, baby, giant, packing, rgsw, perms)

@@ -256,6 +256,31 @@ void packingRLWEs(RlweCiphertext& result, std::vector<RlweCiphertext>& rlwes, co
     result.setIsNtt(false);
 }
 
+void database_tobsgsntt(std::vector<std::vector<uint64_t> >& result, std::vector<std::vector<uint64_t> >& data,
+                        uint64_t modulus, int32_t N1) {
+    if (modulus != 65537ULL) throw std::runtime_error("database_tobsgsntt: records must be mod p = 65537");
+    if (data.size() != (size_t)N || result.size() != (size_t)N / 2)
+        throw std::runtime_error("database_tobsgsntt: data must be N x N/2, result N/2 x N");
+    std::vector<uint64_t> flat((size_t)N * (N / 2)), out((size_t)(N / 2) * N);
+    for (int i = 0; i < N; ++i) {
+        if (data[i].size() < (size_t)N / 2) throw std::runtime_error("database_tobsgsntt: short data row");
+        std::memcpy(&flat[(size_t)i * (N / 2)], data[i].data(), (N / 2) * 8);
+    }
+    std::lock_guard<std::mutex> lock(g_mu);
+    check(kspir_database_tobsgsntt(ctx(), flat.data(), N1, out.data()), "database_tobsgsntt");
+    for (int i = 0; i < N / 2; ++i) result[i].assign(out.begin() + (size_t)i * N, out.begin() + (size_t)(i + 1) * N);
+}
+
+void database_tocrt(uint64_t* datacrt, std::vector<std::vector<uint64_t> >& data_ntt, int32_t N1) {
+    const int32_t N2 = N / 2 / N1;
+    for (int z = 0; z < N; ++z)
+        for (int j = 0; j < N2; ++j)
+            for (int k = 0; k < N1; ++k) {
+                const uint64_t x = data_ntt[(size_t)j * N1 + k][z];
+                datacrt[(size_t)z * N1 * N2 + (size_t)j * N1 + k] = (x % kspir_params::crtq1) | (x % kspir_params::crtq2) << 32;
+            }
+}
+
 void kspir_pir_forget(const void* p) {
     std::lock_guard<std::mutex> lock(g_mu);
     auto it = g_dbs.find(p);

@@ -7,6 +7,9 @@
 //   externalProduct                        src/lwe.h:292
 //   packingRLWEs                           src/answer.h:79
 //
+// and the server preprocessing of the database (twosteps.h:19
+// database_tobsgsntt on the GPU, crt.h:133 database_tocrt)
+//
 // and the data types they take (src/lwe.h:48-110 RlweCiphertext, :113-169
 // RGSWCiphertext, :175-234 AutoKey, :381-466 AutoKeyBSGSRNS), whose public
 // members (a, b, keyMap) hold the reference's words.  The work runs on the
@@ -128,6 +131,16 @@ void externalProduct(RlweCiphertext& result, RlweCiphertext& firstDim, const RGS
 // answer.h:79.  rlwes (a power of two of them, coefficient or NTT form) are
 // consumed; result is the packed RLWE in coefficient form.
 void packingRLWEs(RlweCiphertext& result, std::vector<RlweCiphertext>& rlwes, const AutoKey& autokey);
+
+// twosteps.h:19-21 database_tobsgsntt: data [N][N/2] record values mod
+// modulus (the plaintext p); result [N/2][N] (pre-sized by the caller) gets
+// the rows mod crtMod in NTT form.  Runs on the GPU.
+void database_tobsgsntt(std::vector<std::vector<uint64_t> >& result, std::vector<std::vector<uint64_t> >& data,
+                        uint64_t modulus, int32_t N1);
+
+// crt.h:133 database_tocrt: pack (x mod crtq1) | (x mod crtq2) << 32 into
+// datacrt[z][N2][N1] (a re-layout; host).
+void database_tocrt(uint64_t* datacrt, std::vector<std::vector<uint64_t> >& data_ntt, int32_t N1);
 
 // Drop the device copies made for a database pointer or key object (call
 // after changing their contents in place; destroyed objects are recognised

@@ -125,10 +125,10 @@ struct kspir_keys {
 namespace {
 
 int build_tables(kspir_context* c) {
-    const uint32_t qs[3] = {kQ1, kQ2, kBs};
+    const uint32_t qs[4] = {kQ1, kQ2, kBs, kP};
     const u64 root_q = 3375402822066082ULL;   // crt.h:81 root_of_unity_crt
-    const u64 roots[3] = {root_q % kQ1, root_q % kQ2, min_root(kBs)};
-    for (int p = 0; p < 3; ++p) {
+    const u64 roots[4] = {root_q % kQ1, root_q % kQ2, min_root(kBs), min_root(kP)};
+    for (int p = 0; p < 4; ++p) {
         const uint32_t q = qs[p];
         const u64 w = roots[p];
         std::vector<uint32_t> fw(kN), fwsh(kN), iv(kN), ivsh(kN), pw(2 * kN);
@@ -565,6 +565,121 @@ int kspir_database_create(kspir_context* c, const uint64_t* datacrt, int32_t r, 
     }
     if (stage) hipFree(stage);
     *out = new kspir_database{c, r, n1, n2, dst};
+    return KSPIR_OK;
+}
+
+// gather table of database_tobsgsntt for N1 rotations: gtab[i][x] =
+// dec[perm_i[x]], where perm_i is row i of compute_permutation_matrix and dec
+// inverts compute_query_encode (utils.cpp:1657-1675): enc[i] =
+// bitrev(rot_i >> 1) with rot_i = 5^i mod 2N for i < N/2 and
+// rot_{N-1-i} = (2N-1) * 5^i mod 2N
+static int db_gather_table(int n1, std::vector<uint16_t>& gtab) {
+    std::vector<int32_t> perm((size_t)n1 * kN);
+    int rc = kspir_compute_permutation_matrix(perm.data(), n1);
+    if (rc) return rc;
+    std::vector<uint32_t> rot(kN), dec(kN);
+    rot[0] = 1;
+    rot[kN - 1] = 2 * kN - 1;
+    for (int i = 1; i < kN / 2; ++i) {
+        rot[i] = rot[i - 1] * 5u % (2 * kN);
+        rot[kN - 1 - i] = rot[kN - i] * 5u % (2 * kN);
+    }
+    for (int i = 0; i < kN; ++i) dec[bitrev12(rot[i] >> 1)] = (uint32_t)i;
+    gtab.resize((size_t)n1 * kN);
+    for (size_t t = 0; t < gtab.size(); ++t) gtab[t] = (uint16_t)dec[perm[t]];
+    return KSPIR_OK;
+}
+
+int kspir_database_create_from_records(kspir_context* c, const uint64_t* records, int64_t block_stride, int32_t r,
+                                       int32_t n1, int on_device, kspir_database** out) {
+    if (!c || !records || !out) return fail(KSPIR_EINVAL, "null argument");
+    if (!pow2(r) || r > kN || !pow2(n1) || n1 < 32 || n1 > kN / 2)
+        return fail(KSPIR_EINVAL, "r must be a power of two <= N, N1 a power of two in [32, N/2]");
+    if (block_stride < 0) return fail(KSPIR_EINVAL, "negative block stride");
+    HIP_OK(hipSetDevice(c->device));
+    const int n2 = kN / 2 / n1;
+    const size_t words = (size_t)kN * kN / 2;
+    std::vector<uint16_t> gh;
+    int rc = db_gather_table(n1, gh);
+    if (rc) return rc;
+    uint64_t* dst = nullptr;
+    uint64_t* stage = nullptr;
+    uint16_t* gtab = nullptr;
+    auto cleanup = [&]() { if (dst) hipFree(dst); if (stage) hipFree(stage); if (gtab) hipFree(gtab); };
+    hipError_t e = hipMalloc(&dst, words * 8 * (size_t)r);
+    if (e == hipSuccess) e = hipMalloc(&gtab, gh.size() * 2);
+    if (e == hipSuccess && !on_device) e = hipMalloc(&stage, words * 8);
+    if (e == hipSuccess) e = hipMemcpyAsync(gtab, gh.data(), gh.size() * 2, hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) { cleanup(); return fail_hip("database_create_from_records", e); }
+    for (int b = 0; b < r; ++b) {
+        const uint64_t* src = records + (size_t)block_stride * b;
+        if (!on_device) {
+            if (b == 0 || block_stride != 0) {
+                e = hipMemcpyAsync(stage, src, words * 8, hipMemcpyHostToDevice, c->stream);
+                if (e != hipSuccess) { cleanup(); return fail_hip("hipMemcpy(records)", e); }
+            }
+            src = stage;
+        }
+        hipLaunchKernelGGL(k_db_prep, dim3(n2 * n1), dim3(kNttThreads), 0, c->stream, src, gtab, dst + words * b, n1,
+                           1, c->T);
+        e = hipGetLastError();
+        if (e == hipSuccess && !on_device) e = hipStreamSynchronize(c->stream);   // stage reused next block
+        if (e != hipSuccess) { cleanup(); return fail_hip("k_db_prep", e); }
+    }
+    e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) { cleanup(); return fail_hip("k_db_prep", e); }
+    if (stage) hipFree(stage);
+    hipFree(gtab);
+    *out = new kspir_database{c, r, n1, n2, dst};
+    return KSPIR_OK;
+}
+
+int kspir_database_tobsgsntt(kspir_context* c, const uint64_t* data, int32_t n1, uint64_t* result) {
+    if (!c || !data || !result) return fail(KSPIR_EINVAL, "null argument");
+    if (!pow2(n1) || n1 < 1 || n1 > kN / 2) return fail(KSPIR_EINVAL, "N1 must be a power of two <= N/2");
+    HIP_OK(hipSetDevice(c->device));
+    const int n2 = kN / 2 / n1;
+    const size_t words = (size_t)kN * kN / 2;
+    std::vector<uint16_t> gh;
+    int rc = db_gather_table(n1, gh);
+    if (rc) return rc;
+    uint64_t *src = nullptr, *dst = nullptr;
+    uint16_t* gtab = nullptr;
+    hipError_t e = hipMalloc(&src, words * 8);
+    if (e == hipSuccess) e = hipMalloc(&dst, words * 8);
+    if (e == hipSuccess) e = hipMalloc(&gtab, gh.size() * 2);
+    if (e == hipSuccess) e = hipMemcpyAsync(gtab, gh.data(), gh.size() * 2, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(src, data, words * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_db_prep, dim3(n2 * n1), dim3(kNttThreads), 0, c->stream, src, gtab, dst, n1, 0, c->T);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(result, dst, words * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (src) hipFree(src);
+    if (dst) hipFree(dst);
+    if (gtab) hipFree(gtab);
+    if (e != hipSuccess) return fail_hip("database_tobsgsntt", e);
+    return KSPIR_OK;
+}
+
+int kspir_database_export(const kspir_database* db, uint64_t* datacrt) {
+    if (!db || !datacrt) return fail(KSPIR_EINVAL, "null argument");
+    kspir_context* c = db->ctx;
+    HIP_OK(hipSetDevice(c->device));
+    const size_t words = (size_t)kN * kN / 2;
+    uint64_t* stage = nullptr;
+    HIP_OK(hipMalloc(&stage, words * 8));
+    hipError_t e = hipSuccess;
+    for (int b = 0; b < db->r && e == hipSuccess; ++b) {
+        hipLaunchKernelGGL(k_db_untranspose, dim3(kN / 32, db->n2 * db->n1 / 32), dim3(256), 0, c->stream, stage,
+                           db->words + words * b, db->n2 * db->n1);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(datacrt + words * b, stage, words * 8, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
+    hipFree(stage);
+    if (e != hipSuccess) return fail_hip("database_export", e);
     return KSPIR_OK;
 }
 

@@ -27,6 +27,7 @@ constexpr int kNttThreads = 256;   // threads per workgroup of every NTT kernel
 constexpr uint32_t kQ1 = 268369921u;
 constexpr uint32_t kQ2 = 249561089u;
 constexpr uint32_t kBs = 16760833u;
+constexpr uint32_t kP = 65537u;          // plaintext modulus (params.h)
 constexpr uint64_t kQ = (uint64_t)kQ1 * kQ2;
 
 enum Prime { P_Q1 = 0, P_Q2 = 1, P_BS = 2 };
@@ -46,7 +47,7 @@ struct PrimeTab {
 };
 
 struct Tabs {
-    PrimeTab p[3];
+    PrimeTab p[4];              // q1, q2, bsMod, plaintext p
     const uint16_t* slot_exp;   // NTT slot s holds psi^(slot_exp[s]) (odd exponent)
     const uint16_t* exp_slot;   // exp_slot[e >> 1] = slot holding psi^e
     // CRT / modulus-switch constants (Shoup pairs)

@@ -171,6 +171,69 @@ __global__ void __launch_bounds__(256) k_db_transpose(uint64_t* __restrict__ out
     for (int k = ty; k < 32; k += 8) out[(size_t)(r0 + k) * kN + z0 + tx] = tile[tx][k];
 }
 
+// slot-minor layout [N2][N1][z] -> reference layout [z][N2][N1] (export)
+__global__ void __launch_bounds__(256) k_db_untranspose(uint64_t* __restrict__ out, const uint64_t* __restrict__ in,
+                                                        int rows /* N2*N1 */) {
+    __shared__ uint64_t tile[32][33];
+    const int z0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+    for (int k = ty; k < 32; k += 8) tile[k][tx] = in[(size_t)(r0 + k) * kN + z0 + tx];
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) out[(size_t)(z0 + k) * rows + r0 + tx] = tile[tx][k];
+}
+
+// Server preprocessing of one database block, one row kk = k*N1 + i per
+// workgroup: database_tobsgsntt (twosteps.cpp:42-179) then database_tocrt
+// (crt.cpp:859-885).  The row gathers the diagonal of the record matrix
+// (result[kk][j] = data[(j - kk) mod N/2][j], mirrored for j >= N/2), then
+// query_encode and permutation row i as one gather table gtab[i][x];
+// INTT mod p, centre lift to Q (x > p/2 -> x + Q - p), NTT mod q1 and q2.
+// pack != 0: words (x mod q1) | (x mod q2) << 32 in the slot-minor layout
+// [N2*N1][N]; pack == 0: x mod Q (the result of database_tobsgsntt).
+// rec: [N][N/2] record values < p (sample_database_bsgs layout).
+__global__ void __launch_bounds__(kNttThreads) k_db_prep(const uint64_t* __restrict__ rec,
+        const uint16_t* __restrict__ gtab, uint64_t* __restrict__ out, int n1, int pack, Tabs T) {
+    __shared__ uint32_t s[2][kPadN];
+    constexpr int H = kN / 2;
+    constexpr int PER = kN / kNttThreads;
+    const int kk = blockIdx.x, i = kk % n1, tid = threadIdx.x;
+    const uint16_t* g = gtab + (size_t)i * kN;
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int x = tid + e * kNttThreads;
+        const int j = g[x];
+        uint64_t v;
+        if (j < H) {
+            const int row = (j - kk + H) & (H - 1);
+            v = rec[(size_t)row * H + j];
+        } else {
+            const int jj = kN - 1 - j;
+            const int row = (jj - kk + H) & (H - 1);
+            v = rec[(size_t)(kN - 1 - row) * H + jj];
+        }
+        s[0][lds_pad(x)] = (uint32_t)v;
+    }
+    __syncthreads();
+    ntt_inv_r16(s[0], T.p[3]);
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int x = tid + e * kNttThreads;
+        const uint32_t v = s[0][lds_pad(x)];
+        const bool lift = v > kP / 2;
+        s[0][lds_pad(x)] = lift ? v + kQ1 - kP : v;
+        s[1][lds_pad(x)] = lift ? v + kQ2 - kP : v;
+    }
+    __syncthreads();
+    ntt_fwd_r16x2(s[0], T.p[0], s[1], T.p[1]);
+    uint64_t* o = out + (size_t)kk * kN;
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int x = tid + e * kNttThreads;
+        const uint32_t x1 = s[0][lds_pad(x)], x2 = s[1][lds_pad(x)];
+        o[x] = pack ? ((uint64_t)x2 << 32 | x1) : crt_q(x1, x2, T);
+    }
+}
+
 // ---------------------------------------------------------- baby steps
 
 // rotation-0 and first-half digit preparation from the host query.

@@ -1158,6 +1158,30 @@ int ko_gen_db(int r, int n1, unsigned seed, int plant, int tcol, int tpack, uint
     return 0;
 }
 
+// ko_gen_db with the records of each block kept as well: records [r][N][N/2]
+// after sampling and planting (the input of database_tobsgsntt), db the
+// database_tocrt words
+int ko_gen_records(int r, int n1, unsigned seed, int plant, int tcol, int tpack, uint64_t* records, uint64_t* db) {
+    if (!valid_shape(r, n1)) return -1;
+    srand(seed);
+    std::vector<u64> data((size_t)N * (N / 2)), data_ntt((size_t)(N / 2) * N);
+    const size_t words = (size_t)N * N / 2;
+    for (int k = 0; k < r; ++k) {
+        make_db_block(db + words * k, n1, k, r, plant, tcol, tpack, data, data_ntt);
+        std::memcpy(records + words * k, data.data(), words * 8);
+    }
+    return 0;
+}
+
+// database_tobsgsntt alone on one block: data [N][N/2] -> result [N/2][N]
+int ko_database_tobsgsntt(int n1, const uint64_t* data, uint64_t* result) {
+    if (n1 < 1 || n1 > N / 2 || (n1 & (n1 - 1))) return -1;
+    std::vector<u64> d(data, data + (size_t)N * N / 2), res((size_t)(N / 2) * N);
+    database_tobsgsntt(res, d, n1);
+    std::memcpy(result, res.data(), res.size() * 8);
+    return 0;
+}
+
 static void gen_client_tail(int r, int n1, int tcol, int tpack, const Secret& s, uint64_t* query,
                             uint64_t* rgsw, uint64_t* baby, uint64_t* giant, uint64_t* packing) {
     // test-pir.cpp:68-89: query, RGSW, BSGS keys, packing keys

@@ -69,6 +69,8 @@ int ko_gen_db(int r, int n1, unsigned seed, int plant, int target_col, int targe
      firstdim   [r][2][N]   result[] after the first dimension (NTT form)
      ext        [r][2][N]   after externalProduct (coefficient form)
    response [2][N] coefficient form. */
+int ko_gen_records(int r, int n1, unsigned seed, int plant, int tcol, int tpack, uint64_t* records, uint64_t* db);
+int ko_database_tobsgsntt(int n1, const uint64_t* data, uint64_t* result);
 int ko_answer(int r, int n1, const uint64_t* db, const uint64_t* query, const uint64_t* rgsw,
               const uint64_t* baby, const uint64_t* giant, const uint64_t* packing,
               const int32_t* perms, uint64_t* response, uint64_t* cipherbuf,

@@ -160,6 +160,24 @@ def answer(inst=None, *, r=None, n1=None, db=None, query=None, rgsw=None, baby=N
     return resp
 
 
+def gen_records(r, n1, seed=1, plant=False, tcol=0, tpack=0):
+    """records [r][N][N/2] (database_tobsgsntt input) and the words [r][N*N/2]
+    of the same database, generated as gen_db does"""
+    rec = np.zeros((r, N, N // 2), dtype=np.uint64)
+    db = np.zeros((r, N * N // 2), dtype=np.uint64)
+    rc = lib().ko_gen_records(r, n1, seed, int(plant), tcol, tpack, p64(rec), p64(db))
+    assert rc == 0
+    return rec, db
+
+
+def database_tobsgsntt(n1, data):
+    """database_tobsgsntt of one block: data [N][N/2] -> [N/2][N] mod Q"""
+    out = np.zeros((N // 2, N), dtype=np.uint64)
+    rc = lib().ko_database_tobsgsntt(n1, p64(np.ascontiguousarray(data, dtype=np.uint64)), p64(out))
+    assert rc == 0
+    return out
+
+
 def answer_parallel(inst, count, threads):
     """`count` independent answers of inst's query on `threads` threads
     (the all-cores CPU baseline); returns the response of the first"""

@@ -49,6 +49,19 @@ int main(int argc, char** argv) {
     int lr = 0;
     while ((1 << lr) < r) ++lr;
     std::vector<uint64_t> datacrt = read_u64(dir + "/db.bin", (size_t)r * N * N / 2);
+    // server preprocessing as test-pir.cpp:58-62 runs it, when records are given
+    if (FILE* f = std::fopen((dir + "/records.bin").c_str(), "rb")) {
+        std::fclose(f);
+        const size_t words = (size_t)N * N / 2;
+        std::vector<uint64_t> rec = read_u64(dir + "/records.bin", (size_t)r * words), pre((size_t)r * words);
+        std::vector<std::vector<uint64_t> > data(N, std::vector<uint64_t>(N / 2)), data_ntt(N / 2, std::vector<uint64_t>(N));
+        for (int b = 0; b < r; ++b) {
+            for (int i = 0; i < N; ++i) data[i].assign(&rec[b * words + (size_t)i * (N / 2)], &rec[b * words + (size_t)(i + 1) * (N / 2)]);
+            database_tobsgsntt(data_ntt, data, 65537, N1);
+            database_tocrt(&pre[b * words], data_ntt, N1);
+        }
+        write_u64(dir + "/datacrt_gpu.bin", pre);
+    }
     std::vector<uint64_t> qw = read_u64(dir + "/query.bin", 4 * (size_t)N);
     std::vector<uint64_t> rg = read_u64(dir + "/rgsw.bin", 8 * (size_t)N);
     std::vector<uint64_t> baby = read_u64(dir + "/baby.bin", (size_t)(N1 / 2) * 12 * N);

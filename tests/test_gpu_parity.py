@@ -225,6 +225,28 @@ def test_fused_giant_steps(ctx, oracle, r, n1, nq):
     db.close()
 
 
+@pytest.mark.parametrize("r,n1", [(2, 128), (1, 64), (2, 2048)])
+def test_database_preprocessing_on_gpu(ctx, oracle, r, n1):
+    """database_tobsgsntt + database_tocrt on the GPU (records -> resident
+    words) equal the oracle's words, export returns them in the reference
+    layout, and an export loads back to the same words; block stride 0
+    repeats block 0"""
+    say(f"[gpu] database preprocessing r={r} n1={n1}")
+    rec, want = oracle.gen_records(r, n1, seed=17, plant=True, tcol=5 % (N // 2), tpack=r - 1)
+    db = ctx.database_from_records(rec, r, n1)
+    got = db.export()
+    assert np.array_equal(got, want)
+    db2 = ctx.database(got, r, n1)
+    assert np.array_equal(db2.export(), want)
+    db3 = ctx.database_from_records(rec, r, n1, block_stride=0)
+    rep = db3.export()
+    assert all(np.array_equal(rep[b], want[0]) for b in range(r))
+    one = ctx.database_tobsgsntt(rec[0], n1)
+    assert np.array_equal(one, oracle.database_tobsgsntt(n1, rec[0]))
+    for d in (db, db2, db3):
+        d.close()
+
+
 def test_cpp_dropin_matches_oracle(oracle, tmp_path):
     """tests/native/dropin_test calls matrix_vector_mul_bsgs_rns_crt_large,
     externalProduct and packingRLWEs through kspir_pir.h exactly as
@@ -240,6 +262,8 @@ def test_cpp_dropin_matches_oracle(oracle, tmp_path):
     inst, resp, st = instance(oracle, r, n1, 21)
     for name in ("db", "query", "rgsw", "baby", "giant", "packing", "perms"):
         getattr(inst, name).tofile(str(tmp_path / f"{name}.bin"))
+    rec, rec_words = oracle.gen_records(r, n1, seed=33, plant=True, tcol=7, tpack=1)
+    rec.tofile(str(tmp_path / "records.bin"))
     out = subprocess.run([exe, str(tmp_path), str(r), str(n1)], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     say(out.stdout.strip())
@@ -256,3 +280,6 @@ def test_cpp_dropin_matches_oracle(oracle, tmp_path):
     assert "input[0] ntt=1 result ntt=0" in out.stdout
     msg = oracle.decrypt(r, inst.secret, got)
     assert np.array_equal(msg, np.arange(1, N + 1, dtype=np.uint64))
+    # database_tobsgsntt + database_tocrt through the drop-in give the oracle's words
+    pre = np.fromfile(str(tmp_path / "datacrt_gpu.bin"), dtype=np.uint64).reshape(r, N * N // 2)
+    assert np.array_equal(pre, rec_words)

@@ -288,3 +288,19 @@ def test_planted_mistake_breaks_kat(oracle):
     resp = oracle.answer(inst)
     msg = oracle.decrypt(2, inst.secret, resp)
     assert not np.array_equal(msg, np.arange(1, N + 1, dtype=np.uint64))
+
+
+def test_gen_records_matches_gen_db_and_tocrt(oracle):
+    """the records the oracle keeps are the input of its preprocessing: the
+    same seed gives the words of gen_db, and database_tobsgsntt of a block
+    re-laid as database_tocrt (crt.cpp:859-885) gives that block's words"""
+    r, n1 = 2, 256
+    n2 = N // 2 // n1
+    rec, words = oracle.gen_records(r, n1, seed=3)
+    assert np.array_equal(words, oracle.gen_db(r, n1, seed=3))
+    assert rec.max() < 65537
+    nt = oracle.database_tobsgsntt(n1, rec[1])
+    q1, q2 = np.uint64(268369921), np.uint64(249561089)
+    packed = (nt % q1) | ((nt % q2) << np.uint64(32))          # [n2*n1][N]
+    want = packed.reshape(n2, n1, N).transpose(2, 0, 1).reshape(-1)
+    assert np.array_equal(words[1], want)
