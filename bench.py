@@ -40,6 +40,13 @@ def synthetic_db(r, seed=1234):
     return (lo | (hi << np.uint64(32))).reshape(r, -1)
 
 
+def synthetic_records(seed=1234):
+    """one database block of records in the sample_database_bsgs layout:
+    [N][N/2] values < 2^16 (the reference samples rand() & 0xffff)"""
+    rng = np.random.default_rng(seed)
+    return rng.integers(0, 1 << 16, (1, N, N // 2), dtype=np.uint64)
+
+
 def synthetic_client(r, n1, seed):
     """one client's query and keys: uniform residues of the right moduli in
     the reference layouts (see include/kspir_amd.h)"""
@@ -113,6 +120,176 @@ def cpu_baseline(r, n1, reps):
     return single, allc
 
 
+def config_c2(args, log):
+    """C2: single query against a 1 GB database (r = 64, N1 = 128, N2 = 16),
+    the database built on the GPU from one block of records repeated"""
+    import kspir_amd
+    r, n1 = 64, 128
+    ctx = kspir_amd.Context(0)
+    t0 = time.perf_counter()
+    db = ctx.database_from_records(synthetic_records(), r, n1, block_stride=0)
+    prep_s = time.perf_counter() - t0
+    query, baby, giant, packing, rgsw = synthetic_client(r, n1, 7)
+    keys = ctx.keys(r, n1, baby, giant, packing, rgsw, kspir_amd.compute_permutation_matrix(n1))
+    qbuf, rbuf = ctx.alloc(4 * N * 8), ctx.alloc(2 * N * 8)
+    qbuf.upload(query)
+    ms = ctx.time_answer(db, keys, qbuf, rbuf, 5, args.plain_steps) / args.plain_steps
+    lat = [ctx.time_answer(db, keys, qbuf, rbuf, 0, 1) for _ in range(args.lat_samples)]
+    db_ms = ctx.time_db_pass(db, keys, qbuf, 10)
+    db_bytes = 8 * r * N * (N // 2)
+    out = {"config": "C2 single query, 1 GB database (131072 records x 8 KB), r=64, N1=128, N2=16",
+           "qps": round(1000.0 / ms, 2), "ms": round(ms, 4),
+           "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
+                          "p99": round(float(np.percentile(lat, 99)), 4)},
+           "db_pass": {"avg_launch_ms": round(db_ms, 4), "algorithmic_bytes": db_bytes,
+                       "GBps": round(db_bytes / (db_ms * 1e-3) / 1e9, 1),
+                       "frac_of_8TBps": round(db_bytes / (db_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)},
+           "gpu_preprocess_s_incl_alloc": round(prep_s, 3)}
+    keys.close()
+    db.close()
+    ctx.close()
+    return out
+
+
+def config_c4(args, log):
+    """C4: uneven multi-database batch: 64 MB (r=4), 512 MB (r=32) and 4 GB
+    (r=256) databases with 16, 32 and 48 queries (distinct clients), answered
+    back to back; once with the int8-MFMA database pass, once with the VALU
+    pass (both exact)"""
+    import kspir_amd
+    n1 = 128
+    plan = [(4, 16), (32, 32), (256, 48)]
+    ctx = kspir_amd.Context(0)
+    perms = kspir_amd.compute_permutation_matrix(n1)
+    rec = synthetic_records()
+    groups = []
+    seed = 5000
+    for r, nq in plan:
+        log(f"C4 database r={r}, {nq} clients")
+        db = ctx.database_from_records(rec, r, n1, block_stride=0)
+        keys, qs = [], []
+        for _ in range(nq):
+            query, baby, giant, packing, rgsw = synthetic_client(r, n1, seed)
+            seed += 1
+            keys.append(ctx.keys(r, n1, baby, giant, packing, rgsw, perms))
+            qs.append(query)
+        qbuf, rbuf = ctx.alloc(nq * 4 * N * 8), ctx.alloc(nq * 2 * N * 8)
+        qbuf.upload(np.stack(qs))
+        groups.append((r, nq, db, keys, qbuf, rbuf))
+    total_q = sum(nq for _, nq in plan)
+    out = {"config": "C4 uneven: 64 MB / 512 MB / 4 GB databases (r 4 / 32 / 256, N1 128) with 16 / 32 / 48 "
+                     "queries of distinct clients, one step = all 96 answered back to back"}
+    for mode, min_batch in (("mfma", 2), ("valu", 1 << 30)):
+        ctx.set("mfma_min_batch", min_batch)
+        per = {}
+        for r, nq, db, keys, qbuf, rbuf in groups:       # warm-up (builds limbs, workspace)
+            ctx.answer_batch_device(db, keys, qbuf, rbuf)
+        ctx.synchronize()
+        steps = 3
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            for r, nq, db, keys, qbuf, rbuf in groups:
+                ctx.answer_batch_device(db, keys, qbuf, rbuf)
+        ctx.synchronize()
+        wall = (time.perf_counter() - t0) / steps
+        for r, nq, db, keys, qbuf, rbuf in groups:
+            ms = ctx.time_answer_batch(db, keys, qbuf, rbuf, 1, 3) / 3
+            per[f"r{r}_{nq}q"] = {"ms": round(ms, 3), "qps": round(nq * 1000.0 / ms, 1)}
+        out[mode] = {"step_ms": round(wall * 1e3, 3), "qps": round(total_q / wall, 1), "per_database": per}
+    ctx.set("mfma_min_batch", 2)
+    for r, nq, db, keys, qbuf, rbuf in groups:
+        for k in keys:
+            k.close()
+        db.close()
+        qbuf.free()
+        rbuf.free()
+    ctx.close()
+    return out
+
+
+def config_c5(args, log):
+    """C5: serving loop.  A resident 4 GB database (r=256) and, memory
+    permitting, a 16 GB one (r=1024); 64 clients with their own keys; queries
+    arrive as a Poisson process at each offered rate and go through the
+    library's continuous-batching server (kspir_server_*).  Latency is
+    submit-to-response, measured by the server."""
+    import threading
+    import kspir_amd
+    n1 = 128
+    ctx = kspir_amd.Context(0)
+    perms = kspir_amd.compute_permutation_matrix(n1)
+    rec = synthetic_records()
+    out = {"config": "C5 serving: Poisson arrivals, continuous batching (batch cap 48 at 4 GB, 16 at 16 GB; admission "
+                     "by per-query device footprint), 64 clients with distinct keys, resident 4 GB (r=256) and 16 GB (r=1024) "
+                     "databases, N1=128", "databases": {}}
+    for r, rates, dur, cap in ((256, (200, 500, 1000, 2000), 2.0, 48), (1024, (200, 500), 2.0, 16)):
+        try:
+            db = ctx.database_from_records(rec, r, n1, block_stride=0)
+        except kspir_amd.KspirError as e:
+            out["databases"][f"{r // 64} GB"] = {"skipped": str(e)}
+            continue
+        log(f"C5 database r={r}: 64 clients")
+        keys, queries = [], []
+        for c in range(64):
+            query, baby, giant, packing, rgsw = synthetic_client(r, n1, 9000 + c)
+            keys.append(ctx.keys(r, n1, baby, giant, packing, rgsw, perms))
+            queries.append(query)
+        srv = kspir_amd.Server(ctx, max_batch=cap)
+        # warm-up: one full batch (database limbs, workspace at full size)
+        for t in [srv.submit(db, keys[c % 64], queries[c % 64]) for c in range(cap)]:
+            srv.wait(t)
+        res = {}
+        rng = np.random.default_rng(r)
+        for rate in rates:
+            log(f"C5 r={r}: {rate} queries/s offered for {dur} s")
+            tickets = []
+            done_ms = []
+
+            def collect():
+                i = 0
+                while True:
+                    while i >= len(tickets):
+                        if stop_flag[0] and i >= len(tickets):
+                            return
+                        time.sleep(0.0005)
+                    _, ms = srv.wait(tickets[i])
+                    done_ms.append(ms)
+                    i += 1
+
+            stop_flag = [False]
+            th = threading.Thread(target=collect)
+            th.start()
+            b0 = srv.stats()
+            t_start = time.perf_counter()
+            t_next = t_start
+            c = 0
+            while t_next - t_start < dur:
+                now = time.perf_counter()
+                if now < t_next:
+                    time.sleep(t_next - now)
+                tickets.append(srv.submit(db, keys[c % 64], queries[c % 64]))
+                c += 1
+                t_next += rng.exponential(1.0 / rate)
+            stop_flag[0] = True
+            th.join()
+            wall = time.perf_counter() - t_start
+            b1 = srv.stats()
+            nb = b1["batches"] - b0["batches"]
+            res[str(rate)] = {"offered_qps": rate, "submitted": c, "achieved_qps": round(c / wall, 1),
+                              "latency_ms": {"p50": round(float(np.percentile(done_ms, 50)), 3),
+                                             "p99": round(float(np.percentile(done_ms, 99)), 3)},
+                              "batches": nb, "mean_batch": round(c / max(nb, 1), 2)}
+        srv.close()
+        out["databases"][f"{r // 64} GB"] = {"r": r, "batch_cap": cap,
+                                            "per_query_footprint_MB": round(kspir_amd.query_footprint(r, n1) / 2**20, 1),
+                                            "rates": res}
+        for k in keys:
+            k.close()
+        db.close()
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,16 +302,23 @@ def main():
     ap.add_argument("--lat-samples", type=int, default=50)
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--config", default="c3", choices=("c3", "c2", "c4", "c5"),
+                    help="c3: the headline batched line (default); c2/c4/c5: the other workloads, one JSON line each")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the KsPIR answer here runs on a single MI355X "
               "(multi-GPU sharding by database block is a design note, not built).", file=sys.stderr)
         sys.exit(2)
 
+    log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)
+    if args.config != "c3":
+        fn = {"c2": config_c2, "c4": config_c4, "c5": lambda a, l: config_c5(a, l)}[args.config]
+        print(json.dumps(fn(args, log)), flush=True)
+        return
+
     import kspir_amd
     r, n1, B = args.db_mb // 16, args.n1, args.batch
     n2 = N // 2 // n1
-    log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)
     log(f"synthetic database r={r} n1={n1}")
     ctx = kspir_amd.Context(0)
     db = ctx.database(synthetic_db(r), r, n1)

@@ -149,6 +149,26 @@ int kspir_answer_inspect(kspir_context* ctx, const kspir_database* db, const ksp
                          const uint64_t* query, uint64_t* response, uint64_t* cipherbuf,
                          uint64_t* firstdim, uint64_t* ext);
 
+/* Serving loop (no counterpart in the reference, whose driver answers one
+ * query at a time).  A worker thread answers submitted queries in batches:
+ * whenever the GPU is free it takes the database of the oldest waiting query
+ * and up to max_batch of that database's waiting queries, in arrival order,
+ * as many as workspace_budget_bytes admits at the per-query device footprint
+ * of the database's shape (0 = half of the device's free memory at
+ * creation).  The server drives the context: while it exists, do not call
+ * other functions on that context.  query ([2][2][N], host) and response
+ * ([2][N], host) must stay valid until kspir_server_wait returns for the
+ * ticket; latency_ms is submit-to-completion time.  Responses equal
+ * kspir_answer's for the same query. */
+typedef struct kspir_server kspir_server;
+int kspir_server_create(kspir_context* ctx, int32_t max_batch, size_t workspace_budget_bytes, kspir_server** out);
+int kspir_server_submit(kspir_server* server, const kspir_database* db, const kspir_keys* keys,
+                        const uint64_t* query, uint64_t* response, uint64_t* ticket);
+int kspir_server_wait(kspir_server* server, uint64_t ticket, double* latency_ms);
+int kspir_server_stats(kspir_server* server, uint64_t* batches, uint64_t* answered, uint64_t* pending);
+/* waits for every submitted query, then stops the worker */
+void kspir_server_destroy(kspir_server* server);
+
 #ifdef __cplusplus
 }
 #endif

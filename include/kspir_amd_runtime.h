@@ -28,6 +28,10 @@ int kspir_device_count(int* count);
  *                     (query, block) (default 16); smaller ones spread them
  *                     over one workgroup per (block, row, prime) */
 int kspir_context_set(kspir_context* ctx, const char* key, int64_t value);
+
+/* device bytes one query of shape (r, N1) adds to a batch (workspace and
+ * batched database-pass scratch): the unit of the serving loop's admission */
+int kspir_query_footprint_bytes(int32_t r, int32_t n1, size_t* bytes);
 int kspir_device_alloc(kspir_context* ctx, size_t bytes, void** out);
 int kspir_device_free(kspir_context* ctx, void* ptr);
 int kspir_memcpy_h2d(kspir_context* ctx, void* dst, const void* src, size_t bytes);

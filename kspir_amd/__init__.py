@@ -39,11 +39,14 @@ def _load():
                  "kspir_time_db_pass", "kspir_test_ntt", "kspir_device_count",
                  "kspir_compute_permutation_matrix", "kspir_answer_batch", "kspir_answer_batch_device",
                  "kspir_time_answer_batch", "kspir_context_set", "kspir_time_copy",
-                 "kspir_database_create_from_records", "kspir_database_tobsgsntt", "kspir_database_export"):
+                 "kspir_database_create_from_records", "kspir_database_tobsgsntt", "kspir_database_export",
+                 "kspir_server_create", "kspir_server_submit", "kspir_server_wait", "kspir_server_stats",
+                 "kspir_query_footprint_bytes"):
         getattr(L, name).restype = ctypes.c_int
     L.kspir_context_destroy.argtypes = [_vp]
     L.kspir_database_destroy.argtypes = [_vp]
     L.kspir_keys_destroy.argtypes = [_vp]
+    L.kspir_server_destroy.argtypes = [_vp]
     return L
 
 
@@ -59,6 +62,8 @@ EXPORTS = (
     "kspir_device_count", "kspir_device_alloc", "kspir_device_free", "kspir_memcpy_h2d", "kspir_memcpy_d2h",
     "kspir_synchronize", "kspir_context_stream", "kspir_time_answer", "kspir_time_db_pass", "kspir_test_ntt",
     "kspir_time_copy", "kspir_database_create_from_records", "kspir_database_tobsgsntt", "kspir_database_export",
+    "kspir_server_create", "kspir_server_submit", "kspir_server_wait", "kspir_server_stats", "kspir_server_destroy",
+    "kspir_query_footprint_bytes",
 )
 
 
@@ -160,6 +165,51 @@ class Keys:
             self.h = None
 
 
+def query_footprint(r, n1):
+    """device bytes one query of shape (r, N1) adds to a batch"""
+    out = ctypes.c_size_t(0)
+    _check(lib.kspir_query_footprint_bytes(r, n1, ctypes.byref(out)))
+    return out.value
+
+
+class Server:
+    """Continuous-batching serving loop on a context (see kspir_server_* in
+    include/kspir_amd.h).  While it runs, the context is driven by its worker
+    thread.  submit() returns a ticket; wait(ticket) returns (response,
+    latency_ms)."""
+
+    def __init__(self, ctx, max_batch=64, budget_bytes=0):
+        self.ctx = ctx
+        h = _vp()
+        _check(lib.kspir_server_create(ctx.h, max_batch, ctypes.c_size_t(budget_bytes), ctypes.byref(h)))
+        self.h = h
+        self._live = {}
+
+    def submit(self, db, keys, query):
+        q = np.ascontiguousarray(query, dtype=np.uint64)
+        resp = np.zeros((2, N), dtype=np.uint64)
+        t = ctypes.c_uint64(0)
+        _check(lib.kspir_server_submit(self.h, db.h, keys.h, _p64(q), _p64(resp), ctypes.byref(t)))
+        self._live[t.value] = (q, resp)      # kept alive until wait()
+        return t.value
+
+    def wait(self, ticket):
+        ms = ctypes.c_double(0)
+        _check(lib.kspir_server_wait(self.h, ctypes.c_uint64(ticket), ctypes.byref(ms)))
+        q, resp = self._live.pop(ticket)
+        return resp, ms.value
+
+    def stats(self):
+        b, a, p = ctypes.c_uint64(0), ctypes.c_uint64(0), ctypes.c_uint64(0)
+        _check(lib.kspir_server_stats(self.h, ctypes.byref(b), ctypes.byref(a), ctypes.byref(p)))
+        return {"batches": b.value, "answered": a.value, "pending": p.value}
+
+    def close(self):
+        if self.h:
+            lib.kspir_server_destroy(self.h)
+            self.h = None
+
+
 class Context:
     def __init__(self, device=0):
         h = _vp()
@@ -229,6 +279,13 @@ class Context:
         resp = np.zeros((nq, 2, N), dtype=np.uint64)
         _check(lib.kspir_answer_batch(self.h, db.h, arr, nq, _p64(q), _p64(resp)))
         return resp
+
+    def answer_batch_device(self, db, keys_list, queries_buf, resp_buf, stream=None):
+        """enqueue a batch on device buffers (asynchronous; synchronize() to wait)"""
+        nq = len(keys_list)
+        arr = (_vp * nq)(*[k.h.value for k in keys_list])
+        _check(lib.kspir_answer_batch_device(self.h, db.h, arr, nq, ctypes.cast(_vp(queries_buf.ptr), _u64p),
+                                             ctypes.cast(_vp(resp_buf.ptr), _u64p), _vp(stream)))
 
     def time_answer_batch(self, db, keys_list, queries_buf, resp_buf, warmup, iters):
         nq = len(keys_list)

@@ -3,9 +3,15 @@
 // answer behind the C ABI of include/kspir_amd.h.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <deque>
+#include <mutex>
 #include <string>
+#include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "kspir_kernels.h"
@@ -203,6 +209,20 @@ int build_tables(kspir_context* c) {
     return KSPIR_OK;
 }
 
+
+// device bytes one query of shape (r, n1) adds to a batch: its share of the
+// carved workspace plus its share of the batched database pass's scratch
+// (query limbs and tile-major products)
+size_t query_footprint_bytes(int r, int n1) {
+    const int n2 = kN / 2 / n1;
+    const size_t n = kN;
+    const size_t rh = r / 2 > 0 ? r / 2 : 1;
+    const size_t words = 9 * n * 2 + 3 * n + 6 * n + 3 * n + (size_t)n1 * 4 * n + (size_t)r * n2 * 4 * n * 2 +
+                         (size_t)r * n2 * 2 * n + (size_t)r * 4 * n * 2 + (size_t)r * 16 * n + (size_t)r * 4 * n * 2 +
+                         rh * 2 * n + rh * 16 * n + 2 * 4 * n * 2 + 2 * 2 * n * 2;
+    const size_t mfma = (size_t)kN * 2 * 4 * 2 * n1 + (size_t)2 * r * n2 * 2 * kN * 4;
+    return words * 4 + mfma;
+}
 
 // carve the workspace for nq queries of shape (r, n1); grows on demand
 int carve_workspace(kspir_context* c, int r, int n1, int nq) {
@@ -1060,6 +1080,169 @@ int kspir_time_copy(kspir_context* c, void* dst, const void* src, size_t bytes, 
     HIP_OK(hipEventDestroy(e0));
     HIP_OK(hipEventDestroy(e1));
     return KSPIR_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ serving loop
+
+struct kspir_server {
+    struct Req {
+        uint64_t id;
+        const kspir_database* db;
+        const kspir_keys* keys;
+        const uint64_t* query;
+        uint64_t* response;
+        std::chrono::steady_clock::time_point t0;
+    };
+    struct Done {
+        int rc;
+        double ms;
+    };
+    kspir_context* ctx = nullptr;
+    int max_batch = 64;
+    size_t budget = 0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Req> pending;
+    std::unordered_map<uint64_t, Done> done;
+    uint64_t next_id = 1, batches = 0, answered = 0;
+    bool stop = false;
+    uint64_t* qh = nullptr;   // pinned staging, max_batch queries / responses
+    uint64_t* rh = nullptr;
+    std::thread worker;
+
+    void run() {
+        std::vector<Req> take;
+        std::vector<const kspir_keys*> keys;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lock(mu);
+                cv_work.wait(lock, [&] { return stop || !pending.empty(); });
+                if (pending.empty()) return;   // stop requested and drained
+                // continuous batching: the database of the oldest waiting
+                // query, and as many of its queries (in arrival order) as the
+                // batch cap and the workspace budget admit
+                const kspir_database* db = pending.front().db;
+                const size_t per = query_footprint_bytes(db->r, db->n1);
+                size_t cap = budget / (per ? per : 1);
+                if (cap < 1) cap = 1;
+                if (cap > (size_t)max_batch) cap = (size_t)max_batch;
+                take.clear();
+                for (auto it = pending.begin(); it != pending.end() && take.size() < cap;) {
+                    if (it->db == db) {
+                        take.push_back(*it);
+                        it = pending.erase(it);
+                    } else {
+                        ++it;
+                    }
+                }
+            }
+            const int nq = (int)take.size();
+            keys.resize(nq);
+            for (int i = 0; i < nq; ++i) {
+                std::memcpy(qh + (size_t)i * 4 * kN, take[i].query, 4 * kN * 8);
+                keys[i] = take[i].keys;
+            }
+            const int rc = kspir_answer_batch(ctx, take[0].db, keys.data(), nq, qh, rh);
+            const auto t1 = std::chrono::steady_clock::now();
+            {
+                std::lock_guard<std::mutex> lock(mu);
+                for (int i = 0; i < nq; ++i) {
+                    if (rc == KSPIR_OK) std::memcpy(take[i].response, rh + (size_t)i * 2 * kN, 2 * kN * 8);
+                    done[take[i].id] = Done{rc, std::chrono::duration<double, std::milli>(t1 - take[i].t0).count()};
+                }
+                ++batches;
+                answered += (uint64_t)nq;
+            }
+            cv_done.notify_all();
+        }
+    }
+};
+
+extern "C" {
+
+int kspir_query_footprint_bytes(int32_t r, int32_t n1, size_t* bytes) {
+    if (!bytes || !pow2(r) || r > kN || !pow2(n1) || n1 < 32 || n1 > kN / 2) return fail(KSPIR_EINVAL, "bad shape");
+    *bytes = query_footprint_bytes(r, n1);
+    return KSPIR_OK;
+}
+
+int kspir_server_create(kspir_context* c, int32_t max_batch, size_t workspace_budget_bytes, kspir_server** out) {
+    if (!c || !out || max_batch < 1) return fail(KSPIR_EINVAL, "bad server arguments");
+    HIP_OK(hipSetDevice(c->device));
+    kspir_server* s = new kspir_server();
+    s->ctx = c;
+    s->max_batch = max_batch;
+    if (workspace_budget_bytes == 0) {
+        size_t fr = 0, tot = 0;
+        hipError_t e = hipMemGetInfo(&fr, &tot);
+        if (e != hipSuccess) { delete s; return fail_hip("hipMemGetInfo", e); }
+        workspace_budget_bytes = fr / 2;
+    }
+    s->budget = workspace_budget_bytes;
+    hipError_t e = hipHostMalloc(&s->qh, (size_t)max_batch * 4 * kN * 8, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc(&s->rh, (size_t)max_batch * 2 * kN * 8, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        if (s->qh) hipHostFree(s->qh);
+        delete s;
+        return fail_hip("hipHostMalloc(server staging)", e);
+    }
+    s->worker = std::thread([s] {
+        hipSetDevice(s->ctx->device);
+        s->run();
+    });
+    *out = s;
+    return KSPIR_OK;
+}
+
+int kspir_server_submit(kspir_server* s, const kspir_database* db, const kspir_keys* keys, const uint64_t* query,
+                        uint64_t* response, uint64_t* ticket) {
+    if (!s || !query || !response || !ticket) return fail(KSPIR_EINVAL, "null argument");
+    int rc = check_pair(s->ctx, db, keys);
+    if (rc) return rc;
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        if (s->stop) return fail(KSPIR_EINVAL, "server is stopping");
+        *ticket = s->next_id++;
+        s->pending.push_back(kspir_server::Req{*ticket, db, keys, query, response, std::chrono::steady_clock::now()});
+    }
+    s->cv_work.notify_one();
+    return KSPIR_OK;
+}
+
+int kspir_server_wait(kspir_server* s, uint64_t ticket, double* latency_ms) {
+    if (!s) return fail(KSPIR_EINVAL, "null server");
+    std::unique_lock<std::mutex> lock(s->mu);
+    if (ticket == 0 || ticket >= s->next_id) return fail(KSPIR_EINVAL, "unknown ticket");
+    s->cv_done.wait(lock, [&] { return s->done.count(ticket) != 0; });
+    const kspir_server::Done d = s->done[ticket];
+    s->done.erase(ticket);
+    if (latency_ms) *latency_ms = d.ms;
+    if (d.rc != KSPIR_OK) return fail(d.rc, "batch answer failed in the serving loop");
+    return KSPIR_OK;
+}
+
+int kspir_server_stats(kspir_server* s, uint64_t* batches, uint64_t* answered, uint64_t* pending) {
+    if (!s) return fail(KSPIR_EINVAL, "null server");
+    std::lock_guard<std::mutex> lock(s->mu);
+    if (batches) *batches = s->batches;
+    if (answered) *answered = s->answered;
+    if (pending) *pending = s->pending.size();
+    return KSPIR_OK;
+}
+
+void kspir_server_destroy(kspir_server* s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    hipHostFree(s->qh);
+    hipHostFree(s->rh);
+    delete s;
 }
 
 int kspir_test_ntt(kspir_context* c, uint32_t* polys, int count, int prime, int inverse) {

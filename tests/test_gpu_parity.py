@@ -247,6 +247,50 @@ def test_database_preprocessing_on_gpu(ctx, oracle, r, n1):
         d.close()
 
 
+def test_serving_loop_mixed_databases(ctx, oracle):
+    """queries for two databases submitted interleaved to the serving loop
+    (batch cap 3, a budget that admits two queries of the larger shape) come
+    back equal to the same queries answered alone, and equal to the oracle"""
+    import kspir_amd
+    say("[gpu] serving loop, two databases")
+    shapes = [(2, 128, 61), (4, 64, 62)]
+    dbs, clients, want = [], [], []
+    for r, n1, seed in shapes:
+        dbw = oracle.gen_db(r, n1, seed=seed)
+        perms = oracle.perm_matrix(n1)
+        db = ctx.database(dbw, r, n1)
+        cl = [oracle.gen_client(r, n1, seed * 10 + c, (13 * c) % (N // 2), c % r) for c in range(4)]
+        ks = [ctx.keys(r, n1, x["baby"], x["giant"], x["packing"], x["rgsw"], perms) for x in cl]
+        dbs.append((db, dbw, r, n1, perms))
+        clients.append((cl, ks))
+        want.append([ctx.answer(db, k, x["query"]) for x, k in zip(cl, ks)])
+    w0 = oracle.answer(r=shapes[0][0], n1=shapes[0][1], db=dbs[0][1], query=clients[0][0][0]["query"],
+                       rgsw=clients[0][0][0]["rgsw"], baby=clients[0][0][0]["baby"], giant=clients[0][0][0]["giant"],
+                       packing=clients[0][0][0]["packing"], perms=dbs[0][4])
+    assert np.array_equal(want[0][0], w0)
+    foot = max(kspir_amd.query_footprint(r, n1) for r, n1, _ in shapes)
+    srv = kspir_amd.Server(ctx, max_batch=3, budget_bytes=2 * foot)
+    try:
+        tickets = []
+        for c in range(4):
+            for g in range(2):
+                cl, ks = clients[g]
+                tickets.append((g, c, srv.submit(dbs[g][0], ks[c], cl[c]["query"])))
+        for g, c, t in tickets:
+            resp, ms = srv.wait(t)
+            assert ms > 0
+            assert np.array_equal(resp, want[g][c]), (g, c)
+        st = srv.stats()
+        assert st["answered"] == 8 and st["pending"] == 0 and st["batches"] >= 4
+    finally:
+        srv.close()
+    for db, *_ in dbs:
+        db.close()
+    for cl, ks in clients:
+        for k in ks:
+            k.close()
+
+
 def test_cpp_dropin_matches_oracle(oracle, tmp_path):
     """tests/native/dropin_test calls matrix_vector_mul_bsgs_rns_crt_large,
     externalProduct and packingRLWEs through kspir_pir.h exactly as
