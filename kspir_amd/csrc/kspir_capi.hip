@@ -398,9 +398,12 @@ int enqueue_answer(kspir_context* c, const kspir_database* db, const KeyPtrs* kp
     // baby steps (twosteps.cpp:1206-1214, evalAutoSRNSSmallKeys)
     hipLaunchKernelGGL(k_baby_prep, dim3(16, nq), dim3(kNttThreads), 0, st, q_dev, c->D0, c->bhat0, c->ctw, n1, S, T);
     LAUNCH_OK("k_baby_prep");
+    // first half: rotations 1 .. N1/2-1 from the query's digits, the last of
+    // them (the intermediate) also kept un-switched; second half: rotations
+    // N1/2 .. N1-1 from the intermediate's digits
     if (n1h - 1 > 0) {
         hipLaunchKernelGGL(k_baby_rot, dim3((n1h - 1) * 2 * nq), dim3(kNttThreads), 0, st, c->D0, c->bhat0, kp,
-                           c->ctw, c->mid, n1h - 1, nq, 1, n1h - 2, S.D, S.bhat, S, T);
+                           c->ctw, c->mid, n1h - 1, 0, nq, 1, n1h - 2, S.D, S.bhat, S, T);
         LAUNCH_OK("k_baby_rot(first)");
     }
     hipLaunchKernelGGL(k_baby_mid_coef, dim3(3, nq), dim3(kNttThreads), 0, st, c->mid, c->icoef, S, T);
@@ -408,7 +411,7 @@ int enqueue_answer(kspir_context* c, const kspir_database* db, const KeyPtrs* kp
     hipLaunchKernelGGL(k_baby_mid_digits, dim3(9, nq), dim3(kNttThreads), 0, st, c->icoef, c->D1, S, T);
     LAUNCH_OK("k_baby_mid_digits");
     hipLaunchKernelGGL(k_baby_rot, dim3(n1h * 2 * nq), dim3(kNttThreads), 0, st, c->D1, c->mid + 3 * kN, kp, c->ctw,
-                       (uint32_t*)nullptr, n1h, nq, n1h, -1, S.D, S.mid, S, T);
+                       (uint32_t*)nullptr, n1h, 0, nq, n1h, -1, S.D, S.mid, S, T);
     LAUNCH_OK("k_baby_rot(second)");
     // database pass (crt.cpp:643-768): int8-MFMA limb GEMM for a batch,
     // u64-accumulating VALU stream for a single query

@@ -212,27 +212,10 @@ __host__ __device__ __forceinline__ void load_pass_twiddles(const uint2* tw2, in
 
 __host__ __device__ __forceinline__ uint32_t umin_hd(uint32_t a, uint32_t b) { return a < b ? a : b; }
 
-// 64-bit a * b and a * b + c.  On the GPU each is one v_mad_u64_u32 (full
-// rate on gfx950); only the low words are used by the callers, so the high
-// word of c is whatever the previous product left there (no zeroing moves).
-__host__ __device__ __forceinline__ uint64_t mul_wide(uint32_t a, uint32_t b) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    uint64_t r;
-    asm("v_mad_u64_u32 %0, vcc, %1, %2, 0" : "=v"(r) : "v"(a), "v"(b) : "vcc");
-    return r;
-#else
-    return (uint64_t)a * b;
-#endif
-}
-__host__ __device__ __forceinline__ uint64_t mad_wide(uint32_t a, uint32_t b, uint64_t c) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    uint64_t r;
-    asm("v_mad_u64_u32 %0, vcc, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c) : "vcc");
-    return r;
-#else
-    return (uint64_t)a * b + c;
-#endif
-}
+// 64-bit a * b and a * b + c (the callers use the low words, which the
+// compiler forms with 32-bit multiplies and adds)
+__host__ __device__ __forceinline__ uint64_t mul_wide(uint32_t a, uint32_t b) { return (uint64_t)a * b; }
+__host__ __device__ __forceinline__ uint64_t mad_wide(uint32_t a, uint32_t b, uint64_t c) { return (uint64_t)a * b + c; }
 
 // x * w mod q up to one extra q: result in [0, 2q) for any x < 2^32 (Shoup,
 // without the final correction), w < q, wp = floor(w * 2^32 / q):

@@ -308,13 +308,75 @@ __device__ __forceinline__ int xcd_job(int bid, int total) {
 // The three digit planes D[0..2][p] of a prime are staged in LDS with
 // coalesced loads and permuted from there.  Jobs: nrot * 2 * nq, rotation
 // fastest, then a|b, then query (one workgroup each, 1-D grid).
+// one rotation's key switch for prime p, after the digit planes of p are
+// staged in s[0..2]: acc[e] = -sum_j D[j][p][perm(x)] * key[j][ab][p][x]
+// (+ bhat[p][perm(x)] for the b part)
+__device__ __forceinline__ void baby_key_mult(uint32_t* acc, const u32x4 (&kv)[3][4], const int* ps,
+                                              uint32_t (*s)[kPadN], const uint32_t* bh, int ab, const PrimeTab& P) {
+    constexpr int PER = kN / kNttThreads;
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int g = e >> 2, i = e & 3;
+        uint64_t t = 0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) t += (uint64_t)s[j][lds_pad(ps[e])] * kv[j][g][i];
+        uint32_t r = neg_mod(mont_reduce(t, P.q, P.mont_qinv), P.q);
+        if (ab) r = add_mod(r, bh[e], P.q);
+        acc[e] = r;
+    }
+}
+
+// modulus switch of one rotation and its store: c = INTT_bs(acc_bs),
+// out_p = (acc_p - NTT_p(c)) * bsMod^-1 -> ctw[k][ab][p]
+__device__ __forceinline__ void baby_switch_store(const uint32_t (&acc)[3][kN / kNttThreads], uint32_t (*s)[kPadN],
+                                                  uint32_t* ctw, int k, int ab, int tid, const Tabs& T) {
+    constexpr int PER = kN / kNttThreads;
+    const int x0 = 4 * tid;
+#pragma unroll
+    for (int e = 0; e < PER; ++e) s[0][lds_pad((e >> 2) * 1024 + x0 + (e & 3))] = acc[2][e];
+    __syncthreads();
+    // c in [0, bsMod) is below q1 and q2: the same words are c mod q1, q2
+    ntt_inv_r16(s[0], T.p[2]);
+#pragma unroll
+    for (int e = 0; e < PER; ++e) s[1][lds_pad(tid + e * kNttThreads)] = s[0][lds_pad(tid + e * kNttThreads)];
+    __syncthreads();
+    ntt_fwd_r16x2(s[0], T.p[0], s[1], T.p[1]);
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        uint32_t o1[4], o2[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int e = 4 * g + i, x = g * 1024 + x0 + i;
+            o1[i] = mul_shoup(sub_mod(acc[0][e], s[0][lds_pad(x)], kQ1), T.bsinv_q[0], T.bsinv_q_sh[0], kQ1);
+            o2[i] = mul_shoup(sub_mod(acc[1][e], s[1][lds_pad(x)], kQ2), T.bsinv_q[1], T.bsinv_q_sh[1], kQ2);
+        }
+        *reinterpret_cast<uint4*>(ctw + ((size_t)(k * 2 + ab) * 2 + 0) * kN + g * 1024 + x0) = make_uint4(o1[0], o1[1], o1[2], o1[3]);
+        *reinterpret_cast<uint4*>(ctw + ((size_t)(k * 2 + ab) * 2 + 1) * kN + g * 1024 + x0) = make_uint4(o2[0], o2[1], o2[2], o2[3]);
+    }
+    __syncthreads();
+}
+
+// Baby-step rotations, one workgroup per (rotation, a|b, query); rotation
+// rot applies tau_{5^(rot+1)} with key index rot and permutation row rot+1
+// (twosteps.cpp:937-1204):
+//   acc_p = -sum_j D[j][p][perm(s)] * key[rot][j][ab][p][s]       (p = q1,q2,bs)
+//   b-part adds bhat[p][perm(s)]  (tau(b) in the NTT domain)
+//   c = INTT_bs(acc_bs) in [0,bsMod);  out_p = (acc_p - NTT_p(c)) * bsMod^-1
+// Rotations rot_base .. rot_base+nrot-1 of one half, output to
+// ctw[k_base + rot]; rotation `mid` also stores its un-switched NTT-domain
+// accumulators in mid_out[ab][p] (evalAutoSRNSFirst's intermediate).
+// The three digit planes of a prime are staged in LDS with coalesced loads
+// and permuted from there; each thread owns 4 consecutive slots per
+// 1024-slot group (e = 4g + i), so keys, permutation rows and outputs move
+// as 16-byte (8-byte) vectors, the keys streamed past the caches.  Jobs:
+// nrot * 2 * nq, rotation fastest, then a|b, then query (1-D grid).
 __global__ void __launch_bounds__(kNttThreads) k_baby_rot(const uint32_t* __restrict__ D,
-        const uint32_t* __restrict__ bhat, const KeyPtrs* __restrict__ kp,
-        uint32_t* __restrict__ ctw, uint32_t* __restrict__ mid_out, int nrot, int nq, int k_base, int mid,
-        size_t D_stride, size_t bhat_stride, QS S, Tabs T) {
+        const uint32_t* __restrict__ bhat, const KeyPtrs* __restrict__ kp, uint32_t* __restrict__ ctw,
+        uint32_t* __restrict__ mid_out, int nrot, int rot_base, int nq, int k_base, int mid, size_t D_stride,
+        size_t bhat_stride, QS S, Tabs T) {
     __shared__ uint32_t s[3][kPadN];
     const int L = xcd_job(blockIdx.x, nrot * 2 * nq);
-    const int rot = L % nrot, ab = (L / nrot) & 1;
+    const int rot = rot_base + L % nrot, ab = (L / nrot) & 1;
     const size_t qi = L / (2 * nrot);
     D += qi * D_stride;
     bhat += qi * bhat_stride;
@@ -324,8 +386,6 @@ __global__ void __launch_bounds__(kNttThreads) k_baby_rot(const uint32_t* __rest
     const uint32_t* key = kp[qi].baby + (size_t)rot * 3 * 2 * 3 * kN;
     const int tid = threadIdx.x;
     constexpr int PER = kN / kNttThreads;
-    // a thread owns 4 consecutive slots in each 1024-slot group g (e = 4g + i),
-    // so keys, permutation rows and outputs move as 16-byte (8-byte) vectors
     const int x0 = 4 * tid;
     int ps[PER];
     uint32_t acc[3][PER];
@@ -345,7 +405,8 @@ __global__ void __launch_bounds__(kNttThreads) k_baby_rot(const uint32_t* __rest
         for (int j = 0; j < 3; ++j)
 #pragma unroll
             for (int g = 0; g < 4; ++g)
-                kv[j][g] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(key + ((j * 2 + ab) * 3 + p) * kN + g * 1024 + x0));
+                kv[j][g] = __builtin_nontemporal_load(
+                    reinterpret_cast<const u32x4*>(key + ((j * 2 + ab) * 3 + p) * kN + g * 1024 + x0));
         uint32_t bh[PER];
         if (ab) {
 #pragma unroll
@@ -358,45 +419,14 @@ __global__ void __launch_bounds__(kNttThreads) k_baby_rot(const uint32_t* __rest
             for (int j = 0; j < 3; ++j) s[j][lds_pad(x)] = D[(j * 3 + p) * kN + x];
         }
         __syncthreads();
-        const uint32_t q = T.p[p].q;
-#pragma unroll
-        for (int e = 0; e < PER; ++e) {
-            const int g = e >> 2, i = e & 3;
-            uint64_t t = 0;
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                const uint32_t kw = kv[j][g][i];
-                t += (uint64_t)s[j][lds_pad(ps[e])] * kw;
-            }
-            uint32_t r = neg_mod(mont_reduce(t, q, T.p[p].mont_qinv), q);
-            if (ab) r = add_mod(r, bh[e], q);
-            if (rot == mid) mid_out[(ab * 3 + p) * kN + g * 1024 + x0 + i] = r;
-            acc[p][e] = r;
-        }
+        baby_key_mult(acc[p], kv, ps, s, bh, ab, T.p[p]);
         __syncthreads();
-    }
+        if (rot == mid) {
 #pragma unroll
-    for (int e = 0; e < PER; ++e) s[0][lds_pad((e >> 2) * 1024 + x0 + (e & 3))] = acc[2][e];
-    __syncthreads();
-    // c in [0, bsMod) is below q1 and q2: the same words are c mod q1, q2
-    ntt_inv_r16(s[0], T.p[2]);
-#pragma unroll
-    for (int e = 0; e < PER; ++e) s[1][lds_pad(tid + e * kNttThreads)] = s[0][lds_pad(tid + e * kNttThreads)];
-    __syncthreads();
-    ntt_fwd_r16x2(s[0], T.p[0], s[1], T.p[1]);
-    const int k = k_base + rot;
-#pragma unroll
-    for (int g = 0; g < 4; ++g) {
-        uint32_t o1[4], o2[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int e = 4 * g + i, x = g * 1024 + x0 + i;
-            o1[i] = mul_shoup(sub_mod(acc[0][e], s[0][lds_pad(x)], kQ1), T.bsinv_q[0], T.bsinv_q_sh[0], kQ1);
-            o2[i] = mul_shoup(sub_mod(acc[1][e], s[1][lds_pad(x)], kQ2), T.bsinv_q[1], T.bsinv_q_sh[1], kQ2);
+            for (int e = 0; e < PER; ++e) mid_out[(ab * 3 + p) * kN + (e >> 2) * 1024 + x0 + (e & 3)] = acc[p][e];
         }
-        *reinterpret_cast<uint4*>(ctw + ((size_t)(k * 2 + ab) * 2 + 0) * kN + g * 1024 + x0) = make_uint4(o1[0], o1[1], o1[2], o1[3]);
-        *reinterpret_cast<uint4*>(ctw + ((size_t)(k * 2 + ab) * 2 + 1) * kN + g * 1024 + x0) = make_uint4(o2[0], o2[1], o2[2], o2[3]);
     }
+    baby_switch_store(acc, s, ctw, k_base + rot, ab, tid, T);
 }
 
 // second half, step 1: INTT of the intermediate's a in prime p (blockIdx.x)
