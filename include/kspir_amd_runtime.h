@@ -26,7 +26,10 @@ int kspir_device_count(int* count);
  *   "fused_min_batch" batches of at least this many queries run the giant
  *                     steps and first-dimension sum as one workgroup per
  *                     (query, block) (default 16); smaller ones spread them
- *                     over one workgroup per (block, row, prime) */
+ *                     over one workgroup per (block, row, prime)
+ *   "fused_pack"      1 (default): for batches of at least fused_min_batch
+ *                     queries each packing level is one kernel with a
+ *                     workgroup per (pair, query); 0: three kernels per level */
 int kspir_context_set(kspir_context* ctx, const char* key, int64_t value);
 
 /* device bytes one query of shape (r, N1) adds to a batch (workspace and

@@ -107,6 +107,7 @@ struct kspir_context {
     int mfma_min_batch = 2;         // batches of at least this many queries use the MFMA pass
     int mfma_max_chunk = 1 << 30;   // cap on queries per MFMA database sweep (LDS also caps it)
     int fused_min_batch = 16;       // batches of at least this many queries use k_giant_fused
+    int fused_pack = 1;             // one k_pack_level per packing level (0: three kernels per level)
 };
 
 struct kspir_database {
@@ -371,14 +372,19 @@ int enqueue_packing(kspir_context* c, const KeyPtrs* kp, int r, int nq, hipStrea
         const int li = ilog2(i);
         const uint32_t g = (uint32_t)(kN / i + 1);
         const uint32_t ginv = (uint32_t)powmod(g, kN - 1, 2 * kN);   // odd g has order | N in (Z/2N)*
-        hipLaunchKernelGGL(k_pack_coef, dim3(i, 2 * nq), dim3(kNttThreads), 0, st, cur, c->pcoef, i, S, T);
-        LAUNCH_OK("k_pack_coef");
-        hipLaunchKernelGGL(k_pack_digit, dim3(i, 4, 2 * nq), dim3(kNttThreads), 0, st, c->pcoef, kp, li, c->ppart,
-                           ginv, S, T);
-        LAUNCH_OK("k_pack_digit");
-        hipLaunchKernelGGL(k_pack_combine, dim3(i, 2, 2 * nq), dim3(kNttThreads), 0, st, cur, c->ppart, nxt, i, g,
-                           S, T);
-        LAUNCH_OK("k_pack_combine");
+        if (c->fused_pack && nq >= c->fused_min_batch) {
+            hipLaunchKernelGGL(k_pack_level, dim3(i * nq), dim3(512), 0, st, cur, kp, li, nxt, i, g, ginv, S, T);
+            LAUNCH_OK("k_pack_level");
+        } else {
+            hipLaunchKernelGGL(k_pack_coef, dim3(i, 2 * nq), dim3(kNttThreads), 0, st, cur, c->pcoef, i, S, T);
+            LAUNCH_OK("k_pack_coef");
+            hipLaunchKernelGGL(k_pack_digit, dim3(i, 4, 2 * nq), dim3(kNttThreads), 0, st, c->pcoef, kp, li,
+                               c->ppart, ginv, S, T);
+            LAUNCH_OK("k_pack_digit");
+            hipLaunchKernelGGL(k_pack_combine, dim3(i, 2, 2 * nq), dim3(kNttThreads), 0, st, cur, c->ppart, nxt, i,
+                               g, S, T);
+            LAUNCH_OK("k_pack_combine");
+        }
         uint32_t* t = cur; cur = nxt; nxt = t;
     }
     *out = cur;
@@ -434,9 +440,12 @@ int enqueue_answer(kspir_context* c, const kspir_database* db, const KeyPtrs* kp
                                c->gks, n1, n2, S, T);
             LAUNCH_OK("k_giant_ks");
         }
-        hipLaunchKernelGGL(k_ext_coef, dim3(r, 2, 2 * nq), dim3(kNttThreads), 0, st, c->dbout, c->gks, c->fd,
-                           c->ecoef, n2, S, T);
-        LAUNCH_OK("k_ext_coef");
+        // the sum over giant rows on every lane of the GPU, then the INTTs
+        hipLaunchKernelGGL(k_fd_sum, dim3(grid1d((size_t)nq * r * 4 * kN)), dim3(256), 0, st, c->dbout, c->gks, c->fd,
+                           r, n2, nq, S, T);
+        LAUNCH_OK("k_fd_sum");
+        hipLaunchKernelGGL(k_fd_intt, dim3(r, 2, 2 * nq), dim3(kNttThreads), 0, st, c->fd, c->ecoef, S, T);
+        LAUNCH_OK("k_fd_intt");
     }
     if (fd_u64) {
         hipLaunchKernelGGL(k_rns_to_u64, dim3(grid1d((size_t)r * 2 * kN)), dim3(256), 0, st, c->fd, fd_u64, r, T);
@@ -970,6 +979,8 @@ int kspir_context_set(kspir_context* c, const char* key, int64_t value) {
     } else if (k == "fused_min_batch") {
         if (value < 1) return fail(KSPIR_EINVAL, "fused_min_batch must be >= 1");
         c->fused_min_batch = (int)value;
+    } else if (k == "fused_pack") {
+        c->fused_pack = value != 0;
     } else {
         return fail(KSPIR_EINVAL, "unknown option " + k);
     }

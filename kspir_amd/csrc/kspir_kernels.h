@@ -797,6 +797,8 @@ __global__ void __launch_bounds__(kNttThreads) k_ext_coef(const uint32_t* __rest
     const uint32_t q = T.p[p].q;
     for (int x = threadIdx.x; x < kN; x += kNttThreads) {
         uint32_t v = dbout[(((size_t)blk * n2 + 0) * 4 + ab * 2 + p) * kN + x];
+        // independent loads: unrolled so that they are in flight together
+#pragma unroll 16
         for (int c = 1; c < n2; ++c) v = add_mod(v, gks[(((size_t)blk * n2 + c) * 4 + ab * 2 + p) * kN + x], q);
         s[lds_pad(x)] = v;
         if (fd) fd[(((size_t)blk * 2 + ab) * 2 + p) * kN + x] = v;
@@ -804,6 +806,42 @@ __global__ void __launch_bounds__(kNttThreads) k_ext_coef(const uint32_t* __rest
     __syncthreads();
     ntt_inv_r16(s, T.p[p]);
     for (int x = threadIdx.x; x < kN; x += kNttThreads) ecoef[(((size_t)blk * 2 + ab) * 2 + p) * kN + x] = s[lds_pad(x)];
+}
+
+// first-dimension sum for few queries, one lane per word (grid-stride):
+// fd[blk][ab][p][x] =
+// dbout row 0 + sum over c >= 1 of the key-switched rows gks[blk][c]
+// (NTT form; the sum k_ext_coef forms, spread over the whole GPU)
+__global__ void __launch_bounds__(256) k_fd_sum(const uint32_t* __restrict__ dbout, const uint32_t* __restrict__ gks,
+                                                uint32_t* __restrict__ fd, int r, int n2, int nq, QS S, Tabs T) {
+    const size_t per = (size_t)r * 4 * kN, total = per * nq;
+    for (size_t gidx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; gidx < total;
+         gidx += (size_t)gridDim.x * blockDim.x) {
+        const size_t qi = gidx / per, idx = gidx % per;
+        const size_t blk = idx / (4 * kN), w = idx % (4 * kN);   // w = (ab * 2 + p) * N + x
+        const uint32_t q = T.p[(w / kN) & 1].q;
+        const uint32_t* d = dbout + qi * S.dbout + blk * n2 * 4 * kN + w;
+        const uint32_t* gk = gks + qi * S.gks + blk * n2 * 4 * kN + w;
+        uint32_t v = d[0];
+#pragma unroll 16
+        for (int c = 1; c < n2; ++c) v = add_mod(v, gk[(size_t)c * 4 * kN], q);
+        fd[qi * S.fd + idx] = v;
+    }
+}
+
+// INTT of each first-dimension plane: fd -> ecoef, grid (r, 2 ab, 2 p * nq)
+__global__ void __launch_bounds__(kNttThreads) k_fd_intt(const uint32_t* __restrict__ fd, uint32_t* __restrict__ ecoef,
+                                                         QS S, Tabs T) {
+    __shared__ uint32_t s[kPadN];
+    const int blk = blockIdx.x, ab = blockIdx.y, p = blockIdx.z & 1;
+    const size_t qi = blockIdx.z >> 1;
+    const size_t off = (((size_t)blk * 2 + ab) * 2 + p) * kN;
+    const uint32_t* src = fd + qi * S.fd + off;
+    for (int x = threadIdx.x; x < kN; x += kNttThreads) s[lds_pad(x)] = src[x];
+    __syncthreads();
+    ntt_inv_r16(s, T.p[p]);
+    uint32_t* dst = ecoef + qi * S.ecoef + off;
+    for (int x = threadIdx.x; x < kN; x += kNttThreads) dst[x] = s[lds_pad(x)];
 }
 
 // Batched giant steps and first-dimension sum, one block of one query per
@@ -1024,6 +1062,124 @@ __global__ void __launch_bounds__(kNttThreads) k_pack_digit(const uint32_t* __re
         const uint64_t d = s[lds_pad(x)];
         o[(0 * 2 + p) * kN + x] = mont_reduce(d * pkey[((k * 2 + 0) * 2 + p) * kN + x], P.q, P.mont_qinv);
         o[(1 * 2 + p) * kN + x] = mont_reduce(d * pkey[((k * 2 + 1) * 2 + p) * kN + x], P.q, P.mont_qinv);
+    }
+}
+
+// One whole packing level (answer.cpp:800-835) per (pair j, query): the
+// work of k_pack_coef, k_pack_digit and k_pack_combine in one workgroup of
+// 512 threads, the two 256-thread halves running the transforms of the two
+// primes.  a_j - a_{i+j} X^i is INTT'd in LDS, tau_g of it is read back from
+// LDS and decomposed (ell 4, PP 0, Bg 2^14, answer.cpp:576-648), the digits
+// are NTT'd two at a time and their key products summed in registers; then
+// convertTwoRLWEs (answer.cpp:720-739) writes next[j].  Same words as the
+// three kernels (sums of Montgomery products are reduced once, an exact
+// identity mod q).  Scheduling barriers keep each phase's loads in that
+// phase.  1-D grid of i * nq workgroups.
+__global__ void __launch_bounds__(512) k_pack_level(const uint32_t* __restrict__ ext,
+        const KeyPtrs* __restrict__ kp, int li, uint32_t* __restrict__ next, int i, uint32_t g, uint32_t ginv, QS S,
+        Tabs T) {
+    __shared__ uint32_t s[6][kPadN];
+    constexpr int PER = kN / 512;
+    const int j = blockIdx.x % i;
+    const size_t qi = blockIdx.x / i;
+    ext += qi * S.ext;
+    next += qi * S.ext;
+    const uint32_t* pkey = kp[qi].packing + (size_t)li * 4 * 2 * 2 * kN;   // [k][ab][p][N]
+    const int tid = threadIdx.x;
+    const int half = __builtin_amdgcn_readfirstlane(tid >> 8), ht = tid & 255;
+    const uint32_t* ej = ext + (size_t)j * 4 * kN;         // [ab][p][N]
+    const uint32_t* ei = ext + (size_t)(i + j) * 4 * kN;
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int x = tid + e * 512;
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const PrimeTab& P = T.p[p];
+            s[p][lds_pad(x)] = sub_mod(ej[p * kN + x], mont_mul(ei[p * kN + x], xpow_m(x, i, P, T), P), P.q);
+        }
+    }
+    __syncthreads();
+    __builtin_amdgcn_sched_barrier(0);
+    ntt_inv_r16(s[half], T.p[half], ht);
+    __builtin_amdgcn_sched_barrier(0);
+    // digits of tau_g: 0 and 1 into planes 2..5 now, 2 and 3 held for round two
+    uint32_t dg2[4][PER];   // (digit 2 | 3) x (q1 | q2) at [p * 2 + (k - 2)]
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int x = tid + e * 512;
+        const uint32_t i0 = ((uint32_t)x * ginv) & (2 * kN - 1);
+        const bool neg = i0 >= (uint32_t)kN;
+        const int ix = neg ? (int)i0 - kN : (int)i0;
+        uint64_t v = crt_q(s[0][lds_pad(ix)], s[1][lds_pad(ix)], T);
+        if (neg) v = v ? kQ - v : 0;
+        int32_t d[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) d[k] = tfhe_digit<14, 0, 4>(v, k);
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            s[2 + 2 * p][lds_pad(x)] = digit_residue(d[0], T.p[p].q);
+            s[3 + 2 * p][lds_pad(x)] = digit_residue(d[1], T.p[p].q);
+            dg2[p * 2 + 0][e] = digit_residue(d[2], T.p[p].q);
+            dg2[p * 2 + 1][e] = digit_residue(d[3], T.p[p].q);
+        }
+    }
+    __syncthreads();
+    __builtin_amdgcn_sched_barrier(0);
+    uint32_t acc[2][2][PER];   // [ab][p][e]: sum_k digit_k * key_k
+#pragma unroll
+    for (int round = 0; round < 2; ++round) {
+        if (round == 1) {
+#pragma unroll
+            for (int e = 0; e < PER; ++e) {
+                const int x = tid + e * 512;
+#pragma unroll
+                for (int t = 0; t < 4; ++t) s[2 + t][lds_pad(x)] = dg2[t][e];
+            }
+            __syncthreads();
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        ntt_fwd_r16x2_same(s[2 + 2 * half], s[3 + 2 * half], T.p[half], ht);
+        __builtin_amdgcn_sched_barrier(0);
+        const int k0 = 2 * round;
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int x = tid + e * 512;
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const PrimeTab& P = T.p[p];
+                const uint64_t d0 = s[2 + 2 * p][lds_pad(x)], d1 = s[3 + 2 * p][lds_pad(x)];
+#pragma unroll
+                for (int ab = 0; ab < 2; ++ab) {
+                    const uint64_t t = d0 * pkey[((k0 * 2 + ab) * 2 + p) * kN + x] +
+                                       d1 * pkey[(((k0 + 1) * 2 + ab) * 2 + p) * kN + x];
+                    const uint32_t m = mont_reduce(t, P.q, P.mont_qinv);
+                    acc[ab][p][e] = round ? add_mod(acc[ab][p][e], m, P.q) : m;
+                }
+            }
+        }
+        __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    // convertTwoRLWEs: c' = ext[i+j] * X^i + ext[j]; new = -sum + c' (+ tau_g of
+    // ext[j].b - ext[i+j].b X^i for b)
+    uint32_t* o = next + (size_t)j * 4 * kN;
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int x = tid + e * 512;
+        const int ps = perm_slot(x, g, T);
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const PrimeTab& P = T.p[p];
+#pragma unroll
+            for (int ab = 0; ab < 2; ++ab) {
+                const uint32_t* ejp = ej + (ab * 2 + p) * kN;
+                const uint32_t* eip = ei + (ab * 2 + p) * kN;
+                const uint32_t c = add_mod(mont_mul(eip[x], xpow_m(x, i, P, T), P), ejp[x], P.q);
+                uint32_t v = neg_mod(acc[ab][p][e], P.q);
+                if (ab) v = add_mod(v, sub_mod(ejp[ps], mont_mul(eip[ps], xpow_m(ps, i, P, T), P), P.q), P.q);
+                o[(ab * 2 + p) * kN + x] = add_mod(v, c, P.q);
+            }
+        }
     }
 }
 

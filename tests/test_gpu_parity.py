@@ -291,6 +291,35 @@ def test_serving_loop_mixed_databases(ctx, oracle):
             k.close()
 
 
+def test_packing_paths_agree(ctx, oracle):
+    """the fused packing level (default) and the three-kernel level give the
+    same response words, single and batched, and equal the oracle"""
+    say("[gpu] packing paths r=8 n1=128")
+    r, n1 = 8, 128
+    db_words = oracle.gen_db(r, n1, seed=71)
+    perms = oracle.perm_matrix(n1)
+    cl = [oracle.gen_client(r, n1, 700 + c, (29 * c + 1) % (N // 2), (3 * c) % r) for c in range(3)]
+    db = ctx.database(db_words, r, n1)
+    keys = [ctx.keys(r, n1, x["baby"], x["giant"], x["packing"], x["rgsw"], perms) for x in cl]
+    queries = np.stack([x["query"] for x in cl])
+    fused_b = ctx.answer_batch(db, keys, queries)
+    fused_1 = ctx.answer(db, keys[0], cl[0]["query"])
+    ctx.set("fused_pack", 0)
+    try:
+        plain_b = ctx.answer_batch(db, keys, queries)
+        plain_1 = ctx.answer(db, keys[0], cl[0]["query"])
+    finally:
+        ctx.set("fused_pack", 1)
+    assert np.array_equal(fused_b, plain_b)
+    assert np.array_equal(fused_1, plain_1) and np.array_equal(fused_1, fused_b[0])
+    want = oracle.answer(r=r, n1=n1, db=db_words, query=cl[0]["query"], rgsw=cl[0]["rgsw"], baby=cl[0]["baby"],
+                         giant=cl[0]["giant"], packing=cl[0]["packing"], perms=perms)
+    assert np.array_equal(fused_1, want)
+    for k in keys:
+        k.close()
+    db.close()
+
+
 def test_cpp_dropin_matches_oracle(oracle, tmp_path):
     """tests/native/dropin_test calls matrix_vector_mul_bsgs_rns_crt_large,
     externalProduct and packingRLWEs through kspir_pir.h exactly as
