@@ -149,6 +149,16 @@ int kspir_answer_inspect(kspir_context* ctx, const kspir_database* db, const ksp
                          const uint64_t* query, uint64_t* response, uint64_t* cipherbuf,
                          uint64_t* firstdim, uint64_t* ext);
 
+/* Client recover: decrypt_bsgs_total (twosteps.h:30, twosteps.cpp:388-418)
+ * for nq responses on the GPU.  secrets [nq][N] (the secret's coefficients
+ * mod crtMod), responses [nq][2][N] (coefficient form, as packingRLWEs
+ * returns them; not modified, where the reference turns its argument into
+ * NTT form), messages [nq][N]; host pointers.  r^-1 mod crtMod is computed
+ * for every power-of-two r (the reference's QInv table has no entry for
+ * r = 128 and 1024, where it multiplies by 0). */
+int kspir_decrypt_bsgs_total(kspir_context* ctx, int32_t r, int32_t nq, const uint64_t* secrets,
+                             const uint64_t* responses, uint64_t* messages);
+
 /* Serving loop (no counterpart in the reference, whose driver answers one
  * query at a time).  A worker thread answers submitted queries in batches:
  * whenever the GPU is free it takes the database of the oldest waiting query

@@ -41,7 +41,7 @@ def _load():
                  "kspir_time_answer_batch", "kspir_context_set", "kspir_time_copy",
                  "kspir_database_create_from_records", "kspir_database_tobsgsntt", "kspir_database_export",
                  "kspir_server_create", "kspir_server_submit", "kspir_server_wait", "kspir_server_stats",
-                 "kspir_query_footprint_bytes"):
+                 "kspir_query_footprint_bytes", "kspir_decrypt_bsgs_total"):
         getattr(L, name).restype = ctypes.c_int
     L.kspir_context_destroy.argtypes = [_vp]
     L.kspir_database_destroy.argtypes = [_vp]
@@ -63,7 +63,7 @@ EXPORTS = (
     "kspir_synchronize", "kspir_context_stream", "kspir_time_answer", "kspir_time_db_pass", "kspir_test_ntt",
     "kspir_time_copy", "kspir_database_create_from_records", "kspir_database_tobsgsntt", "kspir_database_export",
     "kspir_server_create", "kspir_server_submit", "kspir_server_wait", "kspir_server_stats", "kspir_server_destroy",
-    "kspir_query_footprint_bytes",
+    "kspir_query_footprint_bytes", "kspir_decrypt_bsgs_total",
 )
 
 
@@ -342,6 +342,15 @@ class Context:
         _check(lib.kspir_time_copy(self.h, _vp(dst_buf.ptr), _vp(src_buf.ptr), ctypes.c_size_t(nbytes), iters,
                                    ctypes.byref(ms)))
         return ms.value
+
+    def decrypt_bsgs_total(self, r, secrets, responses):
+        """recover on the GPU: secrets [nq][N], responses [nq][2][N] -> messages [nq][N]"""
+        sk = np.ascontiguousarray(secrets, dtype=np.uint64).reshape(-1, N)
+        rs = np.ascontiguousarray(responses, dtype=np.uint64).reshape(-1, 2, N)
+        assert sk.shape[0] == rs.shape[0]
+        out = np.zeros((sk.shape[0], N), dtype=np.uint64)
+        _check(lib.kspir_decrypt_bsgs_total(self.h, r, sk.shape[0], _p64(sk), _p64(rs), _p64(out)))
+        return out
 
     def test_ntt(self, polys, prime, inverse):
         x = np.ascontiguousarray(polys, dtype=np.uint32).copy()

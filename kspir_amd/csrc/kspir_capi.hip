@@ -1176,6 +1176,38 @@ struct kspir_server {
 
 extern "C" {
 
+int kspir_decrypt_bsgs_total(kspir_context* c, int32_t r, int32_t nq, const uint64_t* secrets,
+                             const uint64_t* responses, uint64_t* messages) {
+    if (!c || !secrets || !responses || !messages || nq < 1) return fail(KSPIR_EINVAL, "bad argument");
+    if (r < 1 || !pow2(r) || r > kN) return fail(KSPIR_EINVAL, "r must be a power of two <= N");
+    HIP_OK(hipSetDevice(c->device));
+    // r^-1 mod q_p (Q is odd, so every power of two is invertible)
+    uint2 qv[2];
+    for (int p = 0; p < 2; ++p) {
+        const u64 q = c->T.p[p].q;
+        const uint32_t inv = (uint32_t)invmod((u64)r % q, q);
+        qv[p] = make_uint2(inv, shoup(inv, (uint32_t)q));
+    }
+    const size_t sb = (size_t)nq * kN * 8, rb = (size_t)nq * 2 * kN * 8;
+    uint64_t *ds = nullptr, *dr = nullptr, *dm = nullptr;
+    hipError_t e = hipMalloc(&ds, sb);
+    if (e == hipSuccess) e = hipMalloc(&dr, rb);
+    if (e == hipSuccess) e = hipMalloc(&dm, sb);
+    if (e == hipSuccess) e = hipMemcpyAsync(ds, secrets, sb, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dr, responses, rb, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_decrypt, dim3(nq), dim3(512), 0, c->stream, ds, dr, dm, qv[0], qv[1], c->T);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(messages, dm, sb, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (ds) hipFree(ds);
+    if (dr) hipFree(dr);
+    if (dm) hipFree(dm);
+    if (e != hipSuccess) return fail_hip("decrypt_bsgs_total", e);
+    return KSPIR_OK;
+}
+
 int kspir_query_footprint_bytes(int32_t r, int32_t n1, size_t* bytes) {
     if (!bytes || !pow2(r) || r > kN || !pow2(n1) || n1 < 32 || n1 > kN / 2) return fail(KSPIR_EINVAL, "bad shape");
     *bytes = query_footprint_bytes(r, n1);

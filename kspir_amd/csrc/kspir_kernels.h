@@ -1293,6 +1293,55 @@ __global__ void __launch_bounds__(256) k_add_planes(uint32_t* __restrict__ a, co
     }
 }
 
+// Client recover, decrypt_bsgs_total (twosteps.cpp:388-418 with
+// encrypt.cpp:528-573 decrypt): m = (b - a*s) * r^-1 mod Q, rounded to
+// round(m / Delta) (m > Q/2: round((m - Q) / Delta) + p).  One workgroup of
+// 512 threads per response; each 256-thread half works in one prime.
+// secret, response: coefficient form mod Q (u64); msg: N values.
+// qinv_r: (r^-1 mod q_p, Shoup companion) for p = 0, 1.
+__global__ void __launch_bounds__(512) k_decrypt(const uint64_t* __restrict__ secret, const uint64_t* __restrict__ resp,
+                                                 uint64_t* __restrict__ msg, uint2 qinv0, uint2 qinv1, Tabs T) {
+    __shared__ uint32_t s[6][kPadN];     // per prime: a, b, s (planes 3p .. 3p+2)
+    const int tid = threadIdx.x;
+    const int half = __builtin_amdgcn_readfirstlane(tid >> 8), ht = tid & 255;
+    const uint64_t* a = resp + (size_t)blockIdx.x * 2 * kN;
+    const uint64_t* b = a + kN;
+    const uint64_t* sk = secret + (size_t)blockIdx.x * kN;
+    uint64_t* out = msg + (size_t)blockIdx.x * kN;
+    const PrimeTab& P = T.p[half];
+    uint32_t* sa = s[3 * half];
+    uint32_t* sb = s[3 * half + 1];
+    uint32_t* ss = s[3 * half + 2];
+    for (int x = ht; x < kN; x += kNttThreads) {
+        const uint64_t va = a[x], vb = b[x], vs = sk[x];
+        sa[lds_pad(x)] = half ? (uint32_t)(va % kQ2) : (uint32_t)(va % kQ1);
+        sb[lds_pad(x)] = half ? (uint32_t)(vb % kQ2) : (uint32_t)(vb % kQ1);
+        ss[lds_pad(x)] = half ? (uint32_t)(vs % kQ2) : (uint32_t)(vs % kQ1);
+    }
+    __syncthreads();
+    ntt_fwd_r16x2_same(sa, sb, P, ht);
+    ntt_fwd_r16(ss, P, ht);
+    const uint2 qi = half ? qinv1 : qinv0;
+    for (int x = ht; x < kN; x += kNttThreads) {
+        const uint32_t as = half ? (uint32_t)(((uint64_t)sa[lds_pad(x)] * ss[lds_pad(x)]) % kQ2)
+                                 : (uint32_t)(((uint64_t)sa[lds_pad(x)] * ss[lds_pad(x)]) % kQ1);
+        const uint32_t m = sub_mod(sb[lds_pad(x)], as, P.q);
+        sa[lds_pad(x)] = mul_shoup(m, qi.x, qi.y, P.q);      // (b - a s) * r^-1 mod q_p, NTT form
+    }
+    __syncthreads();
+    ntt_inv_r16(sa, P, ht);
+    constexpr uint64_t kDelta = 1021937069741ULL;   // params.h bsgsDelta
+    for (int x = tid; x < kN; x += 512) {
+        const uint64_t me = crt_q(s[0][lds_pad(x)], s[3][lds_pad(x)], T);
+        // round half away from zero of me / Delta, exactly (long double holds
+        // these quotients without a rounding that could cross a half)
+        uint64_t v;
+        if (me > kQ / 2) v = (uint64_t)kP - (2 * (kQ - me) + kDelta) / (2 * kDelta);
+        else v = (2 * me + kDelta) / (2 * kDelta);
+        out[x] = v;
+    }
+}
+
 // plain copy, 16 bytes per lane per step (reference kernel for HBM counters)
 __global__ void __launch_bounds__(256) k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)

@@ -320,6 +320,26 @@ def test_packing_paths_agree(ctx, oracle):
     db.close()
 
 
+def test_recover_on_gpu(ctx, oracle):
+    """decrypt_bsgs_total on the GPU equals the oracle's messages word for
+    word, for the test-pir default and for a batch of clients at r 4"""
+    say("[gpu] recover")
+    inst, resp, _ = instance(oracle, 16, 128, 1)
+    got = ctx.decrypt_bsgs_total(16, inst.secret[None], resp[None])[0]
+    assert np.array_equal(got, oracle.decrypt(16, inst.secret, resp))
+    assert np.array_equal(got, np.arange(1, N + 1, dtype=np.uint64))
+    r, n1 = 4, 64
+    db_words = oracle.gen_db(r, n1, seed=81)
+    perms = oracle.perm_matrix(n1)
+    cl = [oracle.gen_client(r, n1, 810 + c, (11 * c) % (N // 2), c % r) for c in range(3)]
+    resps = np.stack([oracle.answer(r=r, n1=n1, db=db_words, query=x["query"], rgsw=x["rgsw"], baby=x["baby"],
+                                    giant=x["giant"], packing=x["packing"], perms=perms) for x in cl])
+    secrets = np.stack([x["secret"] for x in cl])
+    got = ctx.decrypt_bsgs_total(r, secrets, resps)
+    for c in range(3):
+        assert np.array_equal(got[c], oracle.decrypt(r, secrets[c], resps[c])), c
+
+
 def test_cpp_dropin_matches_oracle(oracle, tmp_path):
     """tests/native/dropin_test calls matrix_vector_mul_bsgs_rns_crt_large,
     externalProduct and packingRLWEs through kspir_pir.h exactly as
