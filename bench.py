@@ -302,6 +302,7 @@ def main():
     ap.add_argument("--lat-samples", type=int, default=50)
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true", help="launch kernel by kernel instead of replaying hipGraphs")
     ap.add_argument("--config", default="c3", choices=("c3", "c2", "c4", "c5"),
                     help="c3: the headline batched line (default); c2/c4/c5: the other workloads, one JSON line each")
     args = ap.parse_args()
@@ -321,6 +322,8 @@ def main():
     n2 = N // 2 // n1
     log(f"synthetic database r={r} n1={n1}")
     ctx = kspir_amd.Context(0)
+    if args.no_graphs:
+        ctx.set("graphs", 0)
     db = ctx.database(synthetic_db(r), r, n1)
     perms = kspir_amd.compute_permutation_matrix(n1)
     log(f"{B} synthetic clients")
@@ -338,6 +341,11 @@ def main():
     ctx.synchronize()
     plain_ms = ctx.time_answer(db, keys[0], qbuf, rbuf, 10, args.plain_steps) / args.plain_steps
     plain_lat = [ctx.time_answer(db, keys[0], qbuf, rbuf, 0, 1) for _ in range(args.lat_samples)]
+    plain_eager_ms = None
+    if not args.no_graphs:
+        ctx.set("graphs", 0)
+        plain_eager_ms = ctx.time_answer(db, keys[0], qbuf, rbuf, 10, args.plain_steps) / args.plain_steps
+        ctx.set("graphs", 1)
 
     # batched answer: B queries of B clients per step
     log(f"timing batched answers (B={B})")
@@ -380,6 +388,7 @@ def main():
                         f"one {args.db_mb} MB database ({r * N // 2} records x 8 KB), N=4096, r={r}, "
                         f"N1={n1}, N2={n2}",
             "records": r * N // 2, "record_bytes": 8192, "r": r, "N1": n1, "N2": n2, "queries_per_step": B,
+            "launch": "kernel by kernel" if args.no_graphs else "hipGraph replay",
         },
         "db_GBps_raw": round(r * N * (N // 2) * 2 * qps / 1e9, 2),
         "latency_ms": {"p50": round(float(np.percentile(batch_lat, 50)), 4),
@@ -388,6 +397,7 @@ def main():
             "qps": round(plain_qps, 2), "ms": round(plain_ms, 4),
             "db_GBps_raw": round(r * N * (N // 2) * 2 * plain_qps / 1e9, 2),
             "db_GBps_streamed": round(db_bytes * plain_qps / 1e9, 2),
+            "ms_kernel_by_kernel": None if plain_eager_ms is None else round(plain_eager_ms, 4),
             "latency_ms": {"p50": round(float(np.percentile(plain_lat, 50)), 4),
                            "p99": round(float(np.percentile(plain_lat, 99)), 4)},
         },

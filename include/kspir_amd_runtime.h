@@ -29,7 +29,12 @@ int kspir_device_count(int* count);
  *                     over one workgroup per (block, row, prime)
  *   "fused_pack"      1 (default): for batches of at least fused_min_batch
  *                     queries each packing level is one kernel with a
- *                     workgroup per (pair, query); 0: three kernels per level */
+ *                     workgroup per (pair, query); 0: three kernels per level
+ *   "graphs"          1 (default): kspir_answer_device / _batch_device replay
+ *                     a captured hipGraph of the whole pipeline for buffers
+ *                     and batch sizes seen before (captured after a first,
+ *                     eager call); 0: launch kernel by kernel
+ * Setting an option waits for the context stream and drops captured graphs. */
 int kspir_context_set(kspir_context* ctx, const char* key, int64_t value);
 
 /* device bytes one query of shape (r, N1) adds to a batch (workspace and

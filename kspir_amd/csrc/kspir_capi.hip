@@ -108,7 +108,30 @@ struct kspir_context {
     int mfma_max_chunk = 1 << 30;   // cap on queries per MFMA database sweep (LDS also caps it)
     int fused_min_batch = 16;       // batches of at least this many queries use k_giant_fused
     int fused_pack = 1;             // one k_pack_level per packing level (0: three kernels per level)
+    // captured pipelines: one hipGraph per (database, key table, batch size,
+    // query / response buffers, stream); dropped whenever a buffer they name
+    // is reallocated or freed, or an option changes
+    struct Graph {
+        const void* db;
+        const void* kp;
+        int nq;
+        const void* q;
+        void* resp;
+        hipStream_t st;
+        hipGraphExec_t exec;   // null: capture failed, run eagerly
+    };
+    std::vector<Graph> graphs;
+    int use_graphs = 1;
 };
+
+static void drop_graphs(kspir_context* c, const void* only = nullptr) {
+    std::vector<kspir_context::Graph> keep;
+    for (auto& g : c->graphs) {
+        if (only && g.db != only && g.kp != only) { keep.push_back(g); continue; }
+        if (g.exec) hipGraphExecDestroy(g.exec);
+    }
+    c->graphs.swap(keep);
+}
 
 struct kspir_database {
     kspir_context* ctx;
@@ -254,6 +277,7 @@ int carve_workspace(kspir_context* c, int r, int n1, int nq) {
     if (total * 4 > c->ws_bytes) {
         if (c->ws) {
             HIP_OK(hipStreamSynchronize(c->stream));
+            drop_graphs(c);
             HIP_OK(hipFree(c->ws));
             c->ws = nullptr;
             c->ws_bytes = 0;
@@ -317,7 +341,7 @@ int launch_db_batched(kspir_context* c, const kspir_database* dbc, int nq, hipSt
     const size_t ql_bytes = (size_t)kN * 2 * 4 * C_full * n1;
     const size_t mf_bytes = (size_t)2 * R * C_full * kN * 4;
     if (ql_bytes + mf_bytes > c->mws_bytes) {
-        if (c->mws) { HIP_OK(hipStreamSynchronize(st)); HIP_OK(hipFree(c->mws)); c->mws = nullptr; }
+        if (c->mws) { HIP_OK(hipStreamSynchronize(st)); drop_graphs(c); HIP_OK(hipFree(c->mws)); c->mws = nullptr; }
         HIP_OK(hipMalloc(&c->mws, ql_bytes + mf_bytes));
         c->mws_bytes = ql_bytes + mf_bytes;
     }
@@ -483,10 +507,42 @@ int check_pair(const kspir_context* c, const kspir_database* db, const kspir_key
     return KSPIR_OK;
 }
 
+// enqueue the answer pipeline through a cached hipGraph: the first call with
+// a given (database, key table, batch size, buffers, stream) runs eagerly
+// (which also makes every lazy allocation) and then captures the same
+// launches; later calls replay the graph
+int run_answer(kspir_context* c, const kspir_database* db, const KeyPtrs* kp, int nq, const uint64_t* q,
+               uint64_t* resp, hipStream_t st) {
+    if (!c->use_graphs) return enqueue_answer(c, db, kp, nq, q, resp, st, nullptr, nullptr, false);
+    for (auto& g : c->graphs)
+        if (g.db == db && g.kp == kp && g.nq == nq && g.q == q && g.resp == resp && g.st == st) {
+            if (!g.exec) return enqueue_answer(c, db, kp, nq, q, resp, st, nullptr, nullptr, false);
+            HIP_OK(hipGraphLaunch(g.exec, st));
+            return KSPIR_OK;
+        }
+    int rc = enqueue_answer(c, db, kp, nq, q, resp, st, nullptr, nullptr, false);
+    if (rc) return rc;
+    kspir_context::Graph g{db, kp, nq, q, resp, st, nullptr};
+    hipGraph_t graph = nullptr;
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) == hipSuccess) {
+        const int crc = enqueue_answer(c, db, kp, nq, q, resp, st, nullptr, nullptr, false);
+        const hipError_t e = hipStreamEndCapture(st, &graph);
+        if (crc == KSPIR_OK && e == hipSuccess && graph) {
+            if (hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0) != hipSuccess) g.exec = nullptr;
+        }
+        if (graph) hipGraphDestroy(graph);
+    }
+    (void)hipGetLastError();   // a failed capture leaves the eager path in place
+    if (c->graphs.size() >= 64) drop_graphs(c);
+    c->graphs.push_back(g);
+    return KSPIR_OK;
+}
+
 // device key-pointer table for a batch; re-uploaded only when it changes
 int stage_batch_keys(kspir_context* c, const kspir_keys* const* keys, int nq) {
     if (nq > c->kp_cap) {
         HIP_OK(hipStreamSynchronize(c->stream));
+        drop_graphs(c);
         if (c->kp_dev) HIP_OK(hipFree(c->kp_dev));
         if (c->kp_host) HIP_OK(hipHostFree(c->kp_host));
         c->kp_dev = nullptr;
@@ -557,6 +613,7 @@ void kspir_context_destroy(kspir_context* c) {
     if (!c) return;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
+    drop_graphs(c);
     for (void* p : c->tables) hipFree(p);
     if (c->ws) hipFree(c->ws);
     if (c->kp_dev) hipFree(c->kp_dev);
@@ -719,6 +776,7 @@ void kspir_database_destroy(kspir_database* db) {
     if (!db) return;
     hipSetDevice(db->ctx->device);
     hipStreamSynchronize(db->ctx->stream);
+    drop_graphs(db->ctx, db);
     hipFree(db->words);
     if (db->limbs) hipFree(db->limbs);
     delete db;
@@ -791,6 +849,7 @@ void kspir_keys_destroy(kspir_keys* k) {
     if (!k) return;
     hipSetDevice(k->ctx->device);
     hipStreamSynchronize(k->ctx->stream);
+    drop_graphs(k->ctx, k->dev_ptrs);
     // a batch table naming these keys must not be reused
     k->ctx->kp_count = 0;
     hipFree(k->baby);
@@ -811,7 +870,7 @@ int kspir_answer_device(kspir_context* c, const kspir_database* db, const kspir_
     rc = carve_workspace(c, db->r, db->n1, 1);
     if (rc) return rc;
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
-    return enqueue_answer(c, db, k->dev_ptrs, 1, query_dev, response_dev, st, nullptr, nullptr, false);
+    return run_answer(c, db, k->dev_ptrs, 1, query_dev, response_dev, st);
 }
 
 int kspir_answer_batch_device(kspir_context* c, const kspir_database* db, const kspir_keys* const* keys, int32_t nq,
@@ -829,7 +888,7 @@ int kspir_answer_batch_device(kspir_context* c, const kspir_database* db, const 
     if (rc) return rc;
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
     if (st != c->stream) HIP_OK(hipStreamSynchronize(c->stream));   // key table upload runs on the context stream
-    return enqueue_answer(c, db, c->kp_dev, nq, queries_dev, responses_dev, st, nullptr, nullptr, false);
+    return run_answer(c, db, c->kp_dev, nq, queries_dev, responses_dev, st);
 }
 
 int kspir_answer_batch(kspir_context* c, const kspir_database* db, const kspir_keys* const* keys, int32_t nq,
@@ -862,8 +921,9 @@ int kspir_answer_inspect(kspir_context* c, const kspir_database* db, const kspir
     uint64_t* ext_dev = nullptr;
     if (ext) HIP_OK(hipMalloc(&ext_dev, (size_t)r * 2 * kN * 8));
     HIP_OK(hipMemcpyAsync(c->q_dev, query, 4 * kN * 8, hipMemcpyHostToDevice, c->stream));
-    rc = enqueue_answer(c, db, k->dev_ptrs, 1, c->q_dev, c->resp_dev, c->stream, firstdim ? fd_dev : nullptr,
-                        ext_dev, false);
+    rc = (firstdim || ext) ? enqueue_answer(c, db, k->dev_ptrs, 1, c->q_dev, c->resp_dev, c->stream,
+                                            firstdim ? fd_dev : nullptr, ext_dev, false)
+                           : run_answer(c, db, k->dev_ptrs, 1, c->q_dev, c->resp_dev, c->stream);
     if (rc) { if (ext_dev) hipFree(ext_dev); return rc; }
     HIP_OK(hipMemcpyAsync(response, c->resp_dev, 2 * kN * 8, hipMemcpyDeviceToHost, c->stream));
     if (cipherbuf) {
@@ -969,6 +1029,9 @@ int kspir_packing_rlwes(kspir_context* c, const kspir_keys* k, uint64_t* result,
 
 int kspir_context_set(kspir_context* c, const char* key, int64_t value) {
     if (!c || !key) return fail(KSPIR_EINVAL, "null argument");
+    HIP_OK(hipSetDevice(c->device));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    drop_graphs(c);   // captured launches carry the old choices
     const std::string k(key);
     if (k == "mfma_min_batch") {
         if (value < 1) return fail(KSPIR_EINVAL, "mfma_min_batch must be >= 1");
@@ -981,6 +1044,8 @@ int kspir_context_set(kspir_context* c, const char* key, int64_t value) {
         c->fused_min_batch = (int)value;
     } else if (k == "fused_pack") {
         c->fused_pack = value != 0;
+    } else if (k == "graphs") {
+        c->use_graphs = value != 0;
     } else {
         return fail(KSPIR_EINVAL, "unknown option " + k);
     }

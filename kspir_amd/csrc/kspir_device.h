@@ -154,17 +154,32 @@ __device__ __forceinline__ void ntt_inv_lds(uint32_t* s, const PrimeTab& P) {
 // ------------------------------------------------- register-blocked NTT
 // The same transforms as ntt_fwd_lds / ntt_inv_lds, organised as three
 // passes of four radix-2 stages.  In pass P a thread holds 16 elements whose
-// indices differ only in bits 4P..4P+3 reversed into its register index k
-// (pass 0: e = tid + 256k, pass 1: e = (tid>>4)*256 + (tid&15) + 16k,
-// pass 2: e = 16*tid + k), so the four stages of a pass run in registers and
-// the passes exchange through LDS: two barriers per transform instead of
-// twelve.  LDS words are padded by one per 16 (kPadN words) so that every
-// pass reads and writes without bank conflicts.
-constexpr int kPadN = kN + kN / 16;
-__host__ __device__ __forceinline__ int lds_pad(int e) { return e + (e >> 4); }
+// indices differ only in bits 4P..4P+3 reversed into its register index k,
+// so the four stages of a pass run in registers and the passes exchange
+// through LDS: two barriers per transform instead of twelve.
+//   pass 0: e = tid + 256k
+//   pass 1: e = 256*g + (tid & 15) + 16k,  g = pass_base(1, tid)
+//   pass 2: e = 16*c + k,                  c = pass_base(2, tid)
+// LDS words carry one pad word per 64 (kPadN words).  With that padding a
+// wave's 64 lanes hit 64 distinct banks in every pass (and each half-wave
+// 32 distinct banks of 32) because pass 1 puts the groups w, w+4, w+8,
+// w+12 in wave w and pass 2 orders a wave's columns so that each half-wave
+// holds the columns c with c & 2 fixed; consecutive-index loops
+// (x = tid + 256i) are conflict-free as well.
+constexpr int kPadN = kN + kN / 64;
+__host__ __device__ __forceinline__ int lds_pad(int e) { return e + (e >> 6); }
+
+// pass 1: group g (its 256-element block); pass 2: column c; pass 0: 0
+__host__ __device__ __forceinline__ int pass_base(int pass, int tid) {
+    return pass == 0 ? 0
+         : pass == 1 ? ((((tid >> 4) & 3) << 2) | (tid >> 6))
+                     : ((tid & ~63) | (tid & 1) | (((tid >> 5) & 1) << 1) | (((tid >> 1) & 15) << 2));
+}
 
 __host__ __device__ __forceinline__ int pass_elem(int pass, int tid, int k) {
-    return pass == 0 ? tid + 256 * k : pass == 1 ? ((tid >> 4) << 8) + (tid & 15) + 16 * k : 16 * tid + k;
+    return pass == 0 ? tid + 256 * k
+         : pass == 1 ? (pass_base(1, tid) << 8) + (tid & 15) + 16 * k
+                     : 16 * pass_base(2, tid) + k;
 }
 
 __host__ __device__ __forceinline__ uint32_t mulhi32_hd(uint32_t a, uint32_t b) {
@@ -178,13 +193,13 @@ __host__ __device__ __forceinline__ uint32_t mul_shoup_hd(uint32_t x, uint32_t w
 
 // Twiddles of pass P for thread tid (forward tables hold 2^32 - w): stage s (s = 0..3) of the pass uses the
 // 2^s consecutive table entries starting at 2^(4P+s) + base * 2^s, with
-// base = 0, tid >> 4, tid for P = 0, 1, 2, i.e. the entry for register k is
+// base = pass_base(P, tid), i.e. the entry for register k is
 // table[2^(4P+s) + base * 2^s + (k >> (4 - s))].  tw2 interleaves (w, w')
 // pairs so each stage's group is one aligned vector load; w[], ws[] hold
 // the 15 pairs as stage s at offset 2^s - 1.
 __host__ __device__ __forceinline__ void load_pass_twiddles(const uint2* tw2, int pass, int tid, uint32_t w[15],
                                                             uint32_t ws[15]) {
-    const int base = pass == 0 ? 0 : pass == 1 ? (tid >> 4) : tid;
+    const int base = pass_base(pass, tid);
     const int lg0 = 4 * pass;
     // s = 0: one pair
     {

@@ -340,6 +340,54 @@ def test_recover_on_gpu(ctx, oracle):
         assert np.array_equal(got[c], oracle.decrypt(r, secrets[c], resps[c])), c
 
 
+def test_graph_replays_match_eager(ctx, oracle):
+    """the captured pipeline, replayed four times on the same device buffers
+    with new queries and key sets each time and eager answers (the
+    inspection path, never captured) in between, gives every replay the words
+    of the kernel-by-kernel launch; single-query replays likewise"""
+    say("[gpu] graph replays r=4 n1=128 nq=3")
+    r, n1 = 4, 128
+    db_words = oracle.gen_db(r, n1, seed=91)
+    perms = oracle.perm_matrix(n1)
+    cl = [oracle.gen_client(r, n1, 900 + c, (53 * c + 7) % (N // 2), c % r) for c in range(5)]
+    db = ctx.database(db_words, r, n1)
+    keys = [ctx.keys(r, n1, x["baby"], x["giant"], x["packing"], x["rgsw"], perms) for x in cl]
+    queries = np.stack([x["query"] for x in cl])
+    ctx.set("graphs", 0)
+    try:
+        eager = np.stack([ctx.answer(db, keys[c], queries[c]) for c in range(5)])
+    finally:
+        ctx.set("graphs", 1)
+    want0 = oracle.answer(r=r, n1=n1, db=db_words, query=cl[0]["query"], rgsw=cl[0]["rgsw"], baby=cl[0]["baby"],
+                          giant=cl[0]["giant"], packing=cl[0]["packing"], perms=perms)
+    assert np.array_equal(eager[0], want0)
+    nq = 3
+    qbuf, rbuf = ctx.alloc(nq * 4 * N * 8), ctx.alloc(nq * 2 * N * 8)
+    q1buf, r1buf = ctx.alloc(4 * N * 8), ctx.alloc(2 * N * 8)
+    for rep in range(4):
+        pick = [(rep + j) % 5 for j in range(nq)]
+        qbuf.upload(queries[pick])
+        ctx.answer_batch_device(db, [keys[c] for c in pick], qbuf, rbuf)
+        ctx.synchronize()
+        got = rbuf.download(np.zeros((nq, 2, N), dtype=np.uint64))
+        for j, c in enumerate(pick):
+            assert np.array_equal(got[j], eager[c]), (rep, j, c)
+        one = (2 * rep + 1) % 5
+        q1buf.upload(queries[one])
+        ctx.answer_device(db, keys[one], q1buf, r1buf)
+        ctx.synchronize()
+        assert np.array_equal(r1buf.download(np.zeros((2, N), dtype=np.uint64)), eager[one]), (rep, one)
+        # eager work on the same context between replays
+        other = (rep + 3) % 5
+        resp, _ = ctx.answer_inspect(db, keys[other], queries[other])
+        assert np.array_equal(resp, eager[other]), (rep, other)
+    for b in (qbuf, rbuf, q1buf, r1buf):
+        b.free()
+    for k in keys:
+        k.close()
+    db.close()
+
+
 def test_cpp_dropin_matches_oracle(oracle, tmp_path):
     """tests/native/dropin_test calls matrix_vector_mul_bsgs_rns_crt_large,
     externalProduct and packingRLWEs through kspir_pir.h exactly as
