@@ -795,6 +795,16 @@ int kspir_keys_create(kspir_context* c, int32_t r, int32_t n1, const uint64_t* b
         if (permutations[i] < 0 || permutations[i] >= kN) return fail(KSPIR_EINVAL, "permutation entry out of range");
         p16[i] = (uint16_t)permutations[i];
     }
+    // the baby-step keys are stored through rows 1 .. N1/2, which must
+    // therefore be permutations (compute_permutation_matrix's rows are)
+    for (int row = 1; row <= n1 / 2 && !p16.empty(); ++row) {
+        std::vector<uint8_t> seen(kN, 0);
+        for (int x = 0; x < kN; ++x) {
+            uint8_t& f = seen[p16[(size_t)row * kN + x]];
+            if (f) return fail(KSPIR_EINVAL, "permutation row is not a permutation of the N slots");
+            f = 1;
+        }
+    }
     HIP_OK(hipSetDevice(c->device));
     kspir_keys* k = new kspir_keys();
     k->ctx = c;
@@ -815,11 +825,15 @@ int kspir_keys_create(kspir_context* c, int32_t r, int32_t n1, const uint64_t* b
     hipError_t e = hipMalloc(&raw, raw_words * 8);
     if (e != hipSuccess) return cleanup(fail_hip("hipMalloc(raw keys)", e));
 #define K_OK(x) do { hipError_t e2 = (x); if (e2 != hipSuccess) return cleanup(fail_hip(#x, e2)); } while (0)
-    if (baby) {
+    if (!p16.empty()) {
+        K_OK(hipMalloc(&k->perms, p16.size() * 2));
+        K_OK(hipMemcpy(k->perms, p16.data(), p16.size() * 2, hipMemcpyHostToDevice));
+    }
+    if (baby) {   // stored in each rotation's slot order (needs the permutation rows)
         K_OK(hipMalloc(&k->baby, (size_t)nb * 3 * 2 * 3 * kN * 4));
         K_OK(hipMemcpy(raw, baby, (size_t)nb * 6 * 2 * kN * 8, hipMemcpyHostToDevice));
-        hipLaunchKernelGGL(k_keys_baby, dim3(grid1d((size_t)nb * 6 * kN)), dim3(256), 0, c->stream, k->baby, raw, nb,
-                           c->T);
+        hipLaunchKernelGGL(k_keys_baby, dim3(grid1d((size_t)nb * 6 * kN)), dim3(256), 0, c->stream, k->baby, raw,
+                           k->perms, nb, c->T);
         K_OK(hipStreamSynchronize(c->stream));
     }
     auto upload_q = [&](const uint64_t* src, int rows, uint32_t** dst) -> hipError_t {
@@ -833,10 +847,6 @@ int kspir_keys_create(kspir_context* c, int32_t r, int32_t n1, const uint64_t* b
     if (giant && ng > 0) K_OK(upload_q(giant, ng * 2, &k->giant));
     if (packing && np > 0) K_OK(upload_q(packing, np * 4, &k->packing));
     if (rgsw) K_OK(upload_q(rgsw, 4, &k->rgsw));
-    if (!p16.empty()) {
-        K_OK(hipMalloc(&k->perms, p16.size() * 2));
-        K_OK(hipMemcpy(k->perms, p16.data(), p16.size() * 2, hipMemcpyHostToDevice));
-    }
     k->ptrs = KeyPtrs{k->baby, k->giant, k->packing, k->rgsw, k->perms};
     K_OK(hipMalloc(&k->dev_ptrs, sizeof(KeyPtrs)));
     K_OK(hipMemcpy(k->dev_ptrs, &k->ptrs, sizeof(KeyPtrs), hipMemcpyHostToDevice));

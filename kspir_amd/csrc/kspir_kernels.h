@@ -139,9 +139,11 @@ __global__ void __launch_bounds__(256) k_keys_q(uint32_t* __restrict__ out, cons
     }
 }
 
-// baby keys: [nb][6][2][N] (rows 0..2 mod Q, 3..5 mod bsMod) -> [nb][3][2][3 primes][N]
+// baby keys: [nb][6][2][N] (rows 0..2 mod Q, 3..5 mod bsMod) -> [nb][3][2][3 primes][N],
+// key i stored in the slot order of its rotation: word s goes to slot
+// perms[i+1][s] (perms [N1][N], each row a permutation of the slots)
 __global__ void __launch_bounds__(256) k_keys_baby(uint32_t* __restrict__ out, const uint64_t* __restrict__ in,
-                                                   int nb, Tabs T) {
+                                                   const uint16_t* __restrict__ perms, int nb, Tabs T) {
     const size_t total = (size_t)nb * 3 * 2 * kN;
     for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
          idx += (size_t)gridDim.x * blockDim.x) {
@@ -155,7 +157,7 @@ __global__ void __launch_bounds__(256) k_keys_baby(uint32_t* __restrict__ out, c
             const uint64_t v = in[((key * 6 + row) * 2 + ab) * kN + s];
             const uint32_t q = T.p[p].q;
             const uint64_t m = ((v % q) << 32) % q;
-            out[(((key * 3 + j) * 2 + ab) * 3 + p) * kN + s] = (uint32_t)m;
+            out[(((key * 3 + j) * 2 + ab) * 3 + p) * kN + perms[(key + 1) * kN + s]] = (uint32_t)m;
         }
     }
 }
@@ -308,22 +310,25 @@ __device__ __forceinline__ int xcd_job(int bid, int total) {
 // The three digit planes D[0..2][p] of a prime are staged in LDS with
 // coalesced loads and permuted from there.  Jobs: nrot * 2 * nq, rotation
 // fastest, then a|b, then query (one workgroup each, 1-D grid).
-// one rotation's key switch for prime p, after the digit planes of p are
-// staged in s[0..2]: acc[e] = -sum_j D[j][p][perm(x)] * key[j][ab][p][x]
-// (+ bhat[p][perm(x)] for the b part)
-__device__ __forceinline__ void baby_key_mult(uint32_t* acc, const u32x4 (&kv)[3][4], const int* ps,
-                                              uint32_t (*s)[kPadN], const uint32_t* bh, int ab, const PrimeTab& P) {
-    constexpr int PER = kN / kNttThreads;
+// one rotation's key product for prime p in the rotation's slot order y
+// (the order the keys are stored in): for the 16 slots
+// y = 1024g + x0 + i (e = 4g + i) of this thread,
+//   a[e] = -sum_j D[j][p][y] * key'[j][ab][p][y]   (+ bhat[p][y] for the b part)
+// with D and bhat read in natural order; slot x of the rotation is then
+// a[perm(x)]
+__device__ __forceinline__ void baby_key_mult(uint32_t* a, const u32x4 (&kv)[3][4], const u32x4 (&dv)[3][4],
+                                              const u32x4 (&bv)[4], int ab, const PrimeTab& P) {
 #pragma unroll
-    for (int e = 0; e < PER; ++e) {
-        const int g = e >> 2, i = e & 3;
-        uint64_t t = 0;
+    for (int g = 0; g < 4; ++g)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) t += (uint64_t)s[j][lds_pad(ps[e])] * kv[j][g][i];
-        uint32_t r = neg_mod(mont_reduce(t, P.q, P.mont_qinv), P.q);
-        if (ab) r = add_mod(r, bh[e], P.q);
-        acc[e] = r;
-    }
+        for (int i = 0; i < 4; ++i) {
+            uint64_t t = 0;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) t += (uint64_t)dv[j][g][i] * kv[j][g][i];
+            uint32_t r = neg_mod(mont_reduce(t, P.q, P.mont_qinv), P.q);
+            if (ab) r = add_mod(r, bv[g][i], P.q);
+            a[4 * g + i] = r;
+        }
 }
 
 // modulus switch of one rotation and its store: c = INTT_bs(acc_bs),
@@ -359,17 +364,19 @@ __device__ __forceinline__ void baby_switch_store(const uint32_t (&acc)[3][kN / 
 // Baby-step rotations, one workgroup per (rotation, a|b, query); rotation
 // rot applies tau_{5^(rot+1)} with key index rot and permutation row rot+1
 // (twosteps.cpp:937-1204):
-//   acc_p = -sum_j D[j][p][perm(s)] * key[rot][j][ab][p][s]       (p = q1,q2,bs)
-//   b-part adds bhat[p][perm(s)]  (tau(b) in the NTT domain)
+//   acc_p[x] = -sum_j D[j][p][perm(x)] * key[rot][j][ab][p][x]     (p = q1,q2,bs)
+//   b-part adds bhat[p][perm(x)]  (tau(b) in the NTT domain)
 //   c = INTT_bs(acc_bs) in [0,bsMod);  out_p = (acc_p - NTT_p(c)) * bsMod^-1
 // Rotations rot_base .. rot_base+nrot-1 of one half, output to
 // ctw[k_base + rot]; rotation `mid` also stores its un-switched NTT-domain
 // accumulators in mid_out[ab][p] (evalAutoSRNSFirst's intermediate).
-// The three digit planes of a prime are staged in LDS with coalesced loads
-// and permuted from there; each thread owns 4 consecutive slots per
-// 1024-slot group (e = 4g + i), so keys, permutation rows and outputs move
-// as 16-byte (8-byte) vectors, the keys streamed past the caches.  Jobs:
-// nrot * 2 * nq, rotation fastest, then a|b, then query (1-D grid).
+// The keys are stored permuted (key'[perm(x)] = key[x]), so the products
+// run at y = perm(x) with the digits, bhat and keys all read as contiguous
+// 16-byte vectors (keys streamed past the caches that hold the query's
+// digits); one permuted LDS read per prime and slot then brings them to slot
+// order.  Each thread owns 4 consecutive slots per 1024-slot group
+// (e = 4g + i).  Jobs: nrot * 2 * nq, rotation fastest, then a|b, then
+// query (1-D grid).
 __global__ void __launch_bounds__(kNttThreads) k_baby_rot(const uint32_t* __restrict__ D,
         const uint32_t* __restrict__ bhat, const KeyPtrs* __restrict__ kp, uint32_t* __restrict__ ctw,
         uint32_t* __restrict__ mid_out, int nrot, int rot_base, int nq, int k_base, int mid, size_t D_stride,
@@ -387,44 +394,42 @@ __global__ void __launch_bounds__(kNttThreads) k_baby_rot(const uint32_t* __rest
     const int tid = threadIdx.x;
     constexpr int PER = kN / kNttThreads;
     const int x0 = 4 * tid;
-    int ps[PER];
     uint32_t acc[3][PER];
 #pragma unroll
-    for (int g = 0; g < 4; ++g) {
-        const uint2 v = *reinterpret_cast<const uint2*>(pm + g * 1024 + x0);
-        ps[4 * g + 0] = (int)(v.x & 0xFFFFu);
-        ps[4 * g + 1] = (int)(v.x >> 16);
-        ps[4 * g + 2] = (int)(v.y & 0xFFFFu);
-        ps[4 * g + 3] = (int)(v.y >> 16);
-    }
-#pragma unroll
     for (int p = 0; p < 3; ++p) {
-        // this prime's key words first, so they are in flight with the staging
-        u32x4 kv[3][4];     // read once: streamed past the caches that hold D
+        u32x4 kv[3][4], dv[3][4], bv[4];
 #pragma unroll
         for (int j = 0; j < 3; ++j)
 #pragma unroll
-            for (int g = 0; g < 4; ++g)
+            for (int g = 0; g < 4; ++g) {
                 kv[j][g] = __builtin_nontemporal_load(
                     reinterpret_cast<const u32x4*>(key + ((j * 2 + ab) * 3 + p) * kN + g * 1024 + x0));
-        uint32_t bh[PER];
-        if (ab) {
+                dv[j][g] = *reinterpret_cast<const u32x4*>(D + (j * 3 + p) * kN + g * 1024 + x0);
+            }
 #pragma unroll
-            for (int e = 0; e < PER; ++e) bh[e] = bhat[p * kN + ps[e]];
-        }
+        for (int g = 0; g < 4; ++g)
+            bv[g] = ab ? *reinterpret_cast<const u32x4*>(bhat + p * kN + g * 1024 + x0) : u32x4{0, 0, 0, 0};
+        uint32_t a[PER];
+        baby_key_mult(a, kv, dv, bv, ab, T.p[p]);
 #pragma unroll
-        for (int e = 0; e < PER; ++e) {
-            const int x = tid + e * kNttThreads;
+        for (int e = 0; e < PER; ++e) s[p][lds_pad((e >> 2) * 1024 + x0 + (e & 3))] = a[e];
+    }
+    __syncthreads();
 #pragma unroll
-            for (int j = 0; j < 3; ++j) s[j][lds_pad(x)] = D[(j * 3 + p) * kN + x];
-        }
-        __syncthreads();
-        baby_key_mult(acc[p], kv, ps, s, bh, ab, T.p[p]);
-        __syncthreads();
-        if (rot == mid) {
+    for (int g = 0; g < 4; ++g) {
+        const uint2 v = *reinterpret_cast<const uint2*>(pm + g * 1024 + x0);
+        const int ps[4] = {(int)(v.x & 0xFFFFu), (int)(v.x >> 16), (int)(v.y & 0xFFFFu), (int)(v.y >> 16)};
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int p = 0; p < 3; ++p) acc[p][4 * g + i] = s[p][lds_pad(ps[i])];
+    }
+    __syncthreads();
+    if (rot == mid) {
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
 #pragma unroll
             for (int e = 0; e < PER; ++e) mid_out[(ab * 3 + p) * kN + (e >> 2) * 1024 + x0 + (e & 3)] = acc[p][e];
-        }
     }
     baby_switch_store(acc, s, ctw, k_base + rot, ab, tid, T);
 }

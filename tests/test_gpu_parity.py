@@ -128,6 +128,10 @@ def test_keys_and_db_shape_checks(ctx, oracle):
     bad[1, 5] = N
     with pytest.raises(kspir_amd.KspirError):
         ctx.keys(4, 128, inst.baby, inst.giant, inst.packing, inst.rgsw, bad)
+    dup = inst.perms.copy()
+    dup[3, 7] = dup[3, 8]      # in range, but row 3 no longer a permutation
+    with pytest.raises(kspir_amd.KspirError):
+        ctx.keys(4, 128, inst.baby, inst.giant, inst.packing, inst.rgsw, dup)
     db = ctx.database(inst.db[:2].copy(), 2, 128)
     keys = ctx.keys(4, 128, inst.baby, inst.giant, inst.packing, inst.rgsw, inst.perms)
     with pytest.raises(kspir_amd.KspirError):
