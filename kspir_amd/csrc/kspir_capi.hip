@@ -428,21 +428,20 @@ int enqueue_answer(kspir_context* c, const kspir_database* db, const KeyPtrs* kp
     // baby steps (twosteps.cpp:1206-1214, evalAutoSRNSSmallKeys)
     hipLaunchKernelGGL(k_baby_prep, dim3(16, nq), dim3(kNttThreads), 0, st, q_dev, c->D0, c->bhat0, c->ctw, n1, S, T);
     LAUNCH_OK("k_baby_prep");
-    // first half: rotations 1 .. N1/2-1 from the query's digits, the last of
-    // them (the intermediate) also kept un-switched; second half: rotations
-    // N1/2 .. N1-1 from the intermediate's digits
-    if (n1h - 1 > 0) {
-        hipLaunchKernelGGL(k_baby_rot, dim3((n1h - 1) * 2 * nq), dim3(kNttThreads), 0, st, c->D0, c->bhat0, kp,
-                           c->ctw, c->mid, n1h - 1, 0, nq, 1, n1h - 2, S.D, S.bhat, S, T);
-        LAUNCH_OK("k_baby_rot(first)");
-    }
+    // rotation N1/2-1 of the query first: its switched words and the
+    // un-switched intermediate that the second half rotates
+    hipLaunchKernelGGL(k_baby_rot, dim3(2 * nq), dim3(kNttThreads), 0, st, c->D0, c->bhat0, kp, c->ctw, c->mid, 1,
+                       n1h - 2, nq, 1, n1h - 2, S.D, S.bhat, S, T);
+    LAUNCH_OK("k_baby_rot(intermediate)");
     hipLaunchKernelGGL(k_baby_mid_coef, dim3(3, nq), dim3(kNttThreads), 0, st, c->mid, c->icoef, S, T);
     LAUNCH_OK("k_baby_mid_coef");
     hipLaunchKernelGGL(k_baby_mid_digits, dim3(9, nq), dim3(kNttThreads), 0, st, c->icoef, c->D1, S, T);
     LAUNCH_OK("k_baby_mid_digits");
-    hipLaunchKernelGGL(k_baby_rot, dim3(n1h * 2 * nq), dim3(kNttThreads), 0, st, c->D1, c->mid + 3 * kN, kp, c->ctw,
-                       (uint32_t*)nullptr, n1h, 0, nq, n1h, -1, S.D, S.mid, S, T);
-    LAUNCH_OK("k_baby_rot(second)");
+    // the other rotations, both halves, one read of each key: rotations
+    // 1 .. N1/2-2 of the query and N1/2 .. N1-1 (from the intermediate)
+    hipLaunchKernelGGL(k_baby_rot_pair, dim3(n1h * 2 * nq), dim3(kNttThreads), 0, st, c->D0, c->bhat0, c->D1,
+                       c->mid + 3 * kN, kp, c->ctw, n1h, nq, S, T);
+    LAUNCH_OK("k_baby_rot_pair");
     // database pass (crt.cpp:643-768): int8-MFMA limb GEMM for a batch,
     // u64-accumulating VALU stream for a single query
     int rc = (nq >= c->mfma_min_batch && mfma_shape(db)) ? launch_db_batched(c, db, nq, st)

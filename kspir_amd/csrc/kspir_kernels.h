@@ -11,7 +11,7 @@
 //                  u64 accumulation, one lane per NTT slot
 //   giant_coef / giant_ks     INTT, automorphism, gadget decomposition and
 //                  key switch of the N2-1 giant-step rows of every block
-//   ext_coef / ext_digit / ext_sum    RGSW external product
+//   ext_digit / ext_sum    RGSW external product
 //   pack_coef / pack_digit / pack_combine   one level of the packing tree
 //   response       INTT + CRT to the u64 response words
 // All polynomials are u32 residue planes of N = 4096 words.
@@ -332,13 +332,16 @@ __device__ __forceinline__ void baby_key_mult(uint32_t* a, const u32x4 (&kv)[3][
 }
 
 // modulus switch of one rotation and its store: c = INTT_bs(acc_bs),
-// out_p = (acc_p - NTT_p(c)) * bsMod^-1 -> ctw[k][ab][p]
-__device__ __forceinline__ void baby_switch_store(const uint32_t (&acc)[3][kN / kNttThreads], uint32_t (*s)[kPadN],
-                                                  uint32_t* ctw, int k, int ab, int tid, const Tabs& T) {
+// out_p = (acc_p - NTT_p(c)) * bsMod^-1 -> ctw[k][ab][p]; acc_bs in
+// registers, acc_p (p = q1, q2) from q12(e, p); uses LDS planes 0 and 1
+template <class Q12>
+__device__ __forceinline__ void baby_switch_core(const uint32_t (&acc_bs)[kN / kNttThreads], Q12 q12,
+                                                 uint32_t (*s)[kPadN], uint32_t* ctw, int k, int ab, int tid,
+                                                 const Tabs& T) {
     constexpr int PER = kN / kNttThreads;
     const int x0 = 4 * tid;
 #pragma unroll
-    for (int e = 0; e < PER; ++e) s[0][lds_pad((e >> 2) * 1024 + x0 + (e & 3))] = acc[2][e];
+    for (int e = 0; e < PER; ++e) s[0][lds_pad((e >> 2) * 1024 + x0 + (e & 3))] = acc_bs[e];
     __syncthreads();
     // c in [0, bsMod) is below q1 and q2: the same words are c mod q1, q2
     ntt_inv_r16(s[0], T.p[2]);
@@ -352,13 +355,18 @@ __device__ __forceinline__ void baby_switch_store(const uint32_t (&acc)[3][kN / 
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int e = 4 * g + i, x = g * 1024 + x0 + i;
-            o1[i] = mul_shoup(sub_mod(acc[0][e], s[0][lds_pad(x)], kQ1), T.bsinv_q[0], T.bsinv_q_sh[0], kQ1);
-            o2[i] = mul_shoup(sub_mod(acc[1][e], s[1][lds_pad(x)], kQ2), T.bsinv_q[1], T.bsinv_q_sh[1], kQ2);
+            o1[i] = mul_shoup(sub_mod(q12(e, 0), s[0][lds_pad(x)], kQ1), T.bsinv_q[0], T.bsinv_q_sh[0], kQ1);
+            o2[i] = mul_shoup(sub_mod(q12(e, 1), s[1][lds_pad(x)], kQ2), T.bsinv_q[1], T.bsinv_q_sh[1], kQ2);
         }
         *reinterpret_cast<uint4*>(ctw + ((size_t)(k * 2 + ab) * 2 + 0) * kN + g * 1024 + x0) = make_uint4(o1[0], o1[1], o1[2], o1[3]);
         *reinterpret_cast<uint4*>(ctw + ((size_t)(k * 2 + ab) * 2 + 1) * kN + g * 1024 + x0) = make_uint4(o2[0], o2[1], o2[2], o2[3]);
     }
     __syncthreads();
+}
+
+__device__ __forceinline__ void baby_switch_store(const uint32_t (&acc)[3][kN / kNttThreads], uint32_t (*s)[kPadN],
+                                                  uint32_t* ctw, int k, int ab, int tid, const Tabs& T) {
+    baby_switch_core(acc[2], [&](int e, int p) { return acc[p][e]; }, s, ctw, k, ab, tid, T);
 }
 
 // Baby-step rotations, one workgroup per (rotation, a|b, query); rotation
@@ -432,6 +440,93 @@ __global__ void __launch_bounds__(kNttThreads) k_baby_rot(const uint32_t* __rest
             for (int e = 0; e < PER; ++e) mid_out[(ab * 3 + p) * kN + (e >> 2) * 1024 + x0 + (e & 3)] = acc[p][e];
     }
     baby_switch_store(acc, s, ctw, k_base + rot, ab, tid, T);
+}
+
+// The first and second halves of the baby steps with one read of each key:
+// workgroup (rot, a|b, query) applies key rot / permutation row rot+1 both
+// to the query's digits D0 (rotation rot+1 -> ctw[1 + rot], for
+// rot < N1/2-2; rotation N1/2-1, which yields the intermediate, runs
+// beforehand in k_baby_rot) and to the intermediate's digits D1
+// (-> ctw[N1/2 + rot]).  Per prime the keys are read once (in the
+// rotation's slot order) for both products.  The query's products come
+// back permuted into registers; of the intermediate's, the bsMod one does
+// too and the q1, q2 ones stay in LDS planes 2, 3 in slot order y until its
+// modulus switch reads them permuted.  Jobs: N1/2 * 2 * nq, rotation
+// fastest, then a|b, then query (1-D grid); two workgroups per CU.
+__global__ void __launch_bounds__(kNttThreads) __attribute__((amdgpu_waves_per_eu(2)))
+k_baby_rot_pair(const uint32_t* __restrict__ D0, const uint32_t* __restrict__ bhat0,
+                const uint32_t* __restrict__ D1, const uint32_t* __restrict__ bhat1,
+                const KeyPtrs* __restrict__ kp, uint32_t* __restrict__ ctw, int n1h, int nq, QS S, Tabs T) {
+    __shared__ uint32_t s[4][kPadN];
+    const int L = xcd_job(blockIdx.x, n1h * 2 * nq);
+    const int rot = L % n1h, ab = (L / n1h) & 1;
+    const size_t qi = L / (2 * n1h);
+    const bool first = rot < n1h - 2;     // uniform over the workgroup
+    D0 += qi * S.D;
+    D1 += qi * S.D;
+    bhat0 += qi * S.bhat;
+    bhat1 += qi * S.mid;
+    ctw += qi * S.ctw;
+    const uint16_t* pm = kp[qi].perms + (size_t)(rot + 1) * kN;
+    const uint32_t* key = kp[qi].baby + (size_t)rot * 3 * 2 * 3 * kN;
+    const int tid = threadIdx.x;
+    constexpr int PER = kN / kNttThreads;
+    const int x0 = 4 * tid;
+    uint2 pv[4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) pv[g] = *reinterpret_cast<const uint2*>(pm + g * 1024 + x0);
+    auto perm_of = [&](int e) -> int {
+        const uint32_t w = (e & 2) ? pv[e >> 2].y : pv[e >> 2].x;
+        return (int)((e & 1) ? (w >> 16) : (w & 0xFFFFu));
+    };
+    uint32_t accA[3][PER], accBbs[PER];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        u32x4 kv[3][4], dv[3][4], bv[4];
+        uint32_t a[PER];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                kv[j][g] = __builtin_nontemporal_load(
+                    reinterpret_cast<const u32x4*>(key + ((j * 2 + ab) * 3 + p) * kN + g * 1024 + x0));
+                dv[j][g] = *reinterpret_cast<const u32x4*>(D1 + (j * 3 + p) * kN + g * 1024 + x0);
+            }
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+            bv[g] = ab ? *reinterpret_cast<const u32x4*>(bhat1 + p * kN + g * 1024 + x0) : u32x4{0, 0, 0, 0};
+        baby_key_mult(a, kv, dv, bv, ab, T.p[p]);
+        uint32_t* yb = p == 2 ? s[1] : s[2 + p];
+#pragma unroll
+        for (int e = 0; e < PER; ++e) yb[lds_pad((e >> 2) * 1024 + x0 + (e & 3))] = a[e];
+        __builtin_amdgcn_sched_barrier(0);
+        if (first) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int g = 0; g < 4; ++g)
+                    dv[j][g] = *reinterpret_cast<const u32x4*>(D0 + (j * 3 + p) * kN + g * 1024 + x0);
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+                bv[g] = ab ? *reinterpret_cast<const u32x4*>(bhat0 + p * kN + g * 1024 + x0) : u32x4{0, 0, 0, 0};
+            baby_key_mult(a, kv, dv, bv, ab, T.p[p]);
+#pragma unroll
+            for (int e = 0; e < PER; ++e) s[0][lds_pad((e >> 2) * 1024 + x0 + (e & 3))] = a[e];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int ps = perm_of(e);
+            if (first) accA[p][e] = s[0][lds_pad(ps)];
+            if (p == 2) accBbs[e] = s[1][lds_pad(ps)];
+        }
+        __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    if (first) baby_switch_store(accA, s, ctw, 1 + rot, ab, tid, T);
+    __builtin_amdgcn_sched_barrier(0);
+    baby_switch_core(accBbs, [&](int e, int p) { return s[2 + p][lds_pad(perm_of(e))]; }, s, ctw, n1h + rot, ab,
+                     tid, T);
 }
 
 // second half, step 1: INTT of the intermediate's a in prime p (blockIdx.x)
