@@ -303,6 +303,7 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="launch kernel by kernel instead of replaying hipGraphs")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the library's default kernel shapes and paths")
     ap.add_argument("--config", default="c3", choices=("c3", "c2", "c4", "c5"),
                     help="c3: the headline batched line (default); c2/c4/c5: the other workloads, one JSON line each")
     args = ap.parse_args()
@@ -335,6 +336,13 @@ def main():
     qbuf = ctx.alloc(B * 4 * N * 8)
     qbuf.upload(np.stack(queries))
     rbuf = ctx.alloc(B * 2 * N * 8)
+    tuned = None
+    if not args.no_autotune:
+        log("autotuning kernel shapes and paths for this database")
+        serve_b, rep = ctx.autotune(db, keys[:8], max_batch=B)
+        tuned = {k: ctx.get(k) for k in ("db_waves", "mfma_min_batch", "mfma_max_chunk", "fused_min_batch",
+                                          "fused_pack")}
+        tuned["serve_batch"] = serve_b
 
     # plain single-query answer (client 0), one query in flight
     log("timing plain single-query answers")
@@ -389,6 +397,7 @@ def main():
                         f"N1={n1}, N2={n2}",
             "records": r * N // 2, "record_bytes": 8192, "r": r, "N1": n1, "N2": n2, "queries_per_step": B,
             "launch": "kernel by kernel" if args.no_graphs else "hipGraph replay",
+            "tuned": tuned,
         },
         "db_GBps_raw": round(r * N * (N // 2) * 2 * qps / 1e9, 2),
         "latency_ms": {"p50": round(float(np.percentile(batch_lat, 50)), 4),

@@ -34,8 +34,26 @@ int kspir_device_count(int* count);
  *                     a captured hipGraph of the whole pipeline for buffers
  *                     and batch sizes seen before (captured after a first,
  *                     eager call); 0: launch kernel by kernel
+ *   "db_waves"        workgroup shape of the single-query database pass:
+ *                     2, 4 (default), 8 or 16 waves on 64 slots, each wave
+ *                     taking N2 / waves rows
  * Setting an option waits for the context stream and drops captured graphs. */
 int kspir_context_set(kspir_context* ctx, const char* key, int64_t value);
+int kspir_context_get(kspir_context* ctx, const char* key, int64_t* value);
+
+/* Autotuner (no counterpart in the reference, whose shapes are constants of
+ * a CPU loop).  Times the answer pipeline for this database on this device,
+ * with the given client keys reused in turn to fill batches of up to
+ * max_batch queries, and sets the options above: db_waves (fastest database
+ * pass), mfma_min_batch and fused_min_batch (smallest batch size from which
+ * that path stays faster, over powers of two up to max_batch),
+ * mfma_max_chunk and fused_pack (fastest at max_batch).  *serve_batch (may be
+ * NULL) receives the batch size with the most queries/s whose step time is
+ * within latency_budget_ms (0: no bound).  report (may be NULL): every
+ * timing as JSON text, cut to report_len bytes.  Runs for a few seconds. */
+int kspir_autotune(kspir_context* ctx, const kspir_database* db, const kspir_keys* const* keys, int32_t nkeys,
+                   int32_t max_batch, double latency_budget_ms, int32_t* serve_batch, char* report,
+                   size_t report_len);
 
 /* device bytes one query of shape (r, N1) adds to a batch (workspace and
  * batched database-pass scratch): the unit of the serving loop's admission */

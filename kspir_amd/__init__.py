@@ -41,8 +41,11 @@ def _load():
                  "kspir_time_answer_batch", "kspir_context_set", "kspir_time_copy",
                  "kspir_database_create_from_records", "kspir_database_tobsgsntt", "kspir_database_export",
                  "kspir_server_create", "kspir_server_submit", "kspir_server_wait", "kspir_server_stats",
-                 "kspir_query_footprint_bytes", "kspir_decrypt_bsgs_total"):
+                 "kspir_query_footprint_bytes", "kspir_decrypt_bsgs_total", "kspir_context_get",
+                 "kspir_autotune"):
         getattr(L, name).restype = ctypes.c_int
+    L.kspir_autotune.argtypes = [_vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                                 ctypes.POINTER(ctypes.c_int32), ctypes.c_char_p, ctypes.c_size_t]
     L.kspir_context_destroy.argtypes = [_vp]
     L.kspir_database_destroy.argtypes = [_vp]
     L.kspir_keys_destroy.argtypes = [_vp]
@@ -63,7 +66,7 @@ EXPORTS = (
     "kspir_synchronize", "kspir_context_stream", "kspir_time_answer", "kspir_time_db_pass", "kspir_test_ntt",
     "kspir_time_copy", "kspir_database_create_from_records", "kspir_database_tobsgsntt", "kspir_database_export",
     "kspir_server_create", "kspir_server_submit", "kspir_server_wait", "kspir_server_stats", "kspir_server_destroy",
-    "kspir_query_footprint_bytes", "kspir_decrypt_bsgs_total",
+    "kspir_query_footprint_bytes", "kspir_decrypt_bsgs_total", "kspir_context_get", "kspir_autotune",
 )
 
 
@@ -229,6 +232,22 @@ class Context:
 
     def set(self, key, value):
         _check(lib.kspir_context_set(self.h, key.encode(), ctypes.c_int64(value)))
+
+    def get(self, key):
+        v = ctypes.c_int64(0)
+        _check(lib.kspir_context_get(self.h, key.encode(), ctypes.byref(v)))
+        return v.value
+
+    def autotune(self, db, keys_list, max_batch=64, latency_budget_ms=0.0):
+        """time the pipeline for this database and set the tuning options;
+        returns (serving batch size, report dict of every timing)"""
+        arr = (_vp * len(keys_list))(*[k.h.value for k in keys_list])
+        sb = ctypes.c_int32(0)
+        buf = ctypes.create_string_buffer(1 << 16)
+        _check(lib.kspir_autotune(self.h, db.h, ctypes.cast(arr, _vp), len(keys_list), int(max_batch),
+                                  float(latency_budget_ms), ctypes.byref(sb), buf, ctypes.c_size_t(len(buf))))
+        import json
+        return sb.value, json.loads(buf.value.decode())
 
     @property
     def stream(self):

@@ -7,7 +7,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 OUT = os.path.join(HERE, "libkspir_amd.so")
-SOURCES = [os.path.join(HERE, "csrc", "kspir_capi.hip"), os.path.join(HERE, "cpp", "kspir_pir.cpp")]
+SOURCES = [os.path.join(HERE, "csrc", "kspir_capi.hip"), os.path.join(HERE, "csrc", "kspir_tune.cpp"),
+           os.path.join(HERE, "cpp", "kspir_pir.cpp")]
 DEPS = SOURCES + [os.path.join(HERE, "csrc", f) for f in ("kspir_device.h", "kspir_kernels.h")] + [os.path.join(HERE, "cpp", "kspir_pir.h")] + [
     os.path.join(ROOT, "include", f) for f in ("kspir_amd.h", "kspir_amd_runtime.h")]
 

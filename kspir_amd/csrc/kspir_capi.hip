@@ -108,6 +108,7 @@ struct kspir_context {
     int mfma_max_chunk = 1 << 30;   // cap on queries per MFMA database sweep (LDS also caps it)
     int fused_min_batch = 16;       // batches of at least this many queries use k_giant_fused
     int fused_pack = 1;             // one k_pack_level per packing level (0: three kernels per level)
+    int db_waves = 4;               // waves per workgroup of the single-query database pass (2, 4, 8, 16)
     // captured pipelines: one hipGraph per (database, key table, batch size,
     // query / response buffers, stream); dropped whenever a buffer they name
     // is reallocated or freed, or an option changes
@@ -296,26 +297,37 @@ int grid1d(size_t total) {
     return (int)(g > 65535 ? 65535 : (g ? g : 1));
 }
 
-int launch_db_product(kspir_context* c, const kspir_database* db, int nq, hipStream_t st) {
+// workgroup shape of the single-query database pass: WAVES waves of RPW
+// rows each (RPW = N2 / WAVES, at least 1); waves = 0 picks 4
+template <int WAVES>
+int launch_db_product_w(kspir_context* c, const kspir_database* db, int nq, hipStream_t st) {
     const int n1 = db->n1, n2 = db->n2, r = db->r;
     const size_t bw = (size_t)n2 * n1 * kN, obw = (size_t)n2 * 4 * kN;
-    const dim3 dg(kN / 64, r, nq);
+    const dim3 dg(kN / 64, r, nq), bd(64 * WAVES);
     const QS& S = c->S;
     const Tabs& T = c->T;
-    // rows per wave: N2 / 4 (N2 a power of two), 1 for N2 < 4 (4 waves per workgroup)
-    const int rpw = n2 <= 4 ? 1 : n2 / 4;
-    if (n2 > 4 && n2 % 4) return fail(KSPIR_EINVAL, "N2 must be a power of two");
+    const int rpw = n2 <= WAVES ? 1 : n2 / WAVES;
+    if (n2 > WAVES && n2 % WAVES) return fail(KSPIR_EINVAL, "N2 must be a power of two");
     switch (rpw) {
-        case 1: hipLaunchKernelGGL(k_db_product<1>, dg, dim3(256), 0, st, db->words, c->ctw, c->dbout, n1, n2, bw, obw, S, T); break;
-        case 2: hipLaunchKernelGGL(k_db_product<2>, dg, dim3(256), 0, st, db->words, c->ctw, c->dbout, n1, n2, bw, obw, S, T); break;
-        case 4: hipLaunchKernelGGL(k_db_product<4>, dg, dim3(256), 0, st, db->words, c->ctw, c->dbout, n1, n2, bw, obw, S, T); break;
-        case 8: hipLaunchKernelGGL(k_db_product<8>, dg, dim3(256), 0, st, db->words, c->ctw, c->dbout, n1, n2, bw, obw, S, T); break;
-        case 16: hipLaunchKernelGGL(k_db_product<16>, dg, dim3(256), 0, st, db->words, c->ctw, c->dbout, n1, n2, bw, obw, S, T); break;
-        case 32: hipLaunchKernelGGL(k_db_product<32>, dg, dim3(256), 0, st, db->words, c->ctw, c->dbout, n1, n2, bw, obw, S, T); break;
-        default: return fail(KSPIR_EINVAL, "unsupported N2 for the database pass");
+        case 1: hipLaunchKernelGGL((k_db_product<1, WAVES>), dg, bd, 0, st, db->words, c->ctw, c->dbout, n1, n2, bw, obw, S, T); break;
+        case 2: hipLaunchKernelGGL((k_db_product<2, WAVES>), dg, bd, 0, st, db->words, c->ctw, c->dbout, n1, n2, bw, obw, S, T); break;
+        case 4: hipLaunchKernelGGL((k_db_product<4, WAVES>), dg, bd, 0, st, db->words, c->ctw, c->dbout, n1, n2, bw, obw, S, T); break;
+        case 8: hipLaunchKernelGGL((k_db_product<8, WAVES>), dg, bd, 0, st, db->words, c->ctw, c->dbout, n1, n2, bw, obw, S, T); break;
+        case 16: hipLaunchKernelGGL((k_db_product<16, WAVES>), dg, bd, 0, st, db->words, c->ctw, c->dbout, n1, n2, bw, obw, S, T); break;
+        case 32: hipLaunchKernelGGL((k_db_product<32, WAVES>), dg, bd, 0, st, db->words, c->ctw, c->dbout, n1, n2, bw, obw, S, T); break;
+        default: return fail(KSPIR_EINVAL, "unsupported N2 for the database pass (rows per wave above 32)");
     }
     LAUNCH_OK("k_db_product");
     return KSPIR_OK;
+}
+
+int launch_db_product(kspir_context* c, const kspir_database* db, int nq, hipStream_t st) {
+    switch (c->db_waves) {
+        case 2: return launch_db_product_w<2>(c, db, nq, st);
+        case 8: return launch_db_product_w<8>(c, db, nq, st);
+        case 16: return launch_db_product_w<16>(c, db, nq, st);
+        default: return launch_db_product_w<4>(c, db, nq, st);
+    }
 }
 
 bool mfma_shape(const kspir_database* db) {
@@ -1055,9 +1067,26 @@ int kspir_context_set(kspir_context* c, const char* key, int64_t value) {
         c->fused_pack = value != 0;
     } else if (k == "graphs") {
         c->use_graphs = value != 0;
+    } else if (k == "db_waves") {
+        if (value != 2 && value != 4 && value != 8 && value != 16)
+            return fail(KSPIR_EINVAL, "db_waves must be 2, 4, 8 or 16");
+        c->db_waves = (int)value;
     } else {
         return fail(KSPIR_EINVAL, "unknown option " + k);
     }
+    return KSPIR_OK;
+}
+
+int kspir_context_get(kspir_context* c, const char* key, int64_t* value) {
+    if (!c || !key || !value) return fail(KSPIR_EINVAL, "null argument");
+    const std::string k(key);
+    if (k == "mfma_min_batch") *value = c->mfma_min_batch;
+    else if (k == "mfma_max_chunk") *value = c->mfma_max_chunk;
+    else if (k == "fused_min_batch") *value = c->fused_min_batch;
+    else if (k == "fused_pack") *value = c->fused_pack;
+    else if (k == "graphs") *value = c->use_graphs;
+    else if (k == "db_waves") *value = c->db_waves;
+    else return fail(KSPIR_EINVAL, "unknown option " + k);
     return KSPIR_OK;
 }
 

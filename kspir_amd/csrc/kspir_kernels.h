@@ -563,15 +563,15 @@ __global__ void __launch_bounds__(kNttThreads) k_baby_mid_digits(const uint32_t*
 // ------------------------------------------------------------- database
 
 // crt.cpp:643-768 restated for a slot-minor layout: one lane per NTT slot z.
-// A workgroup is 4 waves on the same 64 slots; wave w takes rows w, w+4, ...
-// so the rotation words of a slot are fetched once per workgroup (L1) and
-// every database word exactly once.  Sums are exact u64 (N1 * (q-1)^2 < 2^64
-// for N1 <= 256) and reduced once per output.
+// A workgroup is WAVES waves on the same 64 slots; wave w takes rows w,
+// w+WAVES, ... (RPW of them) so the rotation words of a slot are fetched
+// once per workgroup (L1) and every database word exactly once.  Sums are
+// exact u64 (N1 * (q-1)^2 < 2^64 for N1 <= 256) and reduced once per output.
 // db: [N2][N1][N] packed (x mod q1) | (x mod q2) << 32 for one block
 // ctw: [N1][2 ab][2 p][N] rotation residues, slot-minor (k = 0..N1-1)
-// out: [N2][2 ab][2 primes][N] residues.  grid (N/64, r, nq), 256 threads.
-template <int RPW>
-__global__ void __launch_bounds__(256) k_db_product(const uint64_t* __restrict__ db, const uint32_t* __restrict__ ctw,
+// out: [N2][2 ab][2 primes][N] residues.  grid (N/64, r, nq), 64*WAVES threads.
+template <int RPW, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES) k_db_product(const uint64_t* __restrict__ db, const uint32_t* __restrict__ ctw,
                                                     uint32_t* __restrict__ out, int n1, int n2,
                                                     size_t block_words, size_t out_block_words, QS S, Tabs T) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -581,8 +581,8 @@ __global__ void __launch_bounds__(256) k_db_product(const uint64_t* __restrict__
     const uint32_t* ct = ctw + qi * S.ctw + z;
     const uint64_t* dbb = db + blk * block_words + z;
     uint32_t* ob = out + qi * S.dbout + blk * out_block_words + z;
-    // RPW = N2 / 4 for N2 >= 4, so every row of the wave exists; for N2 < 4
-    // (RPW = 1) the waves past the last row have nothing to do
+    // RPW = N2 / WAVES for N2 >= WAVES, so every row of the wave exists; for
+    // N2 < WAVES (RPW = 1) the waves past the last row have nothing to do
     if (wv >= n2) return;
     uint64_t a1[RPW], a2[RPW], b1[RPW], b2[RPW];
 #pragma unroll
@@ -594,7 +594,7 @@ __global__ void __launch_bounds__(256) k_db_product(const uint64_t* __restrict__
         const uint32_t ca1 = c[0], ca2 = c[kN], cb1 = c[2 * kN], cb2 = c[3 * kN];
 #pragma unroll
         for (int i = 0; i < RPW; ++i) {
-            const int row = wv + 4 * i;
+            const int row = wv + WAVES * i;
             const uint64_t d = __builtin_nontemporal_load(dbb + ((size_t)row * n1 + k) * kN);
             const uint32_t d1 = (uint32_t)d, d2 = (uint32_t)(d >> 32);
             a1[i] += (uint64_t)d1 * ca1;
@@ -605,7 +605,7 @@ __global__ void __launch_bounds__(256) k_db_product(const uint64_t* __restrict__
     }
 #pragma unroll
     for (int i = 0; i < RPW; ++i) {
-        const int row = wv + 4 * i;
+        const int row = wv + WAVES * i;
         uint32_t* o = ob + (size_t)row * 4 * kN;
         o[0 * kN] = (uint32_t)(a1[i] % kQ1);
         o[1 * kN] = (uint32_t)(a2[i] % kQ2);

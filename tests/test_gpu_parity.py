@@ -115,6 +115,22 @@ def test_stage_entry_points(ctx, oracle):
     keys.close()
 
 
+@pytest.mark.parametrize("waves", [2, 8, 16])
+def test_db_pass_workgroup_shapes(ctx, oracle, waves):
+    """every workgroup shape of the single-query database pass gives the
+    oracle's response (r 4, N1 64: N2 32 rows over 2..16 waves)"""
+    say(f"[gpu] db_waves={waves} r=4 n1=64 seed=4")
+    inst, resp, _ = instance(oracle, 4, 64, 4)
+    db, keys = upload(ctx, inst)
+    ctx.set("db_waves", waves)
+    try:
+        assert np.array_equal(ctx.answer(db, keys, inst.query), resp)
+    finally:
+        ctx.set("db_waves", 4)
+    db.close()
+    keys.close()
+
+
 def test_keys_and_db_shape_checks(ctx, oracle):
     """mismatched handles and bad shapes are rejected with an error, not run"""
     import kspir_amd
@@ -390,6 +406,40 @@ def test_graph_replays_match_eager(ctx, oracle):
     for k in keys:
         k.close()
     db.close()
+
+
+def test_autotune_sets_options_and_answers_stay_exact(ctx, oracle):
+    """the autotuner times every path for one database shape, leaves valid
+    options behind and a report of its timings; answers under the tuned
+    options still equal the oracle's and the single-query answers"""
+    say("[gpu] autotune r=4 n1=128 max_batch=8")
+    r, n1 = 4, 128
+    db_words = oracle.gen_db(r, n1, seed=61)
+    perms = oracle.perm_matrix(n1)
+    cl = [oracle.gen_client(r, n1, 610 + c, (17 * c + 3) % (N // 2), c % r) for c in range(3)]
+    db = ctx.database(db_words, r, n1)
+    keys = [ctx.keys(r, n1, x["baby"], x["giant"], x["packing"], x["rgsw"], perms) for x in cl]
+    try:
+        sb, rep = ctx.autotune(db, keys, max_batch=8)
+        assert 1 <= sb <= 8 and rep["serve_batch"] == sb
+        assert set(rep["db_waves_ms"]) == {"2", "4", "8", "16"} and ctx.get("db_waves") == rep["db_waves"]
+        assert ctx.get("mfma_min_batch") == rep["mfma_min_batch"]
+        assert ctx.get("fused_min_batch") == rep["fused_min_batch"]
+        assert all(v > 0 for v in rep["step_ms"].values())
+        pick = [0, 1, 2, 0, 1]
+        got = ctx.answer_batch(db, [keys[c] for c in pick], np.stack([cl[c]["query"] for c in pick]))
+        for j, c in enumerate(pick):
+            assert np.array_equal(got[j], ctx.answer(db, keys[c], cl[c]["query"])), j
+        want = oracle.answer(r=r, n1=n1, db=db_words, query=cl[1]["query"], rgsw=cl[1]["rgsw"], baby=cl[1]["baby"],
+                             giant=cl[1]["giant"], packing=cl[1]["packing"], perms=perms)
+        assert np.array_equal(got[1], want)
+    finally:
+        for k, v in (("mfma_min_batch", 2), ("mfma_max_chunk", 1 << 30), ("fused_min_batch", 16),
+                     ("fused_pack", 1), ("db_waves", 4)):
+            ctx.set(k, v)
+        for k in keys:
+            k.close()
+        db.close()
 
 
 def test_cpp_dropin_matches_oracle(oracle, tmp_path):
