@@ -207,6 +207,43 @@ def config_c4(args, log):
     return out
 
 
+def config_keygen(args, log):
+    """client side on the GPU: secrets, queries and all keys of 64 clients
+    at the 256 MB shape (r 16, N1 128), against the oracle's generation of
+    one client on the host"""
+    import kspir_amd
+    r, n1, nc = 16, 128, 64
+    ctx = kspir_amd.Context(0)
+    seeds = list(range(1, nc + 1))
+    tc = [(37 * i) % (N // 2) for i in range(nc)]
+    tp = [i % r for i in range(nc)]
+    ctx.client_generate(r, n1, seeds[:2], tc[:2], tp[:2])      # warm-up (tables, code)
+    reps = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        ctx.client_generate(r, n1, seeds, tc, tp)
+        reps.append(time.perf_counter() - t0)
+    ctx.close()
+    gpu_s = float(np.median(reps))
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle
+        t0 = time.perf_counter()
+        oracle.gen_client(r, n1, 1, tc[0], tp[0])
+        cpu = time.perf_counter() - t0
+    key_mb = (N + 4 * N + 8 * N + (n1 // 2) * 12 * N + (N // 2 // n1 - 1) * 4 * N + 4 * 8 * N) * 8 / 1e6
+    return {"metric": "client key generation (secret, query, RGSW, BSGS and packing keys) per second",
+            "value": round(nc / gpu_s, 2), "unit": "clients/s", "n_gpus": 1, "higher_is_better": True,
+            "dtype": "uint64", "data": "synthetic seeds 1..64",
+            "config": {"workload": f"{nc} clients at r={r}, N1={n1}, glibc rand() stream by jump-ahead on the GPU, "
+                                   f"{key_mb:.1f} MB of key and query words per client returned to host memory",
+                       "clients": nc},
+            "ms_per_batch": round(gpu_s * 1e3, 2),
+            "cpu_baseline": None if cpu is None else
+            {"value": round(1.0 / cpu, 3), "unit": "clients/s", "cores": 1, "kind": "port",
+             "sample": f"one client (seed 1) by the oracle on the host, {cpu * 1e3:.0f} ms"}}
+
+
 def config_c5(args, log):
     """C5: serving loop.  A resident 4 GB database (r=256) and, memory
     permitting, a 16 GB one (r=1024); 64 clients with their own keys; queries
@@ -304,8 +341,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="launch kernel by kernel instead of replaying hipGraphs")
     ap.add_argument("--no-autotune", action="store_true", help="keep the library's default kernel shapes and paths")
-    ap.add_argument("--config", default="c3", choices=("c3", "c2", "c4", "c5"),
-                    help="c3: the headline batched line (default); c2/c4/c5: the other workloads, one JSON line each")
+    ap.add_argument("--config", default="c3", choices=("c3", "c2", "c4", "c5", "keygen"),
+                    help="c3: the headline batched line (default); c2/c4/c5/keygen: the other workloads, one JSON line each")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the KsPIR answer here runs on a single MI355X "
@@ -314,7 +351,7 @@ def main():
 
     log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)
     if args.config != "c3":
-        fn = {"c2": config_c2, "c4": config_c4, "c5": lambda a, l: config_c5(a, l)}[args.config]
+        fn = {"c2": config_c2, "c4": config_c4, "c5": lambda a, l: config_c5(a, l), "keygen": config_keygen}[args.config]
         print(json.dumps(fn(args, log)), flush=True)
         return
 

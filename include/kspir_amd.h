@@ -159,6 +159,25 @@ int kspir_answer_inspect(kspir_context* ctx, const kspir_database* db, const ksp
 int kspir_decrypt_bsgs_total(kspir_context* ctx, int32_t r, int32_t nq, const uint64_t* secrets,
                              const uint64_t* responses, uint64_t* messages);
 
+/* Client side of test-pir.cpp:20-24,68-89 for nc clients at once, on the
+ * GPU.  For client i, as after srand(seeds[i]): the Secret (secret.cpp:13-60),
+ * query_bsgs_rns(target_col[i]) (twosteps.h:26, twosteps.cpp:279-318),
+ * RGSWCiphertext::keyGen(secret, target_packing[i]) (lwe.h:141,
+ * lwe.cpp:297-352), AutoKeyBSGSRNS::bsgsKeyGen (lwe.h:434, lwe.cpp:1143-1160)
+ * and AutoKey::keyGen(secret, r) for packing (lwe.h:213, lwe.cpp:509-550),
+ * drawing glibc rand() in exactly the reference's order (the generator's
+ * stream is produced on the device by jump-ahead).  Host outputs, any may be
+ * NULL:  secrets [nc][N] (coefficients mod crtMod), queries [nc][2][2][N],
+ * rgsw [nc][4][2][N], baby [nc][N1/2][6][2][N], giant [nc][N2-1][2][2][N],
+ * packing [nc][log2 r][4][2][N]: the layouts kspir_keys_create and
+ * kspir_answer take.  The query message rounding reproduces the
+ * reference's long double round(m * crtMod / p). */
+int kspir_client_generate(kspir_context* ctx, int32_t r, int32_t n1, int32_t nc, const uint32_t* seeds,
+                          const int32_t* target_col, const int32_t* target_packing, uint64_t* secrets,
+                          uint64_t* queries, uint64_t* rgsw, uint64_t* baby, uint64_t* giant, uint64_t* packing);
+/* rand() draws one client of shape (r, N1) consumes after srand (host-only) */
+int kspir_client_draws(int32_t r, int32_t n1, uint64_t* draws);
+
 /* Serving loop (no counterpart in the reference, whose driver answers one
  * query at a time).  A worker thread answers submitted queries in batches:
  * whenever the GPU is free it takes the database of the oldest waiting query

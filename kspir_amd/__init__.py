@@ -42,7 +42,7 @@ def _load():
                  "kspir_database_create_from_records", "kspir_database_tobsgsntt", "kspir_database_export",
                  "kspir_server_create", "kspir_server_submit", "kspir_server_wait", "kspir_server_stats",
                  "kspir_query_footprint_bytes", "kspir_decrypt_bsgs_total", "kspir_context_get",
-                 "kspir_autotune"):
+                 "kspir_autotune", "kspir_client_generate", "kspir_client_draws"):
         getattr(L, name).restype = ctypes.c_int
     L.kspir_autotune.argtypes = [_vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                  ctypes.POINTER(ctypes.c_int32), ctypes.c_char_p, ctypes.c_size_t]
@@ -67,6 +67,7 @@ EXPORTS = (
     "kspir_time_copy", "kspir_database_create_from_records", "kspir_database_tobsgsntt", "kspir_database_export",
     "kspir_server_create", "kspir_server_submit", "kspir_server_wait", "kspir_server_stats", "kspir_server_destroy",
     "kspir_query_footprint_bytes", "kspir_decrypt_bsgs_total", "kspir_context_get", "kspir_autotune",
+    "kspir_client_generate", "kspir_client_draws",
 )
 
 
@@ -232,6 +233,25 @@ class Context:
 
     def set(self, key, value):
         _check(lib.kspir_context_set(self.h, key.encode(), ctypes.c_int64(value)))
+
+    def client_generate(self, r, n1, seeds, target_cols, target_packings):
+        """secrets, queries and keys of len(seeds) clients on the GPU, each as
+        after srand(seed) in test-pir; returns a dict of arrays in the
+        reference layouts (leading axis: client)"""
+        nc = len(seeds)
+        n2, lr = N // 2 // n1, int(r).bit_length() - 1
+        out = {"secret": np.zeros((nc, N), np.uint64), "query": np.zeros((nc, 2, 2, N), np.uint64),
+               "rgsw": np.zeros((nc, 4, 2, N), np.uint64), "baby": np.zeros((nc, n1 // 2, 6, 2, N), np.uint64),
+               "giant": np.zeros((nc, max(n2 - 1, 0), 2, 2, N), np.uint64),
+               "packing": np.zeros((nc, lr, 4, 2, N), np.uint64)}
+        sd = np.ascontiguousarray(seeds, dtype=np.uint32)
+        tc = np.ascontiguousarray(target_cols, dtype=np.int32)
+        tp = np.ascontiguousarray(target_packings, dtype=np.int32)
+        _check(lib.kspir_client_generate(self.h, int(r), int(n1), nc, sd.ctypes.data_as(_vp), tc.ctypes.data_as(_vp),
+                                         tp.ctypes.data_as(_vp), *[_p64(out[k]) for k in
+                                                                   ("secret", "query", "rgsw", "baby", "giant",
+                                                                    "packing")]))
+        return out
 
     def get(self, key):
         v = ctypes.c_int64(0)

@@ -15,6 +15,8 @@
 #include <vector>
 
 #include "kspir_kernels.h"
+#include "kspir_client.h"
+#include "kspir_rng.h"
 #include "../../include/kspir_amd.h"
 #include "../../include/kspir_amd_runtime.h"
 
@@ -123,6 +125,10 @@ struct kspir_context {
     };
     std::vector<Graph> graphs;
     int use_graphs = 1;
+    // client generation: M^kRngChunk of the rand() window, and the query
+    // encoding round(m * Q / p) in long double for every m < p
+    uint32_t* rng_jump = nullptr;
+    uint64_t* enc_tab = nullptr;
 };
 
 static void drop_graphs(kspir_context* c, const void* only = nullptr) {
@@ -625,6 +631,8 @@ void kspir_context_destroy(kspir_context* c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     drop_graphs(c);
+    if (c->rng_jump) hipFree(c->rng_jump);
+    if (c->enc_tab) hipFree(c->enc_tab);
     for (void* p : c->tables) hipFree(p);
     if (c->ws) hipFree(c->ws);
     if (c->kp_dev) hipFree(c->kp_dev);
@@ -1278,6 +1286,200 @@ struct kspir_server {
 };
 
 extern "C" {
+
+int kspir_client_draws(int32_t r, int32_t n1, uint64_t* draws) {
+    if (!draws || !pow2(r) || r > kN || !pow2(n1) || n1 < 2 || n1 > kN / 2)
+        return fail(KSPIR_EINVAL, "r must be a power of two <= N, N1 a power of two in [2, N/2]");
+    const int n2 = kN / 2 / n1;
+    *draws = (uint64_t)kN * (18 + 15 * (n1 / 2) + 6 * (n2 - 1) + 12 * ilog2(r));
+    return KSPIR_OK;
+}
+
+int kspir_client_generate(kspir_context* c, int32_t r, int32_t n1, int32_t nc, const uint32_t* seeds,
+                          const int32_t* target_col, const int32_t* target_packing, uint64_t* secrets,
+                          uint64_t* queries, uint64_t* rgsw, uint64_t* baby, uint64_t* giant, uint64_t* packing) {
+    if (!c || nc < 1 || !seeds || !target_col || !target_packing) return fail(KSPIR_EINVAL, "null argument");
+    uint64_t draws = 0;
+    int rc = kspir_client_draws(r, n1, &draws);
+    if (rc) return rc;
+    const int n2 = kN / 2 / n1, nb = n1 / 2, lr = ilog2(r);
+    for (int i = 0; i < nc; ++i)
+        if (target_col[i] < 0 || target_col[i] >= kN / 2 || target_packing[i] < 0 || target_packing[i] >= r)
+            return fail(KSPIR_EINVAL, "target column must be < N/2 and target packing < r");
+    HIP_OK(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    if (!c->rng_jump) {
+        const kspir_rng::Mat jm = kspir_rng::power(kspir::kRngChunk);
+        HIP_OK(hipMalloc(&c->rng_jump, jm.size() * 4));
+        HIP_OK(hipMemcpy(c->rng_jump, jm.data(), jm.size() * 4, hipMemcpyHostToDevice));
+        std::vector<uint64_t> enc(kP);
+        for (uint32_t m = 0; m < kP; ++m) enc[m] = (uint64_t)roundl((long double)m * kQ / kP);
+        HIP_OK(hipMalloc(&c->enc_tab, enc.size() * 8));
+        HIP_OK(hipMemcpy(c->enc_tab, enc.data(), enc.size() * 8, hipMemcpyHostToDevice));
+    }
+    const int nchunks = (int)((draws + kspir::kRngChunk - 1) / kspir::kRngChunk);
+    const size_t stride = (size_t)nchunks * kspir::kRngChunk;
+    // host inputs: seed windows and the two message slots of each query
+    // (compute_query_encode, utils.cpp:1657-1675: slot of psi^(5^i), psi^(-5^i))
+    std::vector<uint32_t> seedw((size_t)nc * kspir_rng::kLag);
+    for (int i = 0; i < nc; ++i) kspir_rng::seed_window(seeds[i], &seedw[(size_t)i * kspir_rng::kLag]);
+    std::vector<int32_t> slot_of(kN), slots(2 * (size_t)nc);
+    for (int s = 0; s < kN; ++s) slot_of[(2 * bitrev12(s) + 1) >> 1] = s;
+    std::vector<uint32_t> rot(kN);
+    rot[0] = 1;
+    rot[kN - 1] = 2 * kN - 1;
+    for (int i = 1; i < kN / 2; ++i) {
+        rot[i] = rot[i - 1] * 5 % (2 * kN);
+        rot[kN - 1 - i] = rot[kN - i] * 5 % (2 * kN);
+    }
+    for (int i = 0; i < nc; ++i) {
+        slots[2 * i] = slot_of[rot[target_col[i]] >> 1];
+        slots[2 * i + 1] = slot_of[rot[kN - 1 - target_col[i]] >> 1];
+    }
+    // device buffers
+    const size_t n_sec = (size_t)nc * kN, n_q = (size_t)nc * 4 * kN, n_rg = (size_t)nc * 8 * kN;
+    const size_t n_baby = (size_t)nc * nb * 12 * kN, n_giant = (size_t)nc * (n2 - 1) * 4 * kN;
+    const size_t n_pack = (size_t)nc * lr * 8 * kN;
+    std::vector<void*> mem;
+    auto alloc = [&](void** p, size_t bytes) -> hipError_t {
+        hipError_t e = hipMalloc(p, bytes ? bytes : 8);
+        if (e == hipSuccess) mem.push_back(*p);
+        return e;
+    };
+    auto cleanup = [&](int code) {
+        hipStreamSynchronize(st);
+        for (void* p : mem) hipFree(p);
+        return code;
+    };
+#define C_OK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return cleanup(fail_hip(#x, e_)); } while (0)
+    uint32_t *d_seedw, *d_win, *d_stream, *d_sm;
+    int32_t *d_scoef, *d_slots;
+    uint64_t *d_sec, *d_q, *d_rg, *d_baby, *d_giant, *d_pack;
+    kspir::ClientJob* d_jobs;
+    C_OK(alloc((void**)&d_seedw, seedw.size() * 4));
+    C_OK(alloc((void**)&d_win, (size_t)nc * nchunks * kspir_rng::kLag * 4));
+    C_OK(alloc((void**)&d_stream, (size_t)nc * stride * 4));
+    C_OK(alloc((void**)&d_scoef, n_sec * 4));
+    C_OK(alloc((void**)&d_sm, n_sec * 3 * 4));
+    C_OK(alloc((void**)&d_slots, slots.size() * 4));
+    C_OK(alloc((void**)&d_sec, n_sec * 8));
+    C_OK(alloc((void**)&d_q, n_q * 8));
+    C_OK(alloc((void**)&d_rg, n_rg * 8));
+    C_OK(alloc((void**)&d_baby, n_baby * 8));
+    C_OK(alloc((void**)&d_giant, n_giant * 8));
+    C_OK(alloc((void**)&d_pack, n_pack * 8));
+    // encryption jobs, with the stream offsets of the reference's draw order:
+    // secret [0, N), query [N, 6N), RGSW from 6N, baby keys from 18N, giant
+    // keys after them, packing keys last (test-pir.cpp:20-24, 68-89)
+    auto inv2n = [](uint32_t x) {   // odd x: x^(N-1) is its inverse mod 2N (the unit group has order N)
+        uint32_t r = 1, b = x & (2 * kN - 1);
+        for (uint32_t e = kN - 1; e; e >>= 1, b = (b * b) & (2 * kN - 1))
+            if (e & 1) r = (r * b) & (2 * kN - 1);
+        return r;
+    };
+    auto pow2mod = [](uint32_t k, uint64_t q) {
+        uint64_t v = 1;
+        for (uint32_t i = 0; i < k; ++i) v = (v << 1) % q;
+        return v;
+    };
+    std::vector<kspir::ClientJob> jobs;
+    const uint64_t N = kN;
+    for (int ci = 0; ci < nc; ++ci) {
+        auto job = [&](int32_t type, uint64_t draw, int32_t msg, uint32_t mfexp, uint32_t num, uint64_t* a,
+                       uint64_t* b, uint64_t* a2, uint64_t* b2) {
+            kspir::ClientJob J{};
+            J.type = type;
+            J.client = ci;
+            J.draw = (uint32_t)draw;
+            J.msg = msg;
+            J.msg_inv = msg >= 1 ? inv2n((uint32_t)msg) : 0;
+            J.mf[0] = (uint32_t)pow2mod(mfexp, kQ1);
+            J.mf[1] = (uint32_t)pow2mod(mfexp, kQ2);
+            J.mf_bs = (uint32_t)pow2mod(mfexp, kBs);
+            J.num = num;
+            J.a = a; J.b = b; J.a2 = a2; J.b2 = b2;
+            jobs.push_back(J);
+        };
+        // RGSW(X^-idx), lwe.cpp:297-352: gadget 2^20, 2^38; row 2+i then row i
+        uint64_t* rg = d_rg + (size_t)ci * 8 * N;
+        for (int i = 0; i < 2; ++i) {
+            const uint64_t d = 6 * N + 6 * N * i;
+            job(kspir::kJobRlwe, d, -target_packing[ci], 20 + 18 * i, 1, rg + (2 + i) * 2 * N, rg + (2 + i) * 2 * N + N,
+                nullptr, nullptr);
+            job(kspir::kJobRgswNeg, d + 3 * N, -target_packing[ci], 20 + 18 * i, 1, rg + i * 2 * N, rg + i * 2 * N + N,
+                nullptr, nullptr);
+        }
+        // baby keys (lwe.cpp:1025-1070): index 5^(k+1), gadget 2^20, 2^40, 2^60
+        uint64_t* bk = d_baby + (size_t)ci * nb * 12 * N;
+        for (int k = 0; k < nb; ++k) {
+            uint32_t idx = 1;
+            for (int t = 0; t <= k; ++t) idx = idx * 5 % (2 * kN);
+            for (int i = 0; i < 3; ++i) {
+                uint64_t* kq = bk + ((size_t)k * 6 + i) * 2 * N;
+                uint64_t* kb = bk + ((size_t)k * 6 + 3 + i) * 2 * N;
+                job(kspir::kJobBaby, 18 * N + 15 * N * k + 5 * N * i, (int32_t)idx, 20 * (i + 1), 1, kq, kq + N, kb,
+                    kb + N);
+            }
+        }
+        // giant keys (lwe.cpp:1072-1101): index 5^(N1*(k+1)), gadget 2^20, 2^38
+        const uint64_t g0 = 18 * N + 15 * N * nb;
+        uint64_t* gk = d_giant + (size_t)ci * (n2 - 1) * 4 * N;
+        for (int k = 0; k < n2 - 1; ++k) {
+            uint32_t idx = 1;
+            for (int t = 0; t < n1 * (k + 1); ++t) idx = idx * 5 % (2 * kN);
+            for (int i = 0; i < 2; ++i) {
+                uint64_t* o = gk + ((size_t)k * 2 + i) * 2 * N;
+                job(kspir::kJobRlwe, g0 + 6 * N * k + 3 * N * i, (int32_t)idx, 20 + 18 * i, 1, o, o + N, nullptr,
+                    nullptr);
+            }
+        }
+        // packing keys (lwe.cpp:472-550): level li, index N/2^li + 1,
+        // gadget 1, 2^14, 2^28, 2^42, error times r
+        const uint64_t p0 = g0 + 6 * N * (n2 - 1);
+        uint64_t* pk = d_pack + (size_t)ci * lr * 8 * N;
+        for (int li = 0; li < lr; ++li)
+            for (int j = 0; j < 4; ++j) {
+                uint64_t* o = pk + ((size_t)li * 4 + j) * 2 * N;
+                job(kspir::kJobRlwe, p0 + 12 * N * li + 3 * N * j, (int32_t)(kN / (1 << li) + 1), 14 * j, (uint32_t)r,
+                    o, o + N, nullptr, nullptr);
+            }
+    }
+    C_OK(alloc((void**)&d_jobs, jobs.size() * sizeof(kspir::ClientJob)));
+    C_OK(hipMemcpyAsync(d_seedw, seedw.data(), seedw.size() * 4, hipMemcpyHostToDevice, st));
+    C_OK(hipMemcpyAsync(d_slots, slots.data(), slots.size() * 4, hipMemcpyHostToDevice, st));
+    C_OK(hipMemcpyAsync(d_jobs, jobs.data(), jobs.size() * sizeof(kspir::ClientJob), hipMemcpyHostToDevice, st));
+    kspir::ClientConsts K{};
+    const uint32_t qs[3] = {kQ1, kQ2, kBs};
+    for (int p = 0; p < 3; ++p) K.r2[p] = (uint32_t)(((u128)1 << 64) % qs[p]);
+    K.bs_mod_q[0] = kBs % kQ1;
+    K.bs_mod_q[1] = kBs % kQ2;
+    const Tabs& T = c->T;
+    hipLaunchKernelGGL(kspir::k_rng_windows, dim3(nc), dim3(64), 0, st, d_seedw, c->rng_jump, d_win, nchunks);
+    if (hipGetLastError() != hipSuccess) return cleanup(fail(KSPIR_EHIP, "k_rng_windows launch"));
+    hipLaunchKernelGGL(kspir::k_rng_stream, dim3((nc * nchunks + 63) / 64), dim3(64), 0, st, d_win, d_stream, nc,
+                       nchunks);
+    if (hipGetLastError() != hipSuccess) return cleanup(fail(KSPIR_EHIP, "k_rng_stream launch"));
+    hipLaunchKernelGGL(kspir::k_client_secret, dim3(nc), dim3(kNttThreads), 0, st, d_stream, stride, d_scoef, d_sm, K,
+                       T);
+    if (hipGetLastError() != hipSuccess) return cleanup(fail(KSPIR_EHIP, "k_client_secret launch"));
+    hipLaunchKernelGGL(kspir::k_client_query, dim3(nc), dim3(kNttThreads), 0, st, d_stream, stride, d_sm, d_slots,
+                       c->enc_tab, d_q, K, T);
+    if (hipGetLastError() != hipSuccess) return cleanup(fail(KSPIR_EHIP, "k_client_query launch"));
+    hipLaunchKernelGGL(kspir::k_client_rlwe, dim3((unsigned)jobs.size()), dim3(kNttThreads), 0, st, d_jobs, d_stream,
+                       stride, d_scoef, d_sm, K, T);
+    if (hipGetLastError() != hipSuccess) return cleanup(fail(KSPIR_EHIP, "k_client_rlwe launch"));
+    hipLaunchKernelGGL(kspir::k_client_secret_words, dim3(grid1d(n_sec)), dim3(256), 0, st, d_scoef, d_sec, n_sec);
+    if (hipGetLastError() != hipSuccess) return cleanup(fail(KSPIR_EHIP, "k_client_secret_words launch"));
+    if (secrets) C_OK(hipMemcpyAsync(secrets, d_sec, n_sec * 8, hipMemcpyDeviceToHost, st));
+    if (queries) C_OK(hipMemcpyAsync(queries, d_q, n_q * 8, hipMemcpyDeviceToHost, st));
+    if (rgsw) C_OK(hipMemcpyAsync(rgsw, d_rg, n_rg * 8, hipMemcpyDeviceToHost, st));
+    if (baby) C_OK(hipMemcpyAsync(baby, d_baby, n_baby * 8, hipMemcpyDeviceToHost, st));
+    if (giant && n_giant) C_OK(hipMemcpyAsync(giant, d_giant, n_giant * 8, hipMemcpyDeviceToHost, st));
+    if (packing && n_pack) C_OK(hipMemcpyAsync(packing, d_pack, n_pack * 8, hipMemcpyDeviceToHost, st));
+    C_OK(hipStreamSynchronize(st));
+#undef C_OK
+    return cleanup(KSPIR_OK);
+}
 
 int kspir_decrypt_bsgs_total(kspir_context* c, int32_t r, int32_t nq, const uint64_t* secrets,
                              const uint64_t* responses, uint64_t* messages) {

@@ -442,6 +442,32 @@ def test_autotune_sets_options_and_answers_stay_exact(ctx, oracle):
         db.close()
 
 
+@pytest.mark.parametrize("r,n1,seeds", [(4, 64, (21, 22, 23)), (16, 128, (5,)), (1, 256, (9,))])
+def test_client_generation_matches_oracle(ctx, oracle, r, n1, seeds):
+    """secret, query, RGSW, baby/giant and packing keys generated on the GPU
+    (rand() stream by jump-ahead) equal the oracle's client generation word
+    for word; the GPU-made client's answer equals the oracle's"""
+    say(f"[gpu] client generation r={r} n1={n1} clients={len(seeds)}")
+    tcols = [(97 * i + 13) % (N // 2) for i in range(len(seeds))]
+    tpacks = [(5 * i + 1) % r for i in range(len(seeds))]
+    got = ctx.client_generate(r, n1, seeds, tcols, tpacks)
+    for i, sd in enumerate(seeds):
+        want = oracle.gen_client(r, n1, sd, tcols[i], tpacks[i])
+        for k in ("secret", "query", "rgsw", "baby", "giant", "packing"):
+            assert np.array_equal(got[k][i].reshape(want[k].shape), want[k]), (sd, k)
+    if r == 4:
+        db_words = oracle.gen_db(r, n1, seed=77)
+        perms = oracle.perm_matrix(n1)
+        db = ctx.database(db_words, r, n1)
+        keys = ctx.keys(r, n1, got["baby"][0], got["giant"][0], got["packing"][0], got["rgsw"][0], perms)
+        resp = ctx.answer(db, keys, got["query"][0])
+        want = oracle.answer(r=r, n1=n1, db=db_words, query=got["query"][0], rgsw=got["rgsw"][0],
+                             baby=got["baby"][0], giant=got["giant"][0], packing=got["packing"][0], perms=perms)
+        assert np.array_equal(resp, want)
+        keys.close()
+        db.close()
+
+
 def test_cpp_dropin_matches_oracle(oracle, tmp_path):
     """tests/native/dropin_test calls matrix_vector_mul_bsgs_rns_crt_large,
     externalProduct and packingRLWEs through kspir_pir.h exactly as
