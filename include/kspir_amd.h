@@ -178,6 +178,30 @@ int kspir_client_generate(kspir_context* ctx, int32_t r, int32_t n1, int32_t nc,
 /* rand() draws one client of shape (r, N1) consumes after srand (host-only) */
 int kspir_client_draws(int32_t r, int32_t n1, uint64_t* draws);
 
+/* The same client operations one at a time, on an explicit secret and on
+ * rand() outputs the caller drew in the reference's order (the C++ drop-ins
+ * of kspir_pir.h call rand() themselves, so the caller's global generator
+ * advances exactly as with the reference).  secret: [N] coefficients mod
+ * crtMod (0, 1 or crtMod-1); draws: ndraws rand() outputs; out (host):
+ *   KSPIR_CLIENT_QUERY   query_bsgs_rns(row = param): 5N draws, [2][2][N]
+ *   KSPIR_CLIENT_RGSW    RGSWCiphertext::keyGen(index = param): 12N draws, [4][2][N]
+ *   KSPIR_CLIENT_BSGS    bsgsKeyGen(N1 = n1): 15N*N1/2 + 6N*(N2-1) draws,
+ *                        baby [N1/2][6][2][N] then giant [N2-1][2][2][N]
+ *   KSPIR_CLIENT_PACKING AutoKey::keyGen(num = param, packing_rlwe = true):
+ *                        indices N/i+1 for i = 1, 2, 4, .. < num; 12N draws and
+ *                        [4][2][N] per index
+ *   KSPIR_CLIENT_RGSW_FORWARD    keyGen(index, is_reverse = false): message X^index
+ *   KSPIR_CLIENT_PACKING_LWE     keyGen(num, packing_rlwe = false): indices
+ *                        i+1 for i = 2, 4, .. <= num+1 (lwe.cpp:509-550) */
+#define KSPIR_CLIENT_QUERY 0
+#define KSPIR_CLIENT_RGSW 1
+#define KSPIR_CLIENT_BSGS 2
+#define KSPIR_CLIENT_PACKING 3
+#define KSPIR_CLIENT_RGSW_FORWARD 4
+#define KSPIR_CLIENT_PACKING_LWE 5
+int kspir_client_encrypt(kspir_context* ctx, int32_t kind, const uint64_t* secret, int64_t param, int32_t n1,
+                         const uint32_t* draws, uint64_t ndraws, uint64_t* out);
+
 /* Serving loop (no counterpart in the reference, whose driver answers one
  * query at a time).  A worker thread answers submitted queries in batches:
  * whenever the GPU is free it takes the database of the oldest waiting query

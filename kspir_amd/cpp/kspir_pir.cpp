@@ -12,6 +12,7 @@
 #include <string>
 
 #include "../../include/kspir_amd.h"
+#include "../../include/kspir_amd_runtime.h"
 
 using kspir_params::N;
 
@@ -34,6 +35,18 @@ namespace {
 
 std::mutex g_mu;
 kspir_context* g_ctx = nullptr;
+
+// While the library runs (device set-up included), rand() is pointed at a
+// private state (initstate), and the caller's state is restored afterwards
+// (setstate), so runtime code that draws rand() cannot move the caller's
+// stream: the drop-ins advance it exactly as the reference does, and only
+// by their own draws.
+struct RandShield {
+    char own[128];
+    char* prev;
+    RandShield() : prev(initstate(1, own, sizeof own)) {}
+    ~RandShield() { setstate(prev); }
+};
 
 void check(int rc, const char* what) {
     if (rc != 0) throw std::runtime_error(std::string(what) + ": " + kspir_last_error());
@@ -196,6 +209,7 @@ void matrix_vector_mul_bsgs_rns_crt_large(std::vector<RlweCiphertext>& result, s
                                           uint64_t* data, const AutoKeyBSGSRNS& autoKey, const int32_t N1,
                                           std::vector<std::vector<int32_t> >& permutations, int32_t r) {
     std::lock_guard<std::mutex> lock(g_mu);
+    RandShield shield;
     if (!data) throw std::runtime_error("matrix_vector_mul_bsgs_rns_crt_large: null database");
     if (!pow2(r) || !pow2(N1) || N1 < 32 || N1 > N / 2)
         throw std::runtime_error("matrix_vector_mul_bsgs_rns_crt_large: r and N1 must be powers of two, N1 in [32, N/2]");
@@ -223,6 +237,7 @@ void matrix_vector_mul_bsgs_rns_crt_large(std::vector<RlweCiphertext>& result, s
 
 void externalProduct(RlweCiphertext& result, RlweCiphertext& firstDim, const RGSWCiphertext& query) {
     std::lock_guard<std::mutex> lock(g_mu);
+    RandShield shield;
     if (!query.getIsNtt()) throw std::runtime_error("externalProduct: RGSW must be in NTT form");   // lwe.cpp:763
     kspir_keys* keys = rgsw_keys(query);
     std::vector<uint64_t> res(2 * (size_t)N), fd(2 * (size_t)N);
@@ -242,6 +257,7 @@ void externalProduct(RlweCiphertext& result, RlweCiphertext& firstDim, const RGS
 
 void packingRLWEs(RlweCiphertext& result, std::vector<RlweCiphertext>& rlwes, const AutoKey& autokey) {
     std::lock_guard<std::mutex> lock(g_mu);
+    RandShield shield;
     const int32_t r = (int32_t)rlwes.size();
     if (!pow2(r)) throw std::runtime_error("packingRLWEs: the number of RLWEs must be a power of two");
     const bool ntt = rlwes[0].getIsNtt();
@@ -267,6 +283,7 @@ void database_tobsgsntt(std::vector<std::vector<uint64_t> >& result, std::vector
         std::memcpy(&flat[(size_t)i * (N / 2)], data[i].data(), (N / 2) * 8);
     }
     std::lock_guard<std::mutex> lock(g_mu);
+    RandShield shield;
     check(kspir_database_tobsgsntt(ctx(), flat.data(), N1, out.data()), "database_tobsgsntt");
     for (int i = 0; i < N / 2; ++i) result[i].assign(out.begin() + (size_t)i * N, out.begin() + (size_t)(i + 1) * N);
 }
@@ -281,8 +298,213 @@ void database_tocrt(uint64_t* datacrt, std::vector<std::vector<uint64_t> >& data
             }
 }
 
+// ---------------------------------------------------------------- client
+
+namespace {
+
+// NTT of one polynomial on the GPU: mod crtMod as the CRT of the q1 and q2
+// transforms (the HEXL root of crtMod is the CRT of their roots), mod bsMod
+// directly
+void ntt_host(std::vector<uint64_t>& v, uint64_t modulus, bool inverse) {
+    using kspir_params::crtq1;
+    using kspir_params::crtq2;
+    std::vector<uint32_t> r((size_t)2 * N);
+    if (modulus == kspir_params::crtMod) {
+        for (int i = 0; i < N; ++i) {
+            r[i] = (uint32_t)(v[i] % crtq1);
+            r[N + i] = (uint32_t)(v[i] % crtq2);
+        }
+        check(kspir_test_ntt(ctx(), r.data(), 1, 0, inverse ? 1 : 0), "kspir_test_ntt");
+        check(kspir_test_ntt(ctx(), r.data() + N, 1, 1, inverse ? 1 : 0), "kspir_test_ntt");
+        // Garner: x = x1 + q1 * ((x2 - x1) * q1^-1 mod q2)
+        uint64_t inv = 1, b = crtq1 % crtq2, e = crtq2 - 2;
+        while (e) {
+            if (e & 1) inv = (unsigned __int128)inv * b % crtq2;
+            b = (unsigned __int128)b * b % crtq2;
+            e >>= 1;
+        }
+        for (int i = 0; i < N; ++i) {
+            const uint64_t x1 = r[i], x2 = r[N + i];
+            const uint64_t t = (unsigned __int128)((x2 + crtq2 - x1 % crtq2) % crtq2) * inv % crtq2;
+            v[i] = x1 + crtq1 * t;
+        }
+    } else if (modulus == kspir_params::bsMod) {
+        for (int i = 0; i < N; ++i) r[i] = (uint32_t)v[i];
+        check(kspir_test_ntt(ctx(), r.data(), 1, 2, inverse ? 1 : 0), "kspir_test_ntt");
+        for (int i = 0; i < N; ++i) v[i] = r[i];
+    } else {
+        throw std::runtime_error("Secret: only crtMod and bsMod are supported");
+    }
+}
+
+// rand() outputs in the reference's consumption order
+std::vector<uint32_t> draw(size_t n) {
+    std::vector<uint32_t> d(n);
+    for (auto& x : d) x = (uint32_t)rand();
+    return d;
+}
+
+const std::vector<uint64_t>& crt_secret(const Secret& s) {
+    if (const_cast<Secret&>(s).getModulus() != kspir_params::crtMod)
+        throw std::runtime_error("client keys need a secret mod crtMod");
+    return s.coefficients();
+}
+
+RlweCiphertext rlwe_from(const uint64_t* p, uint64_t modulus, bool ntt) {
+    RlweCiphertext c(N, modulus);
+    c.a.assign(p, p + N);
+    c.b.assign(p + N, p + 2 * N);
+    c.setIsNtt(ntt);
+    return c;
+}
+
+}  // namespace
+
+Secret::Secret(uint64_t module, bool ntt) : modulus(module), coeff(N), nttform(false) {
+    if (module != kspir_params::crtMod && module != kspir_params::bsMod)
+        throw std::runtime_error("Secret: only crtMod and bsMod are supported");
+    for (int i = 0; i < N; ++i) {   // secret.cpp:54-58
+        const int32_t t = rand() % 3 - 1;
+        coeff[i] = t < 0 ? (uint64_t)(t + (int64_t)module) : (uint64_t)t;
+    }
+    data = coeff;
+    if (ntt) toNttForm();
+}
+
+void Secret::toNttForm() {
+    if (nttform) return;
+    std::lock_guard<std::mutex> g(g_mu);
+    RandShield shield;
+    data = coeff;
+    ntt_host(data, modulus, false);
+    nttform = true;
+}
+
+void Secret::toCoeffForm() {
+    data = coeff;
+    nttform = false;
+}
+
+void query_bsgs_rns(std::vector<RlweCiphertext>& cipher, Secret& queryKey, uint64_t row) {
+    if (cipher.size() < 2) throw std::runtime_error("query_bsgs_rns: cipher needs the crtMod and bsMod ciphertexts");
+    if (row >= (uint64_t)N / 2) throw std::runtime_error("query_bsgs_rns: row must be < N/2");
+    const std::vector<uint64_t>& sc = crt_secret(queryKey);
+    std::vector<uint32_t> d = draw((size_t)5 * N);   // sample_random mod Q, mod bsMod, sample_guass
+    std::vector<uint64_t> out((size_t)4 * N);
+    std::lock_guard<std::mutex> g(g_mu);
+    RandShield shield;
+    check(kspir_client_encrypt(ctx(), KSPIR_CLIENT_QUERY, sc.data(), (int64_t)row, 0, d.data(), d.size(), out.data()),
+          "kspir_client_encrypt");
+    cipher[0] = rlwe_from(out.data(), kspir_params::crtMod, false);
+    cipher[1] = rlwe_from(out.data() + 2 * N, kspir_params::bsMod, false);
+}
+
+void RGSWCiphertext::keyGen(Secret& secret, const uint64_t index, bool is_reverse) {
+    if (index >= length) throw std::runtime_error("RGSWCiphertext::keyGen: index must be < N");
+    if (ellnum != 2 || PP != (1ULL << 20) || BBg != (1ULL << 18) || modulus != kspir_params::crtMod)
+        throw std::runtime_error("RGSWCiphertext::keyGen: only (crtMod, ell 2, 2^20, 2^18) is supported");
+    const std::vector<uint64_t>& sc = crt_secret(secret);
+    std::vector<uint32_t> d = draw((size_t)12 * N);
+    std::vector<uint64_t> out((size_t)8 * N);
+    {
+        std::lock_guard<std::mutex> g(g_mu);
+    RandShield shield;
+        check(kspir_client_encrypt(ctx(), is_reverse ? KSPIR_CLIENT_RGSW : KSPIR_CLIENT_RGSW_FORWARD, sc.data(),
+                                   (int64_t)index, 0, d.data(), d.size(), out.data()),
+              "kspir_client_encrypt");
+    }
+    for (int k = 0; k < 4; ++k) {
+        a[k].assign(out.data() + (size_t)k * 2 * N, out.data() + (size_t)k * 2 * N + N);
+        b[k].assign(out.data() + (size_t)k * 2 * N + N, out.data() + (size_t)(k + 1) * 2 * N);
+    }
+    isntt = true;
+    secret.toNttForm();   // lwe.cpp:317-320
+}
+
+void AutoKeyBSGSRNS::bsgsKeyGen(Secret& secret, int32_t N1) {
+    if (!pow2(N1) || N1 < 2 || N1 > N / 2) throw std::runtime_error("bsgsKeyGen: N1 must be a power of two <= N/2");
+    const int32_t N2 = N / 2 / N1, nb = N1 / 2;
+    const std::vector<uint64_t>& sc = crt_secret(secret);
+    std::vector<uint32_t> d = draw((size_t)15 * N * nb + (size_t)6 * N * (N2 - 1));
+    std::vector<uint64_t> out((size_t)nb * 12 * N + (size_t)(N2 - 1) * 4 * N);
+    {
+        std::lock_guard<std::mutex> g(g_mu);
+    RandShield shield;
+        check(kspir_client_encrypt(ctx(), KSPIR_CLIENT_BSGS, sc.data(), 0, N1, d.data(), d.size(), out.data()),
+              "kspir_client_encrypt");
+    }
+    for (int i = 1; i <= nb; ++i) {
+        std::vector<RlweCiphertext> v;
+        for (int j = 0; j < 6; ++j)
+            v.push_back(rlwe_from(out.data() + ((size_t)(i - 1) * 6 + j) * 2 * N,
+                                  j < 3 ? kspir_params::crtMod : kspir_params::bsMod, true));
+        keyMap.insert(std::make_pair((int32_t)pow5(i), v));
+    }
+    const uint64_t* g0 = out.data() + (size_t)nb * 12 * N;
+    for (int i = 1; i < N2; ++i) {
+        std::vector<RlweCiphertext> v;
+        for (int j = 0; j < 2; ++j)
+            v.push_back(rlwe_from(g0 + ((size_t)(i - 1) * 2 + j) * 2 * N, kspir_params::crtMod, true));
+        keyMap.insert(std::make_pair((int32_t)pow5((int64_t)N1 * i), v));
+    }
+}
+
+void AutoKey::keyGen(Secret& secret, const int32_t num, bool packing_rlwe) {
+    if (num < 1 || num > N) throw std::runtime_error("AutoKey::keyGen: num must be in [1, N]");
+    if (ellnum != 4 || PP != 0 || BBg != (1ULL << 14) || modulus != kspir_params::crtMod)
+        throw std::runtime_error("AutoKey::keyGen: only (crtMod, ell 4, 0, 2^14) is supported");
+    std::vector<int32_t> idx;
+    if (packing_rlwe)
+        for (int32_t i = 1; i < num; i <<= 1) idx.push_back(N / i + 1);
+    else
+        for (int32_t i = 2; i <= num + 1; i <<= 1) idx.push_back(i + 1);
+    const std::vector<uint64_t>& sc = crt_secret(secret);
+    std::vector<uint32_t> d = draw((size_t)12 * N * idx.size());
+    std::vector<uint64_t> out((size_t)8 * N * idx.size());
+    if (!idx.empty()) {
+        std::lock_guard<std::mutex> g(g_mu);
+    RandShield shield;
+        check(kspir_client_encrypt(ctx(), packing_rlwe ? KSPIR_CLIENT_PACKING : KSPIR_CLIENT_PACKING_LWE, sc.data(),
+                                   num, 0, d.data(), d.size(), out.data()),
+              "kspir_client_encrypt");
+    }
+    for (size_t li = 0; li < idx.size(); ++li) {
+        std::vector<RlweCiphertext> v;
+        for (int j = 0; j < 4; ++j) v.push_back(rlwe_from(out.data() + (li * 4 + j) * 2 * N, modulus, true));
+        keyMap.insert(std::make_pair(idx[li], v));
+    }
+}
+
+void decrypt_bsgs_total(std::vector<uint64_t>& message, RlweCiphertext& cipher, Secret& secret,
+                        const int32_t rlwesNum) {
+    const std::vector<uint64_t>& sc = crt_secret(secret);
+    if (cipher.a.size() != (size_t)N || cipher.b.size() != (size_t)N)
+        throw std::runtime_error("decrypt_bsgs_total: RLWE of wrong length");
+    std::vector<uint64_t> resp((size_t)2 * N);
+    std::copy(cipher.a.begin(), cipher.a.end(), resp.begin());
+    std::copy(cipher.b.begin(), cipher.b.end(), resp.begin() + N);
+    message.assign(N, 0);
+    std::lock_guard<std::mutex> g(g_mu);
+    RandShield shield;
+    if (cipher.getIsNtt()) {   // the C ABI takes coefficient form
+        std::vector<uint64_t> a(cipher.a), b(cipher.b);
+        ntt_host(a, kspir_params::crtMod, true);
+        ntt_host(b, kspir_params::crtMod, true);
+        std::copy(a.begin(), a.end(), resp.begin());
+        std::copy(b.begin(), b.end(), resp.begin() + N);
+    }
+    check(kspir_decrypt_bsgs_total(ctx(), rlwesNum, 1, sc.data(), resp.data(), message.data()),
+          "kspir_decrypt_bsgs_total");
+    if (!cipher.getIsNtt()) {   // twosteps.cpp:393-401: the response is turned into NTT form
+        ntt_host(cipher.a, kspir_params::crtMod, false);
+        ntt_host(cipher.b, kspir_params::crtMod, false);
+        cipher.setIsNtt(true);
+    }
+}
+
 void kspir_pir_forget(const void* p) {
     std::lock_guard<std::mutex> lock(g_mu);
+    RandShield shield;
     auto it = g_dbs.find(p);
     if (it != g_dbs.end()) {
         kspir_database_destroy(it->second.h);

@@ -8,7 +8,10 @@
 //   packingRLWEs                           src/answer.h:79
 //
 // and the server preprocessing of the database (twosteps.h:19
-// database_tobsgsntt on the GPU, crt.h:133 database_tocrt)
+// database_tobsgsntt on the GPU, crt.h:133 database_tocrt), the client side
+// of test-pir.cpp:20-24,68-89 (Secret, query_bsgs_rns, RGSW / BSGS /
+// packing keyGen: rand() drawn on the host in the reference's order, the
+// ring arithmetic on the GPU) and recover (decrypt_bsgs_total)
 //
 // and the data types they take (src/lwe.h:48-110 RlweCiphertext, :113-169
 // RGSWCiphertext, :175-234 AutoKey, :381-466 AutoKeyBSGSRNS), whose public
@@ -32,6 +35,31 @@ constexpr uint64_t crtq1 = 268369921ULL, crtq2 = 249561089ULL;  // crt.h:16-17
 constexpr uint64_t crtMod = crtq1 * crtq2;                       // crt.h:18
 constexpr uint64_t bsMod = 16760833ULL;                          // crt.h:32
 }  // namespace kspir_params
+
+// src/secret.h:16-61: the RLWE secret.  Secret(module, ntt) draws N
+// values rand() % 3 - 1 exactly as secret.cpp:13-60 does (so the caller's
+// rand() stream advances as with the reference) and keeps them mod module,
+// in NTT form when ntt is set.  Moduli crtMod and bsMod are supported (the
+// reference's default bigMod is not used on the test-pir path and throws).
+class Secret {
+  private:
+    uint64_t length = kspir_params::N;
+    uint64_t modulus;
+    std::vector<uint64_t> coeff;   // coefficient form, the source of every conversion
+    std::vector<uint64_t> data;    // current form
+    bool nttform;
+
+  public:
+    explicit Secret(uint64_t module, bool ntt = true);
+    std::vector<uint64_t> getData() { return data; }
+    uint64_t getData(int32_t index) { return data.at(index); }
+    uint64_t getModulus() { return modulus; }
+    int32_t getLength() { return (int32_t)length; }
+    bool isNttForm() { return nttform; }
+    void toCoeffForm();
+    void toNttForm();
+    const std::vector<uint64_t>& coefficients() const { return coeff; }
+};
 
 // src/lwe.h:48-110
 class RlweCiphertext {
@@ -73,6 +101,9 @@ class RGSWCiphertext {
     uint64_t getBase() const { return PP; }
     bool getIsNtt() const { return isntt; }
     void setIsNtt(bool is) { isntt = is; }
+    // lwe.h:141, lwe.cpp:297-352: RGSW(X^-index) (is_reverse) or RGSW(X^index),
+    // 12N rand() draws; like the reference it leaves the secret in NTT form
+    void keyGen(Secret& secret, const uint64_t index, bool is_reverse = false);
 };
 
 // src/lwe.h:175-234 (packing keys; keyMap[N/i + 1] = ell RLWE, NTT form)
@@ -87,6 +118,10 @@ class AutoKey {
     std::map<int32_t, std::vector<RlweCiphertext> > keyMap;
 
     AutoKey(int32_t len, uint64_t module, int32_t elln, uint64_t base, uint64_t bg);
+    // lwe.h:213, lwe.cpp:509-550: keys for N/i+1, i = 1, 2, .. < num
+    // (packing_rlwe) or i+1, i = 2, 4, .. <= num+1, 12N rand() draws each;
+    // keyMap.insert keeps existing entries as the reference's does
+    void keyGen(Secret& secret, const int32_t num, bool packing_rlwe = false);
     uint64_t getModulus() const { return modulus; }
     uint64_t getLength() const { return length; }
     uint64_t getEllnum() const { return (uint64_t)ellnum; }
@@ -106,6 +141,9 @@ class AutoKeyBSGSRNS {
     std::map<int32_t, std::vector<RlweCiphertext> > keyMap;
 
     AutoKeyBSGSRNS(int32_t len, uint64_t module, uint64_t bsModule);
+    // lwe.h:434, lwe.cpp:1143-1160: baby keys 5^i (i <= N1/2), giant keys
+    // 5^(N1*i) (i < N2), in the reference's rand() order
+    void bsgsKeyGen(Secret& secret, int32_t N1);
     uint64_t getModulus() const { return modulus; }
     uint64_t getBSModulus() const { return bsModulus; }
     uint64_t getLength() const { return length; }
@@ -145,6 +183,15 @@ void database_tocrt(uint64_t* datacrt, std::vector<std::vector<uint64_t> >& data
 // Drop the device copies made for a database pointer or key object (call
 // after changing their contents in place; destroyed objects are recognised
 // by a fingerprint of their words and re-uploaded automatically).
+// twosteps.h:26, twosteps.cpp:279-318: cipher[0] (mod crtMod) and cipher[1]
+// (mod bsMod) receive the query for `row`, coefficient form; 5N rand() draws
+void query_bsgs_rns(std::vector<RlweCiphertext>& cipher, Secret& queryKey, uint64_t row);
+
+// twosteps.h:30, twosteps.cpp:388-418: message (resized to N) gets the
+// recovered plaintext; like the reference, cipher is left in NTT form
+void decrypt_bsgs_total(std::vector<uint64_t>& message, RlweCiphertext& cipher, Secret& secret,
+                        const int32_t rlwesNum);
+
 void kspir_pir_forget(const void* object_or_data);
 
 #endif

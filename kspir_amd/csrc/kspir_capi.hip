@@ -1295,6 +1295,171 @@ int kspir_client_draws(int32_t r, int32_t n1, uint64_t* draws) {
     return KSPIR_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// odd x: x^(N-1) is its inverse mod 2N (the units mod 2N form a group of order N)
+uint32_t inv2n(uint32_t x) {
+    uint32_t r = 1, b = x & (2 * kN - 1);
+    for (uint32_t e = kN - 1; e; e >>= 1, b = (b * b) & (2 * kN - 1))
+        if (e & 1) r = (r * b) & (2 * kN - 1);
+    return r;
+}
+
+uint32_t pow2mod(uint32_t k, uint64_t q) {
+    uint64_t v = 1;
+    for (uint32_t i = 0; i < k; ++i) v = (v << 1) % q;
+    return (uint32_t)v;
+}
+
+uint32_t pow5(uint64_t e) {   // 5^e mod 2N
+    uint32_t v = 1;
+    for (uint64_t i = 0; i < e; ++i) v = v * 5 % (2 * kN);
+    return v;
+}
+
+// Encryption jobs of one client, draws counted from `d` (its first draw),
+// in the reference's order.  Outputs are device pointers in the reference
+// layouts; each helper returns the draw count it used.
+struct JobSink {
+    std::vector<kspir::ClientJob>& jobs;
+    int ci;
+    void add(int32_t type, uint64_t draw, int32_t msg, uint32_t mfexp, uint32_t num, uint64_t* a, uint64_t* b,
+             uint64_t* a2, uint64_t* b2) {
+        kspir::ClientJob J{};
+        J.type = type;
+        J.client = ci;
+        J.draw = (uint32_t)draw;
+        J.msg = msg;
+        J.msg_inv = msg >= 1 ? inv2n((uint32_t)msg) : 0;
+        J.mf[0] = pow2mod(mfexp, kQ1);
+        J.mf[1] = pow2mod(mfexp, kQ2);
+        J.mf_bs = pow2mod(mfexp, kBs);
+        J.num = num;
+        J.a = a; J.b = b; J.a2 = a2; J.b2 = b2;
+        jobs.push_back(J);
+    }
+    // RGSWCiphertext::keyGen(secret, index), lwe.cpp:297-352: gadget 2^20,
+    // 2^38; per level the positive row 2+i, then the negated row i
+    uint64_t rgsw(uint64_t d, int32_t index, uint64_t* out, bool reverse = true) {
+        const uint64_t N = kN;
+        for (int i = 0; i < 2; ++i) {
+            add(kspir::kJobRlwe, d + 6 * N * i, -index, 20 + 18 * i, 1, out + (2 + i) * 2 * N,
+                out + (2 + i) * 2 * N + N, nullptr, nullptr);
+            jobs.back().msg_plus = !reverse;
+            add(kspir::kJobRgswNeg, d + 6 * N * i + 3 * N, -index, 20 + 18 * i, 1, out + i * 2 * N,
+                out + i * 2 * N + N, nullptr, nullptr);
+            jobs.back().msg_plus = !reverse;
+        }
+        return 12 * N;
+    }
+    // bsgsKeyGen (lwe.cpp:1143-1160): baby keys (lwe.cpp:1025-1070, index
+    // 5^(k+1), gadget 2^20, 2^40, 2^60) then giant keys (lwe.cpp:1072-1101,
+    // index 5^(N1*(k+1)), gadget 2^20, 2^38)
+    uint64_t bsgs(uint64_t d, int n1, uint64_t* baby, uint64_t* giant) {
+        const uint64_t N = kN;
+        const int nb = n1 / 2, n2 = kN / 2 / n1;
+        for (int k = 0; k < nb; ++k) {
+            const uint32_t idx = pow5(k + 1);
+            for (int i = 0; i < 3; ++i) {
+                uint64_t* kq = baby + ((size_t)k * 6 + i) * 2 * N;
+                uint64_t* kb = baby + ((size_t)k * 6 + 3 + i) * 2 * N;
+                add(kspir::kJobBaby, d + 15 * N * k + 5 * N * i, (int32_t)idx, 20 * (i + 1), 1, kq, kq + N, kb,
+                    kb + N);
+            }
+        }
+        const uint64_t g0 = d + 15 * N * nb;
+        for (int k = 0; k < n2 - 1; ++k) {
+            const uint32_t idx = pow5((uint64_t)n1 * (k + 1));
+            for (int i = 0; i < 2; ++i) {
+                uint64_t* o = giant + ((size_t)k * 2 + i) * 2 * N;
+                add(kspir::kJobRlwe, g0 + 6 * N * k + 3 * N * i, (int32_t)idx, 20 + 18 * i, 1, o, o + N, nullptr,
+                    nullptr);
+            }
+        }
+        return 15 * N * nb + 6 * N * (n2 - 1);
+    }
+    // AutoKey::keyGen(secret, num, packing) (lwe.cpp:472-550): level li,
+    // index N/2^li + 1, gadget 1, 2^14, 2^28, 2^42, error times num
+    uint64_t packing(uint64_t d, int num, uint64_t* out, bool rlwe = true) {
+        const uint64_t N = kN;
+        std::vector<int32_t> idx;
+        if (rlwe)
+            for (int i = 1; i < num; i <<= 1) idx.push_back(kN / i + 1);
+        else
+            for (int i = 2; i <= num + 1; i <<= 1) idx.push_back(i + 1);
+        for (size_t li = 0; li < idx.size(); ++li)
+            for (int j = 0; j < 4; ++j) {
+                uint64_t* o = out + (li * 4 + j) * 2 * N;
+                add(kspir::kJobRlwe, d + 12 * N * li + 3 * N * j, idx[li], 14 * j, (uint32_t)num, o, o + N, nullptr,
+                    nullptr);
+            }
+        return 12 * N * idx.size();
+    }
+};
+
+// compute_query_encode (utils.cpp:1657-1675): the two NTT slots of target column `row`
+void query_slots(int row, int32_t* out2) {
+    static std::vector<int32_t> slot_of;
+    static std::vector<uint32_t> rot;
+    if (slot_of.empty()) {
+        slot_of.assign(kN, 0);
+        for (int s = 0; s < kN; ++s) slot_of[(2 * bitrev12(s) + 1) >> 1] = s;
+        rot.assign(kN, 0);
+        rot[0] = 1;
+        rot[kN - 1] = 2 * kN - 1;
+        for (int i = 1; i < kN / 2; ++i) {
+            rot[i] = rot[i - 1] * 5 % (2 * kN);
+            rot[kN - 1 - i] = rot[kN - i] * 5 % (2 * kN);
+        }
+    }
+    out2[0] = slot_of[rot[row] >> 1];
+    out2[1] = slot_of[rot[kN - 1 - row] >> 1];
+}
+
+kspir::ClientConsts client_consts() {
+    kspir::ClientConsts K{};
+    const uint32_t qs[3] = {kQ1, kQ2, kBs};
+    for (int p = 0; p < 3; ++p) K.r2[p] = (uint32_t)(((u128)1 << 64) % qs[p]);
+    K.bs_mod_q[0] = kBs % kQ1;
+    K.bs_mod_q[1] = kBs % kQ2;
+    return K;
+}
+
+int client_tables(kspir_context* c) {
+    if (c->rng_jump) return KSPIR_OK;
+    const kspir_rng::Mat jm = kspir_rng::power(kspir::kRngChunk);
+    HIP_OK(hipMalloc(&c->rng_jump, jm.size() * 4));
+    HIP_OK(hipMemcpy(c->rng_jump, jm.data(), jm.size() * 4, hipMemcpyHostToDevice));
+    // twosteps.cpp:305: round((long double)m * crtMod / p) in the host's
+    // long double, for every plaintext coefficient m < p
+    std::vector<uint64_t> enc(kP);
+    for (uint32_t m = 0; m < kP; ++m) enc[m] = (uint64_t)roundl((long double)m * kQ / kP);
+    HIP_OK(hipMalloc(&c->enc_tab, enc.size() * 8));
+    HIP_OK(hipMemcpy(c->enc_tab, enc.data(), enc.size() * 8, hipMemcpyHostToDevice));
+    return KSPIR_OK;
+}
+
+// device allocations released together
+struct DevScope {
+    hipStream_t st;
+    std::vector<void*> mem;
+    hipError_t alloc(void** p, size_t bytes) {
+        hipError_t e = hipMalloc(p, bytes ? bytes : 8);
+        if (e == hipSuccess) mem.push_back(*p);
+        return e;
+    }
+    ~DevScope() {
+        hipStreamSynchronize(st);
+        for (void* p : mem) hipFree(p);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
 int kspir_client_generate(kspir_context* c, int32_t r, int32_t n1, int32_t nc, const uint32_t* seeds,
                           const int32_t* target_col, const int32_t* target_packing, uint64_t* secrets,
                           uint64_t* queries, uint64_t* rgsw, uint64_t* baby, uint64_t* giant, uint64_t* packing) {
@@ -1307,169 +1472,71 @@ int kspir_client_generate(kspir_context* c, int32_t r, int32_t n1, int32_t nc, c
         if (target_col[i] < 0 || target_col[i] >= kN / 2 || target_packing[i] < 0 || target_packing[i] >= r)
             return fail(KSPIR_EINVAL, "target column must be < N/2 and target packing < r");
     HIP_OK(hipSetDevice(c->device));
+    if ((rc = client_tables(c))) return rc;
     hipStream_t st = c->stream;
-    if (!c->rng_jump) {
-        const kspir_rng::Mat jm = kspir_rng::power(kspir::kRngChunk);
-        HIP_OK(hipMalloc(&c->rng_jump, jm.size() * 4));
-        HIP_OK(hipMemcpy(c->rng_jump, jm.data(), jm.size() * 4, hipMemcpyHostToDevice));
-        std::vector<uint64_t> enc(kP);
-        for (uint32_t m = 0; m < kP; ++m) enc[m] = (uint64_t)roundl((long double)m * kQ / kP);
-        HIP_OK(hipMalloc(&c->enc_tab, enc.size() * 8));
-        HIP_OK(hipMemcpy(c->enc_tab, enc.data(), enc.size() * 8, hipMemcpyHostToDevice));
-    }
     const int nchunks = (int)((draws + kspir::kRngChunk - 1) / kspir::kRngChunk);
     const size_t stride = (size_t)nchunks * kspir::kRngChunk;
-    // host inputs: seed windows and the two message slots of each query
-    // (compute_query_encode, utils.cpp:1657-1675: slot of psi^(5^i), psi^(-5^i))
     std::vector<uint32_t> seedw((size_t)nc * kspir_rng::kLag);
-    for (int i = 0; i < nc; ++i) kspir_rng::seed_window(seeds[i], &seedw[(size_t)i * kspir_rng::kLag]);
-    std::vector<int32_t> slot_of(kN), slots(2 * (size_t)nc);
-    for (int s = 0; s < kN; ++s) slot_of[(2 * bitrev12(s) + 1) >> 1] = s;
-    std::vector<uint32_t> rot(kN);
-    rot[0] = 1;
-    rot[kN - 1] = 2 * kN - 1;
-    for (int i = 1; i < kN / 2; ++i) {
-        rot[i] = rot[i - 1] * 5 % (2 * kN);
-        rot[kN - 1 - i] = rot[kN - i] * 5 % (2 * kN);
-    }
+    std::vector<int32_t> slots(2 * (size_t)nc);
     for (int i = 0; i < nc; ++i) {
-        slots[2 * i] = slot_of[rot[target_col[i]] >> 1];
-        slots[2 * i + 1] = slot_of[rot[kN - 1 - target_col[i]] >> 1];
+        kspir_rng::seed_window(seeds[i], &seedw[(size_t)i * kspir_rng::kLag]);
+        query_slots(target_col[i], &slots[2 * (size_t)i]);
     }
-    // device buffers
-    const size_t n_sec = (size_t)nc * kN, n_q = (size_t)nc * 4 * kN, n_rg = (size_t)nc * 8 * kN;
-    const size_t n_baby = (size_t)nc * nb * 12 * kN, n_giant = (size_t)nc * (n2 - 1) * 4 * kN;
-    const size_t n_pack = (size_t)nc * lr * 8 * kN;
-    std::vector<void*> mem;
-    auto alloc = [&](void** p, size_t bytes) -> hipError_t {
-        hipError_t e = hipMalloc(p, bytes ? bytes : 8);
-        if (e == hipSuccess) mem.push_back(*p);
-        return e;
-    };
-    auto cleanup = [&](int code) {
-        hipStreamSynchronize(st);
-        for (void* p : mem) hipFree(p);
-        return code;
-    };
-#define C_OK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return cleanup(fail_hip(#x, e_)); } while (0)
+    const size_t N = kN;
+    const size_t n_sec = (size_t)nc * N, n_q = (size_t)nc * 4 * N, n_rg = (size_t)nc * 8 * N;
+    const size_t n_baby = (size_t)nc * nb * 12 * N, n_giant = (size_t)nc * (n2 - 1) * 4 * N;
+    const size_t n_pack = (size_t)nc * lr * 8 * N;
+    DevScope ds{st, {}};
+#define C_OK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return fail_hip(#x, e_); } while (0)
     uint32_t *d_seedw, *d_win, *d_stream, *d_sm;
     int32_t *d_scoef, *d_slots;
     uint64_t *d_sec, *d_q, *d_rg, *d_baby, *d_giant, *d_pack;
     kspir::ClientJob* d_jobs;
-    C_OK(alloc((void**)&d_seedw, seedw.size() * 4));
-    C_OK(alloc((void**)&d_win, (size_t)nc * nchunks * kspir_rng::kLag * 4));
-    C_OK(alloc((void**)&d_stream, (size_t)nc * stride * 4));
-    C_OK(alloc((void**)&d_scoef, n_sec * 4));
-    C_OK(alloc((void**)&d_sm, n_sec * 3 * 4));
-    C_OK(alloc((void**)&d_slots, slots.size() * 4));
-    C_OK(alloc((void**)&d_sec, n_sec * 8));
-    C_OK(alloc((void**)&d_q, n_q * 8));
-    C_OK(alloc((void**)&d_rg, n_rg * 8));
-    C_OK(alloc((void**)&d_baby, n_baby * 8));
-    C_OK(alloc((void**)&d_giant, n_giant * 8));
-    C_OK(alloc((void**)&d_pack, n_pack * 8));
-    // encryption jobs, with the stream offsets of the reference's draw order:
-    // secret [0, N), query [N, 6N), RGSW from 6N, baby keys from 18N, giant
-    // keys after them, packing keys last (test-pir.cpp:20-24, 68-89)
-    auto inv2n = [](uint32_t x) {   // odd x: x^(N-1) is its inverse mod 2N (the unit group has order N)
-        uint32_t r = 1, b = x & (2 * kN - 1);
-        for (uint32_t e = kN - 1; e; e >>= 1, b = (b * b) & (2 * kN - 1))
-            if (e & 1) r = (r * b) & (2 * kN - 1);
-        return r;
-    };
-    auto pow2mod = [](uint32_t k, uint64_t q) {
-        uint64_t v = 1;
-        for (uint32_t i = 0; i < k; ++i) v = (v << 1) % q;
-        return v;
-    };
+    C_OK(ds.alloc((void**)&d_seedw, seedw.size() * 4));
+    C_OK(ds.alloc((void**)&d_win, (size_t)nc * nchunks * kspir_rng::kLag * 4));
+    C_OK(ds.alloc((void**)&d_stream, (size_t)nc * stride * 4));
+    C_OK(ds.alloc((void**)&d_scoef, n_sec * 4));
+    C_OK(ds.alloc((void**)&d_sm, n_sec * 3 * 4));
+    C_OK(ds.alloc((void**)&d_slots, slots.size() * 4));
+    C_OK(ds.alloc((void**)&d_sec, n_sec * 8));
+    C_OK(ds.alloc((void**)&d_q, n_q * 8));
+    C_OK(ds.alloc((void**)&d_rg, n_rg * 8));
+    C_OK(ds.alloc((void**)&d_baby, n_baby * 8));
+    C_OK(ds.alloc((void**)&d_giant, n_giant * 8));
+    C_OK(ds.alloc((void**)&d_pack, n_pack * 8));
+    // draws of one client after srand: secret [0, N), query [N, 6N), then
+    // RGSW, BSGS and packing keys (test-pir.cpp:20-24, 68-89)
     std::vector<kspir::ClientJob> jobs;
-    const uint64_t N = kN;
     for (int ci = 0; ci < nc; ++ci) {
-        auto job = [&](int32_t type, uint64_t draw, int32_t msg, uint32_t mfexp, uint32_t num, uint64_t* a,
-                       uint64_t* b, uint64_t* a2, uint64_t* b2) {
-            kspir::ClientJob J{};
-            J.type = type;
-            J.client = ci;
-            J.draw = (uint32_t)draw;
-            J.msg = msg;
-            J.msg_inv = msg >= 1 ? inv2n((uint32_t)msg) : 0;
-            J.mf[0] = (uint32_t)pow2mod(mfexp, kQ1);
-            J.mf[1] = (uint32_t)pow2mod(mfexp, kQ2);
-            J.mf_bs = (uint32_t)pow2mod(mfexp, kBs);
-            J.num = num;
-            J.a = a; J.b = b; J.a2 = a2; J.b2 = b2;
-            jobs.push_back(J);
-        };
-        // RGSW(X^-idx), lwe.cpp:297-352: gadget 2^20, 2^38; row 2+i then row i
-        uint64_t* rg = d_rg + (size_t)ci * 8 * N;
-        for (int i = 0; i < 2; ++i) {
-            const uint64_t d = 6 * N + 6 * N * i;
-            job(kspir::kJobRlwe, d, -target_packing[ci], 20 + 18 * i, 1, rg + (2 + i) * 2 * N, rg + (2 + i) * 2 * N + N,
-                nullptr, nullptr);
-            job(kspir::kJobRgswNeg, d + 3 * N, -target_packing[ci], 20 + 18 * i, 1, rg + i * 2 * N, rg + i * 2 * N + N,
-                nullptr, nullptr);
-        }
-        // baby keys (lwe.cpp:1025-1070): index 5^(k+1), gadget 2^20, 2^40, 2^60
-        uint64_t* bk = d_baby + (size_t)ci * nb * 12 * N;
-        for (int k = 0; k < nb; ++k) {
-            uint32_t idx = 1;
-            for (int t = 0; t <= k; ++t) idx = idx * 5 % (2 * kN);
-            for (int i = 0; i < 3; ++i) {
-                uint64_t* kq = bk + ((size_t)k * 6 + i) * 2 * N;
-                uint64_t* kb = bk + ((size_t)k * 6 + 3 + i) * 2 * N;
-                job(kspir::kJobBaby, 18 * N + 15 * N * k + 5 * N * i, (int32_t)idx, 20 * (i + 1), 1, kq, kq + N, kb,
-                    kb + N);
-            }
-        }
-        // giant keys (lwe.cpp:1072-1101): index 5^(N1*(k+1)), gadget 2^20, 2^38
-        const uint64_t g0 = 18 * N + 15 * N * nb;
-        uint64_t* gk = d_giant + (size_t)ci * (n2 - 1) * 4 * N;
-        for (int k = 0; k < n2 - 1; ++k) {
-            uint32_t idx = 1;
-            for (int t = 0; t < n1 * (k + 1); ++t) idx = idx * 5 % (2 * kN);
-            for (int i = 0; i < 2; ++i) {
-                uint64_t* o = gk + ((size_t)k * 2 + i) * 2 * N;
-                job(kspir::kJobRlwe, g0 + 6 * N * k + 3 * N * i, (int32_t)idx, 20 + 18 * i, 1, o, o + N, nullptr,
-                    nullptr);
-            }
-        }
-        // packing keys (lwe.cpp:472-550): level li, index N/2^li + 1,
-        // gadget 1, 2^14, 2^28, 2^42, error times r
-        const uint64_t p0 = g0 + 6 * N * (n2 - 1);
-        uint64_t* pk = d_pack + (size_t)ci * lr * 8 * N;
-        for (int li = 0; li < lr; ++li)
-            for (int j = 0; j < 4; ++j) {
-                uint64_t* o = pk + ((size_t)li * 4 + j) * 2 * N;
-                job(kspir::kJobRlwe, p0 + 12 * N * li + 3 * N * j, (int32_t)(kN / (1 << li) + 1), 14 * j, (uint32_t)r,
-                    o, o + N, nullptr, nullptr);
-            }
+        JobSink js{jobs, ci};
+        uint64_t d = 6 * N;
+        d += js.rgsw(d, target_packing[ci], d_rg + (size_t)ci * 8 * N);
+        d += js.bsgs(d, n1, d_baby + (size_t)ci * nb * 12 * N, d_giant + (size_t)ci * (n2 - 1) * 4 * N);
+        d += js.packing(d, r, d_pack + (size_t)ci * lr * 8 * N);
+        if (d != draws) return fail(KSPIR_EHIP, "client draw schedule does not add up");
     }
-    C_OK(alloc((void**)&d_jobs, jobs.size() * sizeof(kspir::ClientJob)));
+    C_OK(ds.alloc((void**)&d_jobs, jobs.size() * sizeof(kspir::ClientJob)));
     C_OK(hipMemcpyAsync(d_seedw, seedw.data(), seedw.size() * 4, hipMemcpyHostToDevice, st));
     C_OK(hipMemcpyAsync(d_slots, slots.data(), slots.size() * 4, hipMemcpyHostToDevice, st));
     C_OK(hipMemcpyAsync(d_jobs, jobs.data(), jobs.size() * sizeof(kspir::ClientJob), hipMemcpyHostToDevice, st));
-    kspir::ClientConsts K{};
-    const uint32_t qs[3] = {kQ1, kQ2, kBs};
-    for (int p = 0; p < 3; ++p) K.r2[p] = (uint32_t)(((u128)1 << 64) % qs[p]);
-    K.bs_mod_q[0] = kBs % kQ1;
-    K.bs_mod_q[1] = kBs % kQ2;
+    const kspir::ClientConsts K = client_consts();
     const Tabs& T = c->T;
     hipLaunchKernelGGL(kspir::k_rng_windows, dim3(nc), dim3(64), 0, st, d_seedw, c->rng_jump, d_win, nchunks);
-    if (hipGetLastError() != hipSuccess) return cleanup(fail(KSPIR_EHIP, "k_rng_windows launch"));
+    LAUNCH_OK("k_rng_windows");
     hipLaunchKernelGGL(kspir::k_rng_stream, dim3((nc * nchunks + 63) / 64), dim3(64), 0, st, d_win, d_stream, nc,
                        nchunks);
-    if (hipGetLastError() != hipSuccess) return cleanup(fail(KSPIR_EHIP, "k_rng_stream launch"));
+    LAUNCH_OK("k_rng_stream");
     hipLaunchKernelGGL(kspir::k_client_secret, dim3(nc), dim3(kNttThreads), 0, st, d_stream, stride, d_scoef, d_sm, K,
                        T);
-    if (hipGetLastError() != hipSuccess) return cleanup(fail(KSPIR_EHIP, "k_client_secret launch"));
-    hipLaunchKernelGGL(kspir::k_client_query, dim3(nc), dim3(kNttThreads), 0, st, d_stream, stride, d_sm, d_slots,
-                       c->enc_tab, d_q, K, T);
-    if (hipGetLastError() != hipSuccess) return cleanup(fail(KSPIR_EHIP, "k_client_query launch"));
+    LAUNCH_OK("k_client_secret");
+    hipLaunchKernelGGL(kspir::k_client_query, dim3(nc), dim3(kNttThreads), 0, st, d_stream, stride, (size_t)N, d_sm,
+                       d_slots, c->enc_tab, d_q, K, T);
+    LAUNCH_OK("k_client_query");
     hipLaunchKernelGGL(kspir::k_client_rlwe, dim3((unsigned)jobs.size()), dim3(kNttThreads), 0, st, d_jobs, d_stream,
                        stride, d_scoef, d_sm, K, T);
-    if (hipGetLastError() != hipSuccess) return cleanup(fail(KSPIR_EHIP, "k_client_rlwe launch"));
+    LAUNCH_OK("k_client_rlwe");
     hipLaunchKernelGGL(kspir::k_client_secret_words, dim3(grid1d(n_sec)), dim3(256), 0, st, d_scoef, d_sec, n_sec);
-    if (hipGetLastError() != hipSuccess) return cleanup(fail(KSPIR_EHIP, "k_client_secret_words launch"));
+    LAUNCH_OK("k_client_secret_words");
     if (secrets) C_OK(hipMemcpyAsync(secrets, d_sec, n_sec * 8, hipMemcpyDeviceToHost, st));
     if (queries) C_OK(hipMemcpyAsync(queries, d_q, n_q * 8, hipMemcpyDeviceToHost, st));
     if (rgsw) C_OK(hipMemcpyAsync(rgsw, d_rg, n_rg * 8, hipMemcpyDeviceToHost, st));
@@ -1477,8 +1544,101 @@ int kspir_client_generate(kspir_context* c, int32_t r, int32_t n1, int32_t nc, c
     if (giant && n_giant) C_OK(hipMemcpyAsync(giant, d_giant, n_giant * 8, hipMemcpyDeviceToHost, st));
     if (packing && n_pack) C_OK(hipMemcpyAsync(packing, d_pack, n_pack * 8, hipMemcpyDeviceToHost, st));
     C_OK(hipStreamSynchronize(st));
+    return KSPIR_OK;
+}
+
+int kspir_client_encrypt(kspir_context* c, int32_t kind, const uint64_t* secret, int64_t param, int32_t n1,
+                         const uint32_t* draws, uint64_t ndraws, uint64_t* out) {
+    if (!c || !secret || !draws || !out) return fail(KSPIR_EINVAL, "null argument");
+    const uint64_t N = kN;
+    std::vector<int32_t> sc(kN);
+    for (int i = 0; i < kN; ++i) {
+        if (secret[i] == 0) sc[i] = 0;
+        else if (secret[i] == 1) sc[i] = 1;
+        else if (secret[i] == kQ - 1) sc[i] = -1;
+        else return fail(KSPIR_EINVAL, "secret coefficients must be 0, 1 or crtMod - 1");
+    }
+    uint64_t need = 0;
+    size_t out_words = 0;
+    int n2 = 1;
+    switch (kind) {
+        case KSPIR_CLIENT_QUERY:
+            if (param < 0 || param >= kN / 2) return fail(KSPIR_EINVAL, "query row must be < N/2");
+            need = 5 * N; out_words = 4 * N; break;
+        case KSPIR_CLIENT_RGSW:
+            if (param < 0 || param >= kN) return fail(KSPIR_EINVAL, "RGSW index must be < N");
+            need = 12 * N; out_words = 8 * N; break;
+        case KSPIR_CLIENT_BSGS:
+            if (!pow2(n1) || n1 < 2 || n1 > kN / 2) return fail(KSPIR_EINVAL, "N1 must be a power of two in [2, N/2]");
+            n2 = kN / 2 / n1;
+            need = 15 * N * (n1 / 2) + 6 * N * (n2 - 1);
+            out_words = (size_t)(n1 / 2) * 12 * N + (size_t)(n2 - 1) * 4 * N; break;
+        case KSPIR_CLIENT_RGSW_FORWARD:
+            if (param < 0 || param >= kN) return fail(KSPIR_EINVAL, "RGSW index must be < N");
+            need = 12 * N; out_words = 8 * N; break;
+        case KSPIR_CLIENT_PACKING:
+        case KSPIR_CLIENT_PACKING_LWE: {
+            if (param < 1 || param > kN) return fail(KSPIR_EINVAL, "packing num must be in [1, N]");
+            int nidx = 0;
+            if (kind == KSPIR_CLIENT_PACKING)
+                for (int64_t i = 1; i < param; i <<= 1) ++nidx;
+            else
+                for (int64_t i = 2; i <= param + 1; i <<= 1) ++nidx;
+            need = 12 * N * nidx;
+            out_words = (size_t)nidx * 8 * N;
+            break;
+        }
+        default:
+            return fail(KSPIR_EINVAL, "unknown client operation");
+    }
+    if (ndraws != need) return fail(KSPIR_EINVAL, "draw count does not match the operation");
+    HIP_OK(hipSetDevice(c->device));
+    int rc = client_tables(c);
+    if (rc) return rc;
+    hipStream_t st = c->stream;
+    DevScope ds{st, {}};
+    uint32_t *d_stream, *d_sm;
+    int32_t *d_scoef, *d_slots;
+    uint64_t* d_out;
+    C_OK(ds.alloc((void**)&d_stream, need * 4));
+    C_OK(ds.alloc((void**)&d_scoef, N * 4));
+    C_OK(ds.alloc((void**)&d_sm, 3 * N * 4));
+    C_OK(ds.alloc((void**)&d_slots, 8));
+    C_OK(ds.alloc((void**)&d_out, out_words * 8));
+    C_OK(hipMemcpyAsync(d_stream, draws, need * 4, hipMemcpyHostToDevice, st));
+    C_OK(hipMemcpyAsync(d_scoef, sc.data(), N * 4, hipMemcpyHostToDevice, st));
+    const kspir::ClientConsts K = client_consts();
+    const Tabs& T = c->T;
+    hipLaunchKernelGGL(kspir::k_client_secret_given, dim3(1), dim3(kNttThreads), 0, st, d_scoef, d_sm, K, T);
+    LAUNCH_OK("k_client_secret_given");
+    if (kind == KSPIR_CLIENT_QUERY) {
+        int32_t sl[2];
+        query_slots((int)param, sl);
+        C_OK(hipMemcpyAsync(d_slots, sl, 8, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(kspir::k_client_query, dim3(1), dim3(kNttThreads), 0, st, d_stream, (size_t)need,
+                           (size_t)0, d_sm, d_slots, c->enc_tab, d_out, K, T);
+        LAUNCH_OK("k_client_query");
+    } else {
+        std::vector<kspir::ClientJob> jobs;
+        JobSink js{jobs, 0};
+        if (kind == KSPIR_CLIENT_RGSW || kind == KSPIR_CLIENT_RGSW_FORWARD)
+            js.rgsw(0, (int32_t)param, d_out, kind == KSPIR_CLIENT_RGSW);
+        else if (kind == KSPIR_CLIENT_BSGS) js.bsgs(0, n1, d_out, d_out + (size_t)(n1 / 2) * 12 * N);
+        else js.packing(0, (int)param, d_out, kind == KSPIR_CLIENT_PACKING);
+        if (!jobs.empty()) {
+            kspir::ClientJob* d_jobs;
+            C_OK(ds.alloc((void**)&d_jobs, jobs.size() * sizeof(kspir::ClientJob)));
+            C_OK(hipMemcpyAsync(d_jobs, jobs.data(), jobs.size() * sizeof(kspir::ClientJob), hipMemcpyHostToDevice,
+                                st));
+            hipLaunchKernelGGL(kspir::k_client_rlwe, dim3((unsigned)jobs.size()), dim3(kNttThreads), 0, st, d_jobs,
+                               d_stream, (size_t)need, d_scoef, d_sm, K, T);
+            LAUNCH_OK("k_client_rlwe");
+        }
+    }
+    C_OK(hipMemcpyAsync(out, d_out, out_words * 8, hipMemcpyDeviceToHost, st));
+    C_OK(hipStreamSynchronize(st));
 #undef C_OK
-    return cleanup(KSPIR_OK);
+    return KSPIR_OK;
 }
 
 int kspir_decrypt_bsgs_total(kspir_context* c, int32_t r, int32_t nq, const uint64_t* secrets,

@@ -120,6 +120,22 @@ __global__ void __launch_bounds__(kNttThreads) k_client_secret(const uint32_t* _
         for (int p = 0; p < 3; ++p) sm[((size_t)c * 3 + p) * kN + x] = to_mont(s[p][lds_pad(x)], K.r2[p], T.p[p]);
 }
 
+// the same from given secret coefficients (ternary, as int32): grid nc, 256 threads
+__global__ void __launch_bounds__(kNttThreads) k_client_secret_given(const int32_t* __restrict__ scoef,
+                                                                     uint32_t* __restrict__ sm, ClientConsts K, Tabs T) {
+    __shared__ uint32_t s[3][kPadN];
+    const int c = blockIdx.x;
+    for (int x = threadIdx.x; x < kN; x += kNttThreads) {
+        const int32_t t = scoef[(size_t)c * kN + x];
+        for (int p = 0; p < 3; ++p) s[p][lds_pad(x)] = tern_res(t, T.p[p].q);
+    }
+    __syncthreads();
+    ntt_fwd_r16x2(s[0], T.p[0], s[1], T.p[1]);
+    ntt_fwd_r16(s[2], T.p[2]);
+    for (int x = threadIdx.x; x < kN; x += kNttThreads)
+        for (int p = 0; p < 3; ++p) sm[((size_t)c * 3 + p) * kN + x] = to_mont(s[p][lds_pad(x)], K.r2[p], T.p[p]);
+}
+
 // automorphic(s, index) of a ternary secret at coefficient x (utils.cpp:1355-1379):
 // coefficient i goes to i*index mod 2N, negated past N
 __device__ __forceinline__ int32_t auto_secret(const int32_t* sc, int x, uint32_t index_inv) {
@@ -139,6 +155,7 @@ struct ClientJob {
     uint32_t draw;       // stream offset of this encryption's first draw
     int32_t msg;         // >= 1: m = automorphic(s, msg) (msg_inv its inverse mod 2N); <= 0: m = X^(-(-msg))
     uint32_t msg_inv;
+    int32_t msg_plus;    // 1: for msg <= 0, m = X^(-msg) instead (RGSW keyGen without is_reverse)
     uint32_t mf[2];      // gadget factor mod q1, q2
     uint32_t mf_bs;      // gadget factor mod bsMod (baby keys)
     uint32_t num;        // error multiplier (encrypt_special_rlwe), 1 otherwise
@@ -153,6 +170,7 @@ struct ClientJob {
 __device__ __forceinline__ int32_t job_msg(const ClientJob& J, const int32_t* sc, int x) {
     if (J.msg >= 1) return auto_secret(sc, x, J.msg_inv);
     const int idx = -J.msg;
+    if (J.msg_plus) return x == idx ? 1 : 0;
     if (idx == 0) return x == 0 ? 1 : 0;
     return x == kN - idx ? -1 : 0;
 }
@@ -229,15 +247,15 @@ __global__ void __launch_bounds__(kNttThreads) k_client_rlwe(const ClientJob* __
 // plaintext NTT domain, INTT mod p, each coefficient encoded by enc_tab
 // (round(m * Q / p) in the reference's long double arithmetic); then
 // c0 = (a0, a0*s + NTT(m*bsMod + err)) mod Q, c1 = (a1, a1*s + NTT(lift(err)))
-// mod bsMod, all four returned to coefficient form.  Draws: a0 [N, 3N),
-// a1 [3N, 5N), err [5N, 6N).  out: [nc][4][N] (c0.a, c0.b, c1.a, c1.b).
-// grid nc, 256 threads.
+// mod bsMod, all four returned to coefficient form.  Draws, from `base` of
+// the client's stream (N after the secret): a0 [0, 2N), a1 [2N, 4N),
+// err [4N, 5N).  out: [nc][4][N] (c0.a, c0.b, c1.a, c1.b).  grid nc, 256 threads.
 __global__ void __launch_bounds__(kNttThreads) k_client_query(const uint32_t* __restrict__ stream, size_t stride,
-        const uint32_t* __restrict__ sm, const int32_t* __restrict__ slots, const uint64_t* __restrict__ enc_tab,
-        uint64_t* __restrict__ out, ClientConsts K, Tabs T) {
+        size_t base, const uint32_t* __restrict__ sm, const int32_t* __restrict__ slots,
+        const uint64_t* __restrict__ enc_tab, uint64_t* __restrict__ out, ClientConsts K, Tabs T) {
     __shared__ uint32_t s[4][kPadN];
     const int c = blockIdx.x, tid = threadIdx.x;
-    const uint32_t* o = stream + (size_t)c * stride;
+    const uint32_t* o = stream + (size_t)c * stride + base;   // a0 at o[0], a1 at o[2N], err at o[4N]
     const uint32_t* smc = sm + (size_t)c * 3 * kN;
     uint64_t* oc = out + (size_t)c * 4 * kN;
     const int e0 = slots[2 * c], e1 = slots[2 * c + 1];
@@ -247,7 +265,7 @@ __global__ void __launch_bounds__(kNttThreads) k_client_query(const uint32_t* __
     // NTT_Q(m * bsMod + err) in planes 0, 1; NTT_bs(lift(err)) in plane 2
     for (int x = tid; x < kN; x += kNttThreads) {
         const uint64_t M = enc_tab[s[3][lds_pad(x)]];
-        const int32_t e = ternary(o[5 * kN + x]);
+        const int32_t e = ternary(o[4 * kN + x]);
         for (int p = 0; p < 2; ++p) {
             const uint32_t q = T.p[p].q;
             const uint32_t mv = (uint32_t)(((M % q) * K.bs_mod_q[p]) % q);
@@ -260,13 +278,13 @@ __global__ void __launch_bounds__(kNttThreads) k_client_query(const uint32_t* __
     ntt_fwd_r16(s[2], T.p[2]);
     // b parts in the NTT domain, in place; a parts kept for their INTTs below
     for (int x = tid; x < kN; x += kNttThreads) {
-        const uint64_t A0 = uniform_draw(o + kN + 2 * x, kQ);
+        const uint64_t A0 = uniform_draw(o + 2 * x, kQ);
         for (int p = 0; p < 2; ++p) {
             const PrimeTab& P = T.p[p];
             s[p][lds_pad(x)] = add_mod(mont_reduce((uint64_t)(uint32_t)(A0 % P.q) * smc[(size_t)p * kN + x], P.q,
                                                    P.mont_qinv), s[p][lds_pad(x)], P.q);
         }
-        const uint64_t A1 = uniform_draw(o + 3 * kN + 2 * x, kBs);
+        const uint64_t A1 = uniform_draw(o + 2 * kN + 2 * x, kBs);
         const PrimeTab& P = T.p[2];
         s[2][lds_pad(x)] = add_mod(mont_reduce(A1 * smc[(size_t)2 * kN + x], P.q, P.mont_qinv), s[2][lds_pad(x)], P.q);
     }
@@ -280,10 +298,10 @@ __global__ void __launch_bounds__(kNttThreads) k_client_query(const uint32_t* __
     __syncthreads();
     // the a parts
     for (int x = tid; x < kN; x += kNttThreads) {
-        const uint64_t A0 = uniform_draw(o + kN + 2 * x, kQ);
+        const uint64_t A0 = uniform_draw(o + 2 * x, kQ);
         s[0][lds_pad(x)] = (uint32_t)(A0 % kQ1);
         s[1][lds_pad(x)] = (uint32_t)(A0 % kQ2);
-        s[2][lds_pad(x)] = (uint32_t)uniform_draw(o + 3 * kN + 2 * x, kBs);
+        s[2][lds_pad(x)] = (uint32_t)uniform_draw(o + 2 * kN + 2 * x, kBs);
     }
     __syncthreads();
     ntt_inv_r16x2(s[0], T.p[0], s[1], T.p[1]);

@@ -42,8 +42,82 @@ static RlweCiphertext rlwe(const uint64_t* p, uint64_t q, bool ntt) {
 }
 static uint32_t pow5(int e) { uint32_t r = 1; for (int i = 0; i < e; ++i) r = (r * 5u) & (2 * N - 1); return r; }
 
+// test-pir.cpp:20-24,68-89 through the drop-in: srand(seed), Secret,
+// query_bsgs_rns, RGSW keyGen (is_reverse), bsgsKeyGen, packing keyGen; the
+// words go to <dir> in the layouts of include/kspir_amd.h
+static int client_mode(const std::string& dir, int32_t r, int32_t N1, unsigned seed, int tcol, int tpack) {
+    srand(seed);
+    Secret queryKey(crtMod, false);
+    std::vector<RlweCiphertext> query1(1, RlweCiphertext(N, crtMod));
+    query1.push_back(RlweCiphertext(N, bsMod));
+    query_bsgs_rns(query1, queryKey, (uint64_t)tcol);
+    RGSWCiphertext queryGsw(N, crtMod, 2, 1 << 20, 1 << 18);
+    queryGsw.keyGen(queryKey, (uint64_t)tpack, true);
+    AutoKeyBSGSRNS autoKey(N, crtMod, bsMod);
+    autoKey.bsgsKeyGen(queryKey, N1);
+    AutoKey packingKey(N, crtMod, 4, 0, 1 << 14);
+    packingKey.keyGen(queryKey, r, true);
+    const int N2 = N / 2 / N1;
+    std::vector<uint64_t> q, g, bb, gg, pk;
+    for (int i = 0; i < 2; ++i) {
+        q.insert(q.end(), query1[i].a.begin(), query1[i].a.end());
+        q.insert(q.end(), query1[i].b.begin(), query1[i].b.end());
+    }
+    for (int k = 0; k < 4; ++k) {
+        g.insert(g.end(), queryGsw.a[k].begin(), queryGsw.a[k].end());
+        g.insert(g.end(), queryGsw.b[k].begin(), queryGsw.b[k].end());
+    }
+    for (int i = 1; i <= N1 / 2; ++i)
+        for (const RlweCiphertext& c : autoKey.keyMap.at(pow5(i))) {
+            bb.insert(bb.end(), c.a.begin(), c.a.end());
+            bb.insert(bb.end(), c.b.begin(), c.b.end());
+        }
+    for (int i = 1; i < N2; ++i)
+        for (const RlweCiphertext& c : autoKey.keyMap.at(pow5(N1 * i))) {
+            gg.insert(gg.end(), c.a.begin(), c.a.end());
+            gg.insert(gg.end(), c.b.begin(), c.b.end());
+        }
+    for (int i = 1; i < r; i <<= 1)
+        for (const RlweCiphertext& c : packingKey.keyMap.at(N / i + 1)) {
+            pk.insert(pk.end(), c.a.begin(), c.a.end());
+            pk.insert(pk.end(), c.b.begin(), c.b.end());
+        }
+    const int ntt_after = (int)queryKey.isNttForm();
+    queryKey.toCoeffForm();
+    write_u64(dir + "/secret.bin", queryKey.getData());
+    write_u64(dir + "/query.bin", q);
+    write_u64(dir + "/rgsw.bin", g);
+    write_u64(dir + "/baby.bin", bb);
+    write_u64(dir + "/giant.bin", gg);
+    write_u64(dir + "/packing.bin", pk);
+    std::printf("client seed %u: secret in ntt form after the RGSW keyGen: %d\n", seed, ntt_after);
+    return 0;
+}
+
+// recover through the drop-in: srand(seed), Secret, decrypt_bsgs_total of
+// <dir>/response.bin -> message.bin and the mutated response (NTT form)
+static int decrypt_mode(const std::string& dir, int32_t r, unsigned seed) {
+    srand(seed);
+    Secret queryKey(crtMod, false);
+    std::vector<uint64_t> rw = read_u64(dir + "/response.bin", 2 * (size_t)N);
+    RlweCiphertext resp = rlwe(rw.data(), crtMod, false);
+    std::vector<uint64_t> msg;
+    decrypt_bsgs_total(msg, resp, queryKey, r);
+    write_u64(dir + "/message.bin", msg);
+    std::vector<uint64_t> after(resp.a);
+    after.insert(after.end(), resp.b.begin(), resp.b.end());
+    write_u64(dir + "/response_after.bin", after);
+    std::printf("decrypt: response ntt=%d\n", (int)resp.getIsNtt());
+    return 0;
+}
+
 int main(int argc, char** argv) {
-    if (argc != 4) { std::fprintf(stderr, "usage: dropin_test <dir> <r> <N1>\n"); return 2; }
+    if (argc == 8 && std::string(argv[1]) == "client")
+        return client_mode(argv[2], std::atoi(argv[3]), std::atoi(argv[4]), (unsigned)std::strtoul(argv[5], nullptr, 10),
+                           std::atoi(argv[6]), std::atoi(argv[7]));
+    if (argc == 5 && std::string(argv[1]) == "decrypt")
+        return decrypt_mode(argv[2], std::atoi(argv[3]), (unsigned)std::strtoul(argv[4], nullptr, 10));
+    if (argc != 4) { std::fprintf(stderr, "usage: dropin_test <dir> <r> <N1> | client <dir> <r> <N1> <seed> <col> <pack> | decrypt <dir> <r> <seed>\n"); return 2; }
     const std::string dir = argv[1];
     const int32_t r = std::atoi(argv[2]), N1 = std::atoi(argv[3]), N2 = N / 2 / N1;
     int lr = 0;

@@ -468,6 +468,36 @@ def test_client_generation_matches_oracle(ctx, oracle, r, n1, seeds):
         db.close()
 
 
+def test_cpp_client_dropin_matches_oracle(oracle, tmp_path):
+    """the client side of test-pir through kspir_pir.h (srand, Secret,
+    query_bsgs_rns, RGSW / BSGS / packing keyGen, rand() drawn on the host in
+    the reference's order) gives the oracle's words; decrypt_bsgs_total
+    recovers the planted record and leaves the response in NTT form"""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(__file__), "native", "_build", "dropin_test")
+    if not os.path.exists(exe):
+        pytest.fail("tests/native/_build/dropin_test not built (python kspir_amd/build.py)")
+    r, n1, seed, tcol, tpack = 4, 64, 41, 300, 2
+    say(f"[gpu] C++ client drop-in r={r} n1={n1} seed={seed}")
+    out = subprocess.run([exe, "client", str(tmp_path), str(r), str(n1), str(seed), str(tcol), str(tpack)],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    want = oracle.gen_client(r, n1, seed, tcol, tpack)
+    for k in ("secret", "query", "rgsw", "baby", "giant", "packing"):
+        got = np.fromfile(str(tmp_path / f"{k}.bin"), dtype=np.uint64).reshape(want[k].shape)
+        assert np.array_equal(got, want[k]), k
+    inst, resp, _ = instance(oracle, 2, 128, 3)     # gen_instance(seed 3): its secret is srand(3)'s
+    resp.tofile(str(tmp_path / "response.bin"))
+    out = subprocess.run([exe, "decrypt", str(tmp_path), "2", "3"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    msg = np.fromfile(str(tmp_path / "message.bin"), dtype=np.uint64)
+    assert np.array_equal(msg, np.arange(1, N + 1, dtype=np.uint64))
+    after = np.fromfile(str(tmp_path / "response_after.bin"), dtype=np.uint64).reshape(2, N)
+    assert np.array_equal(after[0], oracle.ntt_forward(0, resp[0]))
+    assert np.array_equal(after[1], oracle.ntt_forward(0, resp[1]))
+
+
 def test_cpp_dropin_matches_oracle(oracle, tmp_path):
     """tests/native/dropin_test calls matrix_vector_mul_bsgs_rns_crt_large,
     externalProduct and packingRLWEs through kspir_pir.h exactly as
