@@ -244,22 +244,70 @@ def config_keygen(args, log):
              "sample": f"one client (seed 1) by the oracle on the host, {cpu * 1e3:.0f} ms"}}
 
 
+def serve_rates(srv, db, keys, queries, rates, dur, seed, log, tag):
+    """Poisson arrivals at each offered rate for `dur` seconds through the
+    server; p50/p99 of the server's submit-to-response latency"""
+    import threading
+    res = {}
+    rng = np.random.default_rng(seed)
+    for rate in rates:
+        log(f"C5 {tag}: {rate} queries/s offered for {dur} s")
+        tickets, done_ms = [], []
+        stop_flag = [False]
+
+        def collect():
+            i = 0
+            while True:
+                while i >= len(tickets):
+                    if stop_flag[0] and i >= len(tickets):
+                        return
+                    time.sleep(0.0005)
+                _, ms = srv.wait(tickets[i])
+                done_ms.append(ms)
+                i += 1
+
+        th = threading.Thread(target=collect)
+        th.start()
+        b0 = srv.stats()
+        t_start = time.perf_counter()
+        t_next = t_start
+        c = 0
+        while t_next - t_start < dur:
+            now = time.perf_counter()
+            if now < t_next:
+                time.sleep(t_next - now)
+            tickets.append(srv.submit(db, keys[c % len(keys)], queries[c % len(queries)]))
+            c += 1
+            t_next += rng.exponential(1.0 / rate)
+        stop_flag[0] = True
+        th.join()
+        wall = time.perf_counter() - t_start
+        nb = srv.stats()["batches"] - b0["batches"]
+        res[str(rate)] = {"offered_qps": rate, "submitted": c, "achieved_qps": round(c / wall, 1),
+                          "latency_ms": {"p50": round(float(np.percentile(done_ms, 50)), 3),
+                                         "p99": round(float(np.percentile(done_ms, 99)), 3)},
+                          "batches": nb, "mean_batch": round(c / max(nb, 1), 2)}
+    return res
+
+
 def config_c5(args, log):
     """C5: serving loop.  A resident 4 GB database (r=256) and, memory
     permitting, a 16 GB one (r=1024); 64 clients with their own keys; queries
     arrive as a Poisson process at each offered rate and go through the
-    library's continuous-batching server (kspir_server_*).  Latency is
+    library's continuous-batching server (kspir_server_*), with one lane and
+    with two lanes (batches overlapping on the device).  Latency is
     submit-to-response, measured by the server."""
-    import threading
     import kspir_amd
     n1 = 128
     ctx = kspir_amd.Context(0)
     perms = kspir_amd.compute_permutation_matrix(n1)
     rec = synthetic_records()
     out = {"config": "C5 serving: Poisson arrivals, continuous batching (batch cap 48 at 4 GB, 16 at 16 GB; admission "
-                     "by per-query device footprint), 64 clients with distinct keys, resident 4 GB (r=256) and 16 GB (r=1024) "
-                     "databases, N1=128", "databases": {}}
-    for r, rates, dur, cap in ((256, (200, 500, 1000, 2000), 2.0, 48), (1024, (200, 500), 2.0, 16)):
+                     "by per-query device footprint; 1 lane, and 2 lanes whose batches overlap on the device), "
+                     "64 clients with distinct keys, resident 4 GB (r=256) and 16 GB (r=1024) databases, N1=128",
+           "databases": {}}
+    for r, rates, dur, cap, lane_set in ((256, (200, 500, 1000, 2000), 2.0, 48, (1, 2)),
+                                         (1024, (200, 500), 2.0, 16, (2,))):
         try:
             db = ctx.database_from_records(rec, r, n1, block_stride=0)
         except kspir_amd.KspirError as e:
@@ -271,61 +319,22 @@ def config_c5(args, log):
             query, baby, giant, packing, rgsw = synthetic_client(r, n1, 9000 + c)
             keys.append(ctx.keys(r, n1, baby, giant, packing, rgsw, perms))
             queries.append(query)
-        srv = kspir_amd.Server(ctx, max_batch=cap)
-        # warm-up: one full batch (database limbs, workspace at full size)
-        for t in [srv.submit(db, keys[c % 64], queries[c % 64]) for c in range(cap)]:
-            srv.wait(t)
-        res = {}
-        rng = np.random.default_rng(r)
-        for rate in rates:
-            log(f"C5 r={r}: {rate} queries/s offered for {dur} s")
-            tickets = []
-            done_ms = []
-
-            def collect():
-                i = 0
-                while True:
-                    while i >= len(tickets):
-                        if stop_flag[0] and i >= len(tickets):
-                            return
-                        time.sleep(0.0005)
-                    _, ms = srv.wait(tickets[i])
-                    done_ms.append(ms)
-                    i += 1
-
-            stop_flag = [False]
-            th = threading.Thread(target=collect)
-            th.start()
-            b0 = srv.stats()
-            t_start = time.perf_counter()
-            t_next = t_start
-            c = 0
-            while t_next - t_start < dur:
-                now = time.perf_counter()
-                if now < t_next:
-                    time.sleep(t_next - now)
-                tickets.append(srv.submit(db, keys[c % 64], queries[c % 64]))
-                c += 1
-                t_next += rng.exponential(1.0 / rate)
-            stop_flag[0] = True
-            th.join()
-            wall = time.perf_counter() - t_start
-            b1 = srv.stats()
-            nb = b1["batches"] - b0["batches"]
-            res[str(rate)] = {"offered_qps": rate, "submitted": c, "achieved_qps": round(c / wall, 1),
-                              "latency_ms": {"p50": round(float(np.percentile(done_ms, 50)), 3),
-                                             "p99": round(float(np.percentile(done_ms, 99)), 3)},
-                              "batches": nb, "mean_batch": round(c / max(nb, 1), 2)}
-        srv.close()
+        by_lanes = {}
+        for lanes in lane_set:
+            srv = kspir_amd.Server(ctx, max_batch=cap, lanes=lanes)
+            # warm-up: full batches on every lane (database limbs, workspaces at full size)
+            for t in [srv.submit(db, keys[c % 64], queries[c % 64]) for c in range(cap * lanes)]:
+                srv.wait(t)
+            by_lanes[f"lanes_{lanes}"] = serve_rates(srv, db, keys, queries, rates, dur, r, log, f"r={r} lanes={lanes}")
+            srv.close()
         out["databases"][f"{r // 64} GB"] = {"r": r, "batch_cap": cap,
                                             "per_query_footprint_MB": round(kspir_amd.query_footprint(r, n1) / 2**20, 1),
-                                            "rates": res}
+                                            **by_lanes}
         for k in keys:
             k.close()
         db.close()
     ctx.close()
     return out
-
 
 def main():
     ap = argparse.ArgumentParser()

@@ -24,6 +24,7 @@
  * success or a negative KSPIR_E* code; kspir_last_error() describes the last
  * failure on the calling thread.  A context is bound to one device and one
  * HIP stream and is not safe for concurrent use from several threads.
+ * Database and key handles may be used by any context of their device.
  */
 #ifndef KSPIR_AMD_H
 #define KSPIR_AMD_H
@@ -215,6 +216,13 @@ int kspir_client_encrypt(kspir_context* ctx, int32_t kind, const uint64_t* secre
  * kspir_answer's for the same query. */
 typedef struct kspir_server kspir_server;
 int kspir_server_create(kspir_context* ctx, int32_t max_batch, size_t workspace_budget_bytes, kspir_server** out);
+/* Same with `lanes` (1..8) worker threads: lane 0 drives ctx, each other
+ * lane a context of its own on ctx's device (with ctx's options), each
+ * taking the next batch as soon as it is free, so that batches overlap on
+ * the device (one batch's database stream beside another's transforms).
+ * The budget is split evenly between the lanes. */
+int kspir_server_create_lanes(kspir_context* ctx, int32_t max_batch, size_t workspace_budget_bytes, int32_t lanes,
+                              kspir_server** out);
 int kspir_server_submit(kspir_server* server, const kspir_database* db, const kspir_keys* keys,
                         const uint64_t* query, uint64_t* response, uint64_t* ticket);
 int kspir_server_wait(kspir_server* server, uint64_t ticket, double* latency_ms);

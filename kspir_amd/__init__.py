@@ -42,7 +42,8 @@ def _load():
                  "kspir_database_create_from_records", "kspir_database_tobsgsntt", "kspir_database_export",
                  "kspir_server_create", "kspir_server_submit", "kspir_server_wait", "kspir_server_stats",
                  "kspir_query_footprint_bytes", "kspir_decrypt_bsgs_total", "kspir_context_get",
-                 "kspir_autotune", "kspir_client_generate", "kspir_client_draws", "kspir_client_encrypt"):
+                 "kspir_autotune", "kspir_client_generate", "kspir_client_draws", "kspir_client_encrypt",
+                 "kspir_server_create_lanes"):
         getattr(L, name).restype = ctypes.c_int
     L.kspir_autotune.argtypes = [_vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                  ctypes.POINTER(ctypes.c_int32), ctypes.c_char_p, ctypes.c_size_t]
@@ -67,7 +68,7 @@ EXPORTS = (
     "kspir_time_copy", "kspir_database_create_from_records", "kspir_database_tobsgsntt", "kspir_database_export",
     "kspir_server_create", "kspir_server_submit", "kspir_server_wait", "kspir_server_stats", "kspir_server_destroy",
     "kspir_query_footprint_bytes", "kspir_decrypt_bsgs_total", "kspir_context_get", "kspir_autotune",
-    "kspir_client_generate", "kspir_client_draws", "kspir_client_encrypt",
+    "kspir_client_generate", "kspir_client_draws", "kspir_client_encrypt", "kspir_server_create_lanes",
 )
 
 
@@ -182,10 +183,11 @@ class Server:
     thread.  submit() returns a ticket; wait(ticket) returns (response,
     latency_ms)."""
 
-    def __init__(self, ctx, max_batch=64, budget_bytes=0):
+    def __init__(self, ctx, max_batch=64, budget_bytes=0, lanes=1):
         self.ctx = ctx
         h = _vp()
-        _check(lib.kspir_server_create(ctx.h, max_batch, ctypes.c_size_t(budget_bytes), ctypes.byref(h)))
+        _check(lib.kspir_server_create_lanes(ctx.h, max_batch, ctypes.c_size_t(budget_bytes), int(lanes),
+                                             ctypes.byref(h)))
         self.h = h
         self._live = {}
 

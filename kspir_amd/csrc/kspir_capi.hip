@@ -124,6 +124,8 @@ struct kspir_context {
         hipGraphExec_t exec;   // null: capture failed, run eagerly
     };
     std::vector<Graph> graphs;
+    std::mutex graph_mu;   // the cache is also dropped from threads that free handles
+    hipStream_t last_stream = nullptr;   // last caller stream work was enqueued on (besides `stream`)
     int use_graphs = 1;
     // client generation: M^kRngChunk of the rand() window, and the query
     // encoding round(m * Q / p) in long double for every m < p
@@ -132,6 +134,7 @@ struct kspir_context {
 };
 
 static void drop_graphs(kspir_context* c, const void* only = nullptr) {
+    std::lock_guard<std::mutex> lock(c->graph_mu);
     std::vector<kspir_context::Graph> keep;
     for (auto& g : c->graphs) {
         if (only && g.db != only && g.kp != only) { keep.push_back(g); continue; }
@@ -140,11 +143,27 @@ static void drop_graphs(kspir_context* c, const void* only = nullptr) {
     c->graphs.swap(keep);
 }
 
+// every live context, so that freeing a database or key handle drops the
+// graphs that name it in all contexts of its device (any context of the
+// device may use a handle)
+static std::mutex g_reg_mu;
+static std::vector<kspir_context*> g_reg;
+
+static void drop_graphs_everywhere(int device, const void* only) {
+    std::lock_guard<std::mutex> lock(g_reg_mu);
+    for (kspir_context* c : g_reg)
+        if (c->device == device) {
+            hipStreamSynchronize(c->stream);
+            drop_graphs(c, only);
+        }
+}
+
 struct kspir_database {
     kspir_context* ctx;
     int r, n1, n2;
     uint64_t* words;            // [r][N2][N1][N]
     int8_t* limbs = nullptr;    // [N][2][4][r*N2][N1] 7-bit limbs, built on the first batched call
+    std::mutex limbs_mu;        // contexts answering batches at once build them once
 };
 
 struct kspir_keys {
@@ -284,6 +303,7 @@ int carve_workspace(kspir_context* c, int r, int n1, int nq) {
     if (total * 4 > c->ws_bytes) {
         if (c->ws) {
             HIP_OK(hipStreamSynchronize(c->stream));
+            if (c->last_stream) HIP_OK(hipStreamSynchronize(c->last_stream));
             drop_graphs(c);
             HIP_OK(hipFree(c->ws));
             c->ws = nullptr;
@@ -346,10 +366,16 @@ int launch_db_batched(kspir_context* c, const kspir_database* dbc, int nq, hipSt
     kspir_database* db = const_cast<kspir_database*>(dbc);
     const int r = db->r, n1 = db->n1, n2 = db->n2, R = r * n2;
     const size_t dbwords = (size_t)r * kN * (kN / 2);
-    if (!db->limbs) {
-        HIP_OK(hipMalloc(&db->limbs, dbwords * 8));
-        hipLaunchKernelGGL(k_db_limbs, dim3(kN / 64, R), dim3(256), 0, st, db->words, db->limbs, n1, R);
-        LAUNCH_OK("k_db_limbs");
+    {
+        std::lock_guard<std::mutex> lock(db->limbs_mu);
+        if (!db->limbs) {
+            int8_t* l = nullptr;
+            HIP_OK(hipMalloc(&l, dbwords * 8));
+            hipLaunchKernelGGL(k_db_limbs, dim3(kN / 64, R), dim3(256), 0, st, db->words, l, n1, R);
+            LAUNCH_OK("k_db_limbs");
+            HIP_OK(hipStreamSynchronize(st));   // complete before another context's stream reads them
+            db->limbs = l;
+        }
     }
     const int lds_cap = 160 * 1024;
     int cmax = lds_cap / (4 * (n1 + 16)) / 16 * 16;
@@ -359,7 +385,14 @@ int launch_db_batched(kspir_context* c, const kspir_database* dbc, int nq, hipSt
     const size_t ql_bytes = (size_t)kN * 2 * 4 * C_full * n1;
     const size_t mf_bytes = (size_t)2 * R * C_full * kN * 4;
     if (ql_bytes + mf_bytes > c->mws_bytes) {
-        if (c->mws) { HIP_OK(hipStreamSynchronize(st)); drop_graphs(c); HIP_OK(hipFree(c->mws)); c->mws = nullptr; }
+        if (c->mws) {
+            HIP_OK(hipStreamSynchronize(st));
+            HIP_OK(hipStreamSynchronize(c->stream));
+            if (c->last_stream) HIP_OK(hipStreamSynchronize(c->last_stream));
+            drop_graphs(c);
+            HIP_OK(hipFree(c->mws));
+            c->mws = nullptr;
+        }
         HIP_OK(hipMalloc(&c->mws, ql_bytes + mf_bytes));
         c->mws_bytes = ql_bytes + mf_bytes;
     }
@@ -515,7 +548,8 @@ int enqueue_answer(kspir_context* c, const kspir_database* db, const KeyPtrs* kp
 // the database and keys fit together; the keys hold what the call needs
 int check_pair(const kspir_context* c, const kspir_database* db, const kspir_keys* k, bool first_dim_only = false) {
     if (!c || !db || !k) return fail(KSPIR_EINVAL, "null handle");
-    if (db->ctx != c || k->ctx != c) return fail(KSPIR_EINVAL, "handle belongs to another context");
+    if (db->ctx->device != c->device || k->ctx->device != c->device)
+        return fail(KSPIR_EINVAL, "handle belongs to another device");
     if (db->r != k->r || db->n1 != k->n1)
         return fail(KSPIR_EINVAL, "database and keys disagree on (r, N1)");
     if (!k->baby || (db->n2 > 1 && !k->giant)) return fail(KSPIR_EINVAL, "keys lack baby- or giant-step keys");
@@ -530,13 +564,24 @@ int check_pair(const kspir_context* c, const kspir_database* db, const kspir_key
 // launches; later calls replay the graph
 int run_answer(kspir_context* c, const kspir_database* db, const KeyPtrs* kp, int nq, const uint64_t* q,
                uint64_t* resp, hipStream_t st) {
+    if (st != c->stream) c->last_stream = st;
     if (!c->use_graphs) return enqueue_answer(c, db, kp, nq, q, resp, st, nullptr, nullptr, false);
-    for (auto& g : c->graphs)
-        if (g.db == db && g.kp == kp && g.nq == nq && g.q == q && g.resp == resp && g.st == st) {
-            if (!g.exec) return enqueue_answer(c, db, kp, nq, q, resp, st, nullptr, nullptr, false);
-            HIP_OK(hipGraphLaunch(g.exec, st));
-            return KSPIR_OK;
-        }
+    bool found = false;
+    hipGraphExec_t exec = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(c->graph_mu);
+        for (auto& g : c->graphs)
+            if (g.db == db && g.kp == kp && g.nq == nq && g.q == q && g.resp == resp && g.st == st) {
+                found = true;
+                exec = g.exec;
+                break;
+            }
+    }
+    if (found) {
+        if (!exec) return enqueue_answer(c, db, kp, nq, q, resp, st, nullptr, nullptr, false);
+        HIP_OK(hipGraphLaunch(exec, st));
+        return KSPIR_OK;
+    }
     int rc = enqueue_answer(c, db, kp, nq, q, resp, st, nullptr, nullptr, false);
     if (rc) return rc;
     kspir_context::Graph g{db, kp, nq, q, resp, st, nullptr};
@@ -550,7 +595,11 @@ int run_answer(kspir_context* c, const kspir_database* db, const KeyPtrs* kp, in
         if (graph) hipGraphDestroy(graph);
     }
     (void)hipGetLastError();   // a failed capture leaves the eager path in place
-    if (c->graphs.size() >= 64) drop_graphs(c);
+    if (c->graphs.size() >= 64) {
+        HIP_OK(hipStreamSynchronize(st));   // earlier replays may still be running
+        drop_graphs(c);
+    }
+    std::lock_guard<std::mutex> lock(c->graph_mu);
     c->graphs.push_back(g);
     return KSPIR_OK;
 }
@@ -559,6 +608,7 @@ int run_answer(kspir_context* c, const kspir_database* db, const KeyPtrs* kp, in
 int stage_batch_keys(kspir_context* c, const kspir_keys* const* keys, int nq) {
     if (nq > c->kp_cap) {
         HIP_OK(hipStreamSynchronize(c->stream));
+        if (c->last_stream) HIP_OK(hipStreamSynchronize(c->last_stream));
         drop_graphs(c);
         if (c->kp_dev) HIP_OK(hipFree(c->kp_dev));
         if (c->kp_host) HIP_OK(hipHostFree(c->kp_host));
@@ -574,6 +624,7 @@ int stage_batch_keys(kspir_context* c, const kspir_keys* const* keys, int nq) {
         same = std::memcmp(&c->kp_host[i], &keys[i]->ptrs, sizeof(KeyPtrs)) == 0;
     if (same) return KSPIR_OK;
     HIP_OK(hipStreamSynchronize(c->stream));   // the previous table may still be in flight
+    if (c->last_stream) HIP_OK(hipStreamSynchronize(c->last_stream));
     for (int i = 0; i < nq; ++i) c->kp_host[i] = keys[i]->ptrs;
     HIP_OK(hipMemcpyAsync(c->kp_dev, c->kp_host, sizeof(KeyPtrs) * nq, hipMemcpyHostToDevice, c->stream));
     c->kp_count = nq;
@@ -620,6 +671,10 @@ int kspir_context_create(int device, kspir_context** out) {
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete c; return fail_hip("hipStreamCreate", e); }
+    {
+        std::lock_guard<std::mutex> lock(g_reg_mu);
+        g_reg.push_back(c);
+    }
     int rc = build_tables(c);
     if (rc != KSPIR_OK) { kspir_context_destroy(c); return rc; }
     *out = c;
@@ -628,6 +683,11 @@ int kspir_context_create(int device, kspir_context** out) {
 
 void kspir_context_destroy(kspir_context* c) {
     if (!c) return;
+    {
+        std::lock_guard<std::mutex> lock(g_reg_mu);
+        for (size_t i = 0; i < g_reg.size(); ++i)
+            if (g_reg[i] == c) { g_reg.erase(g_reg.begin() + i); break; }
+    }
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     drop_graphs(c);
@@ -794,8 +854,7 @@ int kspir_database_export(const kspir_database* db, uint64_t* datacrt) {
 void kspir_database_destroy(kspir_database* db) {
     if (!db) return;
     hipSetDevice(db->ctx->device);
-    hipStreamSynchronize(db->ctx->stream);
-    drop_graphs(db->ctx, db);
+    drop_graphs_everywhere(db->ctx->device, db);
     hipFree(db->words);
     if (db->limbs) hipFree(db->limbs);
     delete db;
@@ -877,8 +936,7 @@ int kspir_keys_create(kspir_context* c, int32_t r, int32_t n1, const uint64_t* b
 void kspir_keys_destroy(kspir_keys* k) {
     if (!k) return;
     hipSetDevice(k->ctx->device);
-    hipStreamSynchronize(k->ctx->stream);
-    drop_graphs(k->ctx, k->dev_ptrs);
+    drop_graphs_everywhere(k->ctx->device, k->dev_ptrs);
     // a batch table naming these keys must not be reused
     k->ctx->kp_count = 0;
     hipFree(k->baby);
@@ -1224,30 +1282,39 @@ struct kspir_server {
         int rc;
         double ms;
     };
+    // one lane per worker thread: its own context (stream, workspace,
+    // graphs) and pinned staging, so batches of different lanes overlap on
+    // the device; lane 0 drives the caller's context
+    struct Lane {
+        kspir_context* ctx = nullptr;
+        bool owned = false;
+        uint64_t* qh = nullptr;   // pinned staging, max_batch queries / responses
+        uint64_t* rh = nullptr;
+        std::thread worker;
+    };
     kspir_context* ctx = nullptr;
     int max_batch = 64;
-    size_t budget = 0;
+    size_t budget = 0;            // per lane
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Req> pending;
     std::unordered_map<uint64_t, Done> done;
     uint64_t next_id = 1, batches = 0, answered = 0;
     bool stop = false;
-    uint64_t* qh = nullptr;   // pinned staging, max_batch queries / responses
-    uint64_t* rh = nullptr;
-    std::thread worker;
+    std::vector<Lane> lanes;
 
-    void run() {
+    void run(Lane& L) {
         std::vector<Req> take;
         std::vector<const kspir_keys*> keys;
         for (;;) {
+            bool more = false;
             {
                 std::unique_lock<std::mutex> lock(mu);
                 cv_work.wait(lock, [&] { return stop || !pending.empty(); });
                 if (pending.empty()) return;   // stop requested and drained
                 // continuous batching: the database of the oldest waiting
                 // query, and as many of its queries (in arrival order) as the
-                // batch cap and the workspace budget admit
+                // batch cap and the lane's workspace budget admit
                 const kspir_database* db = pending.front().db;
                 const size_t per = query_footprint_bytes(db->r, db->n1);
                 size_t cap = budget / (per ? per : 1);
@@ -1262,19 +1329,21 @@ struct kspir_server {
                         ++it;
                     }
                 }
+                more = !pending.empty();
             }
+            if (more) cv_work.notify_one();   // another lane may take the rest
             const int nq = (int)take.size();
             keys.resize(nq);
             for (int i = 0; i < nq; ++i) {
-                std::memcpy(qh + (size_t)i * 4 * kN, take[i].query, 4 * kN * 8);
+                std::memcpy(L.qh + (size_t)i * 4 * kN, take[i].query, 4 * kN * 8);
                 keys[i] = take[i].keys;
             }
-            const int rc = kspir_answer_batch(ctx, take[0].db, keys.data(), nq, qh, rh);
+            const int rc = kspir_answer_batch(L.ctx, take[0].db, keys.data(), nq, L.qh, L.rh);
             const auto t1 = std::chrono::steady_clock::now();
             {
                 std::lock_guard<std::mutex> lock(mu);
                 for (int i = 0; i < nq; ++i) {
-                    if (rc == KSPIR_OK) std::memcpy(take[i].response, rh + (size_t)i * 2 * kN, 2 * kN * 8);
+                    if (rc == KSPIR_OK) std::memcpy(take[i].response, L.rh + (size_t)i * 2 * kN, 2 * kN * 8);
                     done[take[i].id] = Done{rc, std::chrono::duration<double, std::milli>(t1 - take[i].t0).count()};
                 }
                 ++batches;
@@ -1679,8 +1748,9 @@ int kspir_query_footprint_bytes(int32_t r, int32_t n1, size_t* bytes) {
     return KSPIR_OK;
 }
 
-int kspir_server_create(kspir_context* c, int32_t max_batch, size_t workspace_budget_bytes, kspir_server** out) {
-    if (!c || !out || max_batch < 1) return fail(KSPIR_EINVAL, "bad server arguments");
+int kspir_server_create_lanes(kspir_context* c, int32_t max_batch, size_t workspace_budget_bytes, int32_t lanes,
+                              kspir_server** out) {
+    if (!c || !out || max_batch < 1 || lanes < 1 || lanes > 8) return fail(KSPIR_EINVAL, "bad server arguments");
     HIP_OK(hipSetDevice(c->device));
     kspir_server* s = new kspir_server();
     s->ctx = c;
@@ -1691,20 +1761,50 @@ int kspir_server_create(kspir_context* c, int32_t max_batch, size_t workspace_bu
         if (e != hipSuccess) { delete s; return fail_hip("hipMemGetInfo", e); }
         workspace_budget_bytes = fr / 2;
     }
-    s->budget = workspace_budget_bytes;
-    hipError_t e = hipHostMalloc(&s->qh, (size_t)max_batch * 4 * kN * 8, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipHostMalloc(&s->rh, (size_t)max_batch * 2 * kN * 8, hipHostMallocDefault);
-    if (e != hipSuccess) {
-        if (s->qh) hipHostFree(s->qh);
+    s->budget = workspace_budget_bytes / (size_t)lanes;
+    s->lanes.resize(lanes);
+    auto undo = [&](int rc) {
+        for (auto& L : s->lanes) {
+            if (L.qh) hipHostFree(L.qh);
+            if (L.rh) hipHostFree(L.rh);
+            if (L.owned) kspir_context_destroy(L.ctx);
+        }
         delete s;
-        return fail_hip("hipHostMalloc(server staging)", e);
+        return rc;
+    };
+    for (int i = 0; i < lanes; ++i) {
+        kspir_server::Lane& L = s->lanes[i];
+        if (i == 0) {
+            L.ctx = c;
+        } else {
+            int rc = kspir_context_create(c->device, &L.ctx);
+            if (rc) return undo(rc);
+            L.owned = true;
+            // the caller's tuned choices apply to every lane
+            L.ctx->mfma_min_batch = c->mfma_min_batch;
+            L.ctx->mfma_max_chunk = c->mfma_max_chunk;
+            L.ctx->fused_min_batch = c->fused_min_batch;
+            L.ctx->fused_pack = c->fused_pack;
+            L.ctx->use_graphs = c->use_graphs;
+            L.ctx->db_waves = c->db_waves;
+        }
+        hipError_t e = hipHostMalloc(&L.qh, (size_t)max_batch * 4 * kN * 8, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipHostMalloc(&L.rh, (size_t)max_batch * 2 * kN * 8, hipHostMallocDefault);
+        if (e != hipSuccess) return undo(fail_hip("hipHostMalloc(server staging)", e));
     }
-    s->worker = std::thread([s] {
-        hipSetDevice(s->ctx->device);
-        s->run();
-    });
+    for (auto& L : s->lanes) {
+        kspir_server::Lane* lp = &L;
+        L.worker = std::thread([s, lp] {
+            hipSetDevice(lp->ctx->device);
+            s->run(*lp);
+        });
+    }
     *out = s;
     return KSPIR_OK;
+}
+
+int kspir_server_create(kspir_context* c, int32_t max_batch, size_t workspace_budget_bytes, kspir_server** out) {
+    return kspir_server_create_lanes(c, max_batch, workspace_budget_bytes, 1, out);
 }
 
 int kspir_server_submit(kspir_server* s, const kspir_database* db, const kspir_keys* keys, const uint64_t* query,
@@ -1750,9 +1850,13 @@ void kspir_server_destroy(kspir_server* s) {
         s->stop = true;
     }
     s->cv_work.notify_all();
-    if (s->worker.joinable()) s->worker.join();
-    hipHostFree(s->qh);
-    hipHostFree(s->rh);
+    for (auto& L : s->lanes)
+        if (L.worker.joinable()) L.worker.join();
+    for (auto& L : s->lanes) {
+        hipHostFree(L.qh);
+        hipHostFree(L.rh);
+        if (L.owned) kspir_context_destroy(L.ctx);
+    }
     delete s;
 }
 

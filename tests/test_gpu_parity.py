@@ -267,12 +267,15 @@ def test_database_preprocessing_on_gpu(ctx, oracle, r, n1):
         d.close()
 
 
-def test_serving_loop_mixed_databases(ctx, oracle):
+@pytest.mark.parametrize("lanes", [1, 3])
+def test_serving_loop_mixed_databases(ctx, oracle, lanes):
     """queries for two databases submitted interleaved to the serving loop
-    (batch cap 3, a budget that admits two queries of the larger shape) come
-    back equal to the same queries answered alone, and equal to the oracle"""
+    (batch cap 3, a budget that admits two queries of the larger shape per
+    lane; 1 or 3 lanes answering batches at once, the extra lanes on contexts
+    of their own) come back equal to the same queries answered alone, and
+    equal to the oracle"""
     import kspir_amd
-    say("[gpu] serving loop, two databases")
+    say(f"[gpu] serving loop, two databases, lanes={lanes}")
     shapes = [(2, 128, 61), (4, 64, 62)]
     dbs, clients, want = [], [], []
     for r, n1, seed in shapes:
@@ -289,7 +292,7 @@ def test_serving_loop_mixed_databases(ctx, oracle):
                        packing=clients[0][0][0]["packing"], perms=dbs[0][4])
     assert np.array_equal(want[0][0], w0)
     foot = max(kspir_amd.query_footprint(r, n1) for r, n1, _ in shapes)
-    srv = kspir_amd.Server(ctx, max_batch=3, budget_bytes=2 * foot)
+    srv = kspir_amd.Server(ctx, max_batch=3, budget_bytes=2 * foot * lanes, lanes=lanes)
     try:
         tickets = []
         for c in range(4):
