@@ -111,6 +111,7 @@ struct kspir_context {
     int fused_min_batch = 16;       // batches of at least this many queries use k_giant_fused
     int fused_pack = 1;             // one k_pack_level per packing level (0: three kernels per level)
     int db_waves = 4;               // waves per workgroup of the single-query database pass (2, 4, 8, 16)
+    int giant_wgs_per_cu = 1;       // k_giant_fused (1) or its two-per-CU variant k_giant_fused2 (2)
     // captured pipelines: one hipGraph per (database, key table, batch size,
     // query / response buffers, stream); dropped whenever a buffer they name
     // is reallocated or freed, or an option changes
@@ -502,8 +503,12 @@ int enqueue_answer(kspir_context* c, const kspir_database* db, const KeyPtrs* kp
     // fused workgroup per (query, block) for a batch; for few queries, one
     // workgroup per (block, row, prime) keeps the GPU full
     if (nq >= c->fused_min_batch) {
-        hipLaunchKernelGGL(k_giant_fused, dim3(r * nq), dim3(kFusedThreads), 0, st, c->dbout, kp, c->fd, c->ecoef, r, nq,
-                           n1, n2, S, T);
+        if (c->giant_wgs_per_cu == 2)
+            hipLaunchKernelGGL(k_giant_fused2, dim3(r * nq), dim3(kFusedThreads), 0, st, c->dbout, kp, c->fd, c->ecoef,
+                               r, nq, n1, n2, S, T);
+        else
+            hipLaunchKernelGGL(k_giant_fused, dim3(r * nq), dim3(kFusedThreads), 0, st, c->dbout, kp, c->fd, c->ecoef,
+                               r, nq, n1, n2, S, T);
         LAUNCH_OK("k_giant_fused");
     } else {
         if (n2 > 1) {
@@ -1133,6 +1138,9 @@ int kspir_context_set(kspir_context* c, const char* key, int64_t value) {
         c->fused_pack = value != 0;
     } else if (k == "graphs") {
         c->use_graphs = value != 0;
+    } else if (k == "giant_wgs_per_cu") {
+        if (value != 1 && value != 2) return fail(KSPIR_EINVAL, "giant_wgs_per_cu must be 1 or 2");
+        c->giant_wgs_per_cu = (int)value;
     } else if (k == "db_waves") {
         if (value != 2 && value != 4 && value != 8 && value != 16)
             return fail(KSPIR_EINVAL, "db_waves must be 2, 4, 8 or 16");
@@ -1152,6 +1160,7 @@ int kspir_context_get(kspir_context* c, const char* key, int64_t* value) {
     else if (k == "fused_pack") *value = c->fused_pack;
     else if (k == "graphs") *value = c->use_graphs;
     else if (k == "db_waves") *value = c->db_waves;
+    else if (k == "giant_wgs_per_cu") *value = c->giant_wgs_per_cu;
     else return fail(KSPIR_EINVAL, "unknown option " + k);
     return KSPIR_OK;
 }

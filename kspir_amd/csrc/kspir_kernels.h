@@ -1067,6 +1067,117 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused(const uint32_t* _
     }
 }
 
+// k_giant_fused in four LDS planes and at most 128 VGPRs, so that two
+// workgroups share a CU: the b-row of a giant step is gathered from global
+// memory (L2) instead of staged in LDS, and each half runs its transforms one
+// at a time.  Same words as k_giant_fused.
+__global__ void __launch_bounds__(kFusedThreads) __attribute__((amdgpu_waves_per_eu(4))) k_giant_fused2(const uint32_t* __restrict__ dbout,
+        const KeyPtrs* __restrict__ kp, uint32_t* __restrict__ fd, uint32_t* __restrict__ ecoef, int r, int nq,
+        int n1, int n2, QS S, Tabs T) {
+    __shared__ uint32_t s[4][kPadN];     // the transforms; the b-row is gathered from global memory
+    const int L = xcd_job(blockIdx.x, r * nq);
+    const int blk = L % r;
+    const size_t qi = L / r;
+    dbout += qi * S.dbout;
+    if (fd) fd += qi * S.fd;
+    ecoef += qi * S.ecoef;
+    const uint32_t* gkeys = kp[qi].giant;
+    const int tid = threadIdx.x;
+    const int half = __builtin_amdgcn_readfirstlane(tid >> 8), ht = tid & 255;
+    constexpr int PER = kN / kFusedThreads;
+    const uint32_t q1 = T.p[0].q, q2 = T.p[1].q;
+    uint32_t acc[2][2][PER];    // [ab][p][e], slot x = tid + 512 e
+#pragma unroll
+    for (int e = 0; e < PER; ++e) acc[0][0][e] = acc[0][1][e] = acc[1][0][e] = acc[1][1][e] = 0;
+    for (int c = 1; c < n2; ++c) {
+        const uint32_t* row = dbout + ((size_t)blk * n2 + c) * 4 * kN;   // [ab][p][N]
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int x = tid + e * kFusedThreads;
+            s[0][lds_pad(x)] = row[x];
+            s[1][lds_pad(x)] = row[kN + x];
+        }
+        __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
+        ntt_inv_r16(s[half], T.p[half], ht);
+        __builtin_amdgcn_sched_barrier(0);
+        const uint32_t ex = (uint32_t)(n1 * c) % (kN / 2);
+        const uint32_t g = pow5_mod2n(ex);
+        const uint32_t ginv = pow5_mod2n((kN / 2 - ex) % (kN / 2));
+        // digits of tau_g(a): mod q2 straight into planes 2, 3 (not read
+        // here); mod q1 held until the gathers from planes 0, 1 are done
+        uint32_t dg[2][PER];
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int x = tid + e * kFusedThreads;
+            const uint32_t i0 = ((uint32_t)x * ginv) & (2 * kN - 1);
+            const bool neg = i0 >= (uint32_t)kN;
+            const int i = neg ? (int)i0 - kN : (int)i0;
+            uint64_t v = crt_q(s[0][lds_pad(i)], s[1][lds_pad(i)], T);
+            if (neg) v = v ? kQ - v : 0;
+            const int32_t d0 = tfhe_digit<18, 20, 2>(v, 0), d1 = tfhe_digit<18, 20, 2>(v, 1);
+            dg[0][e] = digit_residue(d0, q1);
+            dg[1][e] = digit_residue(d1, q1);
+            s[2][lds_pad(x)] = digit_residue(d0, q2);
+            s[3][lds_pad(x)] = digit_residue(d1, q2);
+        }
+        __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int x = tid + e * kFusedThreads;
+            s[0][lds_pad(x)] = dg[0][e];
+            s[1][lds_pad(x)] = dg[1][e];
+        }
+        __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
+        ntt_fwd_r16(s[2 * half], T.p[half], ht);
+        ntt_fwd_r16(s[2 * half + 1], T.p[half], ht);
+        __builtin_amdgcn_sched_barrier(0);
+        const uint32_t* key = gkeys + (size_t)(c - 1) * 2 * 2 * 2 * kN;   // [j][ab][p][N]
+#pragma unroll 2
+        for (int e = 0; e < PER; ++e) {
+            const int x = tid + e * kFusedThreads;
+            const int px = perm_slot(x, g, T);
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const PrimeTab& P = T.p[p];
+                const uint64_t d0 = s[2 * p][lds_pad(x)], d1 = s[2 * p + 1][lds_pad(x)];
+                const uint64_t ta = d0 * key[((0 * 2 + 0) * 2 + p) * kN + x] + d1 * key[((1 * 2 + 0) * 2 + p) * kN + x];
+                const uint64_t tb = d0 * key[((0 * 2 + 1) * 2 + p) * kN + x] + d1 * key[((1 * 2 + 1) * 2 + p) * kN + x];
+                acc[0][p][e] = sub_mod(acc[0][p][e], mont_reduce(ta, P.q, P.mont_qinv), P.q);
+                const uint32_t bb = add_mod(acc[1][p][e], row[(2 + p) * kN + px], P.q);
+                acc[1][p][e] = sub_mod(bb, mont_reduce(tb, P.q, P.mont_qinv), P.q);
+            }
+        }
+        __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    const uint32_t* row0 = dbout + (size_t)blk * n2 * 4 * kN;
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int x = tid + e * kFusedThreads;
+#pragma unroll
+        for (int ab = 0; ab < 2; ++ab)
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const uint32_t v = add_mod(acc[ab][p][e], row0[(ab * 2 + p) * kN + x], T.p[p].q);
+                s[ab * 2 + p][lds_pad(x)] = v;
+                if (fd) fd[(((size_t)blk * 2 + ab) * 2 + p) * kN + x] = v;
+            }
+    }
+    __syncthreads();
+    __builtin_amdgcn_sched_barrier(0);
+    ntt_inv_r16(s[2 * half], T.p[0], ht);    // half = ab
+    ntt_inv_r16(s[2 * half + 1], T.p[1], ht);
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int x = tid + e * kFusedThreads;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) ecoef[((size_t)blk * 4 + t) * kN + x] = s[t][lds_pad(x)];
+    }
+}
+
 // lwe.cpp:761-808: digit k (0,1 from a; 2,3 from b) of decompose_rlwe
 // (ell 2, PP 2^20, Bg 2^18), NTT, times RGSW row k: grid (r, 4, 2 p)
 // epart: [r][4][2 ab][2 p][N]

@@ -143,6 +143,16 @@ extern "C" int kspir_autotune(kspir_context* c, const kspir_database* db, const 
     js += "}, \"fused_min_batch\": " + std::to_string(fused_from);
     if ((rc = kspir_context_set(c, "fused_min_batch", fused_from))) return rc;
     if (max_batch >= fused_from) {
+        // the fused giant-step kernel at one or two workgroups per CU
+        double g1 = 0, g2 = 0;
+        if ((rc = kspir_context_set(c, "giant_wgs_per_cu", 1))) return rc;
+        if ((rc = step_ms(max_batch, &g1))) return rc;
+        if ((rc = kspir_context_set(c, "giant_wgs_per_cu", 2))) return rc;
+        if ((rc = step_ms(max_batch, &g2))) return rc;
+        const int gw = g2 < g1 ? 2 : 1;
+        js += ", \"giant_wgs_ms\": [" + fmt("%.4f", g1) + ", " + fmt("%.4f", g2) + "], \"giant_wgs_per_cu\": " +
+              std::to_string(gw);
+        if ((rc = kspir_context_set(c, "giant_wgs_per_cu", gw))) return rc;
         double t1 = 0, t0 = 0;
         if ((rc = kspir_context_set(c, "fused_pack", 1))) return rc;
         if ((rc = step_ms(max_batch, &t1))) return rc;
