@@ -34,6 +34,9 @@ int kspir_device_count(int* count);
  *                     a captured hipGraph of the whole pipeline for buffers
  *                     and batch sizes seen before (captured after a first,
  *                     eager call); 0: launch kernel by kernel
+ *   "pack_fused_min_wgs" with fused_pack, a packing level runs fused only when
+ *                     it has at least this many (pair, query) workgroups
+ *                     (default 0: every level)
  *   "giant_wgs_per_cu" 1 (default): the fused giant-step kernel with six LDS
  *                     planes, one workgroup per CU; 2: a variant in four
  *                     planes and 128 VGPRs (b-rows gathered from L2), two per CU
@@ -50,7 +53,8 @@ int kspir_context_get(kspir_context* ctx, const char* key, int64_t* value);
  * max_batch queries, and sets the options above: db_waves (fastest database
  * pass), mfma_min_batch and fused_min_batch (smallest batch size from which
  * that path stays faster, over powers of two up to max_batch),
- * mfma_max_chunk, giant_wgs_per_cu and fused_pack (fastest at max_batch).  *serve_batch (may be
+ * mfma_max_chunk, giant_wgs_per_cu, fused_pack and pack_fused_min_wgs
+ * (fastest at max_batch).  *serve_batch (may be
  * NULL) receives the batch size with the most queries/s whose step time is
  * within latency_budget_ms (0: no bound).  report (may be NULL): every
  * timing as JSON text, cut to report_len bytes.  Runs for a few seconds. */

@@ -112,6 +112,7 @@ struct kspir_context {
     int fused_pack = 1;             // one k_pack_level per packing level (0: three kernels per level)
     int db_waves = 4;               // waves per workgroup of the single-query database pass (2, 4, 8, 16)
     int giant_wgs_per_cu = 1;       // k_giant_fused (1) or its two-per-CU variant k_giant_fused2 (2)
+    int64_t pack_fused_min_wgs = 0; // a packing level is fused only with at least this many (pair, query) workgroups
     // captured pipelines: one hipGraph per (database, key table, batch size,
     // query / response buffers, stream); dropped whenever a buffer they name
     // is reallocated or freed, or an option changes
@@ -448,7 +449,7 @@ int enqueue_packing(kspir_context* c, const KeyPtrs* kp, int r, int nq, hipStrea
         const int li = ilog2(i);
         const uint32_t g = (uint32_t)(kN / i + 1);
         const uint32_t ginv = (uint32_t)powmod(g, kN - 1, 2 * kN);   // odd g has order | N in (Z/2N)*
-        if (c->fused_pack && nq >= c->fused_min_batch) {
+        if (c->fused_pack && nq >= c->fused_min_batch && (int64_t)i * nq >= c->pack_fused_min_wgs) {
             hipLaunchKernelGGL(k_pack_level, dim3(i * nq), dim3(512), 0, st, cur, kp, li, nxt, i, g, ginv, S, T);
             LAUNCH_OK("k_pack_level");
         } else {
@@ -1138,6 +1139,9 @@ int kspir_context_set(kspir_context* c, const char* key, int64_t value) {
         c->fused_pack = value != 0;
     } else if (k == "graphs") {
         c->use_graphs = value != 0;
+    } else if (k == "pack_fused_min_wgs") {
+        if (value < 0) return fail(KSPIR_EINVAL, "pack_fused_min_wgs must be >= 0");
+        c->pack_fused_min_wgs = value;
     } else if (k == "giant_wgs_per_cu") {
         if (value != 1 && value != 2) return fail(KSPIR_EINVAL, "giant_wgs_per_cu must be 1 or 2");
         c->giant_wgs_per_cu = (int)value;
@@ -1161,6 +1165,7 @@ int kspir_context_get(kspir_context* c, const char* key, int64_t* value) {
     else if (k == "graphs") *value = c->use_graphs;
     else if (k == "db_waves") *value = c->db_waves;
     else if (k == "giant_wgs_per_cu") *value = c->giant_wgs_per_cu;
+    else if (k == "pack_fused_min_wgs") *value = c->pack_fused_min_wgs;
     else return fail(KSPIR_EINVAL, "unknown option " + k);
     return KSPIR_OK;
 }
@@ -1794,6 +1799,8 @@ int kspir_server_create_lanes(kspir_context* c, int32_t max_batch, size_t worksp
             L.ctx->mfma_max_chunk = c->mfma_max_chunk;
             L.ctx->fused_min_batch = c->fused_min_batch;
             L.ctx->fused_pack = c->fused_pack;
+            L.ctx->pack_fused_min_wgs = c->pack_fused_min_wgs;
+            L.ctx->giant_wgs_per_cu = c->giant_wgs_per_cu;
             L.ctx->use_graphs = c->use_graphs;
             L.ctx->db_waves = c->db_waves;
         }

@@ -161,6 +161,21 @@ extern "C" int kspir_autotune(kspir_context* c, const kspir_database* db, const 
         const int fp = t1 <= t0 ? 1 : 0;
         js += ", \"pack_ms\": [" + fmt("%.4f", t0) + ", " + fmt("%.4f", t1) + "], \"fused_pack\": " + std::to_string(fp);
         if ((rc = kspir_context_set(c, "fused_pack", fp))) return rc;
+        if (fp) {
+            // levels with few (pair, query) workgroups may do better as three kernels
+            int64_t best_w = 0;
+            double bw = t1;
+            js += ", \"pack_fused_min_wgs_ms\": {\"0\": " + fmt("%.4f", t1);
+            for (int64_t w : {64, 128, 256, 512}) {
+                if ((rc = kspir_context_set(c, "pack_fused_min_wgs", w))) return rc;
+                double t = 0;
+                if ((rc = step_ms(max_batch, &t))) return rc;
+                js += ", \"" + std::to_string(w) + "\": " + fmt("%.4f", t);
+                if (t < bw) { bw = t; best_w = w; }
+            }
+            js += "}, \"pack_fused_min_wgs\": " + std::to_string(best_w);
+            if ((rc = kspir_context_set(c, "pack_fused_min_wgs", best_w))) return rc;
+        }
     }
 
     // 5. serving batch: highest queries/s within the latency budget

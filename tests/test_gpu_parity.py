@@ -329,15 +329,24 @@ def test_packing_paths_agree(ctx, oracle):
     db = ctx.database(db_words, r, n1)
     keys = [ctx.keys(r, n1, x["baby"], x["giant"], x["packing"], x["rgsw"], perms) for x in cl]
     queries = np.stack([x["query"] for x in cl])
-    fused_b = ctx.answer_batch(db, keys, queries)
-    fused_1 = ctx.answer(db, keys[0], cl[0]["query"])
-    ctx.set("fused_pack", 0)
+    ctx.set("fused_min_batch", 1)      # the fused levels run from one query up
     try:
+        fused_b = ctx.answer_batch(db, keys, queries)
+        fused_1 = ctx.answer(db, keys[0], cl[0]["query"])
+        # levels i = 4, 2, 1 carry 12, 6, 3 (pair, query) workgroups: the first
+        # two fused, the last as three kernels
+        ctx.set("pack_fused_min_wgs", 5)
+        mixed_b = ctx.answer_batch(db, keys, queries)
+        ctx.set("pack_fused_min_wgs", 0)
+        ctx.set("fused_pack", 0)
         plain_b = ctx.answer_batch(db, keys, queries)
         plain_1 = ctx.answer(db, keys[0], cl[0]["query"])
     finally:
         ctx.set("fused_pack", 1)
+        ctx.set("pack_fused_min_wgs", 0)
+        ctx.set("fused_min_batch", 16)
     assert np.array_equal(fused_b, plain_b)
+    assert np.array_equal(mixed_b, plain_b)
     assert np.array_equal(fused_1, plain_1) and np.array_equal(fused_1, fused_b[0])
     want = oracle.answer(r=r, n1=n1, db=db_words, query=cl[0]["query"], rgsw=cl[0]["rgsw"], baby=cl[0]["baby"],
                          giant=cl[0]["giant"], packing=cl[0]["packing"], perms=perms)
@@ -442,7 +451,8 @@ def test_autotune_sets_options_and_answers_stay_exact(ctx, oracle):
         assert np.array_equal(got[1], want)
     finally:
         for k, v in (("mfma_min_batch", 2), ("mfma_max_chunk", 1 << 30), ("fused_min_batch", 16),
-                     ("fused_pack", 1), ("db_waves", 4), ("giant_wgs_per_cu", 1)):
+                     ("fused_pack", 1), ("db_waves", 4), ("giant_wgs_per_cu", 1),
+                     ("pack_fused_min_wgs", 0)):
             ctx.set(k, v)
         for k in keys:
             k.close()
