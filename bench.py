@@ -387,7 +387,7 @@ def main():
         log("autotuning kernel shapes and paths for this database")
         serve_b, rep = ctx.autotune(db, keys[:8], max_batch=B)
         tuned = {k: ctx.get(k) for k in ("db_waves", "mfma_min_batch", "mfma_max_chunk", "fused_min_batch",
-                                          "fused_pack", "giant_wgs_per_cu", "pack_fused_min_wgs")}
+                                          "fused_pack", "giant_wgs_per_cu", "giant_dma", "pack_fused_min_wgs")}
         tuned["serve_batch"] = serve_b
         tuned["report"] = rep
 

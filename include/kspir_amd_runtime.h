@@ -40,6 +40,10 @@ int kspir_device_count(int* count);
  *   "giant_wgs_per_cu" 1 (default): the fused giant-step kernel with six LDS
  *                     planes, one workgroup per CU; 2: a variant in four
  *                     planes and 128 VGPRs (b-rows gathered from L2), two per CU
+ *   "giant_dma"       with giant_wgs_per_cu 1: 1 (default) takes the fused
+ *                     giant-step kernel that prefetches each next row into
+ *                     LDS by DMA while the current row is transformed (eight
+ *                     planes); 0: rows loaded through registers (six planes)
  *   "db_waves"        workgroup shape of the single-query database pass:
  *                     2, 4 (default), 8 or 16 waves on 64 slots, each wave
  *                     taking N2 / waves rows
@@ -53,7 +57,7 @@ int kspir_context_get(kspir_context* ctx, const char* key, int64_t* value);
  * max_batch queries, and sets the options above: db_waves (fastest database
  * pass), mfma_min_batch and fused_min_batch (smallest batch size from which
  * that path stays faster, over powers of two up to max_batch),
- * mfma_max_chunk, giant_wgs_per_cu, fused_pack and pack_fused_min_wgs
+ * mfma_max_chunk, giant_wgs_per_cu, giant_dma, fused_pack and pack_fused_min_wgs
  * (fastest at max_batch).  *serve_batch (may be
  * NULL) receives the batch size with the most queries/s whose step time is
  * within latency_budget_ms (0: no bound).  report (may be NULL): every

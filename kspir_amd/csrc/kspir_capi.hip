@@ -112,6 +112,7 @@ struct kspir_context {
     int fused_pack = 1;             // one k_pack_level per packing level (0: three kernels per level)
     int db_waves = 4;               // waves per workgroup of the single-query database pass (2, 4, 8, 16)
     int giant_wgs_per_cu = 1;       // k_giant_fused (1) or its two-per-CU variant k_giant_fused2 (2)
+    int giant_dma = 1;              // one per CU with rows prefetched by LDS-DMA (k_giant_fused_dma)
     int64_t pack_fused_min_wgs = 0; // a packing level is fused only with at least this many (pair, query) workgroups
     // captured pipelines: one hipGraph per (database, key table, batch size,
     // query / response buffers, stream); dropped whenever a buffer they name
@@ -504,7 +505,10 @@ int enqueue_answer(kspir_context* c, const kspir_database* db, const KeyPtrs* kp
     // fused workgroup per (query, block) for a batch; for few queries, one
     // workgroup per (block, row, prime) keeps the GPU full
     if (nq >= c->fused_min_batch) {
-        if (c->giant_wgs_per_cu == 2)
+        if (c->giant_dma && c->giant_wgs_per_cu == 1)
+            hipLaunchKernelGGL(k_giant_fused_dma, dim3(r * nq), dim3(kFusedThreads), 0, st, c->dbout, kp, c->fd,
+                               c->ecoef, r, nq, n1, n2, S, T);
+        else if (c->giant_wgs_per_cu == 2)
             hipLaunchKernelGGL(k_giant_fused2, dim3(r * nq), dim3(kFusedThreads), 0, st, c->dbout, kp, c->fd, c->ecoef,
                                r, nq, n1, n2, S, T);
         else
@@ -1145,6 +1149,8 @@ int kspir_context_set(kspir_context* c, const char* key, int64_t value) {
     } else if (k == "giant_wgs_per_cu") {
         if (value != 1 && value != 2) return fail(KSPIR_EINVAL, "giant_wgs_per_cu must be 1 or 2");
         c->giant_wgs_per_cu = (int)value;
+    } else if (k == "giant_dma") {
+        c->giant_dma = value != 0;
     } else if (k == "db_waves") {
         if (value != 2 && value != 4 && value != 8 && value != 16)
             return fail(KSPIR_EINVAL, "db_waves must be 2, 4, 8 or 16");
@@ -1165,6 +1171,7 @@ int kspir_context_get(kspir_context* c, const char* key, int64_t* value) {
     else if (k == "graphs") *value = c->use_graphs;
     else if (k == "db_waves") *value = c->db_waves;
     else if (k == "giant_wgs_per_cu") *value = c->giant_wgs_per_cu;
+    else if (k == "giant_dma") *value = c->giant_dma;
     else if (k == "pack_fused_min_wgs") *value = c->pack_fused_min_wgs;
     else return fail(KSPIR_EINVAL, "unknown option " + k);
     return KSPIR_OK;
@@ -1801,6 +1808,7 @@ int kspir_server_create_lanes(kspir_context* c, int32_t max_batch, size_t worksp
             L.ctx->fused_pack = c->fused_pack;
             L.ctx->pack_fused_min_wgs = c->pack_fused_min_wgs;
             L.ctx->giant_wgs_per_cu = c->giant_wgs_per_cu;
+            L.ctx->giant_dma = c->giant_dma;
             L.ctx->use_graphs = c->use_graphs;
             L.ctx->db_waves = c->db_waves;
         }

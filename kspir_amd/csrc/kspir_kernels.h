@@ -1067,6 +1067,135 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused(const uint32_t* _
     }
 }
 
+// One giant-step row (4 planes [ab][p][N] of 4096 words) copied from global
+// memory straight into LDS planes dst[0..3] (padded layout) by the wave-wide
+// LDS-DMA load: lane l of an instruction moves word x0 + l to LDS address
+// M0 + 4 l, and 64 consecutive words starting at a multiple of 64 are
+// consecutive in the padded layout.  256 instructions per row, dealt to the 8
+// waves; the copy holds no VGPRs and completes under vmcnt.
+__device__ __forceinline__ void dma_row(uint32_t (*dst)[kPadN], const uint32_t* __restrict__ row, int tid) {
+    typedef __attribute__((address_space(3))) void lds_void;
+    typedef __attribute__((address_space(1))) void glb_void;
+    const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+#pragma unroll 8
+    for (int b = wv; b < 4 * kN / 64; b += kFusedThreads / 64) {
+        const int j = b >> 6, x0 = (b & 63) * 64;
+        __builtin_amdgcn_global_load_lds((glb_void*)(row + j * kN + x0 + lane), (lds_void*)(&dst[j][lds_pad(x0)]), 4, 0,
+                                         0);
+    }
+}
+
+// s_waitcnt vmcnt(0): this wave's global loads, LDS-DMA copies included, done
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// k_giant_fused with the rows prefetched: while row c is decomposed,
+// transformed and multiplied, row c+1 (row 0 after the last) is already on
+// its way into LDS planes 4..7 by LDS-DMA.  The b-row of step c is added to
+// the accumulator (by the slot permutation of tau_g) as soon as it is in LDS,
+// and the a-row is inverse-transformed in place in planes 4, 5, so the four
+// digit planes 0..3 are written in one step.  Planes: 0..3 digits [p][j]
+// (0 = d0 mod q1, 1 = d1 mod q1, 2 = d0 mod q2, 3 = d1 mod q2), 4..7 the row
+// [ab][p].  Same words as k_giant_fused: every update is exact mod q.
+__global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_t* __restrict__ dbout,
+        const KeyPtrs* __restrict__ kp, uint32_t* __restrict__ fd, uint32_t* __restrict__ ecoef, int r, int nq,
+        int n1, int n2, QS S, Tabs T) {
+    __shared__ uint32_t s[8][kPadN];
+    const int L = xcd_job(blockIdx.x, r * nq);
+    const int blk = L % r;
+    const size_t qi = L / r;
+    dbout += qi * S.dbout;
+    if (fd) fd += qi * S.fd;
+    ecoef += qi * S.ecoef;
+    const uint32_t* gkeys = kp[qi].giant;
+    const int tid = threadIdx.x;
+    const int half = __builtin_amdgcn_readfirstlane(tid >> 8), ht = tid & 255;
+    constexpr int PER = kN / kFusedThreads;
+    const uint32_t q1 = T.p[0].q, q2 = T.p[1].q;
+    const uint32_t* rows = dbout + (size_t)blk * n2 * 4 * kN;
+    uint32_t acc[2][2][PER];    // [ab][p][e], slot x = tid + 512 e
+#pragma unroll
+    for (int e = 0; e < PER; ++e) acc[0][0][e] = acc[0][1][e] = acc[1][0][e] = acc[1][1][e] = 0;
+    dma_row(s + 4, rows + (size_t)(n2 > 1 ? 1 : 0) * 4 * kN, tid);
+    for (int c = 1; c < n2; ++c) {
+        wait_vmem();
+        __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
+        const uint32_t ex = (uint32_t)(n1 * c) % (kN / 2);
+        const uint32_t g = pow5_mod2n(ex);
+        const uint32_t ginv = pow5_mod2n((kN / 2 - ex) % (kN / 2));
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int px = perm_slot(tid + e * kFusedThreads, g, T);
+            acc[1][0][e] = add_mod(acc[1][0][e], s[6][lds_pad(px)], q1);
+            acc[1][1][e] = add_mod(acc[1][1][e], s[7][lds_pad(px)], q2);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        ntt_inv_r16(s[4 + half], T.p[half], ht);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int x = tid + e * kFusedThreads;
+            const uint32_t i0 = ((uint32_t)x * ginv) & (2 * kN - 1);
+            const bool neg = i0 >= (uint32_t)kN;
+            const int i = neg ? (int)i0 - kN : (int)i0;
+            uint64_t v = crt_q(s[4][lds_pad(i)], s[5][lds_pad(i)], T);
+            if (neg) v = v ? kQ - v : 0;
+            const int32_t d0 = tfhe_digit<18, 20, 2>(v, 0), d1 = tfhe_digit<18, 20, 2>(v, 1);
+            s[0][lds_pad(x)] = digit_residue(d0, q1);
+            s[1][lds_pad(x)] = digit_residue(d1, q1);
+            s[2][lds_pad(x)] = digit_residue(d0, q2);
+            s[3][lds_pad(x)] = digit_residue(d1, q2);
+        }
+        __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
+        dma_row(s + 4, rows + (size_t)(c + 1 < n2 ? c + 1 : 0) * 4 * kN, tid);
+        __builtin_amdgcn_sched_barrier(0);
+        ntt_fwd_r16x2_same(s[2 * half], s[2 * half + 1], T.p[half], ht);
+        __builtin_amdgcn_sched_barrier(0);
+        const uint32_t* key = gkeys + (size_t)(c - 1) * 2 * 2 * 2 * kN;   // [j][ab][p][N]
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int x = tid + e * kFusedThreads;
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const PrimeTab& P = T.p[p];
+                const uint64_t d0 = s[2 * p][lds_pad(x)], d1 = s[2 * p + 1][lds_pad(x)];
+                const uint64_t ta = d0 * key[((0 * 2 + 0) * 2 + p) * kN + x] + d1 * key[((1 * 2 + 0) * 2 + p) * kN + x];
+                const uint64_t tb = d0 * key[((0 * 2 + 1) * 2 + p) * kN + x] + d1 * key[((1 * 2 + 1) * 2 + p) * kN + x];
+                acc[0][p][e] = sub_mod(acc[0][p][e], mont_reduce(ta, P.q, P.mont_qinv), P.q);
+                acc[1][p][e] = sub_mod(acc[1][p][e], mont_reduce(tb, P.q, P.mont_qinv), P.q);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    // row 0 is in planes 4..7; each thread writes planes 0..3 only at its own
+    // slots, which no other thread reads until the barrier below
+    wait_vmem();
+    __syncthreads();
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int x = tid + e * kFusedThreads;
+#pragma unroll
+        for (int ab = 0; ab < 2; ++ab)
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const uint32_t v = add_mod(acc[ab][p][e], s[4 + ab * 2 + p][lds_pad(x)], T.p[p].q);
+                s[ab * 2 + p][lds_pad(x)] = v;
+                if (fd) fd[(((size_t)blk * 2 + ab) * 2 + p) * kN + x] = v;
+            }
+    }
+    __syncthreads();
+    __builtin_amdgcn_sched_barrier(0);
+    ntt_inv_r16x2(s[2 * half], T.p[0], s[2 * half + 1], T.p[1], ht);    // half = ab
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int x = tid + e * kFusedThreads;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) ecoef[((size_t)blk * 4 + t) * kN + x] = s[t][lds_pad(x)];
+    }
+}
+
 // k_giant_fused in four LDS planes and at most 128 VGPRs, so that two
 // workgroups share a CU: the b-row of a giant step is gathered from global
 // memory (L2) instead of staged in LDS, and each half runs its transforms one

@@ -143,16 +143,23 @@ extern "C" int kspir_autotune(kspir_context* c, const kspir_database* db, const 
     js += "}, \"fused_min_batch\": " + std::to_string(fused_from);
     if ((rc = kspir_context_set(c, "fused_min_batch", fused_from))) return rc;
     if (max_batch >= fused_from) {
-        // the fused giant-step kernel at one or two workgroups per CU
-        double g1 = 0, g2 = 0;
+        // the fused giant-step kernel: one workgroup per CU with rows loaded
+        // through registers or prefetched by LDS-DMA, or two per CU
+        double g1 = 0, gd = 0, g2 = 0;
         if ((rc = kspir_context_set(c, "giant_wgs_per_cu", 1))) return rc;
+        if ((rc = kspir_context_set(c, "giant_dma", 0))) return rc;
         if ((rc = step_ms(max_batch, &g1))) return rc;
+        if ((rc = kspir_context_set(c, "giant_dma", 1))) return rc;
+        if ((rc = step_ms(max_batch, &gd))) return rc;
+        if ((rc = kspir_context_set(c, "giant_dma", 0))) return rc;
         if ((rc = kspir_context_set(c, "giant_wgs_per_cu", 2))) return rc;
         if ((rc = step_ms(max_batch, &g2))) return rc;
-        const int gw = g2 < g1 ? 2 : 1;
-        js += ", \"giant_wgs_ms\": [" + fmt("%.4f", g1) + ", " + fmt("%.4f", g2) + "], \"giant_wgs_per_cu\": " +
-              std::to_string(gw);
+        const int gw = g2 < std::min(g1, gd) ? 2 : 1, dma = gw == 1 && gd < g1 ? 1 : 0;
+        js += ", \"giant_variant_ms\": {\"1\": " + fmt("%.4f", g1) + ", \"1_dma\": " + fmt("%.4f", gd) + ", \"2\": " +
+              fmt("%.4f", g2) + "}, \"giant_wgs_per_cu\": " + std::to_string(gw) + ", \"giant_dma\": " +
+              std::to_string(dma);
         if ((rc = kspir_context_set(c, "giant_wgs_per_cu", gw))) return rc;
+        if ((rc = kspir_context_set(c, "giant_dma", dma))) return rc;
         double t1 = 0, t0 = 0;
         if ((rc = kspir_context_set(c, "fused_pack", 1))) return rc;
         if ((rc = step_ms(max_batch, &t1))) return rc;

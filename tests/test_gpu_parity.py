@@ -233,12 +233,16 @@ def test_fused_giant_steps(ctx, oracle, r, n1, nq):
     ctx.set("fused_min_batch", 1)
     try:
         fused = ctx.answer_batch(db, keys, queries)
+        ctx.set("giant_dma", 0)            # rows loaded through registers
+        fused_regs = ctx.answer_batch(db, keys, queries)
         ctx.set("giant_wgs_per_cu", 2)     # the four-plane, two-per-CU variant
         fused2 = ctx.answer_batch(db, keys, queries)
     finally:
         ctx.set("fused_min_batch", 16)
         ctx.set("giant_wgs_per_cu", 1)
+        ctx.set("giant_dma", 1)
     assert np.array_equal(fused, plain)
+    assert np.array_equal(fused_regs, plain)
     assert np.array_equal(fused2, plain)
     want0 = oracle.answer(r=r, n1=n1, db=db_words, query=clients[0]["query"], rgsw=clients[0]["rgsw"],
                           baby=clients[0]["baby"], giant=clients[0]["giant"], packing=clients[0]["packing"],
@@ -451,7 +455,7 @@ def test_autotune_sets_options_and_answers_stay_exact(ctx, oracle):
         assert np.array_equal(got[1], want)
     finally:
         for k, v in (("mfma_min_batch", 2), ("mfma_max_chunk", 1 << 30), ("fused_min_batch", 16),
-                     ("fused_pack", 1), ("db_waves", 4), ("giant_wgs_per_cu", 1),
+                     ("fused_pack", 1), ("db_waves", 4), ("giant_wgs_per_cu", 1), ("giant_dma", 1),
                      ("pack_fused_min_wgs", 0)):
             ctx.set(k, v)
         for k in keys:
