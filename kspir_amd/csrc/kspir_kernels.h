@@ -1067,20 +1067,21 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused(const uint32_t* _
     }
 }
 
-// One giant-step row (4 planes [ab][p][N] of 4096 words) copied from global
-// memory straight into LDS planes dst[0..3] (padded layout) by the wave-wide
+// Planes of a giant-step row ([ab][p][N], 4096 words each) copied from
+// global memory straight into padded LDS planes dst[0..np-1] by the wave-wide
 // LDS-DMA load: lane l of an instruction moves word x0 + l to LDS address
 // M0 + 4 l, and 64 consecutive words starting at a multiple of 64 are
-// consecutive in the padded layout.  256 instructions per row, dealt to the 8
-// waves; the copy holds no VGPRs and completes under vmcnt.
-__device__ __forceinline__ void dma_row(uint32_t (*dst)[kPadN], const uint32_t* __restrict__ row, int tid) {
+// consecutive in the padded layout.  64 instructions per plane, dealt to the
+// 8 waves; the copy holds no VGPRs and completes under vmcnt.
+template <int NP>
+__device__ __forceinline__ void dma_planes(uint32_t (*dst)[kPadN], const uint32_t* __restrict__ src, int tid) {
     typedef __attribute__((address_space(3))) void lds_void;
     typedef __attribute__((address_space(1))) void glb_void;
     const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-#pragma unroll 8
-    for (int b = wv; b < 4 * kN / 64; b += kFusedThreads / 64) {
+#pragma unroll
+    for (int b = wv; b < NP * kN / 64; b += kFusedThreads / 64) {
         const int j = b >> 6, x0 = (b & 63) * 64;
-        __builtin_amdgcn_global_load_lds((glb_void*)(row + j * kN + x0 + lane), (lds_void*)(&dst[j][lds_pad(x0)]), 4, 0,
+        __builtin_amdgcn_global_load_lds((glb_void*)(src + j * kN + x0 + lane), (lds_void*)(&dst[j][lds_pad(x0)]), 4, 0,
                                          0);
     }
 }
@@ -1090,12 +1091,16 @@ __device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70)
 
 // k_giant_fused with the rows prefetched: while row c is decomposed,
 // transformed and multiplied, row c+1 (row 0 after the last) is already on
-// its way into LDS planes 4..7 by LDS-DMA.  The b-row of step c is added to
-// the accumulator (by the slot permutation of tau_g) as soon as it is in LDS,
-// and the a-row is inverse-transformed in place in planes 4, 5, so the four
-// digit planes 0..3 are written in one step.  Planes: 0..3 digits [p][j]
-// (0 = d0 mod q1, 1 = d1 mod q1, 2 = d0 mod q2, 3 = d1 mod q2), 4..7 the row
-// [ab][p].  Same words as k_giant_fused: every update is exact mod q.
+// its way into LDS planes 4..7 by LDS-DMA: its b-planes as soon as every wave
+// has gathered row c's (after the first pass of the inverse transform), its
+// a-planes once row c's digits are out.  The b-row of step c is added to the
+// accumulator (by the slot permutation of tau_g) as soon as it is in LDS, and
+// the a-row is inverse-transformed in place in planes 4, 5, so the four digit
+// planes 0..3 are written in one step.  Planes: 0..3 digits [p][j] (0 = d0
+// mod q1, 1 = d1 mod q1, 2 = d0 mod q2, 3 = d1 mod q2), 4..7 the row [ab][p].
+// Each thread owns the accumulator slots 4 (tid + 512 h) + i (h = 0, 1,
+// i = 0..3), so keys, row 0 and fd move as 16-byte vectors.  Same words as
+// k_giant_fused: every update is exact mod q.
 __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_t* __restrict__ dbout,
         const KeyPtrs* __restrict__ kp, uint32_t* __restrict__ fd, uint32_t* __restrict__ ecoef, int r, int nq,
         int n1, int n2, QS S, Tabs T) {
@@ -1112,11 +1117,12 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_
     constexpr int PER = kN / kFusedThreads;
     const uint32_t q1 = T.p[0].q, q2 = T.p[1].q;
     const uint32_t* rows = dbout + (size_t)blk * n2 * 4 * kN;
-    uint32_t acc[2][2][PER];    // [ab][p][e], slot x = tid + 512 e
+    uint32_t acc[2][2][PER];    // [ab][p][4 h + i], slot 4 (tid + 512 h) + i
 #pragma unroll
     for (int e = 0; e < PER; ++e) acc[0][0][e] = acc[0][1][e] = acc[1][0][e] = acc[1][1][e] = 0;
-    dma_row(s + 4, rows + (size_t)(n2 > 1 ? 1 : 0) * 4 * kN, tid);
+    dma_planes<4>(s + 4, rows + (size_t)(n2 > 1 ? 1 : 0) * 4 * kN, tid);
     for (int c = 1; c < n2; ++c) {
+        const uint32_t* next = rows + (size_t)(c + 1 < n2 ? c + 1 : 0) * 4 * kN;
         wait_vmem();
         __syncthreads();
         __builtin_amdgcn_sched_barrier(0);
@@ -1125,12 +1131,20 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_
         const uint32_t ginv = pow5_mod2n((kN / 2 - ex) % (kN / 2));
 #pragma unroll
         for (int e = 0; e < PER; ++e) {
-            const int px = perm_slot(tid + e * kFusedThreads, g, T);
+            const int px = perm_slot(4 * (tid + kFusedThreads * (e >> 2)) + (e & 3), g, T);
             acc[1][0][e] = add_mod(acc[1][0][e], s[6][lds_pad(px)], q1);
             acc[1][1][e] = add_mod(acc[1][1][e], s[7][lds_pad(px)], q2);
         }
         __builtin_amdgcn_sched_barrier(0);
-        ntt_inv_r16(s[4 + half], T.p[half], ht);
+        ntt_pass_lds<2, true>(s[4 + half], T.p[half], ht);
+        __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
+        dma_planes<2>(s + 6, next + 2 * kN, tid);     // every wave is past its b-gather
+        __builtin_amdgcn_sched_barrier(0);
+        ntt_pass_lds<1, true>(s[4 + half], T.p[half], ht);
+        __syncthreads();
+        ntt_pass_lds<0, true>(s[4 + half], T.p[half], ht);
+        __syncthreads();
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int e = 0; e < PER; ++e) {
@@ -1148,22 +1162,31 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_
         }
         __syncthreads();
         __builtin_amdgcn_sched_barrier(0);
-        dma_row(s + 4, rows + (size_t)(c + 1 < n2 ? c + 1 : 0) * 4 * kN, tid);
+        dma_planes<2>(s + 4, next, tid);
         __builtin_amdgcn_sched_barrier(0);
         ntt_fwd_r16x2_same(s[2 * half], s[2 * half + 1], T.p[half], ht);
         __builtin_amdgcn_sched_barrier(0);
         const uint32_t* key = gkeys + (size_t)(c - 1) * 2 * 2 * 2 * kN;   // [j][ab][p][N]
 #pragma unroll
-        for (int e = 0; e < PER; ++e) {
-            const int x = tid + e * kFusedThreads;
+        for (int h = 0; h < 2; ++h) {
+            const int x0 = 4 * (tid + kFusedThreads * h);
 #pragma unroll
             for (int p = 0; p < 2; ++p) {
                 const PrimeTab& P = T.p[p];
-                const uint64_t d0 = s[2 * p][lds_pad(x)], d1 = s[2 * p + 1][lds_pad(x)];
-                const uint64_t ta = d0 * key[((0 * 2 + 0) * 2 + p) * kN + x] + d1 * key[((1 * 2 + 0) * 2 + p) * kN + x];
-                const uint64_t tb = d0 * key[((0 * 2 + 1) * 2 + p) * kN + x] + d1 * key[((1 * 2 + 1) * 2 + p) * kN + x];
-                acc[0][p][e] = sub_mod(acc[0][p][e], mont_reduce(ta, P.q, P.mont_qinv), P.q);
-                acc[1][p][e] = sub_mod(acc[1][p][e], mont_reduce(tb, P.q, P.mont_qinv), P.q);
+                const uint4 ka0 = *reinterpret_cast<const uint4*>(key + ((0 * 2 + 0) * 2 + p) * kN + x0);
+                const uint4 ka1 = *reinterpret_cast<const uint4*>(key + ((1 * 2 + 0) * 2 + p) * kN + x0);
+                const uint4 kb0 = *reinterpret_cast<const uint4*>(key + ((0 * 2 + 1) * 2 + p) * kN + x0);
+                const uint4 kb1 = *reinterpret_cast<const uint4*>(key + ((1 * 2 + 1) * 2 + p) * kN + x0);
+                const uint32_t ka0v[4] = {ka0.x, ka0.y, ka0.z, ka0.w}, ka1v[4] = {ka1.x, ka1.y, ka1.z, ka1.w};
+                const uint32_t kb0v[4] = {kb0.x, kb0.y, kb0.z, kb0.w}, kb1v[4] = {kb1.x, kb1.y, kb1.z, kb1.w};
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const uint64_t d0 = s[2 * p][lds_pad(x0) + i], d1 = s[2 * p + 1][lds_pad(x0) + i];
+                    const uint64_t ta = d0 * ka0v[i] + d1 * ka1v[i];
+                    const uint64_t tb = d0 * kb0v[i] + d1 * kb1v[i];
+                    acc[0][p][4 * h + i] = sub_mod(acc[0][p][4 * h + i], mont_reduce(ta, P.q, P.mont_qinv), P.q);
+                    acc[1][p][4 * h + i] = sub_mod(acc[1][p][4 * h + i], mont_reduce(tb, P.q, P.mont_qinv), P.q);
+                }
             }
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -1174,25 +1197,34 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_
     __syncthreads();
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int e = 0; e < PER; ++e) {
-        const int x = tid + e * kFusedThreads;
+    for (int h = 0; h < 2; ++h) {
+        const int x0 = 4 * (tid + kFusedThreads * h);
 #pragma unroll
         for (int ab = 0; ab < 2; ++ab)
 #pragma unroll
             for (int p = 0; p < 2; ++p) {
-                const uint32_t v = add_mod(acc[ab][p][e], s[4 + ab * 2 + p][lds_pad(x)], T.p[p].q);
-                s[ab * 2 + p][lds_pad(x)] = v;
-                if (fd) fd[(((size_t)blk * 2 + ab) * 2 + p) * kN + x] = v;
+                uint32_t v[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    v[i] = add_mod(acc[ab][p][4 * h + i], s[4 + ab * 2 + p][lds_pad(x0) + i], T.p[p].q);
+                    s[ab * 2 + p][lds_pad(x0) + i] = v[i];
+                }
+                if (fd)
+                    *reinterpret_cast<uint4*>(fd + (((size_t)blk * 2 + ab) * 2 + p) * kN + x0) =
+                        make_uint4(v[0], v[1], v[2], v[3]);
             }
     }
     __syncthreads();
     __builtin_amdgcn_sched_barrier(0);
     ntt_inv_r16x2(s[2 * half], T.p[0], s[2 * half + 1], T.p[1], ht);    // half = ab
 #pragma unroll
-    for (int e = 0; e < PER; ++e) {
-        const int x = tid + e * kFusedThreads;
+    for (int h = 0; h < 2; ++h) {
+        const int x0 = 4 * (tid + kFusedThreads * h);
 #pragma unroll
-        for (int t = 0; t < 4; ++t) ecoef[((size_t)blk * 4 + t) * kN + x] = s[t][lds_pad(x)];
+        for (int t = 0; t < 4; ++t) {
+            const uint32_t* sp = &s[t][lds_pad(x0)];
+            *reinterpret_cast<uint4*>(ecoef + ((size_t)blk * 4 + t) * kN + x0) = make_uint4(sp[0], sp[1], sp[2], sp[3]);
+        }
     }
 }
 
