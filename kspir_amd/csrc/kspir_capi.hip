@@ -649,6 +649,9 @@ extern "C" {
 
 const char* kspir_last_error(void) { return g_err.c_str(); }
 
+// error text for the library's other translation units (not exported)
+__attribute__((visibility("hidden"))) int kspir_set_error(int code, const char* msg) { return fail(code, msg); }
+
 int kspir_compute_permutation_matrix(int32_t* out, int32_t n1) {
     if (!out || !pow2(n1) || n1 > kN / 2) return fail(KSPIR_EINVAL, "n1 must be a power of two <= N/2");
     // slot s holds the evaluation at psi^e with e = 2*bitrev(s)+1; tau_g maps

@@ -331,6 +331,26 @@ __device__ __forceinline__ void baby_key_mult(uint32_t* a, const u32x4 (&kv)[3][
         }
 }
 
+// output of a modulus switch: s[0], s[1] hold NTT_q1(c), NTT_q2(c);
+// out_p = (acc_p - NTT_p(c)) * bsMod^-1 -> ctw[k][ab][p], acc_p from q12(e, p)
+template <class Q12>
+__device__ __forceinline__ void baby_switch_finish(Q12 q12, uint32_t (*s)[kPadN], uint32_t* ctw, int k, int ab,
+                                                   int tid, const Tabs& T) {
+    const int x0 = 4 * tid;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        uint32_t o1[4], o2[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int e = 4 * g + i, x = g * 1024 + x0 + i;
+            o1[i] = mul_shoup(sub_mod(q12(e, 0), s[0][lds_pad(x)], kQ1), T.bsinv_q[0], T.bsinv_q_sh[0], kQ1);
+            o2[i] = mul_shoup(sub_mod(q12(e, 1), s[1][lds_pad(x)], kQ2), T.bsinv_q[1], T.bsinv_q_sh[1], kQ2);
+        }
+        *reinterpret_cast<uint4*>(ctw + ((size_t)(k * 2 + ab) * 2 + 0) * kN + g * 1024 + x0) = make_uint4(o1[0], o1[1], o1[2], o1[3]);
+        *reinterpret_cast<uint4*>(ctw + ((size_t)(k * 2 + ab) * 2 + 1) * kN + g * 1024 + x0) = make_uint4(o2[0], o2[1], o2[2], o2[3]);
+    }
+}
+
 // modulus switch of one rotation and its store: c = INTT_bs(acc_bs),
 // out_p = (acc_p - NTT_p(c)) * bsMod^-1 -> ctw[k][ab][p]; acc_bs in
 // registers, acc_p (p = q1, q2) from q12(e, p); uses LDS planes 0 and 1
@@ -349,18 +369,48 @@ __device__ __forceinline__ void baby_switch_core(const uint32_t (&acc_bs)[kN / k
     for (int e = 0; e < PER; ++e) s[1][lds_pad(tid + e * kNttThreads)] = s[0][lds_pad(tid + e * kNttThreads)];
     __syncthreads();
     ntt_fwd_r16x2(s[0], T.p[0], s[1], T.p[1]);
+    baby_switch_finish(q12, s, ctw, k, ab, tid, T);
+    __syncthreads();
+}
+
+// two modulus switches whose bsMod inverse transforms run interleaved:
+// rotation kA from accA (registers, all three residues), rotation kB from
+// accB_bs (registers) with its q1, q2 residues from qB(e, p); uses LDS planes
+// 0 and 1.  Same words as two baby_switch_core calls.
+template <class QB>
+__device__ __forceinline__ void baby_switch_pair(const uint32_t (&accA)[3][kN / kNttThreads],
+                                                 const uint32_t (&accB_bs)[kN / kNttThreads], QB qB,
+                                                 uint32_t (*s)[kPadN], uint32_t* ctw, int kA, int kB, int ab,
+                                                 int tid, const Tabs& T) {
+    constexpr int PER = kN / kNttThreads;
+    const int x0 = 4 * tid;
 #pragma unroll
-    for (int g = 0; g < 4; ++g) {
-        uint32_t o1[4], o2[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int e = 4 * g + i, x = g * 1024 + x0 + i;
-            o1[i] = mul_shoup(sub_mod(q12(e, 0), s[0][lds_pad(x)], kQ1), T.bsinv_q[0], T.bsinv_q_sh[0], kQ1);
-            o2[i] = mul_shoup(sub_mod(q12(e, 1), s[1][lds_pad(x)], kQ2), T.bsinv_q[1], T.bsinv_q_sh[1], kQ2);
-        }
-        *reinterpret_cast<uint4*>(ctw + ((size_t)(k * 2 + ab) * 2 + 0) * kN + g * 1024 + x0) = make_uint4(o1[0], o1[1], o1[2], o1[3]);
-        *reinterpret_cast<uint4*>(ctw + ((size_t)(k * 2 + ab) * 2 + 1) * kN + g * 1024 + x0) = make_uint4(o2[0], o2[1], o2[2], o2[3]);
+    for (int e = 0; e < PER; ++e) {
+        s[0][lds_pad((e >> 2) * 1024 + x0 + (e & 3))] = accA[2][e];
+        s[1][lds_pad((e >> 2) * 1024 + x0 + (e & 3))] = accB_bs[e];
     }
+    __syncthreads();
+    ntt_inv_r16x2(s[0], T.p[2], s[1], T.p[2], tid);
+    uint32_t cB[PER];   // c of rotation kB, held while rotation kA's transforms use both planes
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int x = lds_pad(tid + e * kNttThreads);
+        cB[e] = s[1][x];
+        s[1][x] = s[0][x];
+    }
+    __syncthreads();
+    ntt_fwd_r16x2(s[0], T.p[0], s[1], T.p[1]);
+    baby_switch_finish([&](int e, int p) { return accA[p][e]; }, s, ctw, kA, ab, tid, T);
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int x = lds_pad(tid + e * kNttThreads);
+        s[0][x] = cB[e];
+        s[1][x] = cB[e];
+    }
+    __syncthreads();
+    ntt_fwd_r16x2(s[0], T.p[0], s[1], T.p[1]);
+    baby_switch_finish(qB, s, ctw, kB, ab, tid, T);
     __syncthreads();
 }
 
@@ -523,10 +573,12 @@ k_baby_rot_pair(const uint32_t* __restrict__ D0, const uint32_t* __restrict__ bh
         __syncthreads();
         __builtin_amdgcn_sched_barrier(0);
     }
-    if (first) baby_switch_store(accA, s, ctw, 1 + rot, ab, tid, T);
-    __builtin_amdgcn_sched_barrier(0);
-    baby_switch_core(accBbs, [&](int e, int p) { return s[2 + p][lds_pad(perm_of(e))]; }, s, ctw, n1h + rot, ab,
-                     tid, T);
+    if (first)
+        baby_switch_pair(accA, accBbs, [&](int e, int p) { return s[2 + p][lds_pad(perm_of(e))]; }, s, ctw, 1 + rot,
+                         n1h + rot, ab, tid, T);
+    else
+        baby_switch_core(accBbs, [&](int e, int p) { return s[2 + p][lds_pad(perm_of(e))]; }, s, ctw, n1h + rot, ab,
+                         tid, T);
 }
 
 // second half, step 1: INTT of the intermediate's a in prime p (blockIdx.x)

@@ -13,6 +13,9 @@
 #include "kspir_amd.h"
 #include "kspir_amd_runtime.h"
 
+// sets the text kspir_last_error reports (defined beside it, not exported)
+extern "C" __attribute__((visibility("hidden"))) int kspir_set_error(int code, const char* msg);
+
 namespace {
 
 struct Buf {
@@ -37,11 +40,14 @@ std::string fmt(const char* f, double v) {
 extern "C" int kspir_autotune(kspir_context* c, const kspir_database* db, const kspir_keys* const* keys,
                               int32_t nkeys, int32_t max_batch, double latency_budget_ms, int32_t* serve_batch,
                               char* report, size_t report_len) {
-    if (!c || !db || !keys || nkeys < 1 || max_batch < 1) return KSPIR_EINVAL;
+    if (!c || !db || !keys || nkeys < 1 || max_batch < 1)
+        return kspir_set_error(KSPIR_EINVAL, "kspir_autotune: null handle, nkeys < 1 or max_batch < 1");
+    for (int32_t i = 0; i < nkeys; ++i)
+        if (!keys[i]) return kspir_set_error(KSPIR_EINVAL, "kspir_autotune: null key handle");
     // zero words are valid ciphertext words in every modulus
     const size_t qw = (size_t)4 * KSPIR_N, rw = (size_t)2 * KSPIR_N;
     Buf qd(c, (size_t)max_batch * qw * 8), rd(c, (size_t)max_batch * rw * 8);
-    if (!qd.p || !rd.p) return KSPIR_ENOMEM;
+    if (!qd.p || !rd.p) return kspir_set_error(KSPIR_ENOMEM, "kspir_autotune: no device memory for the timing batch");
     {
         std::vector<uint64_t> zero((size_t)max_batch * qw, 0);
         int rc = kspir_memcpy_h2d(c, qd.p, zero.data(), zero.size() * 8);

@@ -453,6 +453,9 @@ def test_autotune_sets_options_and_answers_stay_exact(ctx, oracle):
         want = oracle.answer(r=r, n1=n1, db=db_words, query=cl[1]["query"], rgsw=cl[1]["rgsw"], baby=cl[1]["baby"],
                              giant=cl[1]["giant"], packing=cl[1]["packing"], perms=perms)
         assert np.array_equal(got[1], want)
+        # argument errors come back with their reason
+        with pytest.raises(RuntimeError, match="max_batch"):   # KspirError
+            ctx.autotune(db, keys, max_batch=0)
     finally:
         for k, v in (("mfma_min_batch", 2), ("mfma_max_chunk", 1 << 30), ("fused_min_batch", 16),
                      ("fused_pack", 1), ("db_waves", 4), ("giant_wgs_per_cu", 1), ("giant_dma", 1),
