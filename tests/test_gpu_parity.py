@@ -322,6 +322,55 @@ def test_serving_loop_mixed_databases(ctx, oracle, lanes):
             k.close()
 
 
+def test_concurrent_contexts_uneven_databases(ctx, oracle):
+    """batches against two databases of uneven size (r 2 and 8), each
+    enqueued on a context of its own before either is waited for, so they
+    overlap on the device, come back equal to the same batches answered one
+    after the other on one context, and equal to the oracle"""
+    import kspir_amd
+    say("[gpu] concurrent contexts, databases r=2 and r=8")
+    shapes = [(8, 128, 81, 17), (2, 64, 82, 3)]   # r, n1, seed, queries (17: the fused giant path)
+    groups = []
+    for r, n1, seed, nq in shapes:
+        dbw = oracle.gen_db(r, n1, seed=seed)
+        perms = oracle.perm_matrix(n1)
+        db = ctx.database(dbw, r, n1)
+        cl = [oracle.gen_client(r, n1, seed * 10 + c, (29 * c) % (N // 2), c % r) for c in range(min(nq, 3))]
+        cl = [cl[c % len(cl)] for c in range(nq)]
+        ks = [ctx.keys(r, n1, x["baby"], x["giant"], x["packing"], x["rgsw"], perms) for x in cl[:3]]
+        ks = [ks[c % len(ks)] for c in range(nq)]
+        q = np.stack([x["query"] for x in cl])
+        qb, rb = ctx.alloc(nq * 4 * N * 8), ctx.alloc(nq * 2 * N * 8)
+        qb.upload(q)
+        groups.append((r, n1, nq, dbw, perms, db, cl, ks, qb, rb))
+    want = []
+    for r, n1, nq, dbw, perms, db, cl, ks, qb, rb in groups:
+        want.append(ctx.answer_batch(db, ks, np.stack([x["query"] for x in cl])))
+    other = kspir_amd.Context(0)
+    try:
+        for _ in range(2):
+            for c, (r, n1, nq, dbw, perms, db, cl, ks, qb, rb) in zip((ctx, other), groups):
+                rb.upload(np.zeros((nq, 2, N), dtype=np.uint64))
+            for c, (r, n1, nq, dbw, perms, db, cl, ks, qb, rb) in zip((ctx, other), groups):
+                c.answer_batch_device(db, ks, qb, rb)
+            ctx.synchronize()
+            other.synchronize()
+            for (r, n1, nq, dbw, perms, db, cl, ks, qb, rb), w in zip(groups, want):
+                assert np.array_equal(rb.download(np.zeros((nq, 2, N), dtype=np.uint64)), w), r
+    finally:
+        other.close()
+    r, n1, nq, dbw, perms, db, cl, ks, qb, rb = groups[1]
+    w0 = oracle.answer(r=r, n1=n1, db=dbw, query=cl[0]["query"], rgsw=cl[0]["rgsw"], baby=cl[0]["baby"],
+                       giant=cl[0]["giant"], packing=cl[0]["packing"], perms=perms)
+    assert np.array_equal(want[1][0], w0)
+    for r, n1, nq, dbw, perms, db, cl, ks, qb, rb in groups:
+        for k in ks[:3]:
+            k.close()
+        db.close()
+        qb.free()
+        rb.free()
+
+
 def test_packing_paths_agree(ctx, oracle):
     """the fused packing level (default) and the three-kernel level give the
     same response words, single and batched, and equal the oracle"""
