@@ -1173,6 +1173,28 @@ int ko_gen_records(int r, int n1, unsigned seed, int plant, int tcol, int tpack,
     return 0;
 }
 
+// inverse_encode (twosteps.cpp:182-198) of one record column of N values, in place
+void ko_inverse_encode(uint64_t* col) {
+    Poly m(col, col + N);
+    inverse_encode(m);
+    std::memcpy(col, m.data(), (size_t)N * 8);
+}
+
+// database_tobsgsntt + database_tocrt (twosteps.cpp:42-179, crt.cpp:859-885)
+// of r blocks of records [N][N/2], block k at records + k * stride (stride 0
+// repeats block 0): words [r][N*N/2], as the preprocessing of test-pir.cpp:46-63
+int ko_records_to_words(int r, int n1, const uint64_t* records, int64_t stride, uint64_t* words) {
+    if (!valid_shape(r, n1) || stride < 0) return -1;
+    std::vector<u64> data((size_t)N * (N / 2)), data_ntt((size_t)(N / 2) * N);
+    const size_t w = (size_t)N * N / 2;
+    for (int k = 0; k < r; ++k) {
+        std::memcpy(data.data(), records + (size_t)stride * k, w * 8);
+        database_tobsgsntt(data_ntt, data, n1);
+        database_tocrt(words + w * k, data_ntt, n1);
+    }
+    return 0;
+}
+
 // database_tobsgsntt alone on one block: data [N][N/2] -> result [N/2][N]
 int ko_database_tobsgsntt(int n1, const uint64_t* data, uint64_t* result) {
     if (n1 < 1 || n1 > N / 2 || (n1 & (n1 - 1))) return -1;

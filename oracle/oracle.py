@@ -170,6 +170,23 @@ def gen_records(r, n1, seed=1, plant=False, tcol=0, tpack=0):
     return rec, db
 
 
+def inverse_encode(col):
+    """inverse_encode of one record column (N values < p)"""
+    out = np.ascontiguousarray(col, dtype=np.uint64).copy()
+    lib().ko_inverse_encode(p64(out))
+    return out
+
+
+def records_to_words(r, n1, records, block_stride=None):
+    """words [r][N*N/2] of records [r or 1][N][N/2] (block_stride 0 repeats block 0)"""
+    rec = np.ascontiguousarray(records, dtype=np.uint64)
+    stride = N * N // 2 if block_stride is None else block_stride
+    out = np.zeros((r, N * N // 2), dtype=np.uint64)
+    rc = lib().ko_records_to_words(r, n1, p64(rec), ctypes.c_int64(stride), p64(out))
+    assert rc == 0
+    return out
+
+
 def database_tobsgsntt(n1, data):
     """database_tobsgsntt of one block: data [N][N/2] -> [N/2][N] mod Q"""
     out = np.zeros((N // 2, N), dtype=np.uint64)

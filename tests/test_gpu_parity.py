@@ -541,6 +541,38 @@ def test_client_generation_matches_oracle(ctx, oracle, r, n1, seeds):
         db.close()
 
 
+@pytest.mark.parametrize("r,nq", [(64, 16), (256, 2)])
+def test_full_size_round_trip(ctx, oracle, r, nq):
+    """1 GB (r 64) and 4 GB (r 256) databases preprocessed on the GPU from one
+    planted record block repeated r times, clients generated on the GPU,
+    answered one by one (VALU database pass) and as one batch (int8-MFMA pass;
+    at r 64 with 16 queries also the fused giant-step and packing kernels),
+    recovered on the GPU: the batch equals the single answers word for word
+    and every client recovers its planted message"""
+    from tests.planted import planted_block, planted_message
+    n1 = 128
+    say(f"[gpu] full-size round trip r={r} ({r * 16} MB), {nq} clients")
+    cols = [(97 * c + 5) % (N // 2) for c in range(nq)]
+    packs = [(13 * c + r - 1) % r for c in range(nq)]
+    shifts = [7 * c for c in range(nq)]
+    rec = planted_block(oracle, r, cols, packs, shifts)
+    db = ctx.database_from_records(rec[None], r, n1, block_stride=0)
+    perms = oracle.perm_matrix(n1)
+    cl = ctx.client_generate(r, n1, [300 + c for c in range(nq)], cols, packs)
+    keys = [ctx.keys(r, n1, cl["baby"][c], cl["giant"][c], cl["packing"][c], cl["rgsw"][c], perms) for c in range(nq)]
+    try:
+        single = np.stack([ctx.answer(db, keys[c], cl["query"][c]) for c in range(nq)])
+        batch = ctx.answer_batch(db, keys, cl["query"])
+        assert np.array_equal(batch, single)
+        msgs = ctx.decrypt_bsgs_total(r, cl["secret"], single)
+        for c in range(nq):
+            assert np.array_equal(msgs[c], planted_message(r, shifts[c])), c
+    finally:
+        for k in keys:
+            k.close()
+        db.close()
+
+
 def test_cpp_client_dropin_matches_oracle(oracle, tmp_path):
     """the client side of test-pir through kspir_pir.h (srand, Secret,
     query_bsgs_rns, RGSW / BSGS / packing keyGen, rand() drawn on the host in

@@ -304,3 +304,30 @@ def test_gen_records_matches_gen_db_and_tocrt(oracle):
     packed = (nt % q1) | ((nt % q2) << np.uint64(32))          # [n2*n1][N]
     want = packed.reshape(n2, n1, N).transpose(2, 0, 1).reshape(-1)
     assert np.array_equal(words[1], want)
+
+
+def test_records_to_words_matches_gen_records(oracle):
+    """records_to_words (the oracle's preprocessing of given records) gives
+    gen_records' words, and block stride 0 repeats block 0's words"""
+    r, n1 = 2, 128
+    rec, words = oracle.gen_records(r, n1, seed=5)
+    assert np.array_equal(oracle.records_to_words(r, n1, rec), words)
+    rep = oracle.records_to_words(r, n1, rec[:1], block_stride=0)
+    assert np.array_equal(rep[0], words[0]) and np.array_equal(rep[1], words[0])
+
+
+@pytest.mark.parametrize("r,n1", [(2, 128), (4, 64)])
+def test_planted_block_round_trip(oracle, r, n1):
+    """one planted block repeated r times (the shape the full-size GPU
+    round trips use): each client recovers m - (m mod r) + 1 + shift; with
+    one block and shift 0 at r = 1 this is test-pir's 1..N"""
+    from tests.planted import planted_block, planted_message
+    cols, packs, shifts = [5, 1000], [r - 1, 0], [0, 9]
+    rec = planted_block(oracle, r, cols, packs, shifts)
+    words = oracle.records_to_words(r, n1, rec[None], block_stride=0)
+    perms = oracle.perm_matrix(n1)
+    for c in range(2):
+        cl = oracle.gen_client(r, n1, 40 + c, cols[c], packs[c])
+        resp = oracle.answer(r=r, n1=n1, db=words, query=cl["query"], rgsw=cl["rgsw"], baby=cl["baby"],
+                             giant=cl["giant"], packing=cl["packing"], perms=perms)
+        assert np.array_equal(oracle.decrypt(r, cl["secret"], resp), planted_message(r, shifts[c])), c
