@@ -541,9 +541,9 @@ def test_client_generation_matches_oracle(ctx, oracle, r, n1, seeds):
         db.close()
 
 
-@pytest.mark.parametrize("r,nq", [(64, 16), (256, 2)])
+@pytest.mark.parametrize("r,nq", [(64, 16), (256, 2), (1024, 1)])
 def test_full_size_round_trip(ctx, oracle, r, nq):
-    """1 GB (r 64) and 4 GB (r 256) databases preprocessed on the GPU from one
+    """1 GB (r 64), 4 GB (r 256) and 16 GB (r 1024) databases preprocessed on the GPU from one
     planted record block repeated r times, clients generated on the GPU,
     answered one by one (VALU database pass) and as one batch (int8-MFMA pass;
     at r 64 with 16 queries also the fused giant-step and packing kernels),
@@ -562,8 +562,9 @@ def test_full_size_round_trip(ctx, oracle, r, nq):
     keys = [ctx.keys(r, n1, cl["baby"][c], cl["giant"][c], cl["packing"][c], cl["rgsw"][c], perms) for c in range(nq)]
     try:
         single = np.stack([ctx.answer(db, keys[c], cl["query"][c]) for c in range(nq)])
-        batch = ctx.answer_batch(db, keys, cl["query"])
-        assert np.array_equal(batch, single)
+        if nq > 1:
+            batch = ctx.answer_batch(db, keys, cl["query"])
+            assert np.array_equal(batch, single)
         msgs = ctx.decrypt_bsgs_total(r, cl["secret"], single)
         for c in range(nq):
             assert np.array_equal(msgs[c], planted_message(r, shifts[c])), c
