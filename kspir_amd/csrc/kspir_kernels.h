@@ -1172,6 +1172,29 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_
     uint32_t acc[2][2][PER];    // [ab][p][4 h + i], slot 4 (tid + 512 h) + i
 #pragma unroll
     for (int e = 0; e < PER; ++e) acc[0][0][e] = acc[0][1][e] = acc[1][0][e] = acc[1][1][e] = 0;
+    // key switch of step cs (>= 1) for slot run h, from its NTT'd digits in planes 0..3
+    auto mac = [&](int cs, int h) {
+        const uint32_t* key = gkeys + (size_t)(cs - 1) * 2 * 2 * 2 * kN;   // [j][ab][p][N]
+        const int x0 = 4 * (tid + kFusedThreads * h);
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const PrimeTab& P = T.p[p];
+            const uint4 ka0 = *reinterpret_cast<const uint4*>(key + ((0 * 2 + 0) * 2 + p) * kN + x0);
+            const uint4 ka1 = *reinterpret_cast<const uint4*>(key + ((1 * 2 + 0) * 2 + p) * kN + x0);
+            const uint4 kb0 = *reinterpret_cast<const uint4*>(key + ((0 * 2 + 1) * 2 + p) * kN + x0);
+            const uint4 kb1 = *reinterpret_cast<const uint4*>(key + ((1 * 2 + 1) * 2 + p) * kN + x0);
+            const uint32_t ka0v[4] = {ka0.x, ka0.y, ka0.z, ka0.w}, ka1v[4] = {ka1.x, ka1.y, ka1.z, ka1.w};
+            const uint32_t kb0v[4] = {kb0.x, kb0.y, kb0.z, kb0.w}, kb1v[4] = {kb1.x, kb1.y, kb1.z, kb1.w};
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const uint64_t d0 = s[2 * p][lds_pad(x0) + i], d1 = s[2 * p + 1][lds_pad(x0) + i];
+                const uint64_t ta = d0 * ka0v[i] + d1 * ka1v[i];
+                const uint64_t tb = d0 * kb0v[i] + d1 * kb1v[i];
+                acc[0][p][4 * h + i] = sub_mod(acc[0][p][4 * h + i], mont_reduce(ta, P.q, P.mont_qinv), P.q);
+                acc[1][p][4 * h + i] = sub_mod(acc[1][p][4 * h + i], mont_reduce(tb, P.q, P.mont_qinv), P.q);
+            }
+        }
+    };
     dma_planes<4>(s + 4, rows + (size_t)(n2 > 1 ? 1 : 0) * 4 * kN, tid);
     for (int c = 1; c < n2; ++c) {
         const uint32_t* next = rows + (size_t)(c + 1 < n2 ? c + 1 : 0) * 4 * kN;
@@ -1188,12 +1211,19 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_
             acc[1][1][e] = add_mod(acc[1][1][e], s[7][lds_pad(px)], q2);
         }
         __builtin_amdgcn_sched_barrier(0);
+        // the key switch of step c-1 (digits still in planes 0..3) rides along
+        // the inverse transform of row c (planes 4, 5): its key loads wait out
+        // while the transform computes.  Both halves end before the transform's
+        // second barrier, so every thread is done with planes 0..3 before the
+        // digits of row c overwrite them.
         ntt_pass_lds<2, true>(s[4 + half], T.p[half], ht);
+        if (c >= 2) mac(c - 1, 0);
         __syncthreads();
         __builtin_amdgcn_sched_barrier(0);
         dma_planes<2>(s + 6, next + 2 * kN, tid);     // every wave is past its b-gather
         __builtin_amdgcn_sched_barrier(0);
         ntt_pass_lds<1, true>(s[4 + half], T.p[half], ht);
+        if (c >= 2) mac(c - 1, 1);
         __syncthreads();
         ntt_pass_lds<0, true>(s[4 + half], T.p[half], ht);
         __syncthreads();
@@ -1218,30 +1248,10 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_
         __builtin_amdgcn_sched_barrier(0);
         ntt_fwd_r16x2_same(s[2 * half], s[2 * half + 1], T.p[half], ht);
         __builtin_amdgcn_sched_barrier(0);
-        const uint32_t* key = gkeys + (size_t)(c - 1) * 2 * 2 * 2 * kN;   // [j][ab][p][N]
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int x0 = 4 * (tid + kFusedThreads * h);
-#pragma unroll
-            for (int p = 0; p < 2; ++p) {
-                const PrimeTab& P = T.p[p];
-                const uint4 ka0 = *reinterpret_cast<const uint4*>(key + ((0 * 2 + 0) * 2 + p) * kN + x0);
-                const uint4 ka1 = *reinterpret_cast<const uint4*>(key + ((1 * 2 + 0) * 2 + p) * kN + x0);
-                const uint4 kb0 = *reinterpret_cast<const uint4*>(key + ((0 * 2 + 1) * 2 + p) * kN + x0);
-                const uint4 kb1 = *reinterpret_cast<const uint4*>(key + ((1 * 2 + 1) * 2 + p) * kN + x0);
-                const uint32_t ka0v[4] = {ka0.x, ka0.y, ka0.z, ka0.w}, ka1v[4] = {ka1.x, ka1.y, ka1.z, ka1.w};
-                const uint32_t kb0v[4] = {kb0.x, kb0.y, kb0.z, kb0.w}, kb1v[4] = {kb1.x, kb1.y, kb1.z, kb1.w};
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const uint64_t d0 = s[2 * p][lds_pad(x0) + i], d1 = s[2 * p + 1][lds_pad(x0) + i];
-                    const uint64_t ta = d0 * ka0v[i] + d1 * ka1v[i];
-                    const uint64_t tb = d0 * kb0v[i] + d1 * kb1v[i];
-                    acc[0][p][4 * h + i] = sub_mod(acc[0][p][4 * h + i], mont_reduce(ta, P.q, P.mont_qinv), P.q);
-                    acc[1][p][4 * h + i] = sub_mod(acc[1][p][4 * h + i], mont_reduce(tb, P.q, P.mont_qinv), P.q);
-                }
-            }
-        }
-        __builtin_amdgcn_sched_barrier(0);
+    }
+    if (n2 > 1) {   // the last step's key switch
+        mac(n2 - 1, 0);
+        mac(n2 - 1, 1);
     }
     // row 0 is in planes 4..7; each thread writes planes 0..3 only at its own
     // slots, which no other thread reads until the barrier below
