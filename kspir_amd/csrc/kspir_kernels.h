@@ -1512,7 +1512,7 @@ __global__ void __launch_bounds__(kNttThreads) k_pack_digit(const uint32_t* __re
 __global__ void __launch_bounds__(512) k_pack_level(const uint32_t* __restrict__ ext,
         const KeyPtrs* __restrict__ kp, int li, uint32_t* __restrict__ next, int i, uint32_t g, uint32_t ginv, QS S,
         Tabs T) {
-    __shared__ uint32_t s[6][kPadN];
+    __shared__ uint32_t s[8][kPadN];
     constexpr int PER = kN / 512;
     const int j = blockIdx.x % i;
     const size_t qi = blockIdx.x / i;
@@ -1536,8 +1536,10 @@ __global__ void __launch_bounds__(512) k_pack_level(const uint32_t* __restrict__
     __builtin_amdgcn_sched_barrier(0);
     ntt_inv_r16(s[half], T.p[half], ht);
     __builtin_amdgcn_sched_barrier(0);
-    // digits of tau_g: 0 and 1 into planes 2..5 now, 2 and 3 held for round two
-    uint32_t dg2[4][PER];   // (digit 2 | 3) x (q1 | q2) at [p * 2 + (k - 2)]
+    // digits of tau_g: digit k of prime p at plane D(k, p): digits 0, 1 in
+    // planes 2, 3 (q1) and 4, 5 (q2), digits 2, 3 in planes 6, 7 (q1) and 0, 1
+    // (q2); the latter two only once every gather from planes 0, 1 is done
+    uint32_t dq2[2][PER];   // digits 2, 3 mod q2, held for planes 0, 1
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
         const int x = tid + e * 512;
@@ -1553,47 +1555,61 @@ __global__ void __launch_bounds__(512) k_pack_level(const uint32_t* __restrict__
         for (int p = 0; p < 2; ++p) {
             s[2 + 2 * p][lds_pad(x)] = digit_residue(d[0], T.p[p].q);
             s[3 + 2 * p][lds_pad(x)] = digit_residue(d[1], T.p[p].q);
-            dg2[p * 2 + 0][e] = digit_residue(d[2], T.p[p].q);
-            dg2[p * 2 + 1][e] = digit_residue(d[3], T.p[p].q);
         }
+        s[6][lds_pad(x)] = digit_residue(d[2], T.p[0].q);
+        s[7][lds_pad(x)] = digit_residue(d[3], T.p[0].q);
+        dq2[0][e] = digit_residue(d[2], T.p[1].q);
+        dq2[1][e] = digit_residue(d[3], T.p[1].q);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int x = tid + e * 512;
+        s[0][lds_pad(x)] = dq2[0][e];
+        s[1][lds_pad(x)] = dq2[1][e];
     }
     __syncthreads();
     __builtin_amdgcn_sched_barrier(0);
     uint32_t acc[2][2][PER];   // [ab][p][e]: sum_k digit_k * key_k
+    // key products of digits k0, k0 + 1 (NTT'd, at planes pl[p][0..1]) for the slots e0 .. e0 + 3
+    auto mac = [&](int k0, const int (&pl)[2][2], int e0) {
 #pragma unroll
-    for (int round = 0; round < 2; ++round) {
-        if (round == 1) {
-#pragma unroll
-            for (int e = 0; e < PER; ++e) {
-                const int x = tid + e * 512;
-#pragma unroll
-                for (int t = 0; t < 4; ++t) s[2 + t][lds_pad(x)] = dg2[t][e];
-            }
-            __syncthreads();
-            __builtin_amdgcn_sched_barrier(0);
-        }
-        ntt_fwd_r16x2_same(s[2 + 2 * half], s[3 + 2 * half], T.p[half], ht);
-        __builtin_amdgcn_sched_barrier(0);
-        const int k0 = 2 * round;
-#pragma unroll
-        for (int e = 0; e < PER; ++e) {
+        for (int e = e0; e < e0 + PER / 2; ++e) {
             const int x = tid + e * 512;
 #pragma unroll
             for (int p = 0; p < 2; ++p) {
                 const PrimeTab& P = T.p[p];
-                const uint64_t d0 = s[2 + 2 * p][lds_pad(x)], d1 = s[3 + 2 * p][lds_pad(x)];
+                const uint64_t d0 = s[pl[p][0]][lds_pad(x)], d1 = s[pl[p][1]][lds_pad(x)];
 #pragma unroll
                 for (int ab = 0; ab < 2; ++ab) {
                     const uint64_t t = d0 * pkey[((k0 * 2 + ab) * 2 + p) * kN + x] +
                                        d1 * pkey[(((k0 + 1) * 2 + ab) * 2 + p) * kN + x];
                     const uint32_t m = mont_reduce(t, P.q, P.mont_qinv);
-                    acc[ab][p][e] = round ? add_mod(acc[ab][p][e], m, P.q) : m;
+                    acc[ab][p][e] = k0 ? add_mod(acc[ab][p][e], m, P.q) : m;
                 }
             }
         }
-        __syncthreads();
-        __builtin_amdgcn_sched_barrier(0);
-    }
+    };
+    constexpr int pl0[2][2] = {{2, 3}, {4, 5}}, pl1[2][2] = {{6, 7}, {0, 1}};
+    // digits 0, 1: half h transforms prime h's two planes
+    ntt_fwd_r16x2_same(s[2 + 2 * half], s[3 + 2 * half], T.p[half], ht);
+    __builtin_amdgcn_sched_barrier(0);
+    // digits 2, 3 (half 0: planes 6, 7 mod q1; half 1: planes 0, 1 mod q2), with
+    // the key products of digits 0, 1 riding along the first two passes
+    uint32_t* r0 = half ? s[0] : s[6];
+    uint32_t* r1 = half ? s[1] : s[7];
+    ntt_pass_lds2<0, false, true>(r0, T.p[half], r1, T.p[half], ht);
+    mac(0, pl0, 0);
+    __syncthreads();
+    ntt_pass_lds2<1, false, true>(r0, T.p[half], r1, T.p[half], ht);
+    mac(0, pl0, PER / 2);
+    __syncthreads();
+    ntt_pass_lds2<2, false, true>(r0, T.p[half], r1, T.p[half], ht);
+    __syncthreads();
+    __builtin_amdgcn_sched_barrier(0);
+    mac(2, pl1, 0);
+    mac(2, pl1, PER / 2);
+    __builtin_amdgcn_sched_barrier(0);
     // convertTwoRLWEs: c' = ext[i+j] * X^i + ext[j]; new = -sum + c' (+ tau_g of
     // ext[j].b - ext[i+j].b X^i for b)
     uint32_t* o = next + (size_t)j * 4 * kN;
