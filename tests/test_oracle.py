@@ -331,3 +331,45 @@ def test_planted_block_round_trip(oracle, r, n1):
         resp = oracle.answer(r=r, n1=n1, db=words, query=cl["query"], rgsw=cl["rgsw"], baby=cl["baby"],
                              giant=cl["giant"], packing=cl["packing"], perms=perms)
         assert np.array_equal(oracle.decrypt(r, cl["secret"], resp), planted_message(r, shifts[c])), c
+
+
+def test_reference_crtbamod_composition_overflows():
+    """why offline key derivation (genAutoKeyFromOffline) is not restated: its
+    crtBaMod half composes each coefficient with crt_inv_bsgs_ba
+    (utils.cpp:555-571) through compute_mult_qinvqi_bsgs_ba (utils.cpp:434-460),
+    whose (temp >> 64) * two128_mod needs up to 142 bits in a 128-bit type.
+    The arithmetic restated here with that wrap-around gives a value that is
+    not the CRT composition for every sampled input, while the crtMod * auxMod
+    half (compute_mult_qinvqi_bsgs_aux, the giant-step keys) stays exact"""
+    import random
+    q1, q2, bs, aux = 268369921, 249561089, 16760833, 268361729
+    Q = q1 * q2
+    w = 1 << 128
+
+    def compose(x1, x2, m2, qinv2, qinv1, two128):
+        mod = Q * m2
+        assert two128 == w % mod
+
+        def term(inp, qinv, qi):
+            f = qi * qinv
+            a, b = f >> 64, f & ((1 << 64) - 1)
+            temp = inp * a % w
+            r = ((temp >> 64) * two128 % w) % mod
+            r += ((temp << 64) % w) % mod
+            r += inp * b % w
+            return r % w % mod
+        return (term(x1, qinv2, m2) + term(x2, qinv1, Q)) % w % mod
+
+    rng = random.Random(1)
+    ba = bs * aux
+    ba_two128 = (14764224294473 << 63) + 1483995998376346528         # crt.h:60-61
+    aux_two128 = (1379708 << 63) + 2086958912630458341               # crt.h:65-66
+    wrong_ba = wrong_aux = 0
+    for _ in range(200):
+        x1, x2 = rng.randrange(Q), rng.randrange(ba)
+        v = compose(x1, x2, ba, 26456327859403970, 2721180491014976, ba_two128)    # crt.h:53-54
+        wrong_ba += v % Q != x1 or v % ba != x2
+        y2 = rng.randrange(aux)
+        u = compose(x1, y2, aux, 48735363086513670, 73083388, aux_two128)          # crt.h:56, 63
+        wrong_aux += u % Q != x1 or u % aux != y2
+    assert wrong_ba == 200 and wrong_aux == 0
