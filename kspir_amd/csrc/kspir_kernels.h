@@ -640,7 +640,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_db_product(const uint64_t* __res
 #pragma unroll
     for (int i = 0; i < RPW; ++i) a1[i] = a2[i] = b1[i] = b2[i] = 0;
     // the row loads of one k carry no condition, so they issue together
-#pragma unroll 2
+#pragma unroll 3
     for (int k = 0; k < n1; ++k) {
         const uint32_t* c = ct + (size_t)k * 4 * kN;
         const uint32_t ca1 = c[0], ca2 = c[kN], cb1 = c[2 * kN], cb2 = c[3 * kN];
