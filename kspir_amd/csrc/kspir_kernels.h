@@ -1151,7 +1151,9 @@ __device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70)
 // planes 0..3 are written in one step.  Planes: 0..3 digits [p][j] (0 = d0
 // mod q1, 1 = d1 mod q1, 2 = d0 mod q2, 3 = d1 mod q2), 4..7 the row [ab][p].
 // Each thread owns the accumulator slots 4 (tid + 512 h) + i (h = 0, 1,
-// i = 0..3), so keys, row 0 and fd move as 16-byte vectors.  Same words as
+// i = 0..3), so keys, row 0 and fd move as 16-byte vectors.  The key switch
+// of step c runs during the inverse transform of row c+1 (its digits are
+// still in planes 0..3 then), the last one after the loop.  Same words as
 // k_giant_fused: every update is exact mod q.
 __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_t* __restrict__ dbout,
         const KeyPtrs* __restrict__ kp, uint32_t* __restrict__ fd, uint32_t* __restrict__ ecoef, int r, int nq,
