@@ -1505,9 +1505,11 @@ __global__ void __launch_bounds__(kNttThreads) k_pack_digit(const uint32_t* __re
 // work of k_pack_coef, k_pack_digit and k_pack_combine in one workgroup of
 // 512 threads, the two 256-thread halves running the transforms of the two
 // primes.  a_j - a_{i+j} X^i is INTT'd in LDS, tau_g of it is read back from
-// LDS and decomposed (ell 4, PP 0, Bg 2^14, answer.cpp:576-648), the digits
-// are NTT'd two at a time and their key products summed in registers; then
-// convertTwoRLWEs (answer.cpp:720-739) writes next[j].  Same words as the
+// LDS and decomposed (ell 4, PP 0, Bg 2^14, answer.cpp:576-648) into eight
+// LDS planes (four digits, two primes), the digits are NTT'd two at a time
+// and their key products summed in registers, the first pair's products
+// during the second pair's transforms; then convertTwoRLWEs
+// (answer.cpp:720-739) writes next[j].  Same words as the
 // three kernels (sums of Montgomery products are reduced once, an exact
 // identity mod q).  Scheduling barriers keep each phase's loads in that
 // phase.  1-D grid of i * nq workgroups.
