@@ -1138,6 +1138,22 @@ __device__ __forceinline__ void dma_planes(uint32_t (*dst)[kPadN], const uint32_
     }
 }
 
+// The same for planes read only by gathers, kept unpadded in LDS: the 16-byte
+// form of the copy moves 256 consecutive words per instruction (lane l to
+// M0 + 16 l), 16 instructions per plane.
+template <int NP>
+__device__ __forceinline__ void dma_planes_flat(uint32_t (*dst)[kPadN], const uint32_t* __restrict__ src, int tid) {
+    typedef __attribute__((address_space(3))) void lds_void;
+    typedef __attribute__((address_space(1))) void glb_void;
+    const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+#pragma unroll
+    for (int b = wv; b < NP * kN / 256; b += kFusedThreads / 64) {
+        const int j = b >> 4, x0 = (b & 15) * 256;
+        __builtin_amdgcn_global_load_lds((glb_void*)(src + j * kN + x0 + 4 * lane), (lds_void*)(&dst[j][x0]), 16, 0,
+                                         0);
+    }
+}
+
 // s_waitcnt vmcnt(0): this wave's global loads, LDS-DMA copies included, done
 __device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
@@ -1149,8 +1165,8 @@ __device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70)
 // accumulator (by the slot permutation of tau_g) as soon as it is in LDS, and
 // the a-row is inverse-transformed in place in planes 4, 5, so the four digit
 // planes 0..3 are written in one step.  Planes: 0..3 digits [p][j] (0 = d0
-// mod q1, 1 = d1 mod q1, 2 = d0 mod q2, 3 = d1 mod q2), 4..7 the row [ab][p].
-// Each thread owns the accumulator slots 4 (tid + 512 h) + i (h = 0, 1,
+// mod q1, 1 = d1 mod q1, 2 = d0 mod q2, 3 = d1 mod q2), 4..7 the row [ab][p]
+// (the b-planes 6, 7 unpadded: they are only gathered).  Each thread owns the accumulator slots 4 (tid + 512 h) + i (h = 0, 1,
 // i = 0..3), so keys, row 0 and fd move as 16-byte vectors.  The key switch
 // of step c runs during the inverse transform of row c+1 (its digits are
 // still in planes 0..3 then), the last one after the loop.  Same words as
@@ -1197,7 +1213,8 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_
             }
         }
     };
-    dma_planes<4>(s + 4, rows + (size_t)(n2 > 1 ? 1 : 0) * 4 * kN, tid);
+    dma_planes<2>(s + 4, rows + (size_t)(n2 > 1 ? 1 : 0) * 4 * kN, tid);
+    dma_planes_flat<2>(s + 6, rows + (size_t)(n2 > 1 ? 1 : 0) * 4 * kN + 2 * kN, tid);
     for (int c = 1; c < n2; ++c) {
         const uint32_t* next = rows + (size_t)(c + 1 < n2 ? c + 1 : 0) * 4 * kN;
         wait_vmem();
@@ -1209,8 +1226,8 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_
 #pragma unroll
         for (int e = 0; e < PER; ++e) {
             const int px = perm_slot(4 * (tid + kFusedThreads * (e >> 2)) + (e & 3), g, T);
-            acc[1][0][e] = add_mod(acc[1][0][e], s[6][lds_pad(px)], q1);
-            acc[1][1][e] = add_mod(acc[1][1][e], s[7][lds_pad(px)], q2);
+            acc[1][0][e] = add_mod(acc[1][0][e], s[6][px], q1);
+            acc[1][1][e] = add_mod(acc[1][1][e], s[7][px], q2);
         }
         __builtin_amdgcn_sched_barrier(0);
         // the key switch of step c-1 (digits still in planes 0..3) rides along
@@ -1222,7 +1239,7 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_
         if (c >= 2) mac(c - 1, 0);
         __syncthreads();
         __builtin_amdgcn_sched_barrier(0);
-        dma_planes<2>(s + 6, next + 2 * kN, tid);     // every wave is past its b-gather
+        dma_planes_flat<2>(s + 6, next + 2 * kN, tid);     // every wave is past its b-gather
         __builtin_amdgcn_sched_barrier(0);
         ntt_pass_lds<1, true>(s[4 + half], T.p[half], ht);
         if (c >= 2) mac(c - 1, 1);
@@ -1270,7 +1287,8 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_
                 uint32_t v[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    v[i] = add_mod(acc[ab][p][4 * h + i], s[4 + ab * 2 + p][lds_pad(x0) + i], T.p[p].q);
+                    const uint32_t r0 = ab ? s[6 + p][x0 + i] : s[4 + p][lds_pad(x0) + i];   // b-planes unpadded
+                    v[i] = add_mod(acc[ab][p][4 * h + i], r0, T.p[p].q);
                     s[ab * 2 + p][lds_pad(x0) + i] = v[i];
                 }
                 if (fd)
