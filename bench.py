@@ -274,6 +274,47 @@ def config_keygen(args, log):
              "sample": f"one client (seed 1) by the oracle on the host, {cpu * 1e3:.0f} ms"}}
 
 
+def config_offline(args, log):
+    """giant-step half of the offline key derivation (genAutoKeyFromOffline)
+    for 64 clients at N1 = 64 (tests/test-two-steps-key.cpp's shape: a chain
+    of 30 key switches by 5^N1, then 31 keys switched to crtMod), against the
+    oracle's derivation for one client on the host"""
+    import kspir_amd
+    from oracle import oracle
+    n1, nc = 64, 64
+    log(f"offline keys for {nc} clients (oracle key generation on the host)")
+    gens = [oracle.gen_offline_giant(n1, 100 + c) for c in range(nc)]
+    off = np.stack([g[1] for g in gens])
+    auto = np.stack([g[2] for g in gens])
+    ctx = kspir_amd.Context(0)
+    ctx.gen_giant_keys_from_offline(n1, off[:2], auto[:2], want_chain=False)   # warm-up
+    reps = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        _, giant = ctx.gen_giant_keys_from_offline(n1, off, auto, want_chain=False)
+        reps.append(time.perf_counter() - t0)
+    ctx.close()
+    gpu_s = float(np.median(reps))
+    cpu = None
+    if not args.no_cpu_baseline:
+        t0 = time.perf_counter()
+        _, want = oracle.giant_from_offline(n1, off[0], auto[0], want_chain=False)
+        cpu = time.perf_counter() - t0
+        assert np.array_equal(giant[0], want)
+    n2 = N // 2 // n1
+    return {"metric": "giant-step keys derived from offline keys per second",
+            "value": round(nc / gpu_s, 2), "unit": "clients/s", "n_gpus": 1, "higher_is_better": True,
+            "dtype": "uint32 RNS (q1, q2, auxMod)", "data": "synthetic keys from seeds 100..163",
+            "config": {"workload": f"{nc} clients at N1={n1}: {n2 - 2} chained key switches by 5^N1 and {n2 - 1} "
+                                   f"modulus switches per client; host keys in, {n2 - 1} giant keys per client "
+                                   f"({(n2 - 1) * 6 * N * 8 / 1e6:.1f} MB) back to host memory, PCIe included",
+                       "clients": nc},
+            "ms_per_batch": round(gpu_s * 1e3, 2),
+            "cpu_baseline": None if cpu is None else
+            {"value": round(1.0 / cpu, 3), "unit": "clients/s", "cores": 1, "kind": "port",
+             "sample": f"one client (seed 100) by the oracle on the host, {cpu * 1e3:.0f} ms, equal to the GPU's words"}}
+
+
 def serve_rates(srv, db, keys, queries, rates, dur, seed, log, tag):
     """Poisson arrivals at each offered rate for `dur` seconds through the
     server; p50/p99 of the server's submit-to-response latency"""
@@ -380,8 +421,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="launch kernel by kernel instead of replaying hipGraphs")
     ap.add_argument("--no-autotune", action="store_true", help="keep the library's default kernel shapes and paths")
-    ap.add_argument("--config", default="c3", choices=("c3", "c2", "c4", "c5", "keygen"),
-                    help="c3: the headline batched line (default); c2/c4/c5/keygen: the other workloads, one JSON line each")
+    ap.add_argument("--config", default="c3", choices=("c3", "c2", "c4", "c5", "keygen", "offline"),
+                    help="c3: the headline batched line (default); c2/c4/c5/keygen/offline: the other workloads, "
+                         "one JSON line each")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the KsPIR answer here runs on a single MI355X "
@@ -390,7 +432,8 @@ def main():
 
     log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)
     if args.config != "c3":
-        fn = {"c2": config_c2, "c4": config_c4, "c5": lambda a, l: config_c5(a, l), "keygen": config_keygen}[args.config]
+        fn = {"c2": config_c2, "c4": config_c4, "c5": lambda a, l: config_c5(a, l), "keygen": config_keygen,
+              "offline": config_offline}[args.config]
         print(json.dumps(fn(args, log)), flush=True)
         return
 

@@ -160,6 +160,23 @@ int kspir_answer_inspect(kspir_context* ctx, const kspir_database* db, const ksp
 int kspir_decrypt_bsgs_total(kspir_context* ctx, int32_t r, int32_t nq, const uint64_t* secrets,
                              const uint64_t* responses, uint64_t* messages);
 
+/* Giant-step half of genAutoKeyFromOffline (twosteps.h, twosteps.cpp:1803-1863,
+ * driven by tests/test-two-steps-key.cpp:118-146) for nc clients at once, on
+ * the GPU: the chain of evalAutoRNSKey (twosteps.cpp:1588-1715, auxMod
+ * branch) by 5^N1 from the auto key at 5^N1, then the modulus switch to
+ * crtMod.  Host pointers:
+ *   offline2 [nc][10][2][N]  keyMap.at(5^N1) of AutoKeyBSGSRNS(N, crtMod, auxMod, 5, 2^14, 2^14)
+ *                            (rows 0..4 mod crtMod, 5..9 mod auxMod, NTT form)
+ *   auto2    [nc][6][2][N]   keyMap.at(5^N1) of AutoKeyBSGSRNS(N, crtMod, auxMod, 3, 2^11*auxMod, 2^15)
+ *   chain    [nc][N2-2][6][2][N]  the keys the reference inserts into the auto key at 5^(k N1),
+ *                            k = 2..N2-1, coefficient form (may be NULL)
+ *   giant    [nc][N2-1][3][2][N]  result.keyMap.at(5^(k N1)), k = 1..N2-1, NTT form mod crtMod
+ * Each RLWE is (a, b).  As in the reference, the key at k = 1 is switched from
+ * auto2's words as stored (NTT form).  N1 is a power of two in [2, N/4].
+ * The crtBaMod (baby-step) half is not provided. */
+int kspir_gen_giant_keys_from_offline(kspir_context* ctx, int32_t n1, int32_t nc, const uint64_t* offline2,
+                                      const uint64_t* auto2, uint64_t* chain, uint64_t* giant);
+
 /* Client side of test-pir.cpp:20-24,68-89 for nc clients at once, on the
  * GPU.  For client i, as after srand(seeds[i]): the Secret (secret.cpp:13-60),
  * query_bsgs_rns(target_col[i]) (twosteps.h:26, twosteps.cpp:279-318),

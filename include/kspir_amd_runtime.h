@@ -102,7 +102,8 @@ int kspir_time_copy(kspir_context* ctx, void* dst_dev, const void* src_dev, size
                     float* ms_per_launch);
 
 /* Test hook: `count` polynomials of N u32 residues in prime p (0: q1,
- * 1: q2, 2: bsMod), forward (inverse != 0: inverse) NTT in place, host memory. */
+ * 1: q2, 2: bsMod, 3: plaintext p, 4: auxMod), forward (inverse != 0:
+ * inverse) NTT in place, host memory. */
 int kspir_test_ntt(kspir_context* ctx, uint32_t* polys, int count, int prime, int inverse);
 
 #ifdef __cplusplus

@@ -43,7 +43,7 @@ def _load():
                  "kspir_server_create", "kspir_server_submit", "kspir_server_wait", "kspir_server_stats",
                  "kspir_query_footprint_bytes", "kspir_decrypt_bsgs_total", "kspir_context_get",
                  "kspir_autotune", "kspir_client_generate", "kspir_client_draws", "kspir_client_encrypt",
-                 "kspir_server_create_lanes"):
+                 "kspir_server_create_lanes", "kspir_gen_giant_keys_from_offline"):
         getattr(L, name).restype = ctypes.c_int
     L.kspir_autotune.argtypes = [_vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                  ctypes.POINTER(ctypes.c_int32), ctypes.c_char_p, ctypes.c_size_t]
@@ -69,6 +69,7 @@ EXPORTS = (
     "kspir_server_create", "kspir_server_submit", "kspir_server_wait", "kspir_server_stats", "kspir_server_destroy",
     "kspir_query_footprint_bytes", "kspir_decrypt_bsgs_total", "kspir_context_get", "kspir_autotune",
     "kspir_client_generate", "kspir_client_draws", "kspir_client_encrypt", "kspir_server_create_lanes",
+    "kspir_gen_giant_keys_from_offline",
 )
 
 
@@ -383,6 +384,22 @@ class Context:
         _check(lib.kspir_time_copy(self.h, _vp(dst_buf.ptr), _vp(src_buf.ptr), ctypes.c_size_t(nbytes), iters,
                                    ctypes.byref(ms)))
         return ms.value
+
+    def gen_giant_keys_from_offline(self, n1, offline2, auto2, want_chain=True):
+        """giant-step half of genAutoKeyFromOffline for nc clients on the GPU.
+        offline2 [nc][10][2][N], auto2 [nc][6][2][N] (NTT form) -> (chain
+        [nc][N2-2][6][2][N] or None, giant [nc][N2-1][3][2][N])"""
+        off = np.ascontiguousarray(offline2, dtype=np.uint64).reshape(-1, 10, 2, N)
+        au = np.ascontiguousarray(auto2, dtype=np.uint64).reshape(-1, 6, 2, N)
+        nc = off.shape[0]
+        assert au.shape[0] == nc
+        n2 = N // 2 // n1
+        giant = np.zeros((nc, max(n2 - 1, 0), 3, 2, N), dtype=np.uint64)
+        chain = np.zeros((nc, max(n2 - 2, 0), 6, 2, N), dtype=np.uint64) if want_chain else None
+        _check(lib.kspir_gen_giant_keys_from_offline(self.h, int(n1), int(nc), _p64(off), _p64(au),
+                                                     _p64(chain) if chain is not None and chain.size else None,
+                                                     _p64(giant)))
+        return chain, giant
 
     def decrypt_bsgs_total(self, r, secrets, responses):
         """recover on the GPU: secrets [nq][N], responses [nq][2][N] -> messages [nq][N]"""

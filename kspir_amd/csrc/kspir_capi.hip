@@ -16,6 +16,7 @@
 
 #include "kspir_kernels.h"
 #include "kspir_client.h"
+#include "kspir_offline.h"
 #include "kspir_rng.h"
 #include "../../include/kspir_amd.h"
 #include "../../include/kspir_amd_runtime.h"
@@ -184,10 +185,10 @@ struct kspir_keys {
 namespace {
 
 int build_tables(kspir_context* c) {
-    const uint32_t qs[4] = {kQ1, kQ2, kBs, kP};
+    const uint32_t qs[5] = {kQ1, kQ2, kBs, kP, kAux};
     const u64 root_q = 3375402822066082ULL;   // crt.h:81 root_of_unity_crt
-    const u64 roots[4] = {root_q % kQ1, root_q % kQ2, min_root(kBs), min_root(kP)};
-    for (int p = 0; p < 4; ++p) {
+    const u64 roots[5] = {root_q % kQ1, root_q % kQ2, min_root(kBs), min_root(kP), min_root(kAux)};
+    for (int p = 0; p < 5; ++p) {
         const uint32_t q = qs[p];
         const u64 w = roots[p];
         std::vector<uint32_t> fw(kN), fwsh(kN), iv(kN), ivsh(kN), pw(2 * kN);
@@ -1766,6 +1767,75 @@ int kspir_decrypt_bsgs_total(kspir_context* c, int32_t r, int32_t nq, const uint
     return KSPIR_OK;
 }
 
+int kspir_gen_giant_keys_from_offline(kspir_context* c, int32_t n1, int32_t nc, const uint64_t* offline2,
+                                      const uint64_t* auto2, uint64_t* chain, uint64_t* giant) {
+    if (!c || !offline2 || !auto2 || !giant || nc < 1) return fail(KSPIR_EINVAL, "bad argument");
+    if (!pow2(n1) || n1 < 2 || n1 > kN / 4) return fail(KSPIR_EINVAL, "N1 must be a power of two in [2, N/4]");
+    const int n2 = kN / 2 / n1, nkeys = n2 - 1, nchain = n2 - 2;
+    HIP_OK(hipSetDevice(c->device));
+    OkdConst K{};
+    for (int p = 0; p < 2; ++p) {
+        const uint32_t q = c->T.p[p].q;
+        K.auxinv[p] = (uint32_t)invmod(kAux % q, q);
+        K.auxinv_sh[p] = shoup(K.auxinv[p], q);
+    }
+    K.qinv_aux = (uint32_t)invmod(kQ % kAux, kAux);
+    K.qinv_aux_sh = shoup(K.qinv_aux, kAux);
+    const u64 g = powmod(5, (u64)n1, 2 * kN);   // 5^N1 mod 2N (odd, so invertible)
+    for (u64 x = 1; x < 2 * kN; x += 2)
+        if (g * x % (2 * kN) == 1) K.ginv = (uint32_t)x;
+    const size_t P = kN;                         // words per plane
+    const size_t st_stride = (size_t)nkeys * 18 * P;
+    const size_t b_off = (size_t)nc * 10 * 2 * P * 8, b_auto = (size_t)nc * 6 * 2 * P * 8;
+    const size_t b_km = (size_t)nc * 30 * P * 4, b_st = (size_t)nc * st_stride * 4, b_c0 = (size_t)nc * 18 * P * 4;
+    const size_t b_dig = (size_t)nc * kOkdRows * kOkdEll * 3 * P * 4, b_tb = (size_t)nc * kOkdRows * 3 * P * 4;
+    const size_t b_giant = (size_t)nc * nkeys * 6 * P * 8;
+    const size_t b_chain = chain && nchain > 0 ? (size_t)nc * nchain * 12 * P * 8 : 0;
+    uint64_t *doff = nullptr, *dauto = nullptr, *dgiant = nullptr, *dchain = nullptr;
+    uint32_t *km = nullptr, *st = nullptr, *c0 = nullptr, *dig = nullptr, *tb = nullptr;
+    hipError_t e = hipMalloc(&doff, b_off);
+    if (e == hipSuccess) e = hipMalloc(&dauto, b_auto);
+    if (e == hipSuccess) e = hipMalloc(&km, b_km);
+    if (e == hipSuccess) e = hipMalloc(&st, b_st);
+    if (e == hipSuccess) e = hipMalloc(&c0, b_c0);
+    if (e == hipSuccess) e = hipMalloc(&dig, b_dig);
+    if (e == hipSuccess) e = hipMalloc(&tb, b_tb);
+    if (e == hipSuccess) e = hipMalloc(&dgiant, b_giant);
+    if (e == hipSuccess && b_chain) e = hipMalloc(&dchain, b_chain);
+    hipStream_t sm = c->stream;
+    if (e == hipSuccess) e = hipMemcpyAsync(doff, offline2, b_off, hipMemcpyHostToDevice, sm);
+    if (e == hipSuccess) e = hipMemcpyAsync(dauto, auto2, b_auto, hipMemcpyHostToDevice, sm);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_okd_keys, dim3(kN / 256, 30, nc), dim3(256), 0, sm, doff, km, K);
+        hipLaunchKernelGGL(k_okd_load, dim3(18, nc), dim3(kNttThreads), 0, sm, dauto, st, st_stride, c0, c->T);
+        for (int k = 1; k < nkeys && e == hipSuccess; ++k) {
+            const uint32_t* in = k == 1 ? c0 : st + (size_t)(k - 1) * 18 * P;
+            const size_t in_stride = k == 1 ? 18 * P : st_stride;
+            hipLaunchKernelGGL(k_okd_digits, dim3(kN / 256, kOkdRows, nc), dim3(256), 0, sm, in, in_stride, dig, tb,
+                               K, c->T);
+            hipLaunchKernelGGL(k_okd_ks, dim3(3, kOkdRows, nc), dim3(kNttThreads), 0, sm, dig, tb, km,
+                               st + (size_t)k * 18 * P, st_stride, c->T);
+            e = hipGetLastError();
+        }
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_okd_modswitch, dim3(nkeys, 6, nc), dim3(512), 0, sm, st, st_stride, dgiant, nkeys, K,
+                           c->T);
+        if (dchain)
+            hipLaunchKernelGGL(k_okd_export, dim3(kN / 256, nchain * 12, nc), dim3(256), 0, sm, st, st_stride, dchain,
+                               nchain, c->T);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(giant, dgiant, b_giant, hipMemcpyDeviceToHost, sm);
+    if (e == hipSuccess && dchain) e = hipMemcpyAsync(chain, dchain, b_chain, hipMemcpyDeviceToHost, sm);
+    if (e == hipSuccess) e = hipStreamSynchronize(sm);
+    for (void* ptr : {(void*)doff, (void*)dauto, (void*)dgiant, (void*)dchain, (void*)km, (void*)st, (void*)c0,
+                      (void*)dig, (void*)tb})
+        if (ptr) hipFree(ptr);
+    if (e != hipSuccess) return fail_hip("gen_giant_keys_from_offline", e);
+    return KSPIR_OK;
+}
+
 int kspir_query_footprint_bytes(int32_t r, int32_t n1, size_t* bytes) {
     if (!bytes || !pow2(r) || r > kN || !pow2(n1) || n1 < 32 || n1 > kN / 2) return fail(KSPIR_EINVAL, "bad shape");
     *bytes = query_footprint_bytes(r, n1);
@@ -1888,7 +1958,7 @@ void kspir_server_destroy(kspir_server* s) {
 }
 
 int kspir_test_ntt(kspir_context* c, uint32_t* polys, int count, int prime, int inverse) {
-    if (!c || !polys || count < 1 || prime < 0 || prime > 2) return fail(KSPIR_EINVAL, "bad argument");
+    if (!c || !polys || count < 1 || prime < 0 || prime > 4) return fail(KSPIR_EINVAL, "bad argument");
     HIP_OK(hipSetDevice(c->device));
     uint32_t* d = nullptr;
     HIP_OK(hipMalloc(&d, (size_t)count * kN * 4));

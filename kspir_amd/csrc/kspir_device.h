@@ -28,6 +28,7 @@ constexpr uint32_t kQ1 = 268369921u;
 constexpr uint32_t kQ2 = 249561089u;
 constexpr uint32_t kBs = 16760833u;
 constexpr uint32_t kP = 65537u;          // plaintext modulus (params.h)
+constexpr uint32_t kAux = 268361729u;    // auxiliary prime of offline key derivation (crt.h:37)
 constexpr uint64_t kQ = (uint64_t)kQ1 * kQ2;
 
 enum Prime { P_Q1 = 0, P_Q2 = 1, P_BS = 2 };
@@ -47,7 +48,7 @@ struct PrimeTab {
 };
 
 struct Tabs {
-    PrimeTab p[4];              // q1, q2, bsMod, plaintext p
+    PrimeTab p[5];              // q1, q2, bsMod, plaintext p, auxMod
     const uint16_t* slot_exp;   // NTT slot s holds psi^(slot_exp[s]) (odd exponent)
     const uint16_t* exp_slot;   // exp_slot[e >> 1] = slot holding psi^e
     // CRT / modulus-switch constants (Shoup pairs)

@@ -1024,6 +1024,191 @@ void packing_rlwes(Rlwe& result, std::vector<Rlwe>& rlwes, const u64* packing) {
     result = rlwes[0];
 }
 
+// ------------------------------------------- offline giant-key derivation
+
+constexpr u64 AUX = 268361729ULL;                 // crt.h:37 auxMod
+constexpr u64 AUX_INV_Q = 48735363086513670ULL;   // crt.h:56 auxModInv (auxMod^-1 mod crtMod)
+constexpr u64 Q_INV_AUX = 73083388ULL;            // crt.h:63 crtModInvAux (crtMod^-1 mod auxMod)
+
+// lwe.cpp:1016-1019: NTT(length, auxMod) takes the minimal root
+const Ntt& ntt_aux() { static Ntt t(AUX, min_root(AUX)); return t; }
+const Mod& maux() { static Mod m(AUX); return m; }
+
+// utils.cpp:462-488 compute_mult_qinvqi_bsgs_aux: input * qinv * qi mod
+// crtMod*auxMod with the factor split at 2^64 and 2^128 mod M folded in
+// (uint128 arithmetic, wrapping where the reference's does)
+u128 mult_qinvqi_aux(u64 input, u64 qinv, u64 qi) {
+    const u128 M = (u128)Q * AUX;
+    const u128 two128 = ((~(u128)0) % M + 1) % M;
+    const u128 mask = ((u128)1 << 64) - 1;
+    const u128 f = (u128)qi * qinv;
+    const u128 temp = (u128)input * (f >> 64);
+    u128 r = (temp >> 64) * two128 % M;
+    r += (temp << 64) % M;
+    r += (u128)input * (f & mask);
+    return r % M;
+}
+
+// utils.cpp:535-553 crt_inv_bsgs_aux
+inline u128 crt_inv_bsgs_aux(u64 x, u64 y) {
+    const u128 M = (u128)Q * AUX;
+    u128 r = mult_qinvqi_aux(x, AUX_INV_Q, AUX);
+    r += mult_qinvqi_aux(y, Q_INV_AUX, Q);
+    return r % M;
+}
+
+// utils.cpp:1191-1256 decompose_bsgs_aux (unsigned digits with offset, `d > M`)
+void decompose_bsgs_aux(std::vector<Poly>& out_q, std::vector<Poly>& out_aux, const u64* in_q,
+                        const u64* in_aux, int ell, u64 base, u64 bg) {
+    const u128 M = (u128)Q * AUX;
+    const int gbits = ilog2(bg);
+    const int basebits = base != 0 ? ilog2(base) : 0;
+    const int nsub = 128 - gbits;
+    u128 offset = 0;
+    for (int i = 0; i < ell; ++i) offset += (u128)1 << (basebits + i * gbits);
+    offset <<= (gbits - 1);
+    offset += base / 2;
+    for (int i = 0; i < N; ++i) {
+        u128 d = crt_inv_bsgs_aux(in_q[i], in_aux[i]) + offset;
+        if (d > M) d -= M;
+        d >>= basebits;
+        for (int j = 0; j < ell; ++j) {
+            i64 r = (i64)(u64)(d << nsub >> nsub);
+            r -= (i64)(bg / 2);
+            d >>= gbits;
+            if (r >= 0) { out_q[j][i] = (u64)r; out_aux[j][i] = (u64)r; }
+            else { out_q[j][i] = (u64)(r + (i64)Q); out_aux[j][i] = (u64)(r + (i64)AUX); }
+        }
+    }
+}
+
+// twosteps.cpp:1588-1715 evalAutoRNSKey, auxMod branch.  key: [2*ell][2][N],
+// rows 0..ell-1 mod crtMod and ell..2ell-1 mod auxMod, NTT form.  The inputs
+// come by value: the reference transforms them in place, on the copies its
+// caller takes from keyMap.  Results are in coefficient form.
+void eval_auto_rns_key(Rlwe& r1, Rlwe& r2, Rlwe in1, Rlwe in2, int32_t index, const u64* key,
+                       int ell, u64 pp, u64 bg) {
+    if (in1.isntt) { ntt_q().inverse(in1.a.data()); ntt_q().inverse(in1.b.data()); in1.isntt = false; }
+    if (in2.isntt) { ntt_aux().inverse(in2.a.data()); ntt_aux().inverse(in2.b.data()); in2.isntt = false; }
+    Poly ta1(N), tb1(N), ta2(N), tb2(N);
+    automorphic_raw(ta1.data(), in1.a.data(), index, Q);
+    automorphic_raw(tb1.data(), in1.b.data(), index, Q);
+    automorphic_raw(ta2.data(), in2.a.data(), index, AUX);
+    automorphic_raw(tb2.data(), in2.b.data(), index, AUX);
+    std::vector<Poly> d1(ell, Poly(N)), d2(ell, Poly(N));
+    decompose_bsgs_aux(d1, d2, ta1.data(), ta2.data(), ell, pp, bg);
+    // crtMod part: (0, b) - g^-1(a) * key
+    std::fill(r1.a.begin(), r1.a.end(), 0);
+    r1.b = tb1;
+    ntt_q().forward(r1.b.data());
+    for (int i = 0; i < ell; ++i) {
+        ntt_q().forward(d1[i].data());
+        const u64* k = key + (size_t)i * 2 * N;
+        ew_submul(r1.a.data(), d1[i].data(), k, mq());
+        ew_submul(r1.b.data(), d1[i].data(), k + N, mq());
+    }
+    ntt_q().inverse(r1.a.data());
+    ntt_q().inverse(r1.b.data());
+    // auxMod part, digits transformed from their auxMod residues
+    std::fill(r2.a.begin(), r2.a.end(), 0);
+    r2.b = tb2;
+    ntt_aux().forward(r2.b.data());
+    for (int i = 0; i < ell; ++i) {
+        ntt_aux().forward(d2[i].data());
+        const u64* k = key + (size_t)(i + ell) * 2 * N;
+        ew_submul(r2.a.data(), d2[i].data(), k, maux());
+        ew_submul(r2.b.data(), d2[i].data(), k + N, maux());
+    }
+    ntt_aux().inverse(r2.a.data());
+    ntt_aux().inverse(r2.b.data());
+}
+
+// twosteps.cpp:1803-1863, the giant-step half of genAutoKeyFromOffline.
+//   auto2    [6][2][N]   keyMap.at(5^N1) of AutoKeyBSGSRNS(N, crtMod, auxMod, 3, 2^11*auxMod, 2^15), NTT form
+//   offline2 [10][2][N]  keyMap.at(5^N1) of AutoKeyBSGSRNS(N, crtMod, auxMod, 5, 2^14, 2^14), NTT form
+//   chain    [N2-2][6][2][N]  keys the reference inserts into result2 at 5^(k*N1), k = 2..N2-1
+//                             (coefficient form), or NULL
+//   giant    [N2-1][3][2][N]  result.keyMap at 5^(k*N1), k = 1..N2-1: auxMod^-1 * (x_crt - x_aux)
+//                             mod crtMod, NTT form.  For k = 1 the inputs are the words of auto2
+//                             as stored (NTT form), which the reference switches as they are.
+void giant_from_offline(int n1, const u64* offline2, const u64* auto2, u64* chain, u64* giant) {
+    const int n2 = N / 2 / n1, ell = 3;
+    const int32_t step = pow_mod2(5, n1, 2 * N);
+    std::vector<std::vector<Rlwe>> keys(1);
+    for (int j = 0; j < 2 * ell; ++j) {
+        Rlwe c(j < ell ? Q : AUX);
+        const u64* src = auto2 + (size_t)j * 2 * N;
+        c.a.assign(src, src + N);
+        c.b.assign(src + N, src + 2 * N);
+        c.isntt = true;
+        keys[0].push_back(c);
+    }
+    for (int i = 1; i < n2 - 1; ++i) {
+        std::vector<Rlwe> next;
+        for (int j = 0; j < 2 * ell; ++j) next.emplace_back(j < ell ? Q : AUX);
+        for (int j = 0; j < ell; ++j)
+            eval_auto_rns_key(next[j], next[j + ell], keys.back()[j], keys.back()[j + ell], step, offline2, 5,
+                              1ULL << 14, 1ULL << 14);
+        keys.push_back(next);
+        if (chain)
+            for (int j = 0; j < 2 * ell; ++j) {
+                u64* o = chain + (((size_t)(i - 1) * 2 * ell + j) * 2) * N;
+                std::memcpy(o, next[j].a.data(), 8 * N);
+                std::memcpy(o + N, next[j].b.data(), 8 * N);
+            }
+    }
+    const Mod& m = mq();
+    for (int i = 0; i < n2 - 1; ++i)
+        for (int j = 0; j < ell; ++j)
+            for (int part = 0; part < 2; ++part) {
+                const Poly& x1 = part ? keys[i][j].b : keys[i][j].a;
+                const Poly& x3 = part ? keys[i][j + ell].b : keys[i][j + ell].a;
+                u64* o = giant + (((size_t)i * ell + j) * 2 + part) * N;
+                for (int k = 0; k < N; ++k) o[k] = m.mul(m.sub(x1[k], x3[k]), AUX_INV_Q);
+                ntt_q().forward(o);
+            }
+}
+
+// encrypt.cpp:300-387 encrypt_rns_bsgs_autokey with modulus2 = auxMod
+void encrypt_rns_aux_autokey(u64* b1, u64* a1, u64* b2, u64* a2, const Secret& s, const Poly& msg1,
+                             const Poly& msg2) {
+    Poly s1 = s.coeff, s2 = s.coeff;
+    guass_to_modulus(s2.data(), Q, AUX);
+    ntt_q().forward(s1.data());
+    ntt_aux().forward(s2.data());
+    const Mod &m1 = mq(), &m2 = maux();
+    sample_random(a1, Q, N);
+    sample_random(a2, AUX, N);
+    ew_mul(b1, a1, s1.data(), m1);
+    ew_mul(b2, a2, s2.data(), m2);
+    Poly err(N), temp(N);
+    sample_guass(err.data(), Q);
+    ew_add(temp.data(), err.data(), msg1.data(), m1);
+    ntt_q().forward(temp.data());
+    ew_add(b1, b1, temp.data(), m1);
+    guass_to_modulus(err.data(), Q, AUX);
+    ew_add(temp.data(), err.data(), msg2.data(), m2);
+    ntt_aux().forward(temp.data());
+    ew_add(b2, b2, temp.data(), m2);
+}
+
+// lwe.cpp:1025-1070 generateSingleKeyBS for AutoKeyBSGSRNS(N, crtMod, auxMod, ell, pp, bg):
+// out [2*ell][2][N], NTT form
+void aux_key(u64* out, int32_t index, const Secret& s, int ell, u64 pp, u64 bg) {
+    Poly autosec(N), autosec2(N), t1(N), t2(N);
+    std::vector<u128> mf = power_of_bg128(pp, bg, ell);
+    automorphic(autosec.data(), s.coeff.data(), index, Q);
+    autosec2 = autosec;
+    guass_to_modulus(autosec2.data(), Q, AUX);
+    for (int i = 0; i < ell; ++i) {
+        ew_mulc(t1.data(), autosec.data(), (u64)(mf[i] % Q), mq());
+        ew_mulc(t2.data(), autosec2.data(), (u64)(mf[i] % AUX), maux());
+        u64* kq = out + (size_t)i * 2 * N;
+        u64* ka = out + (size_t)(i + ell) * 2 * N;
+        encrypt_rns_aux_autokey(kq + N, kq, ka + N, ka, s, t1, t2);
+    }
+}
+
 double now_us() {
     return std::chrono::duration<double, std::micro>(
                std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -1070,6 +1255,10 @@ uint64_t ko_const(int which) {
         case 7: return ROOT_Q;
         case 8: return ntt_bs().root;
         case 9: return ntt_p().root;
+        case 10: return AUX;
+        case 11: return AUX_INV_Q;
+        case 12: return Q_INV_AUX;
+        case 13: return ntt_aux().root;
         default: return 0;
     }
 }
@@ -1085,6 +1274,7 @@ static const Ntt& ntt_sel(int modsel) {
         case 1: return ntt_bs();
         case 2: return ntt_p();
         case 3: return t1;
+        case 5: return ntt_aux();
         default: return t2;
     }
 }
@@ -1358,6 +1548,60 @@ int ko_decrypt(int r, const uint64_t* secret, const uint64_t* response, uint64_t
         else me = (u64)roundl((long double)me / DELTA);
         msg[i] = me;
     }
+    return 0;
+}
+
+void ko_crt_inv_bsgs_aux(uint64_t* out, const uint64_t* in_q, const uint64_t* in_aux) {
+    for (int i = 0; i < N; ++i) {
+        u128 v = crt_inv_bsgs_aux(in_q[i], in_aux[i]);
+        out[2 * i] = (u64)v;
+        out[2 * i + 1] = (u64)(v >> 64);
+    }
+}
+
+void ko_decompose_bsgs_aux(uint64_t* out_q, uint64_t* out_aux, const uint64_t* in_q, const uint64_t* in_aux,
+                           int ell, uint64_t base, uint64_t bg) {
+    std::vector<Poly> oq(ell, Poly(N)), oa(ell, Poly(N));
+    decompose_bsgs_aux(oq, oa, in_q, in_aux, ell, base, bg);
+    for (int j = 0; j < ell; ++j) {
+        std::memcpy(out_q + (size_t)j * N, oq[j].data(), 8 * N);
+        std::memcpy(out_aux + (size_t)j * N, oa[j].data(), 8 * N);
+    }
+}
+
+int ko_gen_offline_giant(int n1, unsigned seed, uint64_t* secret, uint64_t* offline2, uint64_t* auto2) {
+    if (!valid_shape(1, n1) || N / 2 / n1 < 2) return -1;
+    srand(seed);
+    Secret s = make_secret();
+    const int32_t index = pow_mod2(5, n1, 2 * N);
+    aux_key(offline2, index, s, 5, 1ULL << 14, 1ULL << 14);
+    aux_key(auto2, index, s, 3, (1ULL << 11) * AUX, 1ULL << 15);
+    std::memcpy(secret, s.coeff.data(), 8 * N);
+    return 0;
+}
+
+int ko_giant_from_offline(int n1, const uint64_t* offline2, const uint64_t* auto2, uint64_t* chain,
+                          uint64_t* giant) {
+    if (!valid_shape(1, n1) || N / 2 / n1 < 2) return -1;
+    giant_from_offline(n1, offline2, auto2, chain, giant);
+    return 0;
+}
+
+int ko_rlwe_phase(int modsel, const uint64_t* secret, const uint64_t* a, const uint64_t* b, int ntt_form,
+                  uint64_t* out) {
+    // b - a*s in coefficient form, mod crtMod (modsel 0) or auxMod (modsel 5);
+    // the secret is given mod crtMod (coefficients 0, 1, crtMod-1)
+    if (modsel != 0 && modsel != 5) return -1;
+    const u64 q = modsel ? AUX : Q;
+    const Ntt& t = ntt_sel(modsel);
+    const Mod m(q);
+    Poly s(secret, secret + N), x(a, a + N), y(b, b + N);
+    if (modsel) guass_to_modulus(s.data(), Q, AUX);
+    t.forward(s.data());
+    if (!ntt_form) { t.forward(x.data()); t.forward(y.data()); }
+    for (int i = 0; i < N; ++i) y[i] = m.sub(y[i], m.mul(x[i], s[i]));
+    t.inverse(y.data());
+    std::memcpy(out, y.data(), 8 * N);
     return 0;
 }
 

@@ -28,12 +28,13 @@ extern "C" {
 #endif
 
 /* constants */
-uint64_t ko_const(int which);   /* 0:N 1:q1 2:q2 3:Q 4:bsMod 5:p 6:Delta 7:rootQ 8:rootBs 9:rootP */
+uint64_t ko_const(int which);   /* 0:N 1:q1 2:q2 3:Q 4:bsMod 5:p 6:Delta 7:rootQ 8:rootBs 9:rootP
+                                   10:auxMod 11:auxMod^-1 mod Q 12:Q^-1 mod auxMod 13:rootAux */
 uint64_t ko_qinv(int r);        /* utils.cpp:118 QInv(r, crtMod) table (0 when absent) */
 void ko_set_threads(int n);     /* OpenMP threads for offline preprocessing and multi-core timing */
 
 /* arithmetic building blocks */
-void ko_ntt_forward(int modsel, uint64_t* a);   /* modsel 0: Q (root crt.h:81), 1: bsMod, 2: p, 3: q1, 4: q2 */
+void ko_ntt_forward(int modsel, uint64_t* a);   /* modsel 0: Q (root crt.h:81), 1: bsMod, 2: p, 3: q1, 4: q2, 5: auxMod */
 void ko_ntt_inverse(int modsel, uint64_t* a);
 uint64_t ko_min_root(uint64_t q);               /* minimal primitive 2N-th root of unity mod prime q */
 uint64_t ko_crt_compose(uint64_t x, uint64_t y);            /* crt.cpp:368 */
@@ -84,6 +85,21 @@ void ko_last_stage_us(double* out5);
 
 /* decrypt_bsgs_total (twosteps.cpp:388) on a copy of the response */
 int ko_decrypt(int r, const uint64_t* secret, const uint64_t* response, uint64_t* msg);
+
+/* offline giant-key derivation (twosteps.cpp:1588-1863, giant-step half of genAutoKeyFromOffline)
+     offline2 [10][2][N]  keyMap.at(5^N1) of AutoKeyBSGSRNS(N, crtMod, auxMod, 5, 2^14, 2^14), NTT form
+     auto2    [6][2][N]   keyMap.at(5^N1) of AutoKeyBSGSRNS(N, crtMod, auxMod, 3, 2^11*auxMod, 2^15), NTT form
+     chain    [N2-2][6][2][N]  keys added to the auto key at 5^(k*N1), k = 2..N2-1, coefficient form (may be NULL)
+     giant    [N2-1][3][2][N]  result.keyMap at 5^(k*N1), k = 1..N2-1, NTT form mod crtMod */
+void ko_crt_inv_bsgs_aux(uint64_t* out_lohi /*[N][2]*/, const uint64_t* in_q, const uint64_t* in_aux);
+void ko_decompose_bsgs_aux(uint64_t* out_q, uint64_t* out_aux, const uint64_t* in_q, const uint64_t* in_aux,
+                           int ell, uint64_t base, uint64_t bg);
+int ko_gen_offline_giant(int n1, unsigned seed, uint64_t* secret, uint64_t* offline2, uint64_t* auto2);
+int ko_giant_from_offline(int n1, const uint64_t* offline2, const uint64_t* auto2, uint64_t* chain,
+                          uint64_t* giant);
+/* b - a*s in coefficient form mod crtMod (modsel 0) or auxMod (modsel 5) */
+int ko_rlwe_phase(int modsel, const uint64_t* secret, const uint64_t* a, const uint64_t* b, int ntt_form,
+                  uint64_t* out);
 
 #ifdef __cplusplus
 }

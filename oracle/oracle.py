@@ -291,3 +291,51 @@ def ref_db_product(db_block, cbuf, n1, n2):
     out = np.zeros((n2, 2 * N), dtype=np.uint64)
     R.ref_db_product(p64(out), p64(np.ascontiguousarray(db_block)), p64(np.ascontiguousarray(cbuf)), n1, n2)
     return out
+
+
+def crt_inv_bsgs_aux(xq, xaux):
+    out = np.zeros((N, 2), dtype=np.uint64)
+    lib().ko_crt_inv_bsgs_aux(p64(out), p64(np.ascontiguousarray(xq, dtype=np.uint64)),
+                              p64(np.ascontiguousarray(xaux, dtype=np.uint64)))
+    return [int(lo) | (int(hi) << 64) for lo, hi in out]
+
+
+def decompose_bsgs_aux(xq, xaux, ell=5, base=1 << 14, bg=1 << 14):
+    oq = np.zeros((ell, N), dtype=np.uint64)
+    oa = np.zeros((ell, N), dtype=np.uint64)
+    lib().ko_decompose_bsgs_aux(p64(oq), p64(oa), p64(np.ascontiguousarray(xq, dtype=np.uint64)),
+                                p64(np.ascontiguousarray(xaux, dtype=np.uint64)), ell,
+                                ctypes.c_uint64(base), ctypes.c_uint64(bg))
+    return oq, oa
+
+
+def gen_offline_giant(n1, seed):
+    """srand(seed), Secret(crtMod), then the (crtMod, auxMod) offline key
+    (ell 5, 2^14, 2^14) and auto key (ell 3, 2^11*auxMod, 2^15) at 5^N1"""
+    secret = np.zeros(N, dtype=np.uint64)
+    off = np.zeros((10, 2, N), dtype=np.uint64)
+    auto = np.zeros((6, 2, N), dtype=np.uint64)
+    rc = lib().ko_gen_offline_giant(n1, ctypes.c_uint(seed), p64(secret), p64(off), p64(auto))
+    assert rc == 0
+    return secret, off, auto
+
+
+def giant_from_offline(n1, offline2, auto2, want_chain=True):
+    n2 = N // 2 // n1
+    chain = np.zeros((max(n2 - 2, 0), 6, 2, N), dtype=np.uint64) if want_chain else None
+    giant = np.zeros((n2 - 1, 3, 2, N), dtype=np.uint64)
+    null = ctypes.cast(None, _u64p)
+    rc = lib().ko_giant_from_offline(n1, p64(np.ascontiguousarray(offline2, dtype=np.uint64)),
+                                     p64(np.ascontiguousarray(auto2, dtype=np.uint64)),
+                                     p64(chain) if want_chain and chain.size else null, p64(giant))
+    assert rc == 0
+    return chain, giant
+
+
+def rlwe_phase(modsel, secret, a, b, ntt_form):
+    out = np.zeros(N, dtype=np.uint64)
+    rc = lib().ko_rlwe_phase(modsel, p64(np.ascontiguousarray(secret, dtype=np.uint64)),
+                             p64(np.ascontiguousarray(a, dtype=np.uint64)),
+                             p64(np.ascontiguousarray(b, dtype=np.uint64)), int(ntt_form), p64(out))
+    assert rc == 0
+    return out

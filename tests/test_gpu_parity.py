@@ -40,7 +40,8 @@ def upload(ctx, inst):
     return db, keys
 
 
-@pytest.mark.parametrize("prime,modsel,q", [(0, 3, 268369921), (1, 4, 249561089), (2, 1, 16760833)])
+@pytest.mark.parametrize("prime,modsel,q", [(0, 3, 268369921), (1, 4, 249561089), (2, 1, 16760833),
+                                             (4, 5, 268361729)])
 def test_ntt_matches_oracle(ctx, oracle, prime, modsel, q):
     say(f"[gpu] ntt prime={prime}")
     rng = np.random.default_rng(prime)
@@ -427,6 +428,26 @@ def test_recover_on_gpu(ctx, oracle):
     got = ctx.decrypt_bsgs_total(r, secrets, resps)
     for c in range(3):
         assert np.array_equal(got[c], oracle.decrypt(r, secrets[c], resps[c])), c
+
+
+@pytest.mark.parametrize("n1,seeds", [(64, (31, 32)), (512, (33, 34, 35)), (1024, (36,))])
+def test_offline_giant_keys_match_oracle(ctx, oracle, n1, seeds):
+    """giant-step half of genAutoKeyFromOffline on the GPU, for several
+    clients in one call: every chain key (coefficient form, both RNS parts) and
+    every switched giant key (NTT form mod crtMod) equals the oracle's word for
+    word; the clients' results do not depend on being batched"""
+    say(f"[gpu] offline giant keys n1={n1} clients={len(seeds)}")
+    gens = [oracle.gen_offline_giant(n1, s) for s in seeds]
+    off = np.stack([g[1] for g in gens])
+    auto = np.stack([g[2] for g in gens])
+    chain, giant = ctx.gen_giant_keys_from_offline(n1, off, auto)
+    for c in range(len(seeds)):
+        wc, wg = oracle.giant_from_offline(n1, off[c], auto[c])
+        assert np.array_equal(giant[c], wg), c
+        if wc is not None and wc.size:
+            assert np.array_equal(chain[c], wc), c
+    _, g1 = ctx.gen_giant_keys_from_offline(n1, off[-1:], auto[-1:], want_chain=False)
+    assert np.array_equal(g1[0], giant[-1])
 
 
 def test_graph_replays_match_eager(ctx, oracle):

@@ -373,3 +373,109 @@ def test_reference_crtbamod_composition_overflows():
         u = compose(x1, y2, aux, 48735363086513670, 73083388, aux_two128)          # crt.h:56, 63
         wrong_aux += u % Q != x1 or u % aux != y2
     assert wrong_ba == 200 and wrong_aux == 0
+
+
+# ---------------------------------------------------------------- offline giant-step key derivation
+
+AUX = 268361729
+
+
+def _auto_secret(s, g, q):
+    out = [0] * N
+    for i, v in enumerate(s):
+        d = (i * g) % (2 * N)
+        if d >= N:
+            out[d - N] = (q - int(v)) % q
+        else:
+            out[d] = int(v)
+    return out
+
+
+def _centered(x, q):
+    x %= q
+    return x - q if x > q // 2 else x
+
+
+def test_aux_constants_and_root(oracle):
+    """crt.h:37,56,63 constants and the auxMod NTT root (lwe.cpp:1016-1019,
+    HEXL's minimal primitive 2N-th root)"""
+    Q = oracle.Q
+    assert oracle.const(10) == AUX
+    for a in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):   # deterministic Miller-Rabin below 3.3e24
+        d, s = AUX - 1, 0
+        while d % 2 == 0:
+            d, s = d // 2, s + 1
+        x = pow(a, d, AUX)
+        assert x in (1, AUX - 1) or any(pow(x, 2 ** t, AUX) == AUX - 1 for t in range(1, s))
+    assert oracle.const(11) * AUX % Q == 1
+    assert oracle.const(12) * Q % AUX == 1
+    root = oracle.const(13)
+    assert pow(root, N, AUX) == AUX - 1
+    assert all(pow(x, N, AUX) != AUX - 1 for x in range(2, root))
+
+
+def test_crt_inv_bsgs_aux_is_the_crt_composition(oracle):
+    """utils.cpp:535-553 with utils.cpp:462-488: exact CRT into [0, crtMod*auxMod),
+    also for the non-canonical zeros q - 0 an automorphism writes"""
+    Q = oracle.Q
+    rng = np.random.default_rng(3)
+    xq = rng.integers(0, Q, N, dtype=np.uint64)
+    xa = rng.integers(0, AUX, N, dtype=np.uint64)
+    xq[:4] = [0, Q, Q - 1, 0]
+    xa[:4] = [0, AUX, 0, AUX - 1]
+    got = oracle.crt_inv_bsgs_aux(xq, xa)
+    M = Q * AUX
+    for i in range(N):
+        v = got[i]
+        assert 0 <= v < M and v % Q == int(xq[i]) % Q and v % AUX == int(xa[i]) % AUX, i
+
+
+def test_decompose_bsgs_aux_recombines(oracle):
+    """utils.cpp:1191-1256 (ell 5, PP = Bg = 2^14): signed digits in
+    [-2^13, 2^13) whose gadget sum is the composed value to within PP/2 (mod M)"""
+    Q = oracle.Q
+    M = Q * AUX
+    rng = np.random.default_rng(4)
+    xq = rng.integers(0, Q, N, dtype=np.uint64)
+    xa = rng.integers(0, AUX, N, dtype=np.uint64)
+    xq[:3] = [0, Q - 1, 0]
+    xa[:3] = [0, AUX - 1, 0]
+    dq, da = oracle.decompose_bsgs_aux(xq, xa)
+    comp = oracle.crt_inv_bsgs_aux(xq, xa)
+    for i in range(N):
+        s = 0
+        for j in range(5):
+            r = _centered(int(dq[j, i]), Q)
+            assert -(1 << 13) <= r < (1 << 13)
+            assert int(da[j, i]) == r % AUX
+            s += r << (14 + 14 * j)
+        assert abs(_centered(s - comp[i], M)) <= 1 << 13, i
+
+
+@pytest.mark.parametrize("n1", [128, 1024])
+def test_offline_giant_keys_decrypt(oracle, n1):
+    """twosteps.cpp:1803-1863: every key of the derivation chain (5^(k N1),
+    k >= 2) decrypts under the secret to tau(s) * 2^(11+15j) * auxMod plus a
+    noise that is the same integer in its crtMod and auxMod parts; every
+    switched key for k >= 2 decrypts mod crtMod to tau(s) * 2^(11+15j) plus a
+    small noise.  The key at k = 1 is switched from the stored NTT-form words,
+    as the reference does, and does not decrypt."""
+    Q = oracle.Q
+    s, off, auto = oracle.gen_offline_giant(n1, 5)
+    chain, giant = oracle.giant_from_offline(n1, off, auto)
+    n2 = N // 2 // n1
+    assert chain.shape[0] == n2 - 2 and giant.shape[0] == n2 - 1
+    for k in range(2, n2):
+        ts = _auto_secret(s, pow(5, k * n1, 2 * N), Q)
+        for j in range(3):
+            m = (1 << (11 + 15 * j)) * AUX
+            e1 = oracle.rlwe_phase(0, s, chain[k - 2, j, 0], chain[k - 2, j, 1], False)
+            e2 = oracle.rlwe_phase(5, s, chain[k - 2, j + 3, 0], chain[k - 2, j + 3, 1], False)
+            eg = oracle.rlwe_phase(0, s, giant[k - 1, j, 0], giant[k - 1, j, 1], True)
+            for i in range(0, N, 5):
+                e = _centered(int(e1[i]) - ts[i] * (m % Q), Q)
+                assert abs(e) < 1 << 32 and e % AUX == int(e2[i]), (k, j, i)
+                assert abs(_centered(int(eg[i]) - ts[i] * (1 << (11 + 15 * j)), Q)) < 1 << 16, (k, j, i)
+    ts = _auto_secret(s, pow(5, n1, 2 * N), Q)
+    e0 = oracle.rlwe_phase(0, s, giant[0, 0, 0], giant[0, 0, 1], True)
+    assert max(abs(_centered(int(e0[i]) - ts[i] * (1 << 11), Q)) for i in range(N)) > 1 << 40
