@@ -160,7 +160,7 @@ int kspir_answer_inspect(kspir_context* ctx, const kspir_database* db, const ksp
 int kspir_decrypt_bsgs_total(kspir_context* ctx, int32_t r, int32_t nq, const uint64_t* secrets,
                              const uint64_t* responses, uint64_t* messages);
 
-/* Giant-step half of genAutoKeyFromOffline (twosteps.h, twosteps.cpp:1803-1863,
+/* Giant-step half of genAutoKeyFromOffline (twosteps.h, twosteps.cpp:1814-1863,
  * driven by tests/test-two-steps-key.cpp:118-146) for nc clients at once, on
  * the GPU: the chain of evalAutoRNSKey (twosteps.cpp:1588-1715, auxMod
  * branch) by 5^N1 from the auto key at 5^N1, then the modulus switch to

@@ -1,5 +1,5 @@
 // kspir_offline.h — giant-step half of the offline key derivation
-// (genAutoKeyFromOffline, twosteps.cpp:1803-1863) on the GPU, for many clients
+// (genAutoKeyFromOffline, twosteps.cpp:1814-1863) on the GPU, for many clients
 // at once.
 //
 // A key of AutoKeyBSGSRNS(N, crtMod, auxMod, ...) is held as u32 planes in the
@@ -177,7 +177,7 @@ __global__ void __launch_bounds__(kNttThreads) k_okd_ks(const uint32_t* __restri
     }
 }
 
-// twosteps.cpp:1836-1862: key k of every client switched to crtMod,
+// twosteps.cpp:1839-1862: key k of every client switched to crtMod,
 // auxMod^-1 * (x - x_aux) mod crtMod, and transformed; both primes in one
 // workgroup (one per 256-thread half) so the words compose to u64.
 // st: [nc][n2-1][3][2][3][N]; giant: [nc][n2-1][3][2][N] u64.

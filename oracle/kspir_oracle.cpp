@@ -1123,7 +1123,7 @@ void eval_auto_rns_key(Rlwe& r1, Rlwe& r2, Rlwe in1, Rlwe in2, int32_t index, co
     ntt_aux().inverse(r2.b.data());
 }
 
-// twosteps.cpp:1803-1863, the giant-step half of genAutoKeyFromOffline.
+// twosteps.cpp:1814-1863, the giant-step half of genAutoKeyFromOffline.
 //   auto2    [6][2][N]   keyMap.at(5^N1) of AutoKeyBSGSRNS(N, crtMod, auxMod, 3, 2^11*auxMod, 2^15), NTT form
 //   offline2 [10][2][N]  keyMap.at(5^N1) of AutoKeyBSGSRNS(N, crtMod, auxMod, 5, 2^14, 2^14), NTT form
 //   chain    [N2-2][6][2][N]  keys the reference inserts into result2 at 5^(k*N1), k = 2..N2-1

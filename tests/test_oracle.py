@@ -454,7 +454,7 @@ def test_decompose_bsgs_aux_recombines(oracle):
 
 @pytest.mark.parametrize("n1", [128, 1024])
 def test_offline_giant_keys_decrypt(oracle, n1):
-    """twosteps.cpp:1803-1863: every key of the derivation chain (5^(k N1),
+    """twosteps.cpp:1814-1863: every key of the derivation chain (5^(k N1),
     k >= 2) decrypts under the secret to tau(s) * 2^(11+15j) * auxMod plus a
     noise that is the same integer in its crtMod and auxMod parts; every
     switched key for k >= 2 decrypts mod crtMod to tau(s) * 2^(11+15j) plus a
