@@ -448,6 +448,9 @@ def test_offline_giant_keys_match_oracle(ctx, oracle, n1, seeds):
             assert np.array_equal(chain[c], wc), c
     _, g1 = ctx.gen_giant_keys_from_offline(n1, off[-1:], auto[-1:], want_chain=False)
     assert np.array_equal(g1[0], giant[-1])
+    import kspir_amd
+    with pytest.raises(kspir_amd.KspirError, match="N1"):
+        ctx.gen_giant_keys_from_offline(2048, off[:1], auto[:1])   # N2 = 1: no giant steps
 
 
 def test_graph_replays_match_eager(ctx, oracle):
