@@ -111,6 +111,22 @@ int kspir_answer(kspir_context* ctx, const kspir_database* db, const kspir_keys*
 int kspir_answer_device(kspir_context* ctx, const kspir_database* db, const kspir_keys* keys,
                         const uint64_t* query_dev, uint64_t* response_dev, void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): kspir_answer_device with the same arguments and the
+ * same response words, but it calls no hipMalloc, hipFree or allocating
+ * library routine.  Its workspace is a fixed 64 MiB device array per GPU
+ * (held from context creation until the process ends), so shapes whose
+ * single-query workspace exceeds it are rejected with KSPIR_EINVAL: r 16 at
+ * N1 128 (the 256 MB test-pir shape) and r 16 at N1 256 fit, r 32 at N1 128
+ * does not.  Launches are eager (no graph) and use the VALU database pass.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (two streams, two contexts or two threads without ordering between
+ * them) race and give undefined responses; order them on one stream or
+ * synchronize between them.  Asynchronous, like kspir_answer_device. */
+int kspir_answer_device_noalloc(kspir_context* ctx, const kspir_database* db, const kspir_keys* keys,
+                                const uint64_t* query_dev, uint64_t* response_dev, void* stream);
+
 /* Batched online answer: nq queries, each with its own client keys (all for
  * this database's r and N1), against one resident database.  Every database
  * tile read from HBM serves the whole batch.  queries [nq][2][2][N] ->

@@ -43,7 +43,7 @@ def _load():
                  "kspir_server_create", "kspir_server_submit", "kspir_server_wait", "kspir_server_stats",
                  "kspir_query_footprint_bytes", "kspir_decrypt_bsgs_total", "kspir_context_get",
                  "kspir_autotune", "kspir_client_generate", "kspir_client_draws", "kspir_client_encrypt",
-                 "kspir_server_create_lanes", "kspir_gen_giant_keys_from_offline"):
+                 "kspir_server_create_lanes", "kspir_gen_giant_keys_from_offline", "kspir_answer_device_noalloc"):
         getattr(L, name).restype = ctypes.c_int
     L.kspir_autotune.argtypes = [_vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                  ctypes.POINTER(ctypes.c_int32), ctypes.c_char_p, ctypes.c_size_t]
@@ -69,7 +69,7 @@ EXPORTS = (
     "kspir_server_create", "kspir_server_submit", "kspir_server_wait", "kspir_server_stats", "kspir_server_destroy",
     "kspir_query_footprint_bytes", "kspir_decrypt_bsgs_total", "kspir_context_get", "kspir_autotune",
     "kspir_client_generate", "kspir_client_draws", "kspir_client_encrypt", "kspir_server_create_lanes",
-    "kspir_gen_giant_keys_from_offline",
+    "kspir_gen_giant_keys_from_offline", "kspir_answer_device_noalloc",
 )
 
 
@@ -311,6 +311,12 @@ class Context:
     def answer_device(self, db, keys, query_buf, resp_buf, stream=None):
         _check(lib.kspir_answer_device(self.h, db.h, keys.h, ctypes.cast(_vp(query_buf.ptr), _u64p),
                                        ctypes.cast(_vp(resp_buf.ptr), _u64p), _vp(stream)))
+
+    def answer_device_noalloc(self, db, keys, query_buf, resp_buf, stream=None):
+        """kspir_answer_device_noalloc: the same answer on a fixed per-GPU
+        scratch array, with no device allocation (asynchronous)"""
+        _check(lib.kspir_answer_device_noalloc(self.h, db.h, keys.h, ctypes.cast(_vp(query_buf.ptr), _u64p),
+                                               ctypes.cast(_vp(resp_buf.ptr), _u64p), _vp(stream)))
 
     def answer_batch(self, db, keys_list, queries):
         """queries [nq][2][2][N] with one Keys per query -> responses [nq][2][N]"""

@@ -83,6 +83,20 @@ int ilog2(int v) { int b = 0; while (v > 1) { v >>= 1; ++b; } return b; }
 
 }  // namespace
 
+// --------------------------------------------- no-allocation answer scratch
+// Workspace of kspir_answer_device_noalloc: the buffers of one single-query
+// answer, up to the 256 MB test-pir shape (r 16, N1 128 needs 61.6 MB).  It
+// is a file-scope device array, so every device the library runs on holds
+// its own copy until the process ends.  In host code `scratch` only names
+// the symbol: its address on each device comes from hipGetSymbolAddress at
+// context creation (with that device current), and host code passes only
+// that per-device pointer on.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) uint32_t scratch[kScratchBytes / 4];
+static std::mutex g_scratch_mu;
+static uint32_t* g_scratch_ptr[kMaxDevices] = {};
+
 // ------------------------------------------------------------------ handles
 
 struct kspir_context {
@@ -278,8 +292,12 @@ size_t query_footprint_bytes(int r, int n1) {
     return words * 4 + mfma;
 }
 
-// carve the workspace for nq queries of shape (r, n1); grows on demand
-int carve_workspace(kspir_context* c, int r, int n1, int nq) {
+// Layout of the answer workspace for nq queries of shape (r, n1): sets c->S,
+// and every buffer pointer of the context when base is not null; returns
+// the size in 32-bit words.  answer_only leaves out the host-transfer and
+// inspection areas (q_dev, resp_dev, u64tmp, u64tmp2), which the answer
+// kernels themselves never touch.
+size_t layout_workspace(kspir_context* c, int r, int n1, int nq, uint32_t* base, bool answer_only) {
     const int n2 = kN / 2 / n1;
     const size_t n = kN;
     const size_t rh = r / 2 > 0 ? r / 2 : 1;
@@ -299,11 +317,28 @@ int carve_workspace(kspir_context* c, int r, int n1, int nq) {
         {Q * S.icoef, &c->icoef}, {Q * S.ctw, &c->ctw}, {Q * S.dbout, &c->dbout}, {Q * S.gcoef, &c->gcoef},
         {Q * S.gks, &c->gks}, {Q * S.fd, &c->fd}, {Q * S.ecoef, &c->ecoef}, {Q * S.epart, &c->epart},
         {Q * S.ext, &c->extA}, {Q * S.ext, &c->extB}, {Q * S.pcoef, &c->pcoef}, {Q * S.ppart, &c->ppart},
-        {Q * 2 * S.qin64, (uint32_t**)&c->q_dev}, {Q * 2 * S.resp64, (uint32_t**)&c->resp_dev},
-        {R2 * 4 * n, (uint32_t**)&c->u64tmp}, {(size_t)n1 * 4 * n, (uint32_t**)&c->u64tmp2},
     };
+    if (!answer_only) {
+        parts.push_back({Q * 2 * S.qin64, (uint32_t**)&c->q_dev});
+        parts.push_back({Q * 2 * S.resp64, (uint32_t**)&c->resp_dev});
+        parts.push_back({R2 * 4 * n, (uint32_t**)&c->u64tmp});
+        parts.push_back({(size_t)n1 * 4 * n, (uint32_t**)&c->u64tmp2});
+    }
     size_t total = 0;
-    for (auto& p : parts) total += up(p.words);
+    for (auto& p : parts) {
+        if (base) *p.dst = base + total;
+        total += up(p.words);
+    }
+    if (base) {
+        if (answer_only) { c->q_dev = nullptr; c->resp_dev = nullptr; c->u64tmp = nullptr; c->u64tmp2 = nullptr; }
+        c->S = S;
+    }
+    return total;
+}
+
+// carve the workspace for nq queries of shape (r, n1); grows on demand
+int carve_workspace(kspir_context* c, int r, int n1, int nq) {
+    const size_t total = layout_workspace(c, r, n1, nq, nullptr, false);
     if (total * 4 > c->ws_bytes) {
         if (c->ws) {
             HIP_OK(hipStreamSynchronize(c->stream));
@@ -316,9 +351,7 @@ int carve_workspace(kspir_context* c, int r, int n1, int nq) {
         HIP_OK(hipMalloc(&c->ws, total * 4));
         c->ws_bytes = total * 4;
     }
-    uint32_t* base = (uint32_t*)c->ws;
-    for (auto& p : parts) { *p.dst = base; base += up(p.words); }
-    c->S = S;
+    layout_workspace(c, r, n1, nq, (uint32_t*)c->ws, false);
     return KSPIR_OK;
 }
 
@@ -476,7 +509,8 @@ int enqueue_packing(kspir_context* c, const KeyPtrs* kp, int r, int nq, hipStrea
 // Optional inspection outputs for query 0: fd_u64 [r][2][N] (NTT form),
 // ext_u64 [r][2][N] (coefficient form).
 int enqueue_answer(kspir_context* c, const kspir_database* db, const KeyPtrs* kp, int nq, const uint64_t* q_dev,
-                   uint64_t* resp, hipStream_t st, uint64_t* fd_u64, uint64_t* ext_u64, bool first_dim_only) {
+                   uint64_t* resp, hipStream_t st, uint64_t* fd_u64, uint64_t* ext_u64, bool first_dim_only,
+                   bool allow_mfma = true) {
     const int r = db->r, n1 = db->n1, n2 = db->n2, n1h = n1 / 2;
     const Tabs& T = c->T;
     const QS& S = c->S;
@@ -499,7 +533,7 @@ int enqueue_answer(kspir_context* c, const kspir_database* db, const KeyPtrs* kp
     LAUNCH_OK("k_baby_rot_pair");
     // database pass (crt.cpp:643-768): int8-MFMA limb GEMM for a batch,
     // u64-accumulating VALU stream for a single query
-    int rc = (nq >= c->mfma_min_batch && mfma_shape(db)) ? launch_db_batched(c, db, nq, st)
+    int rc = (allow_mfma && nq >= c->mfma_min_batch && mfma_shape(db)) ? launch_db_batched(c, db, nq, st)
                                                           : launch_db_product(c, db, nq, st);
     if (rc) return rc;
     // giant steps (twosteps.cpp:599-677) and the first-dimension sum: one
@@ -680,7 +714,18 @@ int kspir_context_create(int device, kspir_context** out) {
     int n = 0;
     HIP_OK(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return fail(KSPIR_EINVAL, "no such device");
+    if (device >= kMaxDevices) return fail(KSPIR_EINVAL, "device index above the library's limit of 64");
     HIP_OK(hipSetDevice(device));
+    {
+        // this device's copy of the no-allocation scratch array
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (!g_scratch_ptr[device]) {
+            void* p = nullptr;
+            HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+            if (!p) return fail(KSPIR_EHIP, "hipGetSymbolAddress(scratch) returned a null address");
+            g_scratch_ptr[device] = (uint32_t*)p;
+        }
+    }
     kspir_context* c = new kspir_context();
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -972,6 +1017,31 @@ int kspir_answer_device(kspir_context* c, const kspir_database* db, const kspir_
     if (rc) return rc;
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
     return run_answer(c, db, k->dev_ptrs, 1, query_dev, response_dev, st);
+}
+
+int kspir_answer_device_noalloc(kspir_context* c, const kspir_database* db, const kspir_keys* k,
+                                const uint64_t* query_dev, uint64_t* response_dev, void* stream) {
+    if (!c || !db || !k) return fail(KSPIR_EINVAL, "null handle");
+    const size_t need = layout_workspace(c, db->r, db->n1, 1, nullptr, true) * 4;
+    if (need > kScratchBytes)
+        return fail(KSPIR_EINVAL, "kspir_answer_device_noalloc: shape r=" + std::to_string(db->r) + " N1=" +
+                                      std::to_string(db->n1) + " needs " + std::to_string(need) +
+                                      " bytes of workspace, above its fixed scratch of " +
+                                      std::to_string(kScratchBytes) + " bytes (the r 16, N1 128 shape fits)");
+    int rc = check_pair(c, db, k);
+    if (rc) return rc;
+    if (!query_dev || !response_dev) return fail(KSPIR_EINVAL, "null buffer");
+    HIP_OK(hipSetDevice(c->device));
+    uint32_t* ptr = g_scratch_ptr[c->device];   // set when the context was created
+    if (!ptr) return fail(KSPIR_EINVAL, "no scratch address for this device");
+    // every buffer the kernels use is written before it is read, so stale
+    // contents of an earlier call need no clearing
+    layout_workspace(c, db->r, db->n1, 1, ptr, true);
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    if (st != c->stream) c->last_stream = st;
+    // eager launches only (no graph capture), VALU database pass (the MFMA
+    // pass keeps lazily allocated limbs)
+    return enqueue_answer(c, db, k->dev_ptrs, 1, query_dev, response_dev, st, nullptr, nullptr, false, false);
 }
 
 int kspir_answer_batch_device(kspir_context* c, const kspir_database* db, const kspir_keys* const* keys, int32_t nq,
