@@ -50,8 +50,8 @@ typedef struct kspir_keys kspir_keys;
 const char* kspir_last_error(void);
 
 /* compute_permutation_matrix (utils.h:129, utils.cpp:1591-1619): out [n1][N],
- * row i maps NTT slots so that NTT(tau_{5^i}(a))[s] = NTT(a)[out[i][s]].
- * Host-only; needs no device. */
+ * row i maps NTT slots so that NTT(tau_{5^i}(a))[s] = NTT(a)[out[i][s]];
+ * n1 in [1, N/2].  Host-only closed form; needs no device. */
 int kspir_compute_permutation_matrix(int32_t* out, int32_t n1);
 
 /* device context: twiddle tables, stream and scratch for one device */
@@ -82,6 +82,13 @@ int kspir_database_create_from_records(kspir_context* ctx, const uint64_t* recor
  * [N][N/2] record values < p (host), result [N/2][N] values mod Q in NTT
  * form (host), as the reference returns them. */
 int kspir_database_tobsgsntt(kspir_context* ctx, const uint64_t* data, int32_t n1, uint64_t* result);
+
+/* inverse_encode (twosteps.h:22, twosteps.cpp:182-198) of `count` record
+ * columns on the GPU: messages [count][N] values < p = 65537 (host), each
+ * replaced by its forward NTT mod p (HEXL's NTT(N, p), minimal root) in the
+ * slot order of compute_query_decode (utils.cpp:1677-1685).  Values >= p
+ * are rejected with KSPIR_EINVAL. */
+int kspir_inverse_encode(kspir_context* ctx, uint64_t* messages, int32_t count);
 
 /* Copy a resident database back to the host in the reference's datacrt
  * layout (crt.cpp:859-885: per block [N][N2][N1] words): the persisted form

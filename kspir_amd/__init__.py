@@ -43,7 +43,8 @@ def _load():
                  "kspir_server_create", "kspir_server_submit", "kspir_server_wait", "kspir_server_stats",
                  "kspir_query_footprint_bytes", "kspir_decrypt_bsgs_total", "kspir_context_get",
                  "kspir_autotune", "kspir_client_generate", "kspir_client_draws", "kspir_client_encrypt",
-                 "kspir_server_create_lanes", "kspir_gen_giant_keys_from_offline", "kspir_answer_device_noalloc"):
+                 "kspir_server_create_lanes", "kspir_gen_giant_keys_from_offline", "kspir_answer_device_noalloc",
+                 "kspir_inverse_encode"):
         getattr(L, name).restype = ctypes.c_int
     L.kspir_autotune.argtypes = [_vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                  ctypes.POINTER(ctypes.c_int32), ctypes.c_char_p, ctypes.c_size_t]
@@ -69,7 +70,7 @@ EXPORTS = (
     "kspir_server_create", "kspir_server_submit", "kspir_server_wait", "kspir_server_stats", "kspir_server_destroy",
     "kspir_query_footprint_bytes", "kspir_decrypt_bsgs_total", "kspir_context_get", "kspir_autotune",
     "kspir_client_generate", "kspir_client_draws", "kspir_client_encrypt", "kspir_server_create_lanes",
-    "kspir_gen_giant_keys_from_offline", "kspir_answer_device_noalloc",
+    "kspir_gen_giant_keys_from_offline", "kspir_answer_device_noalloc", "kspir_inverse_encode",
 )
 
 
@@ -286,6 +287,12 @@ class Context:
         if not on_device:
             records = np.ascontiguousarray(records, dtype=np.uint64)
         return Database(self, records, r, n1, on_device, records=True, block_stride=block_stride)
+
+    def inverse_encode(self, columns):
+        """inverse_encode of record columns on the GPU: [count][N] values < p"""
+        out = np.ascontiguousarray(columns, dtype=np.uint64).reshape(-1, N).copy()
+        _check(lib.kspir_inverse_encode(self.h, _p64(out), out.shape[0]))
+        return out.reshape(np.shape(columns))
 
     def database_tobsgsntt(self, data, n1):
         """database_tobsgsntt of one block on the GPU: [N][N/2] -> [N/2][N] mod Q"""

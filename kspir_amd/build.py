@@ -34,6 +34,28 @@ def build_dropin_test(verbose=False):
     return out
 
 
+REF_TEST_PIR = "/root/reference/tests/test-pir.cpp"
+
+
+def build_reference_driver(verbose=False):
+    """tests/native/_build/test_pir: the reference's own tests/test-pir.cpp,
+    compiled from where it lies, unchanged, against the drop-in headers
+    (kspir_amd/cpp/compat: pir.h, crt.h) and linked with libkspir_amd.so.
+    Skipped when the reference tree is absent; the binary is never committed."""
+    if not os.path.exists(REF_TEST_PIR):
+        return None
+    outdir = os.path.join(ROOT, "tests", "native", "_build")
+    os.makedirs(outdir, exist_ok=True)
+    out = os.path.join(outdir, "test_pir")
+    cmd = ["g++", "-O2", "-std=c++17", "-w", REF_TEST_PIR, "-I", os.path.join(HERE, "cpp", "compat"),
+           "-I", os.path.join(HERE, "cpp"), "-L", HERE, "-lkspir_amd",
+           "-Wl,-rpath," + os.path.relpath(HERE, outdir).join(["$ORIGIN/", ""]), "-o", out]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    return out
+
+
 def build(verbose=False):
     cmd = [hipcc(), "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -44,6 +66,7 @@ def build(verbose=False):
     subprocess.check_call(cmd)
     os.replace(OUT + ".tmp", OUT)
     build_dropin_test(verbose)
+    build_reference_driver(verbose)
     return OUT
 
 

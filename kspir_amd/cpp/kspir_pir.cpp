@@ -1,13 +1,15 @@
 // kspir_pir.cpp — the drop-in C++ surface (kspir_pir.h) over the C ABI.
 // Reference objects are flattened into the C ABI's layouts; databases and
 // keys are uploaded on first use and cached per object, keyed by address and
-// a fingerprint of sampled words so that a changed or reallocated object is
+// a hash of every word so that a changed or reallocated object is
 // re-uploaded.
 #include "kspir_pir.h"
 
+#include <atomic>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <thread>
 #include <stdexcept>
 #include <string>
 
@@ -68,13 +70,59 @@ uint32_t pow5(int64_t e) {
     return r;
 }
 
+// 64-bit hash of n words, every word: four independent multiply-xorshift
+// lanes (so the loop is not latency-bound), folded at the end
+uint64_t hash_words(const uint64_t* p, size_t n, uint64_t seed) {
+    uint64_t h[4] = {seed ^ 0x243F6A8885A308D3ULL, seed ^ 0x13198A2E03707344ULL, seed ^ 0xA4093822299F31D0ULL,
+                     seed ^ 0x082EFA98EC4E6C89ULL};
+    size_t i = 0;
+    for (; i + 4 <= n; i += 4)
+        for (int l = 0; l < 4; ++l) {
+            uint64_t x = (h[l] ^ p[i + l]) * 0x9E3779B97F4A7C15ULL;
+            h[l] = x ^ (x >> 29);
+        }
+    for (; i < n; ++i) {
+        uint64_t x = (h[0] ^ p[i]) * 0x9E3779B97F4A7C15ULL;
+        h[0] = x ^ (x >> 29);
+    }
+    uint64_t r = n;
+    for (int l = 0; l < 4; ++l) {
+        uint64_t x = (r ^ h[l]) * 0xBF58476D1CE4E5B9ULL;
+        r = x ^ (x >> 31);
+    }
+    return r;
+}
+
+// the same over a large array, in chunks on the host's threads (a 1 GiB
+// database hashes at memory speed)
+uint64_t hash_words_parallel(const uint64_t* p, size_t n) {
+    const size_t chunk = (size_t)1 << 22;   // 32 MiB of words per task
+    const size_t nc = (n + chunk - 1) / chunk;
+    if (nc <= 1) return hash_words(p, n, 1);
+    std::vector<uint64_t> part(nc);
+    unsigned nt = std::thread::hardware_concurrency();
+    if (nt < 1) nt = 1;
+    if (nt > 16) nt = 16;
+    if (nt > nc) nt = (unsigned)nc;
+    std::atomic<size_t> next(0);
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t)
+        th.emplace_back([&] {
+            for (size_t c; (c = next.fetch_add(1)) < nc;) {
+                const size_t b = c * chunk, e = b + chunk < n ? b + chunk : n;
+                part[c] = hash_words(p + b, e - b, c + 1);
+            }
+        });
+    for (auto& x : th) x.join();
+    return hash_words(part.data(), nc, n);
+}
+
 struct Fp {
     uint64_t h = 1469598103934665603ULL;
     void add(uint64_t v) { h = (h ^ v) * 1099511628211ULL; }
     void poly(const std::vector<uint64_t>& p) {
         add(p.size());
-        for (size_t i = 0; i < p.size(); i += 509) add(p[i]);
-        if (!p.empty()) add(p.back());
+        add(hash_words(p.data(), p.size(), 7));
     }
 };
 
@@ -91,31 +139,30 @@ void put_rlwe(uint64_t* dst, const RlweCiphertext& c) {
     std::memcpy(dst + N, c.b.data(), N * 8);
 }
 
-struct DbEntry { int32_t r, n1; uint64_t fp; kspir_database* h; };
+struct DbEntry { int32_t r, n1; uint64_t fp; kspir_database* h; bool pinned; };
 struct KeyEntry { uint64_t fp; kspir_keys* h; };
 std::map<const void*, DbEntry> g_dbs;
 std::map<std::pair<const void*, uint64_t>, KeyEntry> g_keys;   // (object, shape tag)
 
-uint64_t db_fingerprint(const uint64_t* data, int32_t r) {
-    Fp f;
-    const size_t words = (size_t)r * N * (N / 2);
-    f.add(words);
-    for (size_t i = 0; i < words; i += words / 97 + 1) f.add(data[i]);
-    f.add(data[words - 1]);
-    return f.h;
-}
+std::map<const void*, bool> g_pinned;   // databases the caller declared unchanged
 
 kspir_database* database(uint64_t* data, int32_t r, int32_t n1) {
-    const uint64_t fp = db_fingerprint(data, r);
     auto it = g_dbs.find(data);
+    const bool pinned = g_pinned.count(data) != 0;
+    if (it != g_dbs.end() && pinned && it->second.pinned && it->second.r == r && it->second.n1 == n1)
+        return it->second.h;
+    const uint64_t fp = hash_words_parallel(data, (size_t)r * N * (N / 2));
     if (it != g_dbs.end()) {
-        if (it->second.r == r && it->second.n1 == n1 && it->second.fp == fp) return it->second.h;
+        if (it->second.r == r && it->second.n1 == n1 && it->second.fp == fp) {
+            it->second.pinned = pinned;
+            return it->second.h;
+        }
         kspir_database_destroy(it->second.h);
         g_dbs.erase(it);
     }
     kspir_database* h = nullptr;
     check(kspir_database_create(ctx(), data, r, n1, 0, &h), "kspir_database_create");
-    g_dbs[data] = DbEntry{r, n1, fp, h};
+    g_dbs[data] = DbEntry{r, n1, fp, h, pinned};
     return h;
 }
 
@@ -148,7 +195,7 @@ kspir_keys* bsgs_keys(const AutoKeyBSGSRNS& k, int32_t r, int32_t n1,
     if ((int32_t)perms.size() < n1) throw std::runtime_error("permutations has fewer than N1 rows");
     for (int32_t i = 0; i < n1; ++i) {
         if (perms[i].size() != (size_t)N) throw std::runtime_error("permutation row of wrong length");
-        for (size_t j = 0; j < perms[i].size(); j += 251) f.add((uint64_t)perms[i][j]);
+        for (size_t j = 0; j < perms[i].size(); ++j) f.add((uint64_t)(uint32_t)perms[i][j]);
     }
     const uint64_t tag = ((uint64_t)r << 32) | (uint32_t)n1;
     auto it = g_keys.find(std::make_pair((const void*)&k, tag));
@@ -274,7 +321,8 @@ void packingRLWEs(RlweCiphertext& result, std::vector<RlweCiphertext>& rlwes, co
 
 void database_tobsgsntt(std::vector<std::vector<uint64_t> >& result, std::vector<std::vector<uint64_t> >& data,
                         uint64_t modulus, int32_t N1) {
-    if (modulus != 65537ULL) throw std::runtime_error("database_tobsgsntt: records must be mod p = 65537");
+    if (modulus != kspir_params::crtMod)
+        throw std::runtime_error("database_tobsgsntt: the lift modulus must be crtMod (twosteps.cpp:155-166)");
     if (data.size() != (size_t)N || result.size() != (size_t)N / 2)
         throw std::runtime_error("database_tobsgsntt: data must be N x N/2, result N/2 x N");
     std::vector<uint64_t> flat((size_t)N * (N / 2)), out((size_t)(N / 2) * N);
@@ -502,9 +550,55 @@ void decrypt_bsgs_total(std::vector<uint64_t>& message, RlweCiphertext& cipher, 
     }
 }
 
+// ------------------------------------------------- database sampling, helpers
+
+void sample_database_bsgs(std::vector<std::vector<uint64_t> >& data) {
+    if (data.size() < (size_t)N) throw std::runtime_error("sample_database_bsgs: data needs N rows");
+    for (int i = 0; i < N; ++i) {
+        if (data[i].size() < (size_t)N / 2) throw std::runtime_error("sample_database_bsgs: short data row");
+        for (int j = 0; j < N / 2; ++j) {
+            uint64_t r;
+            do {
+                r = (uint64_t)rand() & 0xffff;
+            } while (r > 65537ULL);   // twosteps.cpp:31-35 (bsgsp)
+            data[i][j] = r;
+        }
+    }
+}
+
+void inverse_encode(std::vector<uint64_t>& message) {
+    if (message.size() != (size_t)N) throw std::runtime_error("inverse_encode: message must hold N values");
+    std::lock_guard<std::mutex> lock(g_mu);
+    RandShield shield;
+    check(kspir_inverse_encode(ctx(), message.data(), 1), "inverse_encode");
+}
+
+void compute_permutation_matrix(std::vector<std::vector<int32_t> >& permutations, const int32_t max_indexs,
+                                const int32_t length) {
+    if (length != N) throw std::runtime_error("compute_permutation_matrix: length must be N");
+    if (max_indexs < 1 || max_indexs > N / 2)
+        throw std::runtime_error("compute_permutation_matrix: max_indexs must be in [1, N/2]");
+    if (permutations.size() < (size_t)max_indexs)
+        throw std::runtime_error("compute_permutation_matrix: permutations has fewer than max_indexs rows");
+    std::vector<int32_t> flat((size_t)max_indexs * N);
+    check(kspir_compute_permutation_matrix(flat.data(), max_indexs), "compute_permutation_matrix");
+    for (int32_t i = 0; i < max_indexs; ++i) permutations[i].assign(flat.begin() + (size_t)i * N, flat.begin() + (size_t)(i + 1) * N);
+}
+
+void showLargeVector(std::vector<uint64_t>& vals, std::string ss) {
+    std::cout << ss;
+    showLargeVector<uint64_t>(vals);
+}
+
+void kspir_pir_assume_unchanged(const void* p) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    g_pinned[p] = true;
+}
+
 void kspir_pir_forget(const void* p) {
     std::lock_guard<std::mutex> lock(g_mu);
     RandShield shield;
+    g_pinned.erase(p);
     auto it = g_dbs.find(p);
     if (it != g_dbs.end()) {
         kspir_database_destroy(it->second.h);

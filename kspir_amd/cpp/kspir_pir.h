@@ -26,7 +26,9 @@
 #define KSPIR_PIR_H
 
 #include <stdint.h>
+#include <iostream>
 #include <map>
+#include <string>
 #include <vector>
 
 namespace kspir_params {
@@ -170,9 +172,12 @@ void externalProduct(RlweCiphertext& result, RlweCiphertext& firstDim, const RGS
 // consumed; result is the packed RLWE in coefficient form.
 void packingRLWEs(RlweCiphertext& result, std::vector<RlweCiphertext>& rlwes, const AutoKey& autokey);
 
-// twosteps.h:19-21 database_tobsgsntt: data [N][N/2] record values mod
-// modulus (the plaintext p); result [N/2][N] (pre-sized by the caller) gets
-// the rows mod crtMod in NTT form.  Runs on the GPU.
+// twosteps.h:19-21 database_tobsgsntt: data [N][N/2] record values < p;
+// result [N/2][N] (pre-sized by the caller) gets the rows mod crtMod in NTT
+// form.  modulus is the modulus the centred records are lifted into before
+// the transform (twosteps.cpp:155-166): crtMod, as test-pir.cpp:61 passes it
+// (the reference transforms mod crtMod whatever modulus says, so any other
+// value is rejected).  Runs on the GPU.
 void database_tobsgsntt(std::vector<std::vector<uint64_t> >& result, std::vector<std::vector<uint64_t> >& data,
                         uint64_t modulus, int32_t N1);
 
@@ -180,9 +185,39 @@ void database_tobsgsntt(std::vector<std::vector<uint64_t> >& result, std::vector
 // datacrt[z][N2][N1] (a re-layout; host).
 void database_tocrt(uint64_t* datacrt, std::vector<std::vector<uint64_t> >& data_ntt, int32_t N1);
 
-// Drop the device copies made for a database pointer or key object (call
-// after changing their contents in place; destroyed objects are recognised
-// by a fingerprint of their words and re-uploaded automatically).
+// twosteps.h:11, twosteps.cpp:23-39 sample_database_bsgs: data (N rows of at
+// least N/2 entries) gets rand() & 0xffff per record, drawn on the host in the
+// reference's order (row-major), redrawn above p as the reference does (a
+// 16-bit draw never is)
+void sample_database_bsgs(std::vector<std::vector<uint64_t> >& data);
+
+// twosteps.h:22, twosteps.cpp:182-198 inverse_encode: message (N values < p)
+// becomes its forward NTT mod p in the slot order of compute_query_decode.
+// Runs on the GPU.
+void inverse_encode(std::vector<uint64_t>& message);
+
+// utils.h:129, utils.cpp:1591-1619 compute_permutation_matrix: rows
+// 0..max_indexs-1 of permutations (each resized to length) get the NTT-slot
+// permutation of tau_{5^i}; length must be N, max_indexs in [1, N/2].  Host,
+// closed form.
+void compute_permutation_matrix(std::vector<std::vector<int32_t> >& permutations, const int32_t max_indexs,
+                                const int32_t length);
+
+// utils.h:21-23, utils.cpp:40-69 showLargeVector: "[v0, v1, .., v15, ..., vlast]"
+// (every entry when there are at most 16) on std::cout, after the prefix ss
+template <typename T>
+void showLargeVector(std::vector<T>& vals) {
+    std::cout << "[" << vals[0];
+    if (vals.size() <= 16) {
+        for (size_t i = 1; i < vals.size(); ++i) std::cout << ", " << vals[i];
+    } else {
+        for (size_t i = 1; i < 16; ++i) std::cout << ", " << vals[i];
+        std::cout << ", ..., " << vals[vals.size() - 1];
+    }
+    std::cout << "]" << std::endl;
+}
+void showLargeVector(std::vector<uint64_t>& vals, std::string ss = "");
+
 // twosteps.h:26, twosteps.cpp:279-318: cipher[0] (mod crtMod) and cipher[1]
 // (mod bsMod) receive the query for `row`, coefficient form; 5N rand() draws
 void query_bsgs_rns(std::vector<RlweCiphertext>& cipher, Secret& queryKey, uint64_t row);
@@ -192,6 +227,15 @@ void query_bsgs_rns(std::vector<RlweCiphertext>& cipher, Secret& queryKey, uint6
 void decrypt_bsgs_total(std::vector<uint64_t>& message, RlweCiphertext& cipher, Secret& secret,
                         const int32_t rlwesNum);
 
+// Device copies.  A database pointer or key object is uploaded to HBM on
+// first use and reused while its words hash to the same value: every word is
+// hashed on every call, so a database or key edited in place is re-uploaded
+// and never answered from stale device words.
+//   kspir_pir_forget(p)            drops the device copies made for p
+//   kspir_pir_assume_unchanged(p)  the caller promises not to change the
+//                                  database at p until kspir_pir_forget(p):
+//                                  later calls skip hashing its words
 void kspir_pir_forget(const void* object_or_data);
+void kspir_pir_assume_unchanged(const void* data);
 
 #endif

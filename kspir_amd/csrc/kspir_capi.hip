@@ -12,6 +12,7 @@
 #include <string>
 #include <thread>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "kspir_kernels.h"
@@ -688,7 +689,7 @@ const char* kspir_last_error(void) { return g_err.c_str(); }
 __attribute__((visibility("hidden"))) int kspir_set_error(int code, const char* msg) { return fail(code, msg); }
 
 int kspir_compute_permutation_matrix(int32_t* out, int32_t n1) {
-    if (!out || !pow2(n1) || n1 > kN / 2) return fail(KSPIR_EINVAL, "n1 must be a power of two <= N/2");
+    if (!out || n1 < 1 || n1 > kN / 2) return fail(KSPIR_EINVAL, "the row count must be in [1, N/2]");
     // slot s holds the evaluation at psi^e with e = 2*bitrev(s)+1; tau_g maps
     // it to the slot holding psi^(e*g mod 2N)
     std::vector<int32_t> slot_of(kN);
@@ -858,6 +859,44 @@ int kspir_database_create_from_records(kspir_context* c, const uint64_t* records
     if (stage) hipFree(stage);
     hipFree(gtab);
     *out = new kspir_database{c, r, n1, n2, dst};
+    return KSPIR_OK;
+}
+
+int kspir_inverse_encode(kspir_context* c, uint64_t* messages, int32_t count) {
+    if (!c || !messages || count < 0) return fail(KSPIR_EINVAL, "null argument or negative count");
+    if (count == 0) return KSPIR_OK;
+    const size_t words = (size_t)count * kN;
+    for (size_t i = 0; i < words; ++i)
+        if (messages[i] >= kP) return fail(KSPIR_EINVAL, "inverse_encode: record values must be below p = 65537");
+    HIP_OK(hipSetDevice(c->device));
+    // compute_query_decode (utils.cpp:1677-1685): the inverse of the slot
+    // order db_gather_table uses, dec[bitrev(rot_i >> 1)] = i
+    std::vector<uint16_t> dec(kN);
+    {
+        std::vector<uint32_t> rot(kN);
+        rot[0] = 1;
+        rot[kN - 1] = 2 * kN - 1;
+        for (int i = 1; i < kN / 2; ++i) {
+            rot[i] = rot[i - 1] * 5u % (2 * kN);
+            rot[kN - 1 - i] = rot[kN - i] * 5u % (2 * kN);
+        }
+        for (int i = 0; i < kN; ++i) dec[bitrev12(rot[i] >> 1)] = (uint16_t)i;
+    }
+    uint64_t* d = nullptr;
+    uint16_t* dd = nullptr;
+    hipError_t e = hipMalloc(&d, words * 8);
+    if (e == hipSuccess) e = hipMalloc(&dd, kN * 2);
+    if (e == hipSuccess) e = hipMemcpyAsync(dd, dec.data(), kN * 2, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d, messages, words * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_inverse_encode, dim3(count), dim3(kNttThreads), 0, c->stream, d, dd, c->T);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(messages, d, words * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (d) hipFree(d);
+    if (dd) hipFree(dd);
+    if (e != hipSuccess) return fail_hip("inverse_encode", e);
     return KSPIR_OK;
 }
 
@@ -1394,6 +1433,7 @@ struct kspir_server {
     std::condition_variable cv_work, cv_done;
     std::deque<Req> pending;
     std::unordered_map<uint64_t, Done> done;
+    std::unordered_set<uint64_t> live;   // submitted and not yet collected by a wait
     uint64_t next_id = 1, batches = 0, answered = 0;
     bool stop = false;
     std::vector<Lane> lanes;
@@ -1983,6 +2023,7 @@ int kspir_server_submit(kspir_server* s, const kspir_database* db, const kspir_k
         std::lock_guard<std::mutex> lock(s->mu);
         if (s->stop) return fail(KSPIR_EINVAL, "server is stopping");
         *ticket = s->next_id++;
+        s->live.insert(*ticket);
         s->pending.push_back(kspir_server::Req{*ticket, db, keys, query, response, std::chrono::steady_clock::now()});
     }
     s->cv_work.notify_one();
@@ -1992,10 +2033,13 @@ int kspir_server_submit(kspir_server* s, const kspir_database* db, const kspir_k
 int kspir_server_wait(kspir_server* s, uint64_t ticket, double* latency_ms) {
     if (!s) return fail(KSPIR_EINVAL, "null server");
     std::unique_lock<std::mutex> lock(s->mu);
-    if (ticket == 0 || ticket >= s->next_id) return fail(KSPIR_EINVAL, "unknown ticket");
+    // each ticket is collected once: a second wait (or a ticket never
+    // issued) is an error rather than a wait for a result that cannot come
+    if (!s->live.count(ticket)) return fail(KSPIR_EINVAL, "unknown ticket, or ticket already collected");
     s->cv_done.wait(lock, [&] { return s->done.count(ticket) != 0; });
     const kspir_server::Done d = s->done[ticket];
     s->done.erase(ticket);
+    s->live.erase(ticket);
     if (latency_ms) *latency_ms = d.ms;
     if (d.rc != KSPIR_OK) return fail(d.rc, "batch answer failed in the serving loop");
     return KSPIR_OK;

@@ -1820,6 +1820,20 @@ __global__ void __launch_bounds__(256) k_copy(const uint4* __restrict__ src, uin
         dst[i] = src[i];
 }
 
+// inverse_encode (twosteps.cpp:182-198) of one record column per workgroup:
+// forward NTT mod p (HEXL NTT(N, p) with its minimal root), then the slot
+// order of compute_query_decode: msg[dec[j]] = NTT(msg)[j].  msg: [count][N]
+// values < p, rewritten in place.
+__global__ void __launch_bounds__(kNttThreads) k_inverse_encode(uint64_t* __restrict__ msg,
+                                                                const uint16_t* __restrict__ dec, Tabs T) {
+    __shared__ uint32_t s[kPadN];
+    uint64_t* m = msg + (size_t)blockIdx.x * kN;
+    for (int x = threadIdx.x; x < kN; x += kNttThreads) s[lds_pad(x)] = (uint32_t)m[x];
+    __syncthreads();
+    ntt_fwd_r16(s, T.p[3]);
+    for (int x = threadIdx.x; x < kN; x += kNttThreads) m[dec[x]] = s[lds_pad(x)];
+}
+
 // test hooks: batched forward / inverse NTT of polynomials in prime p
 __global__ void __launch_bounds__(kNttThreads) k_ntt_test(uint32_t* __restrict__ data, int p, int inverse, Tabs T) {
     __shared__ uint32_t s[kPadN];

@@ -3,7 +3,7 @@
 // reference driver calls it.  TEST INFRASTRUCTURE: inputs come from raw files
 // written by the test (oracle-generated, reference layouts); outputs go back
 // as raw files for word-for-word comparison.
-//   dropin_test <dir> <r> <N1>
+//   dropin_test <dir> <r> <N1>  |  client ...  |  decrypt ...  |  sample <dir> <r> <seed>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -111,7 +111,23 @@ static int decrypt_mode(const std::string& dir, int32_t r, unsigned seed) {
     return 0;
 }
 
+// sample_database_bsgs through the drop-in: srand(seed), r blocks of
+// records drawn in test-pir's order -> <dir>/records.bin ([r][N][N/2])
+static int sample_mode(const std::string& dir, int32_t r, unsigned seed) {
+    srand(seed);
+    std::vector<std::vector<uint64_t> > data(N, std::vector<uint64_t>(N / 2, 0));
+    std::vector<uint64_t> flat;
+    for (int k = 0; k < r; ++k) {
+        sample_database_bsgs(data);
+        for (int i = 0; i < N; ++i) flat.insert(flat.end(), data[i].begin(), data[i].end());
+    }
+    write_u64(dir + "/records.bin", flat);
+    return 0;
+}
+
 int main(int argc, char** argv) {
+    if (argc == 5 && std::string(argv[1]) == "sample")
+        return sample_mode(argv[2], std::atoi(argv[3]), (unsigned)std::strtoul(argv[4], nullptr, 10));
     if (argc == 8 && std::string(argv[1]) == "client")
         return client_mode(argv[2], std::atoi(argv[3]), std::atoi(argv[4]), (unsigned)std::strtoul(argv[5], nullptr, 10),
                            std::atoi(argv[6]), std::atoi(argv[7]));
@@ -131,7 +147,7 @@ int main(int argc, char** argv) {
         std::vector<std::vector<uint64_t> > data(N, std::vector<uint64_t>(N / 2)), data_ntt(N / 2, std::vector<uint64_t>(N));
         for (int b = 0; b < r; ++b) {
             for (int i = 0; i < N; ++i) data[i].assign(&rec[b * words + (size_t)i * (N / 2)], &rec[b * words + (size_t)(i + 1) * (N / 2)]);
-            database_tobsgsntt(data_ntt, data, 65537, N1);
+            database_tobsgsntt(data_ntt, data, kspir_params::crtMod, N1);
             database_tocrt(&pre[b * words], data_ntt, N1);
         }
         write_u64(dir + "/datacrt_gpu.bin", pre);
@@ -197,6 +213,24 @@ int main(int argc, char** argv) {
     std::copy(query1[0].a.begin(), query1[0].a.end(), q0.begin());
     std::copy(query1[0].b.begin(), query1[0].b.end(), q0.begin() + N);
     write_u64(dir + "/query0_after.bin", q0);
+
+    // one database word edited in place, then the first dimension again: the
+    // drop-in must answer from the edited words (every word is hashed on every
+    // call), never from its device copy of the old ones
+    const size_t words = (size_t)N * N / 2;
+    const size_t w = (size_t)(r > 1 ? 1 : 0) * words + ((size_t)100 * N2 + (N2 > 2 ? 2 : 0)) * N1 + 5;
+    const uint64_t lo = datacrt[w] & 0xffffffffULL, hi = datacrt[w] >> 32;
+    datacrt[w] = ((lo + 1) % crtq1) | (hi << 32);
+    write_u64(dir + "/edit_index.bin", std::vector<uint64_t>{w, datacrt[w]});
+    std::vector<RlweCiphertext> query2(1, rlwe(&qw[0], crtMod, false));
+    query2.push_back(rlwe(&qw[2 * N], bsMod, false));
+    std::vector<RlweCiphertext> result2(r, RlweCiphertext(N, crtMod));
+    matrix_vector_mul_bsgs_rns_crt_large(result2, query2, datacrt.data(), autoKey, N1, permutations, r);
+    for (int j = 0; j < r; ++j) {
+        std::copy(result2[j].a.begin(), result2[j].a.end(), fd.begin() + (size_t)j * 2 * N);
+        std::copy(result2[j].b.begin(), result2[j].b.end(), fd.begin() + (size_t)j * 2 * N + N);
+    }
+    write_u64(dir + "/firstdim_edit.bin", fd);
     std::printf("online server response costs %lld us (first call, includes uploads); input[0] ntt=%d result ntt=%d\n",
                 (long long)std::chrono::duration_cast<std::chrono::microseconds>(stop - start).count(),
                 (int)query1[0].getIsNtt(), (int)result_output.getIsNtt());

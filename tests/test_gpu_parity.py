@@ -664,3 +664,68 @@ def test_cpp_dropin_matches_oracle(oracle, tmp_path):
     # database_tobsgsntt + database_tocrt through the drop-in give the oracle's words
     pre = np.fromfile(str(tmp_path / "datacrt_gpu.bin"), dtype=np.uint64).reshape(r, N * N // 2)
     assert np.array_equal(pre, rec_words)
+    # a database word edited in place between two calls: the second answer
+    # follows the edited words (the oracle's first dimension on them)
+    w, val = np.fromfile(str(tmp_path / "edit_index.bin"), dtype=np.uint64)
+    edited = inst.db.copy()
+    assert edited.reshape(-1)[int(w)] != val
+    edited.reshape(-1)[int(w)] = val
+    _, st2 = oracle.answer(r=r, n1=n1, db=edited, query=inst.query, rgsw=inst.rgsw, baby=inst.baby,
+                           giant=inst.giant, packing=inst.packing, perms=inst.perms, want_stages=True)
+    fd2 = np.fromfile(str(tmp_path / "firstdim_edit.bin"), dtype=np.uint64).reshape(r, 2, N)
+    assert not np.array_equal(st2["firstdim"], st["firstdim"])
+    assert np.array_equal(fd2, st2["firstdim"])
+
+
+def test_cpp_sample_database_matches_oracle(oracle, tmp_path):
+    """sample_database_bsgs through kspir_pir.h after srand(seed) draws the
+    oracle's records (test-pir.cpp:58 order, twosteps.cpp:23-39)"""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(__file__), "native", "_build", "dropin_test")
+    if not os.path.exists(exe):
+        pytest.fail("tests/native/_build/dropin_test not built (python kspir_amd/build.py)")
+    say("[gpu] C++ sample_database_bsgs r=2 seed=19")
+    out = subprocess.run([exe, "sample", str(tmp_path), "2", "19"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    got = np.fromfile(str(tmp_path / "records.bin"), dtype=np.uint64).reshape(2, N, N // 2)
+    want, _ = oracle.gen_records(2, 128, seed=19, plant=False)
+    assert np.array_equal(got, want)
+
+
+def test_reference_test_pir_unchanged(tmp_path):
+    """the reference's own tests/test-pir.cpp, compiled unchanged against
+    the drop-in headers (kspir_amd/cpp/compat) and linked with
+    libkspir_amd.so, at its default 256 MB shape: it draws the README's
+    targets and recovers the planted record [1..4096] (README.md:56,65)"""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(__file__), "native", "_build", "test_pir")
+    if not os.path.exists(exe):
+        pytest.skip("the reference's tests/test-pir.cpp was not present when the library was built")
+    say("[gpu] reference test-pir driver on the drop-in")
+    out = subprocess.run([exe], cwd=str(tmp_path), capture_output=True, text=True, timeout=600)
+    say(out.stdout.strip())
+    assert out.returncode == 0, out.stderr
+    assert "target_col: 177, target_packing: 4" in out.stdout
+    assert ("the recovered result = [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, ..., 4096]"
+            in out.stdout)
+
+
+def test_inverse_encode_on_gpu(ctx, oracle):
+    """inverse_encode (forward NTT mod p, query_decode order) on the GPU
+    equals the oracle's, including all-zero and all-(p-1) columns; values
+    >= p are rejected"""
+    import kspir_amd
+    say("[gpu] inverse_encode")
+    rng = np.random.default_rng(23)
+    cols = rng.integers(0, 65537, (4, N), dtype=np.uint64)
+    cols[0] = 0
+    cols[1] = 65536
+    got = ctx.inverse_encode(cols)
+    for i in range(4):
+        assert np.array_equal(got[i], oracle.inverse_encode(cols[i])), i
+    bad = cols[2].copy()
+    bad[7] = 65537
+    with pytest.raises(kspir_amd.KspirError, match="below p"):
+        ctx.inverse_encode(bad)
