@@ -23,6 +23,12 @@ int kspir_device_count(int* count);
  *                     database pass (default 2); smaller ones the VALU pass
  *   "mfma_max_chunk"  most queries per MFMA database sweep (default: as many
  *                     as the LDS holds)
+ *   "mfma_direct"     0 (default): the MFMA pass writes tile-major products
+ *                     that a second kernel moves into the first-dimension
+ *                     layout; 1: one kernel writes that layout directly (16
+ *                     slots per workgroup, query limbs re-read from L2: it
+ *                     moves 1 GiB less through HBM per 64-query step at r 16
+ *                     but waits on those L2 reads and runs slower there)
  *   "fused_min_batch" batches of at least this many queries run the giant
  *                     steps and first-dimension sum as one workgroup per
  *                     (query, block) (default 16); smaller ones spread them

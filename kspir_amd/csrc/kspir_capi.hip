@@ -124,6 +124,7 @@ struct kspir_context {
     size_t mws_bytes = 0;
     int mfma_min_batch = 2;         // batches of at least this many queries use the MFMA pass
     int mfma_max_chunk = 1 << 30;   // cap on queries per MFMA database sweep (LDS also caps it)
+    int mfma_direct = 0;            // 1: MFMA products written straight into dbout (k_db_mfma_direct); 0: tile-major + scatter
     int fused_min_batch = 16;       // batches of at least this many queries use k_giant_fused
     int fused_pack = 1;             // one k_pack_level per packing level (0: three kernels per level)
     int db_waves = 4;               // waves per workgroup of the single-query database pass (2, 4, 8, 16)
@@ -282,14 +283,14 @@ int build_tables(kspir_context* c) {
 // device bytes one query of shape (r, n1) adds to a batch: its share of the
 // carved workspace plus its share of the batched database pass's scratch
 // (query limbs and tile-major products)
-size_t query_footprint_bytes(int r, int n1) {
+size_t query_footprint_bytes(int r, int n1, bool mfma_direct = false) {
     const int n2 = kN / 2 / n1;
     const size_t n = kN;
     const size_t rh = r / 2 > 0 ? r / 2 : 1;
     const size_t words = 9 * n * 2 + 3 * n + 6 * n + 3 * n + (size_t)n1 * 4 * n + (size_t)r * n2 * 4 * n * 2 +
                          (size_t)r * n2 * 2 * n + (size_t)r * 4 * n * 2 + (size_t)r * 16 * n + (size_t)r * 4 * n * 2 +
                          rh * 2 * n + rh * 16 * n + 2 * 4 * n * 2 + 2 * 2 * n * 2;
-    const size_t mfma = (size_t)kN * 2 * 4 * 2 * n1 + (size_t)2 * r * n2 * 2 * kN * 4;
+    const size_t mfma = (size_t)kN * 2 * 4 * 2 * n1 + (mfma_direct ? 0 : (size_t)2 * r * n2 * 2 * kN * 4);
     return words * 4 + mfma;
 }
 
@@ -415,13 +416,17 @@ int launch_db_batched(kspir_context* c, const kspir_database* dbc, int nq, hipSt
             db->limbs = l;
         }
     }
+    // the tile-major kernel stages a slot's query limbs in LDS, which caps
+    // its columns; the direct kernel reads them from L2 (256 columns cap the
+    // limb scratch at 128 queries per sweep either way)
+    const bool direct = c->mfma_direct != 0;
     const int lds_cap = 160 * 1024;
-    int cmax = lds_cap / (4 * (n1 + 16)) / 16 * 16;
+    int cmax = direct ? 256 : lds_cap / (4 * (n1 + 16)) / 16 * 16;
     if (cmax > 256) cmax = 256;
     const int nqc_max = cmax / 2 < c->mfma_max_chunk ? cmax / 2 : c->mfma_max_chunk;
     const int C_full = ((2 * (nq < nqc_max ? nq : nqc_max)) + 15) / 16 * 16;
     const size_t ql_bytes = (size_t)kN * 2 * 4 * C_full * n1;
-    const size_t mf_bytes = (size_t)2 * R * C_full * kN * 4;
+    const size_t mf_bytes = direct ? 0 : (size_t)2 * R * C_full * kN * 4;
     if (ql_bytes + mf_bytes > c->mws_bytes) {
         if (c->mws) {
             HIP_OK(hipStreamSynchronize(st));
@@ -451,6 +456,24 @@ int launch_db_batched(kspir_context* c, const kspir_database* dbc, int nq, hipSt
             default: hipLaunchKernelGGL(k_ct_limbs<256>, dim3(kN / 32, C / 2, 2), dim3(256), 0, st, c->ctw + q0 * S.ctw, ql, nqc, C, S); break;
         }
         LAUNCH_OK("k_ct_limbs");
+        if (direct) {
+            // products straight into dbout: 16 slots x (16 RTW) rows per workgroup
+            const int rtw = (n1 <= 128 && R % 32 == 0) ? 2 : 1;
+            const dim3 g((kN / 16) * (R / (16 * rtw)) * 2);
+            uint32_t* out = c->dbout + q0 * S.dbout;
+            if (n1 == 64 && rtw == 2)
+                hipLaunchKernelGGL((k_db_mfma_direct<1, 2>), g, dim3(512), 0, st, db->limbs, ql, out, R, C, n2, nqc, S, MC, c->T);
+            else if (n1 == 64)
+                hipLaunchKernelGGL((k_db_mfma_direct<1, 1>), g, dim3(512), 0, st, db->limbs, ql, out, R, C, n2, nqc, S, MC, c->T);
+            else if (n1 == 128 && rtw == 2)
+                hipLaunchKernelGGL((k_db_mfma_direct<2, 2>), g, dim3(512), 0, st, db->limbs, ql, out, R, C, n2, nqc, S, MC, c->T);
+            else if (n1 == 128)
+                hipLaunchKernelGGL((k_db_mfma_direct<2, 1>), g, dim3(512), 0, st, db->limbs, ql, out, R, C, n2, nqc, S, MC, c->T);
+            else
+                hipLaunchKernelGGL((k_db_mfma_direct<4, 1>), g, dim3(512), 0, st, db->limbs, ql, out, R, C, n2, nqc, S, MC, c->T);
+            LAUNCH_OK("k_db_mfma_direct");
+            continue;
+        }
         const size_t lds = (size_t)4 * C * (n1 + 16);
         switch (n1) {
             case 64:
@@ -1254,6 +1277,8 @@ int kspir_context_set(kspir_context* c, const char* key, int64_t value) {
         c->fused_min_batch = (int)value;
     } else if (k == "fused_pack") {
         c->fused_pack = value != 0;
+    } else if (k == "mfma_direct") {
+        c->mfma_direct = value != 0;
     } else if (k == "graphs") {
         c->use_graphs = value != 0;
     } else if (k == "pack_fused_min_wgs") {
@@ -1281,6 +1306,7 @@ int kspir_context_get(kspir_context* c, const char* key, int64_t* value) {
     else if (k == "mfma_max_chunk") *value = c->mfma_max_chunk;
     else if (k == "fused_min_batch") *value = c->fused_min_batch;
     else if (k == "fused_pack") *value = c->fused_pack;
+    else if (k == "mfma_direct") *value = c->mfma_direct;
     else if (k == "graphs") *value = c->use_graphs;
     else if (k == "db_waves") *value = c->db_waves;
     else if (k == "giant_wgs_per_cu") *value = c->giant_wgs_per_cu;
@@ -1451,7 +1477,7 @@ struct kspir_server {
                 // query, and as many of its queries (in arrival order) as the
                 // batch cap and the lane's workspace budget admit
                 const kspir_database* db = pending.front().db;
-                const size_t per = query_footprint_bytes(db->r, db->n1);
+                const size_t per = query_footprint_bytes(db->r, db->n1, L.ctx->mfma_direct != 0);
                 size_t cap = budget / (per ? per : 1);
                 if (cap < 1) cap = 1;
                 if (cap > (size_t)max_batch) cap = (size_t)max_batch;
@@ -1989,6 +2015,7 @@ int kspir_server_create_lanes(kspir_context* c, int32_t max_batch, size_t worksp
             L.ctx->mfma_max_chunk = c->mfma_max_chunk;
             L.ctx->fused_min_batch = c->fused_min_batch;
             L.ctx->fused_pack = c->fused_pack;
+            L.ctx->mfma_direct = c->mfma_direct;
             L.ctx->pack_fused_min_wgs = c->pack_fused_min_wgs;
             L.ctx->giant_wgs_per_cu = c->giant_wgs_per_cu;
             L.ctx->giant_dma = c->giant_dma;

@@ -90,8 +90,53 @@ __device__ __forceinline__ uint64_t crt_q(uint32_t x1, uint32_t x2, const Tabs& 
 }
 
 // residue mod q of a signed gadget digit r with |r| < q
-__device__ __forceinline__ uint32_t digit_residue(int32_t r, uint32_t q) {
+__host__ __device__ __forceinline__ uint32_t digit_residue(int32_t r, uint32_t q) {
     return r < 0 ? (uint32_t)(r + (int32_t)q) : (uint32_t)r;
+}
+
+// ------------------------------------------------------ gadget digits
+
+// utils.cpp:1120-1189 decompose_bsgs digit j (ell 3, PP = Bg = 2^20).  The
+// digit helpers below are host-callable as well, so that their boundary cases
+// can be checked against the oracle on the CPU.
+__host__ __device__ __forceinline__ int32_t bsgs_digit(unsigned __int128 v, int j) {
+    const unsigned __int128 M = (unsigned __int128)kQ * kBs;
+    const unsigned __int128 offset = (((unsigned __int128)1 << 20) + ((unsigned __int128)1 << 40) +
+                                      ((unsigned __int128)1 << 60)) << 19 | ((unsigned __int128)1 << 19);
+    unsigned __int128 d = v + offset;
+    if (d > M) d -= M;
+    d >>= 20;
+    uint32_t r = (uint32_t)(d >> (20 * j)) & ((1u << 20) - 1);
+    return (int32_t)r - (1 << 19);
+}
+
+// utils.cpp:754-811 decompose (tfhe variant): digit j of x in [0, Q]
+template <int GBITS, int BASEBITS, int ELL>
+__host__ __device__ __forceinline__ int32_t tfhe_digit(uint64_t x, int j) {
+    uint64_t offset = 0;
+#pragma unroll
+    for (int i = 0; i < ELL; ++i) offset += (uint64_t)1 << (BASEBITS + i * GBITS);
+    offset <<= (GBITS - 1);
+    offset += BASEBITS ? ((uint64_t)1 << (BASEBITS - 1)) : 0;
+    uint64_t d = x + offset;
+    if (d > kQ) d -= kQ;
+    d >>= BASEBITS;
+    uint32_t r = (uint32_t)(d >> (GBITS * j)) & ((1u << GBITS) - 1);
+    return (int32_t)r - (1 << (GBITS - 1));
+}
+
+// utils.cpp:974-1044 decompose_rlwe (signed digits): digit j of x in [0, Q)
+template <int GBITS, int BASEBITS>
+__host__ __device__ __forceinline__ int32_t signed_digit(uint64_t x, int j) {
+    int64_t d = x > (kQ >> 1) ? (int64_t)(x - kQ) : (int64_t)x;
+    d >>= BASEBITS;
+    int32_t r = 0;
+    for (int k = 0; k <= j; ++k) {
+        r = (int32_t)((int64_t)((uint64_t)d << (64 - GBITS)) >> (64 - GBITS));
+        d -= r;
+        d >>= GBITS;
+    }
+    return r;
 }
 
 // ---------------------------------------------------------------- NTT in LDS

@@ -47,47 +47,6 @@ __device__ __forceinline__ unsigned __int128 crt80(uint64_t xq, uint32_t xb, con
     return (unsigned __int128)xq + (unsigned __int128)kQ * t;
 }
 
-// utils.cpp:1120-1189 decompose_bsgs digit j (ell 3, PP = Bg = 2^20)
-__device__ __forceinline__ int32_t bsgs_digit(unsigned __int128 v, int j) {
-    const unsigned __int128 M = (unsigned __int128)kQ * kBs;
-    const unsigned __int128 offset = (((unsigned __int128)1 << 20) + ((unsigned __int128)1 << 40) +
-                                      ((unsigned __int128)1 << 60)) << 19 | ((unsigned __int128)1 << 19);
-    unsigned __int128 d = v + offset;
-    if (d > M) d -= M;
-    d >>= 20;
-    uint32_t r = (uint32_t)(d >> (20 * j)) & ((1u << 20) - 1);
-    return (int32_t)r - (1 << 19);
-}
-
-// utils.cpp:754-811 decompose (tfhe variant): digit j of x in [0, Q]
-template <int GBITS, int BASEBITS, int ELL>
-__device__ __forceinline__ int32_t tfhe_digit(uint64_t x, int j) {
-    uint64_t offset = 0;
-#pragma unroll
-    for (int i = 0; i < ELL; ++i) offset += (uint64_t)1 << (BASEBITS + i * GBITS);
-    offset <<= (GBITS - 1);
-    offset += BASEBITS ? ((uint64_t)1 << (BASEBITS - 1)) : 0;
-    uint64_t d = x + offset;
-    if (d > kQ) d -= kQ;
-    d >>= BASEBITS;
-    uint32_t r = (uint32_t)(d >> (GBITS * j)) & ((1u << GBITS) - 1);
-    return (int32_t)r - (1 << (GBITS - 1));
-}
-
-// utils.cpp:974-1044 decompose_rlwe (signed digits): digit j of x in [0, Q)
-template <int GBITS, int BASEBITS>
-__device__ __forceinline__ int32_t signed_digit(uint64_t x, int j) {
-    int64_t d = x > (kQ >> 1) ? (int64_t)(x - kQ) : (int64_t)x;
-    d >>= BASEBITS;
-    int32_t r = 0;
-    for (int k = 0; k <= j; ++k) {
-        r = (int32_t)((int64_t)((uint64_t)d << (64 - GBITS)) >> (64 - GBITS));
-        d -= r;
-        d >>= GBITS;
-    }
-    return r;
-}
-
 // coefficient s of tau_g(a), a given by its residues in two planes; returns
 // the canonical value in [0, Q).  ginv = g^-1 mod 2N.
 __device__ __forceinline__ uint64_t auto_coeff(const uint32_t* a1, const uint32_t* a2, int s,
@@ -829,6 +788,108 @@ __global__ void __launch_bounds__(256) k_db_mfma(const int8_t* __restrict__ dbl,
             }
         }
         __syncthreads();
+    }
+}
+
+// The same products written straight into the first-dimension layout
+// dbout [q][b][i][ab][p][z], with no tile-major intermediate.  Coalesced
+// writes need runs of consecutive slots, so a workgroup owns 16 slots and
+// RTW*16*8 rows (8 waves, each 2 slots x RTW row tiles): the database limbs
+// of its (slot, row tile) pairs stay in registers for the whole sweep and are
+// read from HBM exactly once; the query limbs of a (slot, column tile) are
+// read from L2, once per row group (the row groups of one 16-slot group run
+// back to back on one XCD).  For each column tile the 16 x 16 products of
+// every (slot, row tile) go through an LDS tile [slot][row][col] and leave as
+// 64-byte runs of 16 slots per (row, column), the buffers alternating so one
+// barrier per column tile suffices.  Strides are chosen so that the MFMA
+// result writes and the run reads touch each bank at most twice per wave.
+// Requires R % (16 RTW * 8 / 8) == 0 row tiles per group, i.e. R % (16 RTW) == 0.
+// grid: 256 slot groups x (R / (16 RTW)) row groups x 2 primes, 512 threads.
+constexpr int kMfmaRhoStride = 20;                    // words per staged row (16 columns + pad)
+constexpr int kMfmaZStride = 32 * kMfmaRhoStride + 4; // words per staged slot (up to 32 rows)
+template <int KS, int RTW>
+__global__ void __launch_bounds__(512) k_db_mfma_direct(const int8_t* __restrict__ dbl, const int8_t* __restrict__ ql,
+                                                        uint32_t* __restrict__ dbout, int R, int C, int n2, int nq,
+                                                        QS S, MfmaConst MC, Tabs T) {
+    constexpr int n1 = KS * 64;
+    __shared__ uint32_t stage[2][16 * kMfmaZStride];
+    const int ngroups = R / (16 * RTW);
+    const int L = xcd_job(blockIdx.x, (kN / 16) * ngroups * 2);
+    const int rg = L % ngroups, p = (L / ngroups) & 1, zg = L / (2 * ngroups);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lr = lane & 15, lk = (lane >> 4) * 16;
+    const PrimeTab& P = T.p[p];
+    const int CT = C / 16;
+    // database limbs of this wave's two slots and RTW row tiles
+    i8x16 a[2][RTW][4][KS];
+#pragma unroll
+    for (int zi = 0; zi < 2; ++zi) {
+        const int z = zg * 16 + 2 * wv + zi;
+        const int8_t* asrc = dbl + ((size_t)z * 2 + p) * 4 * R * n1;
+#pragma unroll
+        for (int t = 0; t < RTW; ++t)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int ks = 0; ks < KS; ++ks)
+                    a[zi][t][u][ks] = *reinterpret_cast<const i8x16*>(
+                        asrc + ((size_t)u * R + (rg * RTW + t) * 16 + lr) * n1 + ks * 64 + lk);
+    }
+    for (int ct = 0; ct < CT; ++ct) {
+        uint32_t* st = stage[ct & 1];
+#pragma unroll
+        for (int zi = 0; zi < 2; ++zi) {
+            const int zl = 2 * wv + zi, z = zg * 16 + zl;
+            const int8_t* bsrc = ql + ((size_t)z * 2 + p) * 4 * C * n1 + (size_t)(ct * 16 + lr) * n1 + lk;
+            i32x4 acc[RTW][7];
+#pragma unroll
+            for (int t = 0; t < RTW; ++t)
+#pragma unroll
+                for (int w = 0; w < 7; ++w) acc[t][w] = i32x4{0, 0, 0, 0};
+            // one query limb at a time: its KS fragments serve every row tile
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                i8x16 b[KS];
+#pragma unroll
+                for (int ks = 0; ks < KS; ++ks)
+                    b[ks] = *reinterpret_cast<const i8x16*>(bsrc + (size_t)v * C * n1 + ks * 64);
+#pragma unroll
+                for (int t = 0; t < RTW; ++t)
+#pragma unroll
+                    for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            acc[t][u + v] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[zi][t][u][ks], b[ks],
+                                                                                  acc[t][u + v], 0, 0, 0);
+            }
+#pragma unroll
+            for (int t = 0; t < RTW; ++t) {
+                // lane: column lr, rows 4 (lane / 16) + j of row tile t
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    uint64_t s = 0;
+#pragma unroll
+                    for (int w = 0; w < 7; ++w) s += (uint64_t)(uint32_t)acc[t][w][j] * MC.cw[p][w];
+                    const int rho = t * 16 + 4 * (lane >> 4) + j;
+                    st[zl * kMfmaZStride + rho * kMfmaRhoStride + lr] = mont_reduce(s, P.q, P.mont_qinv);
+                }
+            }
+        }
+        __syncthreads();
+        // runs: (row, column) pairs, four lanes per run of 16 slots
+#pragma unroll
+        for (int i = 0; i < RTW * 16 * 16 * 4 / 512; ++i) {
+            const int idx = threadIdx.x + 512 * i, run = idx >> 2, qz = idx & 3;
+            const int rho = run >> 4, col = run & 15;
+            const int cg = ct * 16 + col, q = cg >> 1, ab = cg & 1;
+            if (q < nq) {
+                const uint32_t* sp = st + (4 * qz) * kMfmaZStride + rho * kMfmaRhoStride + col;
+                const uint4 v = make_uint4(sp[0], sp[kMfmaZStride], sp[2 * kMfmaZStride], sp[3 * kMfmaZStride]);
+                const int rhog = (rg * RTW) * 16 + rho, blk = rhog / n2, row = rhog % n2;
+                *reinterpret_cast<uint4*>(dbout + q * S.dbout + ((size_t)(blk * n2 + row) * 4 + ab * 2 + p) * kN +
+                                          zg * 16 + 4 * qz) = v;
+            }
+        }
     }
 }
 

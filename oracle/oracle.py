@@ -286,6 +286,15 @@ def db_product(db_block, cbuf, n1, n2):
     return out
 
 
+def ref_database_tocrt(data_ntt, n1):
+    """the reference's own database_tocrt (crt.cpp:859-885) of one block:
+    data_ntt [N/2][N] -> datacrt words [N*N/2]"""
+    R = ref_lib()
+    out = np.zeros(N * N // 2, dtype=np.uint64)
+    R.ref_database_tocrt(p64(out), p64(np.ascontiguousarray(data_ntt, dtype=np.uint64)), n1)
+    return out
+
+
 def ref_db_product(db_block, cbuf, n1, n2):
     R = ref_lib()
     out = np.zeros((n2, 2 * N), dtype=np.uint64)

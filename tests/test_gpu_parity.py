@@ -217,6 +217,35 @@ def test_batch_mfma_shapes_and_chunks(ctx, oracle, n1, chunk):
     db.close()
 
 
+@pytest.mark.parametrize("r,n1,nq", [(4, 128, 5), (2, 64, 3), (4, 256, 4), (1, 128, 2)])
+def test_batch_mfma_direct_matches_tile_major(ctx, oracle, r, n1, nq):
+    """the MFMA database pass that writes its products straight into the
+    first-dimension layout (option mfma_direct; two row tiles per wave where
+    r*N2 allows, one otherwise) gives the words of the tile-major pass and
+    the oracle's"""
+    say(f"[gpu] mfma direct r={r} n1={n1} nq={nq}")
+    db_words = oracle.gen_db(r, n1, seed=13)
+    perms = oracle.perm_matrix(n1)
+    clients = [oracle.gen_client(r, n1, 131 + c, (41 * c + 9) % (N // 2), c % r) for c in range(nq)]
+    db = ctx.database(db_words, r, n1)
+    keys = [ctx.keys(r, n1, cl["baby"], cl["giant"], cl["packing"], cl["rgsw"], perms) for cl in clients]
+    queries = np.stack([cl["query"] for cl in clients])
+    tile = ctx.answer_batch(db, keys, queries)
+    ctx.set("mfma_direct", 1)
+    try:
+        direct = ctx.answer_batch(db, keys, queries)
+    finally:
+        ctx.set("mfma_direct", 0)
+    assert np.array_equal(direct, tile)
+    want0 = oracle.answer(r=r, n1=n1, db=db_words, query=clients[0]["query"], rgsw=clients[0]["rgsw"],
+                          baby=clients[0]["baby"], giant=clients[0]["giant"], packing=clients[0]["packing"],
+                          perms=perms)
+    assert np.array_equal(direct[0], want0)
+    for k in keys:
+        k.close()
+    db.close()
+
+
 @pytest.mark.parametrize("r,n1,nq", [(4, 128, 2), (4, 64, 5), (2, 512, 3), (8, 256, 1)])
 def test_fused_giant_steps(ctx, oracle, r, n1, nq):
     """the fused per-(query, block) giant-step kernel (forced on with

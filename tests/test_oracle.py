@@ -306,6 +306,19 @@ def test_gen_records_matches_gen_db_and_tocrt(oracle):
     assert np.array_equal(words[1], want)
 
 
+@pytest.mark.parametrize("n1", [64, 256])
+def test_database_tocrt_matches_reference(oracle, n1):
+    """the oracle's database words of a block equal the reference's own
+    database_tocrt (crt.cpp:859-885, built into oracle/_ref) applied to the
+    oracle's database_tobsgsntt of that block's records"""
+    if oracle.ref_lib() is None:
+        pytest.skip("oracle/_ref not built (needs the reference sources)")
+    rec, words = oracle.gen_records(2, n1, seed=11)
+    for b in range(2):
+        nt = oracle.database_tobsgsntt(n1, rec[b])
+        assert np.array_equal(oracle.ref_database_tocrt(nt, n1), words[b]), b
+
+
 def test_records_to_words_matches_gen_records(oracle):
     """records_to_words (the oracle's preprocessing of given records) gives
     gen_records' words, and block stride 0 repeats block 0's words"""
