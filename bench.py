@@ -101,16 +101,21 @@ def cpu_baseline(r, n1, reps):
     inst = O.gen_instance(r=r, n1=n1, seed=1)
     O.set_threads(1)
     first = O.answer(inst)   # warm-up
-    times = []
+    times, stages = [], []
     for _ in range(reps):
         t0 = time.perf_counter()
         O.answer(inst)
         times.append(time.perf_counter() - t0)
+        stages.append(O.last_stage_us())
     sec = float(np.median(times))
+    names = ("baby_steps", "db_pass", "giant_steps", "external_products", "packing")
+    split = {n: round(float(np.median([st[i] for st in stages])) / 1e3, 2) for i, n in enumerate(names)}
+    loop = "AVX-512 database loop (crt.cpp:660-768 shape)" if O.db_product_vectorised() else "scalar database loop"
     single = {"value": round(1.0 / sec, 3), "unit": "queries/s", "cores": 1, "kind": "port",
               "sample": f"{reps} single-query online answers (median {sec * 1e3:.1f} ms) at r={r}, N1={n1} "
-                        f"on oracle-generated test-pir inputs, seed 1; host {model}, {ncpu} logical CPUs; "
-                        f"the reference README reports 236,635 us per answer on unknown hardware (reported only)"}
+                        f"on oracle-generated test-pir inputs, seed 1, {loop}; host {model}, {ncpu} logical CPUs; "
+                        f"the reference README reports 236,635 us per answer on unknown hardware (reported only)",
+              "stages_ms": split}
     t0 = time.perf_counter()
     resp = O.answer_parallel(inst, threads, threads)
     wall = time.perf_counter() - t0
@@ -118,6 +123,39 @@ def cpu_baseline(r, n1, reps):
     allc = {"value": round(threads / wall, 3), "unit": "queries/s", "cores": threads, "kind": "port",
             "sample": f"{threads} answers of the same inputs, one per thread, {wall * 1e3:.0f} ms wall"}
     return single, allc
+
+
+MFMA_I8_PEAK_TOPS = 5000.0     # MI355X dense int8 MFMA, 5.0 POP/s
+
+
+def batched_roofline(r, n1, B, db3):
+    """the batched step's database pass (three kernels, timed live with HIP
+    events) against its bounds: k_db_mfma's int8-MFMA rate, and the pass's
+    algorithmic HBM bytes over the three kernels' time; VALU utilisation of
+    the transform kernels from the committed PMC summary"""
+    if not db3:
+        return None
+    n2 = N // 2 // n1
+    R, C = r * n2, (2 * B + 15) // 16 * 16
+    ops = 2 * N * 2 * R * n1 * C * 16          # 2 primes x N slots x R x C outputs x n1 x 16 limb MACs x 2 ops
+    limbs_ms, mfma_ms, move_ms = db3
+    tops = ops / (mfma_ms * 1e-3) / 1e12
+    # database limbs (8 B per word) + the batch's rotation words (4 B per
+    # residue) in, the first-dimension products (4 B per residue) out
+    alg = 8 * r * N * (N // 2) + B * n1 * 4 * N * 4 + B * R * 4 * N * 4
+    pass_ms = limbs_ms + mfma_ms + move_ms
+    out = {"kernel": "k_db_mfma (batched int8-MFMA database pass)", "bound": "mfma",
+           "achieved": round(tops, 1), "peak": MFMA_I8_PEAK_TOPS, "unit": "TOP/s (int8)",
+           "frac": round(tops / MFMA_I8_PEAK_TOPS, 4), "ops_per_launch": ops, "avg_launch_ms": round(mfma_ms, 4),
+           "db_pass": {"kernels_ms": {"k_ct_limbs": round(limbs_ms, 4), "k_db_mfma": round(mfma_ms, 4),
+                                      "k_mfma_scatter": round(move_ms, 4)},
+                       "algorithmic_bytes": alg, "achieved_GBps": round(alg / (pass_ms * 1e-3) / 1e9, 1),
+                       "peak_GBps": HBM_PEAK_GBPS, "frac": round(alg / (pass_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)}}
+    path = os.path.join(ROOT, "profiles", "batched_step_utilisation.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            out["transform_kernels_valu"] = json.load(f)
+    return out
 
 
 def config_c2(args, log):
@@ -416,7 +454,7 @@ def main():
     ap.add_argument("--db-mb", type=int, default=256, help="database size: 16 MB per block (r = MB/16)")
     ap.add_argument("--n1", type=int, default=128)
     ap.add_argument("--plain-steps", type=int, default=100)
-    ap.add_argument("--lat-samples", type=int, default=50)
+    ap.add_argument("--lat-samples", type=int, default=1000, help="latency samples (plain and batched)")
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="launch kernel by kernel instead of replaying hipGraphs")
@@ -464,6 +502,21 @@ def main():
         tuned["serve_batch"] = serve_b
         tuned["report"] = rep
 
+    # every response of one batched step equals its query's single answer
+    log("checking one batched step against the single-query answers")
+    ctx.answer_batch_device(db, keys, qbuf, rbuf)
+    ctx.synchronize()
+    batch_resp = rbuf.download(np.zeros((B, 2, N), dtype=np.uint64))
+    q1buf, r1buf = ctx.alloc(4 * N * 8), ctx.alloc(2 * N * 8)
+    for c in range(B):
+        q1buf.upload(queries[c])
+        ctx.answer_device(db, keys[c], q1buf, r1buf)
+        ctx.synchronize()
+        if not np.array_equal(r1buf.download(np.zeros((2, N), dtype=np.uint64)), batch_resp[c]):
+            raise SystemExit(f"batched response {c} differs from its single-query answer")
+    q1buf.free()
+    r1buf.free()
+
     # plain single-query answer (client 0), one query in flight
     log("timing plain single-query answers")
     ctx.synchronize()
@@ -480,7 +533,10 @@ def main():
     total_ms = ctx.time_answer_batch(db, keys, qbuf, rbuf, args.warmup, args.steps)
     ms_per_step = total_ms / args.steps
     qps = B * 1000.0 / ms_per_step
-    batch_lat = [ctx.time_answer_batch(db, keys, qbuf, rbuf, 0, 1) for _ in range(max(5, args.lat_samples // 5))]
+    batch_lat = [ctx.time_answer_batch(db, keys, qbuf, rbuf, 0, 1) for _ in range(args.lat_samples)]
+    # the batched database pass, kernel by kernel (HIP events): query limbs,
+    # int8-MFMA products, move into the first-dimension layout
+    db3 = ctx.time_db_pass_batch(db, keys, qbuf, 10) if B >= 2 else None
 
     # dominant single-query kernel: the database pass, timed alone with HIP
     # events on the context stream; algorithmic bytes = 8 B per word, r*N*N/2 words
@@ -517,20 +573,22 @@ def main():
                         f"N1={n1}, N2={n2}",
             "records": r * N // 2, "record_bytes": 8192, "r": r, "N1": n1, "N2": n2, "queries_per_step": B,
             "launch": "kernel by kernel" if args.no_graphs else "hipGraph replay",
+            "checked": "every response of one batched step equals its query's single-query answer",
             "tuned": tuned,
         },
         "db_GBps_raw": round(r * N * (N // 2) * 2 * qps / 1e9, 2),
         "latency_ms": {"p50": round(float(np.percentile(batch_lat, 50)), 4),
-                       "p99": round(float(np.percentile(batch_lat, 99)), 4)},
+                       "p99": round(float(np.percentile(batch_lat, 99)), 4), "samples": len(batch_lat)},
         "plain_single_query": {
             "qps": round(plain_qps, 2), "ms": round(plain_ms, 4),
             "db_GBps_raw": round(r * N * (N // 2) * 2 * plain_qps / 1e9, 2),
             "db_GBps_streamed": round(db_bytes * plain_qps / 1e9, 2),
             "ms_kernel_by_kernel": None if plain_eager_ms is None else round(plain_eager_ms, 4),
             "latency_ms": {"p50": round(float(np.percentile(plain_lat, 50)), 4),
-                           "p99": round(float(np.percentile(plain_lat, 99)), 4)},
+                           "p99": round(float(np.percentile(plain_lat, 99)), 4), "samples": len(plain_lat)},
         },
         "batched_over_plain": round(qps / plain_qps, 2),
+        "roofline_batched": batched_roofline(r, n1, B, db3),
         "roofline": {"kernel": "k_db_product (single-query database pass)", "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic_from_profile(),

@@ -44,7 +44,7 @@ def _load():
                  "kspir_query_footprint_bytes", "kspir_decrypt_bsgs_total", "kspir_context_get",
                  "kspir_autotune", "kspir_client_generate", "kspir_client_draws", "kspir_client_encrypt",
                  "kspir_server_create_lanes", "kspir_gen_giant_keys_from_offline", "kspir_answer_device_noalloc",
-                 "kspir_inverse_encode"):
+                 "kspir_inverse_encode", "kspir_time_db_pass_batch"):
         getattr(L, name).restype = ctypes.c_int
     L.kspir_autotune.argtypes = [_vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                  ctypes.POINTER(ctypes.c_int32), ctypes.c_char_p, ctypes.c_size_t]
@@ -71,6 +71,7 @@ EXPORTS = (
     "kspir_query_footprint_bytes", "kspir_decrypt_bsgs_total", "kspir_context_get", "kspir_autotune",
     "kspir_client_generate", "kspir_client_draws", "kspir_client_encrypt", "kspir_server_create_lanes",
     "kspir_gen_giant_keys_from_offline", "kspir_answer_device_noalloc", "kspir_inverse_encode",
+    "kspir_time_db_pass_batch",
 )
 
 
@@ -390,6 +391,16 @@ class Context:
         _check(lib.kspir_time_db_pass(self.h, db.h, keys.h, ctypes.cast(_vp(query_buf.ptr), _u64p), iters,
                                       ctypes.byref(ms)))
         return ms.value
+
+    def time_db_pass_batch(self, db, keys_list, queries_buf, iters):
+        """average ms per launch of the batched database pass's three kernels:
+        (query limbs, MFMA products, move into the first-dimension layout)"""
+        nq = len(keys_list)
+        arr = (_vp * nq)(*[k.h.value for k in keys_list])
+        ms = (ctypes.c_float * 3)()
+        _check(lib.kspir_time_db_pass_batch(self.h, db.h, arr, nq, ctypes.cast(_vp(queries_buf.ptr), _u64p), iters,
+                                            ms))
+        return list(ms)
 
     def time_copy(self, dst_buf, src_buf, nbytes, iters):
         """average ms of a plain 16-byte-per-lane copy of nbytes (device buffers)"""

@@ -401,7 +401,9 @@ bool mfma_shape(const kspir_database* db) {
 
 // batched database pass on int8 MFMA (one database read per chunk of up to
 // nqc_max queries); results land in c->dbout exactly as k_db_product's
-int launch_db_batched(kspir_context* c, const kspir_database* dbc, int nq, hipStream_t st) {
+// parts (timing aid): bit 0 the query limbs, bit 1 the MFMA products, bit 2
+// the move into dbout; the whole pass is 7
+int launch_db_batched(kspir_context* c, const kspir_database* dbc, int nq, hipStream_t st, int parts = 7) {
     kspir_database* db = const_cast<kspir_database*>(dbc);
     const int r = db->r, n1 = db->n1, n2 = db->n2, R = r * n2;
     const size_t dbwords = (size_t)r * kN * (kN / 2);
@@ -450,13 +452,13 @@ int launch_db_batched(kspir_context* c, const kspir_database* dbc, int nq, hipSt
     for (int q0 = 0; q0 < nq; q0 += nqc_max) {
         const int nqc = nq - q0 < nqc_max ? nq - q0 : nqc_max;
         const int C = (2 * nqc + 15) / 16 * 16;
-        switch (n1) {
+        if (parts & 1) switch (n1) {
             case 64: hipLaunchKernelGGL(k_ct_limbs<64>, dim3(kN / 32, C / 2, 2), dim3(256), 0, st, c->ctw + q0 * S.ctw, ql, nqc, C, S); break;
             case 128: hipLaunchKernelGGL(k_ct_limbs<128>, dim3(kN / 32, C / 2, 2), dim3(256), 0, st, c->ctw + q0 * S.ctw, ql, nqc, C, S); break;
             default: hipLaunchKernelGGL(k_ct_limbs<256>, dim3(kN / 32, C / 2, 2), dim3(256), 0, st, c->ctw + q0 * S.ctw, ql, nqc, C, S); break;
         }
         LAUNCH_OK("k_ct_limbs");
-        if (direct) {
+        if (direct && (parts & 2)) {
             // products straight into dbout: 16 slots x (16 RTW) rows per workgroup
             const int rtw = (n1 <= 128 && R % 32 == 0) ? 2 : 1;
             const dim3 g((kN / 16) * (R / (16 * rtw)) * 2);
@@ -474,8 +476,9 @@ int launch_db_batched(kspir_context* c, const kspir_database* dbc, int nq, hipSt
             LAUNCH_OK("k_db_mfma_direct");
             continue;
         }
+        if (direct) continue;
         const size_t lds = (size_t)4 * C * (n1 + 16);
-        switch (n1) {
+        if (parts & 2) switch (n1) {
             case 64:
                 HIP_OK(hipFuncSetAttribute((const void*)k_db_mfma<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
                 hipLaunchKernelGGL(k_db_mfma<1>, dim3(kN), dim3(256), lds, st, db->limbs, ql, mf, R, C, MC, c->T);
@@ -490,8 +493,9 @@ int launch_db_batched(kspir_context* c, const kspir_database* dbc, int nq, hipSt
                 break;
         }
         LAUNCH_OK("k_db_mfma");
-        hipLaunchKernelGGL(k_mfma_scatter, dim3(kN / 32, (R / 16) * (C / 16), 2), dim3(256), 0, st, mf,
-                           c->dbout + q0 * S.dbout, R, C, n2, nqc, S);
+        if (parts & 4)
+            hipLaunchKernelGGL(k_mfma_scatter, dim3(kN / 32, (R / 16) * (C / 16), 2), dim3(256), 0, st, mf,
+                               c->dbout + q0 * S.dbout, R, C, n2, nqc, S);
         LAUNCH_OK("k_mfma_scatter");
     }
     return KSPIR_OK;
@@ -1396,6 +1400,45 @@ int kspir_time_db_pass(kspir_context* c, const kspir_database* db, const kspir_k
     float ms = 0;
     HIP_OK(hipEventElapsedTime(&ms, e0, e1));
     *ms_per_launch = ms / iters;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return KSPIR_OK;
+}
+
+int kspir_time_db_pass_batch(kspir_context* c, const kspir_database* db, const kspir_keys* const* keys, int32_t nq,
+                             const uint64_t* queries_dev, int iters, float* ms3) {
+    if (!ms3 || iters < 1 || !keys || nq < 2) return fail(KSPIR_EINVAL, "bad timing arguments (a batch needs 2 queries)");
+    for (int i = 0; i < nq; ++i) {
+        int rc = check_pair(c, db, keys[i], true);
+        if (rc) return rc;
+    }
+    if (!mfma_shape(db)) return fail(KSPIR_EINVAL, "database shape has no MFMA pass");
+    HIP_OK(hipSetDevice(c->device));
+    int rc = carve_workspace(c, db->r, db->n1, nq);
+    if (rc) return rc;
+    rc = stage_batch_keys(c, keys, nq);
+    if (rc) return rc;
+    // one first dimension fills the rotation words and makes the lazy allocations
+    rc = enqueue_answer(c, db, c->kp_dev, nq, queries_dev, nullptr, c->stream, nullptr, nullptr, true);
+    if (rc) return rc;
+    rc = launch_db_batched(c, db, nq, c->stream, 0);   // makes its lazy allocations, launches nothing
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    const int part[3] = {1, 2, 4};
+    for (int t = 0; t < 3; ++t) {
+        HIP_OK(hipEventRecord(e0, c->stream));
+        for (int i = 0; i < iters; ++i) {
+            rc = launch_db_batched(c, db, nq, c->stream, part[t]);
+            if (rc) return rc;
+        }
+        HIP_OK(hipEventRecord(e1, c->stream));
+        HIP_OK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+        ms3[t] = ms / iters;
+    }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     return KSPIR_OK;

@@ -27,6 +27,8 @@
 
 #include "kspir_oracle.h"
 
+#include <immintrin.h>
+
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -873,8 +875,8 @@ void reorient_cipher(u64* cbuf, const Rlwe& input, const std::vector<Rlwe>& rot,
 
 // crt.cpp:643-768 fastMultiplyQueryByDatabaseDim1InvCRT: u64 sums of the low
 // (q1) and high (q2) 32-bit halves without per-term reduction, then
-// crt_compose.  out: [n2][2N] (a in [0,N), b in [N,2N)).
-void db_product(u64* out, const u64* db, const u64* cbuf, int n1, int n2) {
+// crt_compose.  out: [n2][2N] (a in [0,N), b in [N,2N)).  Scalar form.
+void db_product_scalar(u64* out, const u64* db, const u64* cbuf, int n1, int n2) {
 #pragma omp parallel for num_threads(g_threads) schedule(static)
     for (int z = 0; z < N; ++z) {
         const u64* ct = cbuf + (size_t)z * 2 * n1;
@@ -893,6 +895,44 @@ void db_product(u64* out, const u64* db, const u64* cbuf, int n1, int n2) {
             out[(size_t)i * 2 * N + N + z] = crt_compose(sb0, sb1);
         }
     }
+}
+
+// The same sums in the shape of the reference's AVX-512 branch
+// (crt.cpp:660-768): one 512-bit load holds the packed a|b words of four
+// rotations, each database word is broadcast to its a and b lanes, and
+// _mm512_mul_epu32 forms the q1 (low halves) and q2 (high halves) products,
+// accumulated per lane without reduction.  Lanes 0, 2, 4, 6 carry a and 1, 3,
+// 5, 7 carry b.  Exact for N1 <= 256 like the scalar form (every product is
+// below 2^56).  Requires N1 % 4 == 0.
+__attribute__((target("avx512f"))) void db_product_avx512(u64* out, const u64* db, const u64* cbuf, int n1, int n2) {
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int z = 0; z < N; ++z) {
+        const u64* ct = cbuf + (size_t)z * 2 * n1;
+        for (int i = 0; i < n2; ++i) {
+            const u64* d = db + (size_t)z * n2 * n1 + (size_t)i * n1;
+            __m512i s_lo = _mm512_setzero_si512(), s_hi = _mm512_setzero_si512();
+            for (int k = 0; k < n1; k += 4) {
+                const __m512i c = _mm512_loadu_si512(ct + 2 * k);
+                const __m512i w = _mm512_set_epi64((long long)d[k + 3], (long long)d[k + 3], (long long)d[k + 2],
+                                                   (long long)d[k + 2], (long long)d[k + 1], (long long)d[k + 1],
+                                                   (long long)d[k], (long long)d[k]);
+                s_lo = _mm512_add_epi64(s_lo, _mm512_mul_epu32(c, w));
+                s_hi = _mm512_add_epi64(s_hi, _mm512_mul_epu32(_mm512_srli_epi64(c, 32), _mm512_srli_epi64(w, 32)));
+            }
+            alignas(64) u64 lo[8], hi[8];
+            _mm512_store_si512(lo, s_lo);
+            _mm512_store_si512(hi, s_hi);
+            out[(size_t)i * 2 * N + z] = crt_compose(lo[0] + lo[2] + lo[4] + lo[6], hi[0] + hi[2] + hi[4] + hi[6]);
+            out[(size_t)i * 2 * N + N + z] = crt_compose(lo[1] + lo[3] + lo[5] + lo[7], hi[1] + hi[3] + hi[5] + hi[7]);
+        }
+    }
+}
+
+// the AVX-512 form where the host has it, the scalar form otherwise
+void db_product(u64* out, const u64* db, const u64* cbuf, int n1, int n2) {
+    static const bool avx512 = __builtin_cpu_supports("avx512f");
+    if (avx512 && n1 % 4 == 0) db_product_avx512(out, db, cbuf, n1, n2);
+    else db_product_scalar(out, db, cbuf, n1, n2);
 }
 
 // twosteps.cpp:599-677 evalAutoRNSCRT (giant steps; ell 2, PP 2^20, Bg 2^18)
@@ -1334,6 +1374,11 @@ void ko_permutation_matrix(int32_t* out, int n1) {
 void ko_automorphic_rns(uint64_t* out, const uint64_t* in, int index, uint64_t q) {
     automorphic_rns(out, in, index, q);
 }
+
+void ko_db_product_scalar(uint64_t* out, const uint64_t* db, const uint64_t* cbuf, int n1, int n2) {
+    db_product_scalar(out, db, cbuf, n1, n2);
+}
+int ko_db_product_vectorised(void) { return __builtin_cpu_supports("avx512f") ? 1 : 0; }
 
 void ko_db_product(uint64_t* out, const uint64_t* db, const uint64_t* cbuf, int n1, int n2) {
     db_product(out, db, cbuf, n1, n2);

@@ -50,6 +50,10 @@ void ko_permutation_matrix(int32_t* out, int n1);   /* utils.cpp:1591 */
 void ko_automorphic_rns(uint64_t* out, const uint64_t* in, int index, uint64_t q);
 /* one call of fastMultiplyQueryByDatabaseDim1InvCRT: out [n2][2N] */
 void ko_db_product(uint64_t* out, const uint64_t* db, const uint64_t* cipherbuf, int n1, int n2);
+/* the same with the scalar loop, and whether ko_db_product takes the
+   AVX-512 loop (crt.cpp:660-768's shape) on this host */
+void ko_db_product_scalar(uint64_t* out, const uint64_t* db, const uint64_t* cipherbuf, int n1, int n2);
+int ko_db_product_vectorised(void);
 
 /* instance generation in test-pir order (srand(seed) first; seed 1 == glibc default) */
 int ko_gen_instance(int r, int n1, unsigned seed, int plant,

@@ -280,6 +280,17 @@ def automorphic_rns(x, index, q):
     return out
 
 
+def db_product_scalar(db_block, cbuf, n1, n2):
+    out = np.zeros((n2, 2 * N), dtype=np.uint64)
+    lib().ko_db_product_scalar(p64(out), p64(np.ascontiguousarray(db_block)), p64(np.ascontiguousarray(cbuf)), n1, n2)
+    return out
+
+
+def db_product_vectorised():
+    """whether db_product takes the AVX-512 loop on this host"""
+    return bool(lib().ko_db_product_vectorised())
+
+
 def db_product(db_block, cbuf, n1, n2):
     out = np.zeros((n2, 2 * N), dtype=np.uint64)
     lib().ko_db_product(p64(out), p64(np.ascontiguousarray(db_block)), p64(np.ascontiguousarray(cbuf)), n1, n2)

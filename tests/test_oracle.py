@@ -239,6 +239,24 @@ def test_db_product_matches_reference_kernel(oracle):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("n1,n2", [(128, 16), (256, 8), (4, 512)])
+def test_db_product_vector_loop_matches_scalar(oracle, n1, n2):
+    """the oracle's AVX-512 database loop (the CPU baseline's) and its
+    scalar loop give the same words, including all-(q-1) words where the
+    unreduced sums are largest"""
+    if not oracle.db_product_vectorised():
+        pytest.skip("host without AVX-512: the scalar loop is the only one")
+    rng = np.random.default_rng(n1)
+    q1, q2 = oracle.Q1, oracle.Q2
+    lo = rng.integers(0, q1, (2, N * n2 * n1 + N * n1 * 2), dtype=np.uint64)
+    hi = rng.integers(0, q2, (2, N * n2 * n1 + N * n1 * 2), dtype=np.uint64)
+    words = lo[0] | (hi[0] << np.uint64(32))
+    words[:4096] = np.uint64(q1 - 1) | (np.uint64(q2 - 1) << np.uint64(32))
+    db, cbuf = words[:N * n2 * n1].copy(), words[N * n2 * n1:].copy()
+    cbuf[:2 * n1] = np.uint64(q1 - 1) | (np.uint64(q2 - 1) << np.uint64(32))
+    assert np.array_equal(oracle.db_product(db, cbuf, n1, n2), oracle.db_product_scalar(db, cbuf, n1, n2))
+
+
 # ------------------------------------------------------------ end to end
 
 def test_readme_target_indices(oracle):
