@@ -29,6 +29,13 @@ int kspir_device_count(int* count);
  *                     slots per workgroup, query limbs re-read from L2: it
  *                     moves 1 GiB less through HBM per 64-query step at r 16
  *                     but waits on those L2 reads and runs slower there)
+ *   "ext_fused"       1 (default): with the DMA giant-step kernel, the external
+ *                     products run in the same workgroups (no coefficient
+ *                     planes written out); 0: separate kernels
+ *   "chunk_blocks"    batches on that path run the database pass, giant steps
+ *                     and external products this many blocks at a time (a
+ *                     power of two, default 32; 0: the whole database), so
+ *                     their first-dimension buffers hold one chunk of blocks
  *   "fused_min_batch" batches of at least this many queries run the giant
  *                     steps and first-dimension sum as one workgroup per
  *                     (query, block) (default 16); smaller ones spread them
@@ -72,8 +79,10 @@ int kspir_autotune(kspir_context* ctx, const kspir_database* db, const kspir_key
                    int32_t max_batch, double latency_budget_ms, int32_t* serve_batch, char* report,
                    size_t report_len);
 
-/* device bytes one query of shape (r, N1) adds to a batch (workspace and
- * batched database-pass scratch): the unit of the serving loop's admission */
+/* device bytes per query of a 64-query batch of shape (r, N1) under the
+ * default options (workspace and batched database-pass scratch, divided by
+ * 64).  The serving loop admits each batch by its exact footprint for the
+ * batch size and the lane's options. */
 int kspir_query_footprint_bytes(int32_t r, int32_t n1, size_t* bytes);
 int kspir_device_alloc(kspir_context* ctx, size_t bytes, void** out);
 int kspir_device_free(kspir_context* ctx, void* ptr);

@@ -124,6 +124,9 @@ struct kspir_context {
     size_t mws_bytes = 0;
     int mfma_min_batch = 2;         // batches of at least this many queries use the MFMA pass
     int mfma_max_chunk = 1 << 30;   // cap on queries per MFMA database sweep (LDS also caps it)
+    int ext_fused = 1;              // the DMA giant-step kernel also runs the external products
+    int chunk_blocks = 32;          // blocks per chunk of the batched pipeline (0: the whole database at once)
+    int cur_cb = 0;                 // chunk size the current workspace carve was made for (0: not chunked)
     int mfma_direct = 0;            // 1: MFMA products written straight into dbout (k_db_mfma_direct); 0: tile-major + scatter
     int fused_min_batch = 16;       // batches of at least this many queries use k_giant_fused
     int fused_pack = 1;             // one k_pack_level per packing level (0: three kernels per level)
@@ -280,18 +283,41 @@ int build_tables(kspir_context* c) {
 }
 
 
-// device bytes one query of shape (r, n1) adds to a batch: its share of the
-// carved workspace plus its share of the batched database pass's scratch
-// (query limbs and tile-major products)
-size_t query_footprint_bytes(int r, int n1, bool mfma_direct = false) {
+// Blocks per chunk of the batched pipeline for nq queries of shape (r, n1),
+// or 0 when that batch takes the unchunked path.  A chunked batch runs the
+// MFMA database pass, the giant steps and the external products one chunk of
+// blocks at a time, so its first-dimension buffers hold one chunk; only the
+// int8-MFMA pass with the DMA giant-step kernel and fused external products
+// is chunked.
+int chunk_blocks_for(const kspir_context* c, int r, int n1, int nq) {
     const int n2 = kN / 2 / n1;
-    const size_t n = kN;
-    const size_t rh = r / 2 > 0 ? r / 2 : 1;
-    const size_t words = 9 * n * 2 + 3 * n + 6 * n + 3 * n + (size_t)n1 * 4 * n + (size_t)r * n2 * 4 * n * 2 +
-                         (size_t)r * n2 * 2 * n + (size_t)r * 4 * n * 2 + (size_t)r * 16 * n + (size_t)r * 4 * n * 2 +
-                         rh * 2 * n + rh * 16 * n + 2 * 4 * n * 2 + 2 * 2 * n * 2;
-    const size_t mfma = (size_t)kN * 2 * 4 * 2 * n1 + (mfma_direct ? 0 : (size_t)2 * r * n2 * 2 * kN * 4);
-    return words * 4 + mfma;
+    const bool mfma = n1 % 64 == 0 && n1 <= 256 && (r * n2) % 16 == 0;
+    if (!mfma || nq < c->mfma_min_batch || nq < c->fused_min_batch || !c->giant_dma || c->giant_wgs_per_cu != 1 ||
+        !c->ext_fused)
+        return 0;
+    int cb = c->chunk_blocks > 0 && c->chunk_blocks < r ? c->chunk_blocks : r;
+    while ((cb * n2) % 16) cb *= 2;   // whole 16-row MFMA tiles per chunk
+    return cb > r ? r : cb;
+}
+
+size_t layout_workspace(kspir_context* c, int r, int n1, int nq, uint32_t* base, bool answer_only, int cb = 0);
+int mfma_sweep_queries(const kspir_context* c, int n1);
+
+// Device bytes a batch of nq queries of shape (r, n1) needs beyond the
+// database itself: the carved workspace and the MFMA pass's scratch (query
+// limbs; tile-major products of one chunk's rows).  The serving loop admits
+// batches against it.
+size_t batch_footprint_bytes(kspir_context* c, int r, int n1, int nq) {
+    const int cb = chunk_blocks_for(c, r, n1, nq);
+    size_t bytes = layout_workspace(c, r, n1, nq, nullptr, false, cb) * 4;
+    const int n2 = kN / 2 / n1;
+    if (n1 % 64 == 0 && n1 <= 256 && (r * n2) % 16 == 0 && nq >= c->mfma_min_batch) {
+        const int nqc = nq < mfma_sweep_queries(c, n1) ? nq : mfma_sweep_queries(c, n1);
+        const size_t C = (size_t)(2 * nqc + 15) / 16 * 16;
+        const size_t rows = (size_t)(cb ? cb : r) * n2;
+        bytes += (size_t)kN * 2 * 4 * C * n1 + (c->mfma_direct ? 0 : 2 * rows * C * kN * 4);
+    }
+    return bytes;
 }
 
 // Layout of the answer workspace for nq queries of shape (r, n1): sets c->S,
@@ -299,7 +325,7 @@ size_t query_footprint_bytes(int r, int n1, bool mfma_direct = false) {
 // the size in 32-bit words.  answer_only leaves out the host-transfer and
 // inspection areas (q_dev, resp_dev, u64tmp, u64tmp2), which the answer
 // kernels themselves never touch.
-size_t layout_workspace(kspir_context* c, int r, int n1, int nq, uint32_t* base, bool answer_only) {
+size_t layout_workspace(kspir_context* c, int r, int n1, int nq, uint32_t* base, bool answer_only, int cb) {
     const int n2 = kN / 2 / n1;
     const size_t n = kN;
     const size_t rh = r / 2 > 0 ? r / 2 : 1;
@@ -311,6 +337,13 @@ size_t layout_workspace(kspir_context* c, int r, int n1, int nq, uint32_t* base,
     S.fd = (size_t)r * 4 * n; S.ecoef = (size_t)r * 4 * n; S.epart = (size_t)r * 16 * n;
     S.ext = (size_t)r * 4 * n; S.pcoef = rh * 2 * n; S.ppart = rh * 16 * n;
     S.qin64 = 4 * n; S.resp64 = 2 * n;
+    if (cb > 0) {
+        // chunked batch: first-dimension products of one chunk of blocks;
+        // the per-row giant buffers, the coefficient planes and the external
+        // products' partial sums are not used (fused kernel)
+        S.dbout = (size_t)cb * n2 * 4 * n;
+        S.gcoef = S.gks = S.fd = S.ecoef = S.epart = 0;
+    }
     auto up = [](size_t w) { return (w + 63) / 64 * 64; };
     struct Part { size_t words; uint32_t** dst; };
     const size_t Q = (size_t)nq;
@@ -338,9 +371,12 @@ size_t layout_workspace(kspir_context* c, int r, int n1, int nq, uint32_t* base,
     return total;
 }
 
-// carve the workspace for nq queries of shape (r, n1); grows on demand
-int carve_workspace(kspir_context* c, int r, int n1, int nq) {
-    const size_t total = layout_workspace(c, r, n1, nq, nullptr, false);
+// carve the workspace for nq queries of shape (r, n1); grows on demand.
+// allow_chunk = false keeps whole-database buffers (callers that run the
+// first dimension alone).
+int carve_workspace(kspir_context* c, int r, int n1, int nq, bool allow_chunk = true) {
+    const int cb = allow_chunk ? chunk_blocks_for(c, r, n1, nq) : 0;
+    const size_t total = layout_workspace(c, r, n1, nq, nullptr, false, cb);
     if (total * 4 > c->ws_bytes) {
         if (c->ws) {
             HIP_OK(hipStreamSynchronize(c->stream));
@@ -353,7 +389,8 @@ int carve_workspace(kspir_context* c, int r, int n1, int nq) {
         HIP_OK(hipMalloc(&c->ws, total * 4));
         c->ws_bytes = total * 4;
     }
-    layout_workspace(c, r, n1, nq, (uint32_t*)c->ws, false);
+    layout_workspace(c, r, n1, nq, (uint32_t*)c->ws, false, cb);
+    c->cur_cb = cb;
     return KSPIR_OK;
 }
 
@@ -403,29 +440,41 @@ bool mfma_shape(const kspir_database* db) {
 // nqc_max queries); results land in c->dbout exactly as k_db_product's
 // parts (timing aid): bit 0 the query limbs, bit 1 the MFMA products, bit 2
 // the move into dbout; the whole pass is 7
-int launch_db_batched(kspir_context* c, const kspir_database* dbc, int nq, hipStream_t st, int parts = 7) {
+// queries of one MFMA sweep: the tile-major kernel stages a slot's query
+// limbs in LDS, which caps its columns; the direct kernel reads them from L2
+// (256 columns cap the limb scratch at 128 queries per sweep either way)
+int mfma_sweep_queries(const kspir_context* c, int n1) {
+    const int lds_cap = 160 * 1024;
+    int cmax = c->mfma_direct ? 256 : lds_cap / (4 * (n1 + 16)) / 16 * 16;
+    if (cmax > 256) cmax = 256;
+    return cmax / 2 < c->mfma_max_chunk ? cmax / 2 : c->mfma_max_chunk;
+}
+
+// Queries q_lo .. q_lo + nq (in sweeps of mfma_sweep_queries), database
+// rows of blocks b0 .. b0 + nb (nb 0: every block); the products land in
+// c->dbout, block-relative to b0.  A chunked pipeline calls it once with
+// parts 1 (query limbs) and then per chunk with parts 6, always with the
+// same nq and nb, so every call carves the scratch the same way.
+int launch_db_batched(kspir_context* c, const kspir_database* dbc, int q_lo, int nq, hipStream_t st, int parts = 7,
+                      int b0 = 0, int nb = 0) {
     kspir_database* db = const_cast<kspir_database*>(dbc);
-    const int r = db->r, n1 = db->n1, n2 = db->n2, R = r * n2;
+    const int r = db->r, n1 = db->n1, n2 = db->n2, Rt = r * n2;
+    if (nb <= 0) nb = r;
+    const int R = nb * n2, r0 = b0 * n2;
     const size_t dbwords = (size_t)r * kN * (kN / 2);
     {
         std::lock_guard<std::mutex> lock(db->limbs_mu);
         if (!db->limbs) {
             int8_t* l = nullptr;
             HIP_OK(hipMalloc(&l, dbwords * 8));
-            hipLaunchKernelGGL(k_db_limbs, dim3(kN / 64, R), dim3(256), 0, st, db->words, l, n1, R);
+            hipLaunchKernelGGL(k_db_limbs, dim3(kN / 64, Rt), dim3(256), 0, st, db->words, l, n1, Rt);
             LAUNCH_OK("k_db_limbs");
             HIP_OK(hipStreamSynchronize(st));   // complete before another context's stream reads them
             db->limbs = l;
         }
     }
-    // the tile-major kernel stages a slot's query limbs in LDS, which caps
-    // its columns; the direct kernel reads them from L2 (256 columns cap the
-    // limb scratch at 128 queries per sweep either way)
     const bool direct = c->mfma_direct != 0;
-    const int lds_cap = 160 * 1024;
-    int cmax = direct ? 256 : lds_cap / (4 * (n1 + 16)) / 16 * 16;
-    if (cmax > 256) cmax = 256;
-    const int nqc_max = cmax / 2 < c->mfma_max_chunk ? cmax / 2 : c->mfma_max_chunk;
+    const int nqc_max = mfma_sweep_queries(c, n1);
     const int C_full = ((2 * (nq < nqc_max ? nq : nqc_max)) + 15) / 16 * 16;
     const size_t ql_bytes = (size_t)kN * 2 * 4 * C_full * n1;
     const size_t mf_bytes = direct ? 0 : (size_t)2 * R * C_full * kN * 4;
@@ -449,8 +498,8 @@ int launch_db_batched(kspir_context* c, const kspir_database* dbc, int nq, hipSt
         for (int w = 0; w < 7; ++w) MC.cw[p][w] = (uint32_t)(((u128)powmod(2, 7 * w, q) << 32) % q);
     }
     const QS& S = c->S;
-    for (int q0 = 0; q0 < nq; q0 += nqc_max) {
-        const int nqc = nq - q0 < nqc_max ? nq - q0 : nqc_max;
+    for (int q0 = q_lo; q0 < q_lo + nq; q0 += nqc_max) {
+        const int nqc = q_lo + nq - q0 < nqc_max ? q_lo + nq - q0 : nqc_max;
         const int C = (2 * nqc + 15) / 16 * 16;
         if (parts & 1) switch (n1) {
             case 64: hipLaunchKernelGGL(k_ct_limbs<64>, dim3(kN / 32, C / 2, 2), dim3(256), 0, st, c->ctw + q0 * S.ctw, ql, nqc, C, S); break;
@@ -464,15 +513,15 @@ int launch_db_batched(kspir_context* c, const kspir_database* dbc, int nq, hipSt
             const dim3 g((kN / 16) * (R / (16 * rtw)) * 2);
             uint32_t* out = c->dbout + q0 * S.dbout;
             if (n1 == 64 && rtw == 2)
-                hipLaunchKernelGGL((k_db_mfma_direct<1, 2>), g, dim3(512), 0, st, db->limbs, ql, out, R, C, n2, nqc, S, MC, c->T);
+                hipLaunchKernelGGL((k_db_mfma_direct<1, 2>), g, dim3(512), 0, st, db->limbs, ql, out, R, Rt, r0, C, n2, nqc, S, MC, c->T);
             else if (n1 == 64)
-                hipLaunchKernelGGL((k_db_mfma_direct<1, 1>), g, dim3(512), 0, st, db->limbs, ql, out, R, C, n2, nqc, S, MC, c->T);
+                hipLaunchKernelGGL((k_db_mfma_direct<1, 1>), g, dim3(512), 0, st, db->limbs, ql, out, R, Rt, r0, C, n2, nqc, S, MC, c->T);
             else if (n1 == 128 && rtw == 2)
-                hipLaunchKernelGGL((k_db_mfma_direct<2, 2>), g, dim3(512), 0, st, db->limbs, ql, out, R, C, n2, nqc, S, MC, c->T);
+                hipLaunchKernelGGL((k_db_mfma_direct<2, 2>), g, dim3(512), 0, st, db->limbs, ql, out, R, Rt, r0, C, n2, nqc, S, MC, c->T);
             else if (n1 == 128)
-                hipLaunchKernelGGL((k_db_mfma_direct<2, 1>), g, dim3(512), 0, st, db->limbs, ql, out, R, C, n2, nqc, S, MC, c->T);
+                hipLaunchKernelGGL((k_db_mfma_direct<2, 1>), g, dim3(512), 0, st, db->limbs, ql, out, R, Rt, r0, C, n2, nqc, S, MC, c->T);
             else
-                hipLaunchKernelGGL((k_db_mfma_direct<4, 1>), g, dim3(512), 0, st, db->limbs, ql, out, R, C, n2, nqc, S, MC, c->T);
+                hipLaunchKernelGGL((k_db_mfma_direct<4, 1>), g, dim3(512), 0, st, db->limbs, ql, out, R, Rt, r0, C, n2, nqc, S, MC, c->T);
             LAUNCH_OK("k_db_mfma_direct");
             continue;
         }
@@ -481,15 +530,15 @@ int launch_db_batched(kspir_context* c, const kspir_database* dbc, int nq, hipSt
         if (parts & 2) switch (n1) {
             case 64:
                 HIP_OK(hipFuncSetAttribute((const void*)k_db_mfma<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-                hipLaunchKernelGGL(k_db_mfma<1>, dim3(kN), dim3(256), lds, st, db->limbs, ql, mf, R, C, MC, c->T);
+                hipLaunchKernelGGL(k_db_mfma<1>, dim3(kN), dim3(256), lds, st, db->limbs, ql, mf, R, Rt, r0, C, MC, c->T);
                 break;
             case 128:
                 HIP_OK(hipFuncSetAttribute((const void*)k_db_mfma<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-                hipLaunchKernelGGL(k_db_mfma<2>, dim3(kN), dim3(256), lds, st, db->limbs, ql, mf, R, C, MC, c->T);
+                hipLaunchKernelGGL(k_db_mfma<2>, dim3(kN), dim3(256), lds, st, db->limbs, ql, mf, R, Rt, r0, C, MC, c->T);
                 break;
             default:
                 HIP_OK(hipFuncSetAttribute((const void*)k_db_mfma<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-                hipLaunchKernelGGL(k_db_mfma<4>, dim3(kN), dim3(256), lds, st, db->limbs, ql, mf, R, C, MC, c->T);
+                hipLaunchKernelGGL(k_db_mfma<4>, dim3(kN), dim3(256), lds, st, db->limbs, ql, mf, R, Rt, r0, C, MC, c->T);
                 break;
         }
         LAUNCH_OK("k_db_mfma");
@@ -531,6 +580,39 @@ int enqueue_packing(kspir_context* c, const KeyPtrs* kp, int r, int nq, hipStrea
     return KSPIR_OK;
 }
 
+// The batched pipeline after the baby steps, one chunk of c->cur_cb blocks at
+// a time: per MFMA sweep of queries the query limbs once, then per chunk the
+// database pass over the chunk's rows and the giant steps with the external
+// products of every (query, block) of the chunk (k_giant_fused_dma<true>,
+// block offset b0 into ext); then the packing tree over all blocks.  The
+// database is still read once per sweep, a chunk at a time.
+int enqueue_chunked(kspir_context* c, const kspir_database* db, const KeyPtrs* kp, int nq, uint64_t* resp,
+                    hipStream_t st) {
+    const int r = db->r, n1 = db->n1, n2 = db->n2, cb = c->cur_cb;
+    const QS& S = c->S;
+    const Tabs& T = c->T;
+    const int nqc_max = mfma_sweep_queries(c, n1);
+    for (int q0 = 0; q0 < nq; q0 += nqc_max) {
+        const int nqc = nq - q0 < nqc_max ? nq - q0 : nqc_max;
+        int rc = launch_db_batched(c, db, q0, nqc, st, 1, 0, cb);
+        if (rc) return rc;
+        for (int b0 = 0; b0 < r; b0 += cb) {
+            rc = launch_db_batched(c, db, q0, nqc, st, 6, b0, cb);
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_giant_fused_dma<true>, dim3(cb * nqc), dim3(kFusedThreads), 0, st,
+                               c->dbout + q0 * S.dbout, kp + q0, nullptr, c->extA + q0 * S.ext, cb, nqc, n1, n2, S, T,
+                               b0);
+            LAUNCH_OK("k_giant_fused_dma");
+        }
+    }
+    uint32_t* packed = nullptr;
+    int rc = enqueue_packing(c, kp, r, nq, st, &packed);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_response, dim3(2, nq), dim3(kNttThreads), 0, st, packed, resp, S.ext, T);
+    LAUNCH_OK("k_response");
+    return KSPIR_OK;
+}
+
 // The online answer (test-pir.cpp:107-116) for nq queries against one
 // database, enqueued on `st`.  q_dev: [nq][4][N] u64; resp: [nq][2][N] u64
 // (unused when first_dim_only).  kp: device array of nq key-pointer sets.
@@ -561,16 +643,27 @@ int enqueue_answer(kspir_context* c, const kspir_database* db, const KeyPtrs* kp
     LAUNCH_OK("k_baby_rot_pair");
     // database pass (crt.cpp:643-768): int8-MFMA limb GEMM for a batch,
     // u64-accumulating VALU stream for a single query
-    int rc = (allow_mfma && nq >= c->mfma_min_batch && mfma_shape(db)) ? launch_db_batched(c, db, nq, st)
+    if (c->cur_cb > 0) {
+        if (!allow_mfma || first_dim_only || nq < c->fused_min_batch)
+            return fail(KSPIR_EINVAL, "internal: chunked workspace for an unchunked pipeline");
+        return enqueue_chunked(c, db, kp, nq, resp, st);
+    }
+    int rc = (allow_mfma && nq >= c->mfma_min_batch && mfma_shape(db)) ? launch_db_batched(c, db, 0, nq, st)
                                                           : launch_db_product(c, db, nq, st);
     if (rc) return rc;
     // giant steps (twosteps.cpp:599-677) and the first-dimension sum: one
     // fused workgroup per (query, block) for a batch; for few queries, one
     // workgroup per (block, row, prime) keeps the GPU full
+    bool ext_done = false;   // the giant-step kernel ran the external products too
     if (nq >= c->fused_min_batch) {
-        if (c->giant_dma && c->giant_wgs_per_cu == 1)
-            hipLaunchKernelGGL(k_giant_fused_dma, dim3(r * nq), dim3(kFusedThreads), 0, st, c->dbout, kp, c->fd,
-                               c->ecoef, r, nq, n1, n2, S, T);
+        if (c->giant_dma && c->giant_wgs_per_cu == 1 && c->ext_fused && !first_dim_only) {
+            // the ext output of a query has the ecoef stride's layout [r][ab][p][N]
+            hipLaunchKernelGGL(k_giant_fused_dma<true>, dim3(r * nq), dim3(kFusedThreads), 0, st, c->dbout, kp,
+                               fd_u64 ? c->fd : nullptr, c->extA, r, nq, n1, n2, S, T);
+            ext_done = true;
+        } else if (c->giant_dma && c->giant_wgs_per_cu == 1)
+            hipLaunchKernelGGL(k_giant_fused_dma<false>, dim3(r * nq), dim3(kFusedThreads), 0, st, c->dbout, kp,
+                               fd_u64 ? c->fd : nullptr, c->ecoef, r, nq, n1, n2, S, T);
         else if (c->giant_wgs_per_cu == 2)
             hipLaunchKernelGGL(k_giant_fused2, dim3(r * nq), dim3(kFusedThreads), 0, st, c->dbout, kp, c->fd, c->ecoef,
                                r, nq, n1, n2, S, T);
@@ -600,11 +693,13 @@ int enqueue_answer(kspir_context* c, const kspir_database* db, const KeyPtrs* kp
     }
     if (first_dim_only) return KSPIR_OK;
     // external products (lwe.cpp:761-808)
-    hipLaunchKernelGGL(k_ext_digit, dim3(r, 4, 2 * nq), dim3(kNttThreads), 0, st, c->ecoef, kp, c->epart, S, T);
-    LAUNCH_OK("k_ext_digit");
-    hipLaunchKernelGGL(k_ext_sum, dim3(grid1d((size_t)nq * r * 4 * kN)), dim3(256), 0, st, c->epart, c->extA, r, nq,
-                       S, T);
-    LAUNCH_OK("k_ext_sum");
+    if (!ext_done) {
+        hipLaunchKernelGGL(k_ext_digit, dim3(r, 4, 2 * nq), dim3(kNttThreads), 0, st, c->ecoef, kp, c->epart, S, T);
+        LAUNCH_OK("k_ext_digit");
+        hipLaunchKernelGGL(k_ext_sum, dim3(grid1d((size_t)nq * r * 4 * kN)), dim3(256), 0, st, c->epart, c->extA, r,
+                           nq, S, T);
+        LAUNCH_OK("k_ext_sum");
+    }
     if (ext_u64) {
         hipLaunchKernelGGL(k_response, dim3(2, r), dim3(kNttThreads), 0, st, c->extA, ext_u64, (size_t)4 * kN, T);
         LAUNCH_OK("k_response(ext)");
@@ -1103,6 +1198,7 @@ int kspir_answer_device_noalloc(kspir_context* c, const kspir_database* db, cons
     // every buffer the kernels use is written before it is read, so stale
     // contents of an earlier call need no clearing
     layout_workspace(c, db->r, db->n1, 1, ptr, true);
+    c->cur_cb = 0;
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
     if (st != c->stream) c->last_stream = st;
     // eager launches only (no graph capture), VALU database pass (the MFMA
@@ -1151,7 +1247,7 @@ int kspir_answer_inspect(kspir_context* c, const kspir_database* db, const kspir
     if (rc) return rc;
     if (!query || !response) return fail(KSPIR_EINVAL, "null buffer");
     HIP_OK(hipSetDevice(c->device));
-    rc = carve_workspace(c, db->r, db->n1, 1);
+    rc = carve_workspace(c, db->r, db->n1, 1, false);
     if (rc) return rc;
     const int r = db->r;
     uint64_t* fd_dev = c->u64tmp;
@@ -1187,7 +1283,7 @@ int kspir_matrix_vector_mul_bsgs_rns_crt_large(kspir_context* c, const kspir_dat
     if (rc0) return rc0;
     if (!query || !result) return fail(KSPIR_EINVAL, "null buffer");
     HIP_OK(hipSetDevice(c->device));
-    int rc = carve_workspace(c, db->r, db->n1, 1);
+    int rc = carve_workspace(c, db->r, db->n1, 1, false);
     if (rc) return rc;
     HIP_OK(hipMemcpyAsync(c->q_dev, query, 4 * kN * 8, hipMemcpyHostToDevice, c->stream));
     rc = enqueue_answer(c, db, k->dev_ptrs, 1, c->q_dev, nullptr, c->stream, c->u64tmp, nullptr, true);
@@ -1283,6 +1379,11 @@ int kspir_context_set(kspir_context* c, const char* key, int64_t value) {
         c->fused_pack = value != 0;
     } else if (k == "mfma_direct") {
         c->mfma_direct = value != 0;
+    } else if (k == "ext_fused") {
+        c->ext_fused = value != 0;
+    } else if (k == "chunk_blocks") {
+        if (value < 0 || (value & (value - 1))) return fail(KSPIR_EINVAL, "chunk_blocks must be 0 or a power of two");
+        c->chunk_blocks = (int)value;
     } else if (k == "graphs") {
         c->use_graphs = value != 0;
     } else if (k == "pack_fused_min_wgs") {
@@ -1311,6 +1412,8 @@ int kspir_context_get(kspir_context* c, const char* key, int64_t* value) {
     else if (k == "fused_min_batch") *value = c->fused_min_batch;
     else if (k == "fused_pack") *value = c->fused_pack;
     else if (k == "mfma_direct") *value = c->mfma_direct;
+    else if (k == "ext_fused") *value = c->ext_fused;
+    else if (k == "chunk_blocks") *value = c->chunk_blocks;
     else if (k == "graphs") *value = c->use_graphs;
     else if (k == "db_waves") *value = c->db_waves;
     else if (k == "giant_wgs_per_cu") *value = c->giant_wgs_per_cu;
@@ -1383,7 +1486,7 @@ int kspir_time_db_pass(kspir_context* c, const kspir_database* db, const kspir_k
     int rc = check_pair(c, db, k, true);
     if (rc) return rc;
     HIP_OK(hipSetDevice(c->device));
-    rc = carve_workspace(c, db->r, db->n1, 1);
+    rc = carve_workspace(c, db->r, db->n1, 1, false);
     if (rc) return rc;
     rc = enqueue_answer(c, db, k->dev_ptrs, 1, query_dev, c->resp_dev, c->stream, nullptr, nullptr, true);
     if (rc) return rc;
@@ -1414,14 +1517,14 @@ int kspir_time_db_pass_batch(kspir_context* c, const kspir_database* db, const k
     }
     if (!mfma_shape(db)) return fail(KSPIR_EINVAL, "database shape has no MFMA pass");
     HIP_OK(hipSetDevice(c->device));
-    int rc = carve_workspace(c, db->r, db->n1, nq);
+    int rc = carve_workspace(c, db->r, db->n1, nq, false);
     if (rc) return rc;
     rc = stage_batch_keys(c, keys, nq);
     if (rc) return rc;
     // one first dimension fills the rotation words and makes the lazy allocations
     rc = enqueue_answer(c, db, c->kp_dev, nq, queries_dev, nullptr, c->stream, nullptr, nullptr, true);
     if (rc) return rc;
-    rc = launch_db_batched(c, db, nq, c->stream, 0);   // makes its lazy allocations, launches nothing
+    rc = launch_db_batched(c, db, 0, nq, c->stream, 0);   // makes its lazy allocations, launches nothing
     if (rc) return rc;
     hipEvent_t e0, e1;
     HIP_OK(hipEventCreate(&e0));
@@ -1430,7 +1533,7 @@ int kspir_time_db_pass_batch(kspir_context* c, const kspir_database* db, const k
     for (int t = 0; t < 3; ++t) {
         HIP_OK(hipEventRecord(e0, c->stream));
         for (int i = 0; i < iters; ++i) {
-            rc = launch_db_batched(c, db, nq, c->stream, part[t]);
+            rc = launch_db_batched(c, db, 0, nq, c->stream, part[t]);
             if (rc) return rc;
         }
         HIP_OK(hipEventRecord(e1, c->stream));
@@ -1520,10 +1623,9 @@ struct kspir_server {
                 // query, and as many of its queries (in arrival order) as the
                 // batch cap and the lane's workspace budget admit
                 const kspir_database* db = pending.front().db;
-                const size_t per = query_footprint_bytes(db->r, db->n1, L.ctx->mfma_direct != 0);
-                size_t cap = budget / (per ? per : 1);
-                if (cap < 1) cap = 1;
-                if (cap > (size_t)max_batch) cap = (size_t)max_batch;
+                // the largest batch whose device footprint fits the lane's budget
+                size_t cap = (size_t)max_batch;
+                while (cap > 1 && batch_footprint_bytes(L.ctx, db->r, db->n1, (int)cap) > budget) --cap;
                 take.clear();
                 for (auto it = pending.begin(); it != pending.end() && take.size() < cap;) {
                     if (it->db == db) {
@@ -2017,7 +2119,8 @@ int kspir_gen_giant_keys_from_offline(kspir_context* c, int32_t n1, int32_t nc, 
 
 int kspir_query_footprint_bytes(int32_t r, int32_t n1, size_t* bytes) {
     if (!bytes || !pow2(r) || r > kN || !pow2(n1) || n1 < 32 || n1 > kN / 2) return fail(KSPIR_EINVAL, "bad shape");
-    *bytes = query_footprint_bytes(r, n1);
+    static kspir_context defaults;   // the default options, no device state
+    *bytes = batch_footprint_bytes(&defaults, r, n1, 64) / 64;
     return KSPIR_OK;
 }
 
@@ -2059,6 +2162,8 @@ int kspir_server_create_lanes(kspir_context* c, int32_t max_batch, size_t worksp
             L.ctx->fused_min_batch = c->fused_min_batch;
             L.ctx->fused_pack = c->fused_pack;
             L.ctx->mfma_direct = c->mfma_direct;
+            L.ctx->ext_fused = c->ext_fused;
+            L.ctx->chunk_blocks = c->chunk_blocks;
             L.ctx->pack_fused_min_wgs = c->pack_fused_min_wgs;
             L.ctx->giant_wgs_per_cu = c->giant_wgs_per_cu;
             L.ctx->giant_dma = c->giant_dma;

@@ -727,7 +727,8 @@ struct MfmaConst {
     uint32_t cw[2][7];   // 2^(7w) * 2^32 mod q_p
 };
 
-// one slot z per workgroup (4 waves).  The query limbs of the slot for one
+// one slot z per workgroup (4 waves), rows r0 .. r0 + R of the database's Rt
+// (a chunk of whole blocks; the output's rows are chunk-local).  The query limbs of the slot for one
 // residue are staged in LDS (row pitch n1 + 16 bytes: conflict-free 16-byte
 // reads); each wave streams the database limb tiles of its row tiles from
 // HBM once and sweeps all column tiles.  Output tile-major:
@@ -736,7 +737,8 @@ struct MfmaConst {
 // k-bytes at 16*(l/16); D holds rows 4*(l/16)+j of column l%16.
 template <int KS>
 __global__ void __launch_bounds__(256) k_db_mfma(const int8_t* __restrict__ dbl, const int8_t* __restrict__ ql,
-                                                 uint32_t* __restrict__ mf, int R, int C, MfmaConst MC, Tabs T) {
+                                                 uint32_t* __restrict__ mf, int R, int Rt, int r0, int C, MfmaConst MC,
+                                                 Tabs T) {
     extern __shared__ __attribute__((aligned(16))) int8_t lds_b[];
     constexpr int n1 = KS * 64;
     constexpr int pitch = n1 + 16;
@@ -753,14 +755,15 @@ __global__ void __launch_bounds__(256) k_db_mfma(const int8_t* __restrict__ dbl,
                 *reinterpret_cast<const i8x16*>(qsrc + (size_t)row * n1 + c16 * 16);
         }
         __syncthreads();
-        const int8_t* asrc = dbl + ((size_t)z * 2 + p) * 4 * R * n1;
+        // rows r0 .. r0 + R of the Rt database rows (one chunk of blocks)
+        const int8_t* asrc = dbl + ((size_t)z * 2 + p) * 4 * Rt * n1 + (size_t)r0 * n1;
         for (int rt = wv; rt < RT; rt += 4) {
             i8x16 a[4][KS];
 #pragma unroll
             for (int u = 0; u < 4; ++u)
 #pragma unroll
                 for (int ks = 0; ks < KS; ++ks)
-                    a[u][ks] = *reinterpret_cast<const i8x16*>(asrc + ((size_t)u * R + rt * 16 + lr) * n1 + ks * 64 + lk);
+                    a[u][ks] = *reinterpret_cast<const i8x16*>(asrc + ((size_t)u * Rt + rt * 16 + lr) * n1 + ks * 64 + lk);
             for (int ct = 0; ct < CT; ++ct) {
                 i32x4 acc[7];
 #pragma unroll
@@ -809,7 +812,8 @@ constexpr int kMfmaRhoStride = 20;                    // words per staged row (1
 constexpr int kMfmaZStride = 32 * kMfmaRhoStride + 4; // words per staged slot (up to 32 rows)
 template <int KS, int RTW>
 __global__ void __launch_bounds__(512) k_db_mfma_direct(const int8_t* __restrict__ dbl, const int8_t* __restrict__ ql,
-                                                        uint32_t* __restrict__ dbout, int R, int C, int n2, int nq,
+                                                        uint32_t* __restrict__ dbout, int R, int Rt, int r0, int C,
+                                                        int n2, int nq,
                                                         QS S, MfmaConst MC, Tabs T) {
     constexpr int n1 = KS * 64;
     __shared__ uint32_t stage[2][16 * kMfmaZStride];
@@ -825,7 +829,7 @@ __global__ void __launch_bounds__(512) k_db_mfma_direct(const int8_t* __restrict
 #pragma unroll
     for (int zi = 0; zi < 2; ++zi) {
         const int z = zg * 16 + 2 * wv + zi;
-        const int8_t* asrc = dbl + ((size_t)z * 2 + p) * 4 * R * n1;
+        const int8_t* asrc = dbl + ((size_t)z * 2 + p) * 4 * Rt * n1 + (size_t)r0 * n1;
 #pragma unroll
         for (int t = 0; t < RTW; ++t)
 #pragma unroll
@@ -833,7 +837,7 @@ __global__ void __launch_bounds__(512) k_db_mfma_direct(const int8_t* __restrict
 #pragma unroll
                 for (int ks = 0; ks < KS; ++ks)
                     a[zi][t][u][ks] = *reinterpret_cast<const i8x16*>(
-                        asrc + ((size_t)u * R + (rg * RTW + t) * 16 + lr) * n1 + ks * 64 + lk);
+                        asrc + ((size_t)u * Rt + (rg * RTW + t) * 16 + lr) * n1 + ks * 64 + lk);
     }
     for (int ct = 0; ct < CT; ++ct) {
         uint32_t* st = stage[ct & 1];
@@ -1232,16 +1236,26 @@ __device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70)
 // of step c runs during the inverse transform of row c+1 (its digits are
 // still in planes 0..3 then), the last one after the loop.  Same words as
 // k_giant_fused: every update is exact mod q.
+// With EXT the external product of the block (lwe.cpp:761-808, what k_ext_digit
+// and k_ext_sum compute) follows in the same workgroup, on the coefficient
+// planes still in LDS: the signed digits (ell 2, PP 2^20, Bg 2^18) of a and b,
+// their transforms in both primes (half h takes prime h), and the RGSW
+// products summed over the four rows with one Montgomery reduction (four
+// products below 2^58 < q * 2^32), written NTT-domain to `out` with the ext
+// stride; without EXT `out` receives the coefficient planes (ecoef).
+template <bool EXT>
 __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_t* __restrict__ dbout,
         const KeyPtrs* __restrict__ kp, uint32_t* __restrict__ fd, uint32_t* __restrict__ ecoef, int r, int nq,
-        int n1, int n2, QS S, Tabs T) {
+        int n1, int n2, QS S, Tabs T, int b0 = 0) {
     __shared__ uint32_t s[8][kPadN];
+    // r: blocks of this launch (dbout holds their rows); b0: index of the
+    // first of them in the outputs fd / ecoef / ext
     const int L = xcd_job(blockIdx.x, r * nq);
-    const int blk = L % r;
+    const int blk = L % r, ob = b0 + blk;
     const size_t qi = L / r;
     dbout += qi * S.dbout;
     if (fd) fd += qi * S.fd;
-    ecoef += qi * S.ecoef;
+    ecoef += qi * (EXT ? S.ext : S.ecoef);
     const uint32_t* gkeys = kp[qi].giant;
     const int tid = threadIdx.x;
     const int half = __builtin_amdgcn_readfirstlane(tid >> 8), ht = tid & 255;
@@ -1353,21 +1367,80 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_
                     s[ab * 2 + p][lds_pad(x0) + i] = v[i];
                 }
                 if (fd)
-                    *reinterpret_cast<uint4*>(fd + (((size_t)blk * 2 + ab) * 2 + p) * kN + x0) =
+                    *reinterpret_cast<uint4*>(fd + (((size_t)ob * 2 + ab) * 2 + p) * kN + x0) =
                         make_uint4(v[0], v[1], v[2], v[3]);
             }
     }
     __syncthreads();
     __builtin_amdgcn_sched_barrier(0);
     ntt_inv_r16x2(s[2 * half], T.p[0], s[2 * half + 1], T.p[1], ht);    // half = ab
+    if (!EXT) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int x0 = 4 * (tid + kFusedThreads * h);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const uint32_t* sp = &s[t][lds_pad(x0)];
+                *reinterpret_cast<uint4*>(ecoef + ((size_t)ob * 4 + t) * kN + x0) =
+                    make_uint4(sp[0], sp[1], sp[2], sp[3]);
+            }
+        }
+        return;
+    }
+    // external product: digit k = 2 src + j of part src (a, b)
+    __builtin_amdgcn_sched_barrier(0);
+    int32_t dg[2][4][4];   // [h][k][i]
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int x0 = 4 * (tid + kFusedThreads * h);
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const uint32_t* sp = &s[t][lds_pad(x0)];
-            *reinterpret_cast<uint4*>(ecoef + ((size_t)blk * 4 + t) * kN + x0) = make_uint4(sp[0], sp[1], sp[2], sp[3]);
-        }
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int src = 0; src < 2; ++src) {
+                const uint64_t v = crt_q(s[2 * src][lds_pad(x0) + i], s[2 * src + 1][lds_pad(x0) + i], T);
+                dg[h][2 * src][i] = signed_digit<18, 20>(v, 0);
+                dg[h][2 * src + 1][i] = signed_digit<18, 20>(v, 1);
+            }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int x0 = 4 * (tid + kFusedThreads * h);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int p = 0; p < 2; ++p)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) s[2 * k + p][lds_pad(x0) + i] = digit_residue(dg[h][k][i], T.p[p].q);
+    }
+    __syncthreads();
+    __builtin_amdgcn_sched_barrier(0);
+    ntt_fwd_r16x2_same(s[0 + half], s[2 + half], T.p[half], ht);    // digits 0, 1 in prime `half`
+    ntt_fwd_r16x2_same(s[4 + half], s[6 + half], T.p[half], ht);    // digits 2, 3
+    __builtin_amdgcn_sched_barrier(0);
+    const uint32_t* rg = kp[qi].rgsw;   // [k][ab][p][N], Montgomery form
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int x0 = 4 * (tid + kFusedThreads * h);
+#pragma unroll
+        for (int ab = 0; ab < 2; ++ab)
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const PrimeTab& P = T.p[p];
+                uint64_t acc[4] = {0, 0, 0, 0};
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const uint4 kv = *reinterpret_cast<const uint4*>(rg + ((k * 2 + ab) * 2 + p) * kN + x0);
+                    const uint32_t* sp = &s[2 * k + p][lds_pad(x0)];
+                    acc[0] += (uint64_t)sp[0] * kv.x;
+                    acc[1] += (uint64_t)sp[1] * kv.y;
+                    acc[2] += (uint64_t)sp[2] * kv.z;
+                    acc[3] += (uint64_t)sp[3] * kv.w;
+                }
+                *reinterpret_cast<uint4*>(ecoef + (((size_t)ob * 2 + ab) * 2 + p) * kN + x0) =
+                    make_uint4(mont_reduce(acc[0], P.q, P.mont_qinv), mont_reduce(acc[1], P.q, P.mont_qinv),
+                               mont_reduce(acc[2], P.q, P.mont_qinv), mont_reduce(acc[3], P.q, P.mont_qinv));
+            }
     }
 }
 

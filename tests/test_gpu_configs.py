@@ -167,7 +167,8 @@ def test_serving_burst_4GB(ctx, oracle, db4gb):
     s = db4gb
     say("[gpu] serving loop, 4 GB database, burst of 64")
     order = list(range(s.nq)) + list(range(16))
-    srv = kspir_amd.Server(ctx, max_batch=64)
+    # a budget that admits about 24 queries of this shape per batch
+    srv = kspir_amd.Server(ctx, max_batch=64, budget_bytes=24 * kspir_amd.query_footprint(s.r, s.n1))
     try:
         tickets = [(c, srv.submit(s.db, s.keys[c], s.queries[c])) for c in order]
         lat = []
@@ -176,11 +177,33 @@ def test_serving_burst_4GB(ctx, oracle, db4gb):
             assert np.array_equal(resp, s.single[c]), c
             lat.append(ms)
         st = srv.stats()
-        assert st["answered"] == len(order) and st["pending"] == 0 and st["batches"] >= 2
+        assert st["answered"] == len(order) and st["pending"] == 0 and st["batches"] >= 3
         say(f"[gpu] serving burst: {st['batches']} batches, p50 {np.percentile(lat, 50):.1f} ms, "
             f"max {max(lat):.1f} ms")
     finally:
         srv.close()
+
+
+@pytest.mark.parametrize("chunk", [1, 2, 4, 0])
+def test_chunked_batch_pipeline(ctx, oracle, chunk):
+    """the batched pipeline run a chunk of blocks at a time (chunk_blocks 1,
+    2, 4; 0 = the whole database at once) gives the single answers' words
+    for 16 clients on an r 8 database, and the oracle's for client 0"""
+    say(f"[gpu] chunked batch r=8 n1=128 nq=16 chunk_blocks={chunk}")
+    s = Shape(ctx, oracle, 8, 128, 16, 4000)
+    ctx.set("chunk_blocks", chunk)
+    try:
+        batch = ctx.answer_batch(s.db, s.keys, s.queries)
+    finally:
+        ctx.set("chunk_blocks", 32)
+    try:
+        for c in range(s.nq):
+            assert np.array_equal(batch[c], s.single[c]), c
+        s.check_decrypts(ctx, batch)
+        if chunk == 1:
+            s.check_oracle(oracle, batch, (0,))
+    finally:
+        s.close()
 
 
 def test_serving_rejects_double_wait(ctx, oracle):
