@@ -411,12 +411,12 @@ def config_c5(args, log):
     ctx = kspir_amd.Context(0)
     perms = kspir_amd.compute_permutation_matrix(n1)
     rec = synthetic_records()
-    out = {"config": "C5 serving: Poisson arrivals, continuous batching (batch cap 48 at 4 GB, 16 at 16 GB; admission "
-                     "by per-query device footprint; 1 lane, and 2 lanes whose batches overlap on the device), "
+    out = {"config": "C5 serving: Poisson arrivals, continuous batching (batch cap 128 at 4 GB, 64 at 16 GB; admission "
+                     "by each batch's device footprint; 1 lane, and 2 lanes whose batches overlap on the device), "
                      "64 clients with distinct keys, resident 4 GB (r=256) and 16 GB (r=1024) databases, N1=128",
            "databases": {}}
-    for r, rates, dur, cap, lane_set in ((256, (200, 500, 1000, 2000), 2.0, 48, (1, 2)),
-                                         (1024, (200, 500), 2.0, 16, (2,))):
+    for r, rates, dur, cap, lane_set in ((256, (200, 500, 1000, 2000), 2.0, 128, (1, 2)),
+                                         (1024, (200, 500), 2.0, 64, (1,))):
         try:
             db = ctx.database_from_records(rec, r, n1, block_stride=0)
         except kspir_amd.KspirError as e:

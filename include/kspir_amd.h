@@ -194,9 +194,15 @@ int kspir_decrypt_bsgs_total(kspir_context* ctx, int32_t r, int32_t nq, const ui
  *   chain    [nc][N2-2][6][2][N]  the keys the reference inserts into the auto key at 5^(k N1),
  *                            k = 2..N2-1, coefficient form (may be NULL)
  *   giant    [nc][N2-1][3][2][N]  result.keyMap.at(5^(k N1)), k = 1..N2-1, NTT form mod crtMod
- * Each RLWE is (a, b).  As in the reference, the key at k = 1 is switched from
- * auto2's words as stored (NTT form).  N1 is a power of two in [2, N/4].
- * The crtBaMod (baby-step) half is not provided. */
+ * Each RLWE is (a, b).  N1 is a power of two in [2, N/4].  As in the
+ * reference, the key at k = 1 (giant[:, 0]) is switched from auto2's words
+ * as stored, in NTT form, by a switch that expects coefficient form: its
+ * words equal the reference's, but that key does NOT decrypt and must not be
+ * used as a giant-step key (auto2 itself has another gadget, so it is no
+ * substitute either): a caller that needs the key at 5^N1 generates it
+ * directly (bsgsKeyGen).  Every other giant[:, k] is a valid key.  Device state and pinned
+ * output staging are kept on the context between calls.  The crtBaMod
+ * (baby-step) half is not provided. */
 int kspir_gen_giant_keys_from_offline(kspir_context* ctx, int32_t n1, int32_t nc, const uint64_t* offline2,
                                       const uint64_t* auto2, uint64_t* chain, uint64_t* giant);
 

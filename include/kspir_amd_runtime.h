@@ -23,12 +23,13 @@ int kspir_device_count(int* count);
  *                     database pass (default 2); smaller ones the VALU pass
  *   "mfma_max_chunk"  most queries per MFMA database sweep (default: as many
  *                     as the LDS holds)
- *   "mfma_direct"     0 (default): the MFMA pass writes tile-major products
- *                     that a second kernel moves into the first-dimension
- *                     layout; 1: one kernel writes that layout directly (16
- *                     slots per workgroup, query limbs re-read from L2: it
- *                     moves 1 GiB less through HBM per 64-query step at r 16
- *                     but waits on those L2 reads and runs slower there)
+ *   "mfma_direct"     1 (default): one kernel writes the MFMA products in the
+ *                     first-dimension layout directly (16 slots per
+ *                     workgroup, query limbs streamed from L2 into an LDS ring
+ *                     by DMA, several column tiles ahead); 0: the MFMA pass
+ *                     writes tile-major products that a second kernel moves
+ *                     into that layout (1 GiB more HBM traffic and scratch per
+ *                     64-query step at r 16)
  *   "ext_fused"       1 (default): with the DMA giant-step kernel, the external
  *                     products run in the same workgroups (no coefficient
  *                     planes written out); 0: separate kernels

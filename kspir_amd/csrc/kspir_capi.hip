@@ -127,7 +127,7 @@ struct kspir_context {
     int ext_fused = 1;              // the DMA giant-step kernel also runs the external products
     int chunk_blocks = 32;          // blocks per chunk of the batched pipeline (0: the whole database at once)
     int cur_cb = 0;                 // chunk size the current workspace carve was made for (0: not chunked)
-    int mfma_direct = 0;            // 1: MFMA products written straight into dbout (k_db_mfma_direct); 0: tile-major + scatter
+    int mfma_direct = 1;            // 1: MFMA products written straight into dbout (k_db_mfma_direct); 0: tile-major + scatter
     int fused_min_batch = 16;       // batches of at least this many queries use k_giant_fused
     int fused_pack = 1;             // one k_pack_level per packing level (0: three kernels per level)
     int db_waves = 4;               // waves per workgroup of the single-query database pass (2, 4, 8, 16)
@@ -154,6 +154,11 @@ struct kspir_context {
     // encoding round(m * Q / p) in long double for every m < p
     uint32_t* rng_jump = nullptr;
     uint64_t* enc_tab = nullptr;
+    // offline key derivation: device state and pinned output staging, grown on demand
+    void* ows = nullptr;
+    size_t ows_bytes = 0;
+    void* ohost = nullptr;
+    size_t ohost_bytes = 0;
 };
 
 static void drop_graphs(kspir_context* c, const void* only = nullptr) {
@@ -343,6 +348,8 @@ size_t layout_workspace(kspir_context* c, int r, int n1, int nq, uint32_t* base,
         // products' partial sums are not used (fused kernel)
         S.dbout = (size_t)cb * n2 * 4 * n;
         S.gcoef = S.gks = S.fd = S.ecoef = S.epart = 0;
+        // every packing level fused (k_pack_level): no per-level scratch
+        if (c->fused_pack && c->pack_fused_min_wgs <= nq) S.pcoef = S.ppart = 0;
     }
     auto up = [](size_t w) { return (w + 63) / 64 * 64; };
     struct Part { size_t words; uint32_t** dst; };
@@ -880,6 +887,8 @@ void kspir_context_destroy(kspir_context* c) {
     if (c->kp_dev) hipFree(c->kp_dev);
     if (c->kp_host) hipHostFree(c->kp_host);
     if (c->mws) hipFree(c->mws);
+    if (c->ows) hipFree(c->ows);
+    if (c->ohost) hipHostFree(c->ohost);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -2072,18 +2081,36 @@ int kspir_gen_giant_keys_from_offline(kspir_context* c, int32_t n1, int32_t nc, 
     const size_t b_dig = (size_t)nc * kOkdRows * kOkdEll * 3 * P * 4, b_tb = (size_t)nc * kOkdRows * 3 * P * 4;
     const size_t b_giant = (size_t)nc * nkeys * 6 * P * 8;
     const size_t b_chain = chain && nchain > 0 ? (size_t)nc * nchain * 12 * P * 8 : 0;
-    uint64_t *doff = nullptr, *dauto = nullptr, *dgiant = nullptr, *dchain = nullptr;
-    uint32_t *km = nullptr, *st = nullptr, *c0 = nullptr, *dig = nullptr, *tb = nullptr;
-    hipError_t e = hipMalloc(&doff, b_off);
-    if (e == hipSuccess) e = hipMalloc(&dauto, b_auto);
-    if (e == hipSuccess) e = hipMalloc(&km, b_km);
-    if (e == hipSuccess) e = hipMalloc(&st, b_st);
-    if (e == hipSuccess) e = hipMalloc(&c0, b_c0);
-    if (e == hipSuccess) e = hipMalloc(&dig, b_dig);
-    if (e == hipSuccess) e = hipMalloc(&tb, b_tb);
-    if (e == hipSuccess) e = hipMalloc(&dgiant, b_giant);
-    if (e == hipSuccess && b_chain) e = hipMalloc(&dchain, b_chain);
+    // device state and pinned output staging live on the context and grow
+    // on demand, so repeated calls allocate nothing
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t parts[9] = {b_off, b_auto, b_km, b_st, b_c0, b_dig, b_tb, b_giant, b_chain};
+    size_t total = 0;
+    for (size_t b : parts) total += up(b);
     hipStream_t sm = c->stream;
+    if (total > c->ows_bytes) {
+        HIP_OK(hipStreamSynchronize(sm));
+        if (c->ows) HIP_OK(hipFree(c->ows));
+        c->ows = nullptr;
+        c->ows_bytes = 0;
+        HIP_OK(hipMalloc(&c->ows, total));
+        c->ows_bytes = total;
+    }
+    if (b_giant + b_chain > c->ohost_bytes) {
+        if (c->ohost) HIP_OK(hipHostFree(c->ohost));
+        c->ohost = nullptr;
+        c->ohost_bytes = 0;
+        HIP_OK(hipHostMalloc(&c->ohost, b_giant + b_chain, hipHostMallocDefault));
+        c->ohost_bytes = b_giant + b_chain;
+    }
+    char* base = (char*)c->ows;
+    void* ptrs[9];
+    for (int i = 0; i < 9; ++i) { ptrs[i] = parts[i] ? base : nullptr; base += up(parts[i]); }
+    uint64_t *doff = (uint64_t*)ptrs[0], *dauto = (uint64_t*)ptrs[1], *dgiant = (uint64_t*)ptrs[7],
+             *dchain = (uint64_t*)ptrs[8];
+    uint32_t *km = (uint32_t*)ptrs[2], *st = (uint32_t*)ptrs[3], *c0 = (uint32_t*)ptrs[4], *dig = (uint32_t*)ptrs[5],
+             *tb = (uint32_t*)ptrs[6];
+    hipError_t e = hipSuccess;
     if (e == hipSuccess) e = hipMemcpyAsync(doff, offline2, b_off, hipMemcpyHostToDevice, sm);
     if (e == hipSuccess) e = hipMemcpyAsync(dauto, auto2, b_auto, hipMemcpyHostToDevice, sm);
     if (e == hipSuccess) {
@@ -2107,13 +2134,13 @@ int kspir_gen_giant_keys_from_offline(kspir_context* c, int32_t n1, int32_t nc, 
                                nchain, c->T);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(giant, dgiant, b_giant, hipMemcpyDeviceToHost, sm);
-    if (e == hipSuccess && dchain) e = hipMemcpyAsync(chain, dchain, b_chain, hipMemcpyDeviceToHost, sm);
+    uint64_t* hg = (uint64_t*)c->ohost;
+    if (e == hipSuccess) e = hipMemcpyAsync(hg, dgiant, b_giant, hipMemcpyDeviceToHost, sm);
+    if (e == hipSuccess && dchain) e = hipMemcpyAsync(hg + b_giant / 8, dchain, b_chain, hipMemcpyDeviceToHost, sm);
     if (e == hipSuccess) e = hipStreamSynchronize(sm);
-    for (void* ptr : {(void*)doff, (void*)dauto, (void*)dgiant, (void*)dchain, (void*)km, (void*)st, (void*)c0,
-                      (void*)dig, (void*)tb})
-        if (ptr) hipFree(ptr);
     if (e != hipSuccess) return fail_hip("gen_giant_keys_from_offline", e);
+    std::memcpy(giant, hg, b_giant);
+    if (dchain) std::memcpy(chain, hg + b_giant / 8, b_chain);
     return KSPIR_OK;
 }
 

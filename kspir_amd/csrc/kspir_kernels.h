@@ -797,33 +797,56 @@ __global__ void __launch_bounds__(256) k_db_mfma(const int8_t* __restrict__ dbl,
 // The same products written straight into the first-dimension layout
 // dbout [q][b][i][ab][p][z], with no tile-major intermediate.  Coalesced
 // writes need runs of consecutive slots, so a workgroup owns 16 slots and
-// RTW*16*8 rows (8 waves, each 2 slots x RTW row tiles): the database limbs
-// of its (slot, row tile) pairs stay in registers for the whole sweep and are
-// read from HBM exactly once; the query limbs of a (slot, column tile) are
-// read from L2, once per row group (the row groups of one 16-slot group run
-// back to back on one XCD).  For each column tile the 16 x 16 products of
-// every (slot, row tile) go through an LDS tile [slot][row][col] and leave as
-// 64-byte runs of 16 slots per (row, column), the buffers alternating so one
-// barrier per column tile suffices.  Strides are chosen so that the MFMA
-// result writes and the run reads touch each bank at most twice per wave.
-// Requires R % (16 RTW * 8 / 8) == 0 row tiles per group, i.e. R % (16 RTW) == 0.
-// grid: 256 slot groups x (R / (16 RTW)) row groups x 2 primes, 512 threads.
+// RTW*16 rows (8 waves, each 2 slots x RTW row tiles): the database limbs of
+// its (slot, row tile) pairs stay in registers for the whole sweep and are
+// read from HBM exactly once.  The query limbs of a (slot, column tile) are
+// read from L2 (once per row group; the row groups of one 16-slot group run
+// back to back on one XCD) by LDS-DMA into a ring of NS stages per wave, one
+// stage per (column tile, slot, limb v), issued NS stages ahead of use, so
+// the L2 latency hides behind the MFMAs of the stages before it.  A wave
+// waits for its stage g with vmcnt(KS * stages issued after g): vector loads
+// complete in order, so at most that many outstanding operations means stage
+// g has landed.  For each column tile the 16 x 16 products of every (slot,
+// row tile) go through an LDS tile [slot][row][col] and leave as 64-byte
+// runs of 16 slots per (row, column).  The strides make the MFMA result
+// writes and the run reads touch each bank at most twice per wave.
+// Requires R % (16 RTW) == 0.  grid: 256 slot groups x (R / (16 RTW)) row
+// groups x 2 primes, 512 threads.
 constexpr int kMfmaRhoStride = 20;                    // words per staged row (16 columns + pad)
 constexpr int kMfmaZStride = 32 * kMfmaRhoStride + 4; // words per staged slot (up to 32 rows)
+
+// s_waitcnt vmcnt(n), other counters not waited on (n < 16)
+template <int NV>
+__device__ __forceinline__ void wait_vmcnt() { __builtin_amdgcn_s_waitcnt(0x0F70 | NV); }
+
 template <int KS, int RTW>
 __global__ void __launch_bounds__(512) k_db_mfma_direct(const int8_t* __restrict__ dbl, const int8_t* __restrict__ ql,
                                                         uint32_t* __restrict__ dbout, int R, int Rt, int r0, int C,
-                                                        int n2, int nq,
-                                                        QS S, MfmaConst MC, Tabs T) {
+                                                        int n2, int nq, QS S, MfmaConst MC, Tabs T) {
+    typedef __attribute__((address_space(3))) void lds_void;
+    typedef __attribute__((address_space(1))) void glb_void;
     constexpr int n1 = KS * 64;
-    __shared__ uint32_t stage[2][16 * kMfmaZStride];
+    constexpr int NS = KS == 4 ? 2 : 4;     // ring stages per wave (KS KiB each)
+    __shared__ __attribute__((aligned(16))) int8_t ring[8][NS][KS * 1024];
+    __shared__ uint32_t stage[16 * kMfmaZStride];
     const int ngroups = R / (16 * RTW);
     const int L = xcd_job(blockIdx.x, (kN / 16) * ngroups * 2);
     const int rg = L % ngroups, p = (L / ngroups) & 1, zg = L / (2 * ngroups);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lr = lane & 15, lk = (lane >> 4) * 16;
     const PrimeTab& P = T.p[p];
-    const int CT = C / 16;
+    const int CT = C / 16, G = CT * 8;      // stages: g = (ct * 2 + zi) * 4 + v
+    // this lane's query-limb fragment of stage g: slot 2 wv + zi, limb v,
+    // column ct * 16 + lr, rotations lk .. lk + 15 of each 64-rotation chunk
+    auto issue = [&](int g) {
+        const int ct = g >> 3, zi = (g >> 2) & 1, v = g & 3;
+        const int z = zg * 16 + 2 * wv + zi;
+        const int8_t* src = ql + ((size_t)z * 2 + p) * 4 * C * n1 + ((size_t)v * C + ct * 16 + lr) * n1 + lk;
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+            __builtin_amdgcn_global_load_lds((glb_void*)(src + ks * 64), (lds_void*)&ring[wv][g % NS][ks * 1024], 16,
+                                             0, 0);
+    };
     // database limbs of this wave's two slots and RTW row tiles
     i8x16 a[2][RTW][4][KS];
 #pragma unroll
@@ -839,24 +862,32 @@ __global__ void __launch_bounds__(512) k_db_mfma_direct(const int8_t* __restrict
                     a[zi][t][u][ks] = *reinterpret_cast<const i8x16*>(
                         asrc + ((size_t)u * Rt + (rg * RTW + t) * 16 + lr) * n1 + ks * 64 + lk);
     }
+    __builtin_amdgcn_sched_barrier(0);
+    for (int g = 0; g < NS && g < G; ++g) issue(g);
+    __builtin_amdgcn_sched_barrier(0);
     for (int ct = 0; ct < CT; ++ct) {
-        uint32_t* st = stage[ct & 1];
 #pragma unroll
         for (int zi = 0; zi < 2; ++zi) {
-            const int zl = 2 * wv + zi, z = zg * 16 + zl;
-            const int8_t* bsrc = ql + ((size_t)z * 2 + p) * 4 * C * n1 + (size_t)(ct * 16 + lr) * n1 + lk;
+            const int zl = 2 * wv + zi;
             i32x4 acc[RTW][7];
 #pragma unroll
             for (int t = 0; t < RTW; ++t)
 #pragma unroll
                 for (int w = 0; w < 7; ++w) acc[t][w] = i32x4{0, 0, 0, 0};
-            // one query limb at a time: its KS fragments serve every row tile
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
+                const int g = (ct * 2 + zi) * 4 + v;
+                const int after = G - 1 - g < NS - 1 ? G - 1 - g : NS - 1;   // stages issued after g
+                __builtin_amdgcn_sched_barrier(0);
+                if (after >= 3) wait_vmcnt<3 * KS>();
+                else if (after == 2) wait_vmcnt<2 * KS>();
+                else if (after == 1) wait_vmcnt<KS>();
+                else wait_vmcnt<0>();
+                __builtin_amdgcn_sched_barrier(0);
                 i8x16 b[KS];
 #pragma unroll
                 for (int ks = 0; ks < KS; ++ks)
-                    b[ks] = *reinterpret_cast<const i8x16*>(bsrc + (size_t)v * C * n1 + ks * 64);
+                    b[ks] = *reinterpret_cast<const i8x16*>(&ring[wv][g % NS][ks * 1024 + 16 * lane]);
 #pragma unroll
                 for (int t = 0; t < RTW; ++t)
 #pragma unroll
@@ -865,19 +896,23 @@ __global__ void __launch_bounds__(512) k_db_mfma_direct(const int8_t* __restrict
                         for (int u = 0; u < 4; ++u)
                             acc[t][u + v] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[zi][t][u][ks], b[ks],
                                                                                   acc[t][u + v], 0, 0, 0);
+                // this stage's ring slot takes stage g + NS once its reads are done
+                __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0)
+                __builtin_amdgcn_sched_barrier(0);
+                if (g + NS < G) issue(g + NS);
+                __builtin_amdgcn_sched_barrier(0);
             }
+            // lane: column lr, rows 4 (lane / 16) + j of row tile t
 #pragma unroll
-            for (int t = 0; t < RTW; ++t) {
-                // lane: column lr, rows 4 (lane / 16) + j of row tile t
+            for (int t = 0; t < RTW; ++t)
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    uint64_t s = 0;
+                    uint64_t sum = 0;
 #pragma unroll
-                    for (int w = 0; w < 7; ++w) s += (uint64_t)(uint32_t)acc[t][w][j] * MC.cw[p][w];
+                    for (int w = 0; w < 7; ++w) sum += (uint64_t)(uint32_t)acc[t][w][j] * MC.cw[p][w];
                     const int rho = t * 16 + 4 * (lane >> 4) + j;
-                    st[zl * kMfmaZStride + rho * kMfmaRhoStride + lr] = mont_reduce(s, P.q, P.mont_qinv);
+                    stage[zl * kMfmaZStride + rho * kMfmaRhoStride + lr] = mont_reduce(sum, P.q, P.mont_qinv);
                 }
-            }
         }
         __syncthreads();
         // runs: (row, column) pairs, four lanes per run of 16 slots
@@ -887,13 +922,14 @@ __global__ void __launch_bounds__(512) k_db_mfma_direct(const int8_t* __restrict
             const int rho = run >> 4, col = run & 15;
             const int cg = ct * 16 + col, q = cg >> 1, ab = cg & 1;
             if (q < nq) {
-                const uint32_t* sp = st + (4 * qz) * kMfmaZStride + rho * kMfmaRhoStride + col;
+                const uint32_t* sp = stage + (4 * qz) * kMfmaZStride + rho * kMfmaRhoStride + col;
                 const uint4 v = make_uint4(sp[0], sp[kMfmaZStride], sp[2 * kMfmaZStride], sp[3 * kMfmaZStride]);
                 const int rhog = (rg * RTW) * 16 + rho, blk = rhog / n2, row = rhog % n2;
                 *reinterpret_cast<uint4*>(dbout + q * S.dbout + ((size_t)(blk * n2 + row) * 4 + ab * 2 + p) * kN +
                                           zg * 16 + 4 * qz) = v;
             }
         }
+        __syncthreads();   // the LDS tile is rewritten by the next column tile
     }
 }
 
