@@ -457,13 +457,25 @@ int mfma_sweep_queries(const kspir_context* c, int n1) {
     return cmax / 2 < c->mfma_max_chunk ? cmax / 2 : c->mfma_max_chunk;
 }
 
+// k_db_mfma_direct in either first-dimension layout
+template <int KS, int RTW>
+void launch_direct(bool zmaj, dim3 g, hipStream_t st, const int8_t* dbl, const int8_t* ql, uint32_t* out, int R, int Rt,
+                   int r0, int C, int n2, int nq, const QS& S, const MfmaConst& MC, const Tabs& T) {
+    if (zmaj)
+        hipLaunchKernelGGL((k_db_mfma_direct<KS, RTW, true>), g, dim3(512), 0, st, dbl, ql, out, R, Rt, r0, C, n2, nq, S, MC, T);
+    else
+        hipLaunchKernelGGL((k_db_mfma_direct<KS, RTW, false>), g, dim3(512), 0, st, dbl, ql, out, R, Rt, r0, C, n2, nq, S, MC, T);
+}
+
 // Queries q_lo .. q_lo + nq (in sweeps of mfma_sweep_queries), database
 // rows of blocks b0 .. b0 + nb (nb 0: every block); the products land in
 // c->dbout, block-relative to b0.  A chunked pipeline calls it once with
 // parts 1 (query limbs) and then per chunk with parts 6, always with the
-// same nq and nb, so every call carves the scratch the same way.
+// same nq and nb, so every call carves the scratch the same way.  zmaj
+// (direct pass only): dbout in the slot-group-major layout of the chunked
+// pipeline.
 int launch_db_batched(kspir_context* c, const kspir_database* dbc, int q_lo, int nq, hipStream_t st, int parts = 7,
-                      int b0 = 0, int nb = 0) {
+                      int b0 = 0, int nb = 0, bool zmaj = false) {
     kspir_database* db = const_cast<kspir_database*>(dbc);
     const int r = db->r, n1 = db->n1, n2 = db->n2, Rt = r * n2;
     if (nb <= 0) nb = r;
@@ -519,16 +531,11 @@ int launch_db_batched(kspir_context* c, const kspir_database* dbc, int q_lo, int
             const int rtw = (n1 <= 128 && R % 32 == 0) ? 2 : 1;
             const dim3 g((kN / 16) * (R / (16 * rtw)) * 2);
             uint32_t* out = c->dbout + q0 * S.dbout;
-            if (n1 == 64 && rtw == 2)
-                hipLaunchKernelGGL((k_db_mfma_direct<1, 2>), g, dim3(512), 0, st, db->limbs, ql, out, R, Rt, r0, C, n2, nqc, S, MC, c->T);
-            else if (n1 == 64)
-                hipLaunchKernelGGL((k_db_mfma_direct<1, 1>), g, dim3(512), 0, st, db->limbs, ql, out, R, Rt, r0, C, n2, nqc, S, MC, c->T);
-            else if (n1 == 128 && rtw == 2)
-                hipLaunchKernelGGL((k_db_mfma_direct<2, 2>), g, dim3(512), 0, st, db->limbs, ql, out, R, Rt, r0, C, n2, nqc, S, MC, c->T);
-            else if (n1 == 128)
-                hipLaunchKernelGGL((k_db_mfma_direct<2, 1>), g, dim3(512), 0, st, db->limbs, ql, out, R, Rt, r0, C, n2, nqc, S, MC, c->T);
-            else
-                hipLaunchKernelGGL((k_db_mfma_direct<4, 1>), g, dim3(512), 0, st, db->limbs, ql, out, R, Rt, r0, C, n2, nqc, S, MC, c->T);
+            if (n1 == 64 && rtw == 2) launch_direct<1, 2>(zmaj, g, st, db->limbs, ql, out, R, Rt, r0, C, n2, nqc, S, MC, c->T);
+            else if (n1 == 64) launch_direct<1, 1>(zmaj, g, st, db->limbs, ql, out, R, Rt, r0, C, n2, nqc, S, MC, c->T);
+            else if (n1 == 128 && rtw == 2) launch_direct<2, 2>(zmaj, g, st, db->limbs, ql, out, R, Rt, r0, C, n2, nqc, S, MC, c->T);
+            else if (n1 == 128) launch_direct<2, 1>(zmaj, g, st, db->limbs, ql, out, R, Rt, r0, C, n2, nqc, S, MC, c->T);
+            else launch_direct<4, 1>(zmaj, g, st, db->limbs, ql, out, R, Rt, r0, C, n2, nqc, S, MC, c->T);
             LAUNCH_OK("k_db_mfma_direct");
             continue;
         }
@@ -590,7 +597,7 @@ int enqueue_packing(kspir_context* c, const KeyPtrs* kp, int r, int nq, hipStrea
 // The batched pipeline after the baby steps, one chunk of c->cur_cb blocks at
 // a time: per MFMA sweep of queries the query limbs once, then per chunk the
 // database pass over the chunk's rows and the giant steps with the external
-// products of every (query, block) of the chunk (k_giant_fused_dma<true>,
+// products of every (query, block) of the chunk (k_giant_fused_dma<true, zmaj>,
 // block offset b0 into ext); then the packing tree over all blocks.  The
 // database is still read once per sweep, a chunk at a time.
 int enqueue_chunked(kspir_context* c, const kspir_database* db, const KeyPtrs* kp, int nq, uint64_t* resp,
@@ -599,16 +606,24 @@ int enqueue_chunked(kspir_context* c, const kspir_database* db, const KeyPtrs* k
     const QS& S = c->S;
     const Tabs& T = c->T;
     const int nqc_max = mfma_sweep_queries(c, n1);
+    // the direct MFMA pass and the giant steps share the slot-group-major
+    // first-dimension layout here (dbout is internal to this pipeline)
+    const bool zmaj = c->mfma_direct != 0;
     for (int q0 = 0; q0 < nq; q0 += nqc_max) {
         const int nqc = nq - q0 < nqc_max ? nq - q0 : nqc_max;
         int rc = launch_db_batched(c, db, q0, nqc, st, 1, 0, cb);
         if (rc) return rc;
         for (int b0 = 0; b0 < r; b0 += cb) {
-            rc = launch_db_batched(c, db, q0, nqc, st, 6, b0, cb);
+            rc = launch_db_batched(c, db, q0, nqc, st, 6, b0, cb, zmaj);
             if (rc) return rc;
-            hipLaunchKernelGGL(k_giant_fused_dma<true>, dim3(cb * nqc), dim3(kFusedThreads), 0, st,
-                               c->dbout + q0 * S.dbout, kp + q0, nullptr, c->extA + q0 * S.ext, cb, nqc, n1, n2, S, T,
-                               b0);
+            if (zmaj)
+                hipLaunchKernelGGL((k_giant_fused_dma<true, true>), dim3(cb * nqc), dim3(kFusedThreads), 0, st,
+                                   c->dbout + q0 * S.dbout, kp + q0, nullptr, c->extA + q0 * S.ext, cb, nqc, n1, n2, S,
+                                   T, b0);
+            else
+                hipLaunchKernelGGL((k_giant_fused_dma<true, false>), dim3(cb * nqc), dim3(kFusedThreads), 0, st,
+                                   c->dbout + q0 * S.dbout, kp + q0, nullptr, c->extA + q0 * S.ext, cb, nqc, n1, n2, S,
+                                   T, b0);
             LAUNCH_OK("k_giant_fused_dma");
         }
     }
@@ -665,11 +680,11 @@ int enqueue_answer(kspir_context* c, const kspir_database* db, const KeyPtrs* kp
     if (nq >= c->fused_min_batch) {
         if (c->giant_dma && c->giant_wgs_per_cu == 1 && c->ext_fused && !first_dim_only) {
             // the ext output of a query has the ecoef stride's layout [r][ab][p][N]
-            hipLaunchKernelGGL(k_giant_fused_dma<true>, dim3(r * nq), dim3(kFusedThreads), 0, st, c->dbout, kp,
+            hipLaunchKernelGGL((k_giant_fused_dma<true, false>), dim3(r * nq), dim3(kFusedThreads), 0, st, c->dbout, kp,
                                fd_u64 ? c->fd : nullptr, c->extA, r, nq, n1, n2, S, T);
             ext_done = true;
         } else if (c->giant_dma && c->giant_wgs_per_cu == 1)
-            hipLaunchKernelGGL(k_giant_fused_dma<false>, dim3(r * nq), dim3(kFusedThreads), 0, st, c->dbout, kp,
+            hipLaunchKernelGGL((k_giant_fused_dma<false, false>), dim3(r * nq), dim3(kFusedThreads), 0, st, c->dbout, kp,
                                fd_u64 ? c->fd : nullptr, c->ecoef, r, nq, n1, n2, S, T);
         else if (c->giant_wgs_per_cu == 2)
             hipLaunchKernelGGL(k_giant_fused2, dim3(r * nq), dim3(kFusedThreads), 0, st, c->dbout, kp, c->fd, c->ecoef,

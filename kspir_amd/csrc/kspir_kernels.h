@@ -806,10 +806,20 @@ __global__ void __launch_bounds__(256) k_db_mfma(const int8_t* __restrict__ dbl,
 // the L2 latency hides behind the MFMAs of the stages before it.  A wave
 // waits for its stage g with vmcnt(KS * stages issued after g): vector loads
 // complete in order, so at most that many outstanding operations means stage
-// g has landed.  For each column tile the 16 x 16 products of every (slot,
-// row tile) go through an LDS tile [slot][row][col] and leave as 64-byte
-// runs of 16 slots per (row, column).  The strides make the MFMA result
-// writes and the run reads touch each bank at most twice per wave.
+// g has landed (the wave's global stores of earlier column tiles count in
+// vmcnt too; whatever their order, they can only make such a wait longer).
+// For each column tile the 16 x 16 products of every (slot, row tile) go
+// through an LDS tile [slot][row][col] (two of them, for alternate column
+// tiles) and leave as 64-byte runs of 16 slots per (row, column), after one
+// LDS-only barrier per column tile: no wait for stores or prefetches in
+// flight.  The strides make the MFMA result writes and the run reads touch
+// each bank at most twice per wave.
+// ZMAJ selects the slot-group-major first-dimension layout
+//   dbout [q][b][p][z / 16][i][ab][z % 16]
+// in which each workgroup's output for one (query, block) is one contiguous
+// run of n2 * 2 * 64 bytes, and each store instruction writes whole 128-byte
+// lines; k_giant_fused_dma<EXT, true> reads it.  Without ZMAJ the layout is
+// dbout [q][b][i][ab][p][z] (64-byte runs, one per (row, column)).
 // Requires R % (16 RTW) == 0.  grid: 256 slot groups x (R / (16 RTW)) row
 // groups x 2 primes, 512 threads.
 constexpr int kMfmaRhoStride = 20;                    // words per staged row (16 columns + pad)
@@ -819,7 +829,91 @@ constexpr int kMfmaZStride = 32 * kMfmaRhoStride + 4; // words per staged slot (
 template <int NV>
 __device__ __forceinline__ void wait_vmcnt() { __builtin_amdgcn_s_waitcnt(0x0F70 | NV); }
 
-template <int KS, int RTW>
+// KS 16-byte LDS reads at la + 1024 ks, completed (lgkmcnt(0)) within the
+// same asm statement.  Written as asm so that the compiler sees no LDS read
+// of the DMA ring: with global stores in flight it would otherwise guard each
+// such read with vmcnt(0), which also waits for every store and prefetch.
+// The ring's own vmcnt waits order these reads after their stage's DMA.
+template <int KS>
+__device__ __forceinline__ void ring_read(i8x16 (&b)[KS], uint32_t la) {
+    i32x4 r[KS];
+    if constexpr (KS == 1) {
+        asm volatile("ds_read_b128 %0, %1\n\ts_waitcnt lgkmcnt(0)" : "=v"(r[0]) : "v"(la) : "memory");
+    } else if constexpr (KS == 2) {
+        asm volatile("ds_read_b128 %0, %2\n\tds_read_b128 %1, %2 offset:1024\n\ts_waitcnt lgkmcnt(0)"
+                     : "=v"(r[0]), "=v"(r[1]) : "v"(la) : "memory");
+    } else {
+        static_assert(KS == 4, "ring_read: KS in {1, 2, 4}");
+        asm volatile("ds_read_b128 %0, %4\n\tds_read_b128 %1, %4 offset:1024\n\t"
+                     "ds_read_b128 %2, %4 offset:2048\n\tds_read_b128 %3, %4 offset:3072\n\ts_waitcnt lgkmcnt(0)"
+                     : "=v"(r[0]), "=v"(r[1]), "=v"(r[2]), "=v"(r[3]) : "v"(la) : "memory");
+    }
+#pragma unroll
+    for (int k = 0; k < KS; ++k) b[k] = __builtin_bit_cast(i8x16, r[k]);
+}
+
+// one LDS word write as asm, for the same reason (the staging tile that
+// shares the LDS with the DMA ring); completed by the next lds_barrier
+__device__ __forceinline__ void lds_write_u32(uint32_t la, uint32_t v) {
+    asm volatile("ds_write_b32 %0, %1" : : "v"(la), "v"(v) : "memory");
+}
+
+// The run reads of a column tile as one asm statement (all reads, then
+// lgkmcnt(0)): w[i][k] = word (slot 4 qz + k, row rho0 + 8 i) of the thread's
+// column, la = its (slot 4 qz, row rho0) address in the staging tile.
+template <int NI>
+__device__ __forceinline__ void stage_read(uint32_t (&w)[NI][4], uint32_t la) {
+    static_assert(kMfmaRhoStride == 20 && kMfmaZStride == 644, "stage_read: offsets assume these strides");
+    if constexpr (NI == 2) {
+        asm volatile("ds_read_b32 %0, %8 offset:0\n\t"
+                     "ds_read_b32 %1, %8 offset:2576\n\t"
+                     "ds_read_b32 %2, %8 offset:5152\n\t"
+                     "ds_read_b32 %3, %8 offset:7728\n\t"
+                     "ds_read_b32 %4, %8 offset:640\n\t"
+                     "ds_read_b32 %5, %8 offset:3216\n\t"
+                     "ds_read_b32 %6, %8 offset:5792\n\t"
+                     "ds_read_b32 %7, %8 offset:8368\n\t"
+                     "s_waitcnt lgkmcnt(0)"
+                     : "=v"(w[0][0]), "=v"(w[0][1]), "=v"(w[0][2]), "=v"(w[0][3]), "=v"(w[1][0]), "=v"(w[1][1]), "=v"(w[1][2]), "=v"(w[1][3])
+                     : "v"(la) : "memory");
+    } else {
+        static_assert(NI == 4, "stage_read: NI in {2, 4}");
+        asm volatile("ds_read_b32 %0, %16 offset:0\n\t"
+                     "ds_read_b32 %1, %16 offset:2576\n\t"
+                     "ds_read_b32 %2, %16 offset:5152\n\t"
+                     "ds_read_b32 %3, %16 offset:7728\n\t"
+                     "ds_read_b32 %4, %16 offset:640\n\t"
+                     "ds_read_b32 %5, %16 offset:3216\n\t"
+                     "ds_read_b32 %6, %16 offset:5792\n\t"
+                     "ds_read_b32 %7, %16 offset:8368\n\t"
+                     "ds_read_b32 %8, %16 offset:1280\n\t"
+                     "ds_read_b32 %9, %16 offset:3856\n\t"
+                     "ds_read_b32 %10, %16 offset:6432\n\t"
+                     "ds_read_b32 %11, %16 offset:9008\n\t"
+                     "ds_read_b32 %12, %16 offset:1920\n\t"
+                     "ds_read_b32 %13, %16 offset:4496\n\t"
+                     "ds_read_b32 %14, %16 offset:7072\n\t"
+                     "ds_read_b32 %15, %16 offset:9648\n\t"
+                     "s_waitcnt lgkmcnt(0)"
+                     : "=v"(w[0][0]), "=v"(w[0][1]), "=v"(w[0][2]), "=v"(w[0][3]), "=v"(w[1][0]), "=v"(w[1][1]), "=v"(w[1][2]), "=v"(w[1][3]), "=v"(w[2][0]), "=v"(w[2][1]), "=v"(w[2][2]), "=v"(w[2][3]), "=v"(w[3][0]), "=v"(w[3][1]), "=v"(w[3][2]), "=v"(w[3][3])
+                     : "v"(la) : "memory");
+    }
+}
+
+// Workgroup barrier for LDS data only: this wave's LDS reads and writes are
+// done (lgkmcnt(0)) and every wave has arrived.  Unlike __syncthreads(), whose
+// release fence waits for vmcnt(0), it does not wait for the wave's global
+// stores and LDS-DMA loads in flight (on gfx9 both count in vmcnt), so stores
+// of one column tile and prefetches of the next keep going across it.  The
+// empty asm statements keep the compiler from moving memory accesses across.
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xC07F);
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+template <int KS, int RTW, bool ZMAJ>
 __global__ void __launch_bounds__(512) k_db_mfma_direct(const int8_t* __restrict__ dbl, const int8_t* __restrict__ ql,
                                                         uint32_t* __restrict__ dbout, int R, int Rt, int r0, int C,
                                                         int n2, int nq, QS S, MfmaConst MC, Tabs T) {
@@ -828,7 +922,7 @@ __global__ void __launch_bounds__(512) k_db_mfma_direct(const int8_t* __restrict
     constexpr int n1 = KS * 64;
     constexpr int NS = KS == 4 ? 2 : 4;     // ring stages per wave (KS KiB each)
     __shared__ __attribute__((aligned(16))) int8_t ring[8][NS][KS * 1024];
-    __shared__ uint32_t stage[16 * kMfmaZStride];
+    __shared__ uint32_t stage[2][16 * kMfmaZStride];   // column tile ct in stage[ct & 1]
     const int ngroups = R / (16 * RTW);
     const int L = xcd_job(blockIdx.x, (kN / 16) * ngroups * 2);
     const int rg = L % ngroups, p = (L / ngroups) & 1, zg = L / (2 * ngroups);
@@ -885,9 +979,7 @@ __global__ void __launch_bounds__(512) k_db_mfma_direct(const int8_t* __restrict
                 else wait_vmcnt<0>();
                 __builtin_amdgcn_sched_barrier(0);
                 i8x16 b[KS];
-#pragma unroll
-                for (int ks = 0; ks < KS; ++ks)
-                    b[ks] = *reinterpret_cast<const i8x16*>(&ring[wv][g % NS][ks * 1024 + 16 * lane]);
+                ring_read<KS>(b, (uint32_t)(uintptr_t)(lds_void*)&ring[wv][g % NS][16 * lane]);
 #pragma unroll
                 for (int t = 0; t < RTW; ++t)
 #pragma unroll
@@ -903,6 +995,8 @@ __global__ void __launch_bounds__(512) k_db_mfma_direct(const int8_t* __restrict
                 __builtin_amdgcn_sched_barrier(0);
             }
             // lane: column lr, rows 4 (lane / 16) + j of row tile t
+            const uint32_t la = (uint32_t)(uintptr_t)(lds_void*)&stage[ct & 1][zl * kMfmaZStride +
+                                                                          4 * (lane >> 4) * kMfmaRhoStride + lr];
 #pragma unroll
             for (int t = 0; t < RTW; ++t)
 #pragma unroll
@@ -910,26 +1004,34 @@ __global__ void __launch_bounds__(512) k_db_mfma_direct(const int8_t* __restrict
                     uint64_t sum = 0;
 #pragma unroll
                     for (int w = 0; w < 7; ++w) sum += (uint64_t)(uint32_t)acc[t][w][j] * MC.cw[p][w];
-                    const int rho = t * 16 + 4 * (lane >> 4) + j;
-                    stage[zl * kMfmaZStride + rho * kMfmaRhoStride + lr] = mont_reduce(sum, P.q, P.mont_qinv);
+                    lds_write_u32(la + 4 * (t * 16 + j) * kMfmaRhoStride, mont_reduce(sum, P.q, P.mont_qinv));
                 }
         }
-        __syncthreads();
-        // runs: (row, column) pairs, four lanes per run of 16 slots
-#pragma unroll
-        for (int i = 0; i < RTW * 16 * 16 * 4 / 512; ++i) {
-            const int idx = threadIdx.x + 512 * i, run = idx >> 2, qz = idx & 3;
-            const int rho = run >> 4, col = run & 15;
+        // every wave's products of column tile ct are in stage[ct & 1]; that
+        // buffer is rewritten at column tile ct + 2, after every wave has
+        // passed the barrier of ct + 1 and so finished its run reads of ct
+        lds_barrier();
+        // runs: (row, column) pairs, four lanes per run of 16 slots; thread t
+        // takes slots 4 (t & 3) .. + 3 of column (t >> 2) & 15 in rows
+        // (t >> 6) + 8 i
+        {
+            constexpr int NI = RTW * 2;
+            const int qz = threadIdx.x & 3, col = (threadIdx.x >> 2) & 15, rho0 = threadIdx.x >> 6;
             const int cg = ct * 16 + col, q = cg >> 1, ab = cg & 1;
+            uint32_t w[NI][4];
+            stage_read<NI>(w, (uint32_t)(uintptr_t)(lds_void*)&stage[ct & 1][(4 * qz) * kMfmaZStride +
+                                                                              rho0 * kMfmaRhoStride + col]);
             if (q < nq) {
-                const uint32_t* sp = stage + (4 * qz) * kMfmaZStride + rho * kMfmaRhoStride + col;
-                const uint4 v = make_uint4(sp[0], sp[kMfmaZStride], sp[2 * kMfmaZStride], sp[3 * kMfmaZStride]);
-                const int rhog = (rg * RTW) * 16 + rho, blk = rhog / n2, row = rhog % n2;
-                *reinterpret_cast<uint4*>(dbout + q * S.dbout + ((size_t)(blk * n2 + row) * 4 + ab * 2 + p) * kN +
-                                          zg * 16 + 4 * qz) = v;
+#pragma unroll
+                for (int i = 0; i < NI; ++i) {
+                    const int rhog = (rg * RTW) * 16 + rho0 + 8 * i, blk = rhog / n2, row = rhog % n2;
+                    const size_t o = ZMAJ ? ((size_t)blk * 2 + p) * 2 * n2 * kN + (size_t)zg * 32 * n2 + row * 32 +
+                                                ab * 16 + 4 * qz
+                                          : ((size_t)(blk * n2 + row) * 4 + ab * 2 + p) * kN + zg * 16 + 4 * qz;
+                    *reinterpret_cast<uint4*>(dbout + q * S.dbout + o) = make_uint4(w[i][0], w[i][1], w[i][2], w[i][3]);
+                }
             }
         }
-        __syncthreads();   // the LDS tile is rewritten by the next column tile
     }
 }
 
@@ -1255,6 +1357,38 @@ __device__ __forceinline__ void dma_planes_flat(uint32_t (*dst)[kPadN], const ui
     }
 }
 
+// The same copies of the planes (row i, part ab, primes 0..np-1) of one
+// block from the slot-group-major layout blk [p][z / 16][i][ab][z % 16]
+// (n2 rows): each lane gathers its own words, 16 consecutive slots are
+// contiguous, so every 64-byte source run is read whole.
+template <int NP>
+__device__ __forceinline__ void dma_planes_z(uint32_t (*dst)[kPadN], const uint32_t* __restrict__ blk, int i, int ab,
+                                             int n2, int tid) {
+    typedef __attribute__((address_space(3))) void lds_void;
+    typedef __attribute__((address_space(1))) void glb_void;
+    const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+#pragma unroll
+    for (int b = wv; b < NP * kN / 64; b += kFusedThreads / 64) {
+        const int j = b >> 6, x0 = (b & 63) * 64, z = x0 + lane;
+        const uint32_t* src = blk + (size_t)j * 2 * n2 * kN + (size_t)(z >> 4) * 32 * n2 + i * 32 + ab * 16 + (z & 15);
+        __builtin_amdgcn_global_load_lds((glb_void*)src, (lds_void*)(&dst[j][lds_pad(x0)]), 4, 0, 0);
+    }
+}
+
+template <int NP>
+__device__ __forceinline__ void dma_planes_flat_z(uint32_t (*dst)[kPadN], const uint32_t* __restrict__ blk, int i,
+                                                  int ab, int n2, int tid) {
+    typedef __attribute__((address_space(3))) void lds_void;
+    typedef __attribute__((address_space(1))) void glb_void;
+    const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+#pragma unroll
+    for (int b = wv; b < NP * kN / 256; b += kFusedThreads / 64) {
+        const int j = b >> 4, x0 = (b & 15) * 256, z = x0 + 4 * lane;
+        const uint32_t* src = blk + (size_t)j * 2 * n2 * kN + (size_t)(z >> 4) * 32 * n2 + i * 32 + ab * 16 + (z & 15);
+        __builtin_amdgcn_global_load_lds((glb_void*)src, (lds_void*)(&dst[j][x0]), 16, 0, 0);
+    }
+}
+
 // s_waitcnt vmcnt(0): this wave's global loads, LDS-DMA copies included, done
 __device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
@@ -1272,6 +1406,8 @@ __device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70)
 // of step c runs during the inverse transform of row c+1 (its digits are
 // still in planes 0..3 then), the last one after the loop.  Same words as
 // k_giant_fused: every update is exact mod q.
+// ZMAJ: dbout in the slot-group-major layout that k_db_mfma_direct<KS, RTW, true>
+// writes (rows gathered by dma_planes_z / dma_planes_flat_z).
 // With EXT the external product of the block (lwe.cpp:761-808, what k_ext_digit
 // and k_ext_sum compute) follows in the same workgroup, on the coefficient
 // planes still in LDS: the signed digits (ell 2, PP 2^20, Bg 2^18) of a and b,
@@ -1279,7 +1415,7 @@ __device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70)
 // products summed over the four rows with one Montgomery reduction (four
 // products below 2^58 < q * 2^32), written NTT-domain to `out` with the ext
 // stride; without EXT `out` receives the coefficient planes (ecoef).
-template <bool EXT>
+template <bool EXT, bool ZMAJ>
 __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_t* __restrict__ dbout,
         const KeyPtrs* __restrict__ kp, uint32_t* __restrict__ fd, uint32_t* __restrict__ ecoef, int r, int nq,
         int n1, int n2, QS S, Tabs T, int b0 = 0) {
@@ -1324,10 +1460,19 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_
             }
         }
     };
-    dma_planes<2>(s + 4, rows + (size_t)(n2 > 1 ? 1 : 0) * 4 * kN, tid);
-    dma_planes_flat<2>(s + 6, rows + (size_t)(n2 > 1 ? 1 : 0) * 4 * kN + 2 * kN, tid);
+    // row i's a-planes into planes 4, 5 (padded), its b-planes into 6, 7 (flat)
+    auto dma_a = [&](int i) {
+        if (ZMAJ) dma_planes_z<2>(s + 4, rows, i, 0, n2, tid);
+        else dma_planes<2>(s + 4, rows + (size_t)i * 4 * kN, tid);
+    };
+    auto dma_b = [&](int i) {
+        if (ZMAJ) dma_planes_flat_z<2>(s + 6, rows, i, 1, n2, tid);
+        else dma_planes_flat<2>(s + 6, rows + (size_t)i * 4 * kN + 2 * kN, tid);
+    };
+    dma_a(n2 > 1 ? 1 : 0);
+    dma_b(n2 > 1 ? 1 : 0);
     for (int c = 1; c < n2; ++c) {
-        const uint32_t* next = rows + (size_t)(c + 1 < n2 ? c + 1 : 0) * 4 * kN;
+        const int next = c + 1 < n2 ? c + 1 : 0;
         wait_vmem();
         __syncthreads();
         __builtin_amdgcn_sched_barrier(0);
@@ -1350,7 +1495,7 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_
         if (c >= 2) mac(c - 1, 0);
         __syncthreads();
         __builtin_amdgcn_sched_barrier(0);
-        dma_planes_flat<2>(s + 6, next + 2 * kN, tid);     // every wave is past its b-gather
+        dma_b(next);     // every wave is past its b-gather
         __builtin_amdgcn_sched_barrier(0);
         ntt_pass_lds<1, true>(s[4 + half], T.p[half], ht);
         if (c >= 2) mac(c - 1, 1);
@@ -1374,7 +1519,7 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_
         }
         __syncthreads();
         __builtin_amdgcn_sched_barrier(0);
-        dma_planes<2>(s + 4, next, tid);
+        dma_a(next);
         __builtin_amdgcn_sched_barrier(0);
         ntt_fwd_r16x2_same(s[2 * half], s[2 * half + 1], T.p[half], ht);
         __builtin_amdgcn_sched_barrier(0);
