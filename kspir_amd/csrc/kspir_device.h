@@ -351,12 +351,30 @@ __host__ __device__ __forceinline__ void ntt_inv_finish(uint32_t v[16], uint32_t
     }
 }
 
-// one pass with the pass index known at compile time (constant LDS offsets;
-// pass-0 twiddle addresses are uniform, so they become scalar loads)
+// The twiddles of a pass on the device.  Pass 0's are the same for every
+// thread: read through the constant address space they become scalar loads,
+// counted in lgkmcnt rather than vmcnt, so they never wait behind LDS-DMA
+// copies or other vector loads in flight.
+template <int PASS>
+__device__ __forceinline__ void load_pass_twiddles_dev(const uint2* tw2, int tid, uint32_t w[15], uint32_t ws[15]) {
+    if constexpr (PASS == 0) {
+        typedef const __attribute__((address_space(4))) uint32_t cu32;
+        cu32* t = (cu32*)tw2;
+#pragma unroll
+        for (int k = 0; k < 15; ++k) {   // stage s at entries 2^s .. 2^(s+1) - 1
+            w[k] = t[2 * (1 + k)];
+            ws[k] = t[2 * (1 + k) + 1];
+        }
+    } else {
+        load_pass_twiddles(tw2, PASS, tid, w, ws);
+    }
+}
+
+// one pass with the pass index known at compile time (constant LDS offsets)
 template <int PASS, bool INV>
 __device__ __forceinline__ void ntt_pass_lds(uint32_t* s, const PrimeTab& P, int tid) {
     uint32_t v[16], w[15], ws[15];
-    load_pass_twiddles(INV ? P.iv2 : P.fw2, PASS, tid, w, ws);
+    load_pass_twiddles_dev<PASS>(INV ? P.iv2 : P.fw2, tid, w, ws);
 #pragma unroll
     for (int k = 0; k < 16; ++k) v[k] = s[lds_pad(pass_elem(PASS, tid, k))];
     if (INV) {
@@ -400,12 +418,12 @@ template <int PASS, bool INV, bool SAME = false>
 __device__ __forceinline__ void ntt_pass_lds2(uint32_t* sa, const PrimeTab& PA, uint32_t* sb, const PrimeTab& PB,
                                               int tid) {
     uint32_t va[16], vb[16], wa[15], wsa[15], wb[15], wsb[15];
-    load_pass_twiddles(INV ? PA.iv2 : PA.fw2, PASS, tid, wa, wsa);
+    load_pass_twiddles_dev<PASS>(INV ? PA.iv2 : PA.fw2, tid, wa, wsa);
     if (SAME) {     // one prime for both: share the twiddle registers
 #pragma unroll
         for (int t = 0; t < 15; ++t) { wb[t] = wa[t]; wsb[t] = wsa[t]; }
     } else {
-        load_pass_twiddles(INV ? PB.iv2 : PB.fw2, PASS, tid, wb, wsb);
+        load_pass_twiddles_dev<PASS>(INV ? PB.iv2 : PB.fw2, tid, wb, wsb);
     }
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
@@ -425,6 +443,57 @@ __device__ __forceinline__ void ntt_pass_lds2(uint32_t* sa, const PrimeTab& PA, 
         if (PASS == 2) {
             ntt_fwd_finish(va, PA.q);
             ntt_fwd_finish(vb, PB.q);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        sa[lds_pad(pass_elem(PASS, tid, k))] = va[k];
+        sb[lds_pad(pass_elem(PASS, tid, k))] = vb[k];
+    }
+}
+
+// ntt_pass_lds with the pass's twiddles already loaded
+template <int PASS, bool INV>
+__device__ __forceinline__ void ntt_pass_lds_tw(uint32_t* s, const PrimeTab& P, int tid, const uint32_t (&w)[15],
+                                                const uint32_t (&ws)[15]) {
+    uint32_t v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = s[lds_pad(pass_elem(PASS, tid, k))];
+    if (INV) {
+        ntt_inv_pass(v, w, ws, P.q);
+        if (PASS == 0) ntt_inv_finish(v, P.ninv, P.ninv_sh, P.q);
+    } else {
+        ntt_fwd_pass(v, w, ws, P.q);
+        if (PASS == 2) ntt_fwd_finish(v, P.q);
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) s[lds_pad(pass_elem(PASS, tid, k))] = v[k];
+}
+
+// ntt_pass_lds2 for two transforms in one prime with the pass's twiddles
+// already loaded (w, ws as load_pass_twiddles gives them)
+template <int PASS, bool INV>
+__device__ __forceinline__ void ntt_pass_lds2_tw(uint32_t* sa, uint32_t* sb, const PrimeTab& P, int tid,
+                                                 const uint32_t (&w)[15], const uint32_t (&ws)[15]) {
+    uint32_t va[16], vb[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        va[k] = sa[lds_pad(pass_elem(PASS, tid, k))];
+        vb[k] = sb[lds_pad(pass_elem(PASS, tid, k))];
+    }
+    if (INV) {
+        ntt_inv_pass(va, w, ws, P.q);
+        ntt_inv_pass(vb, w, ws, P.q);
+        if (PASS == 0) {
+            ntt_inv_finish(va, P.ninv, P.ninv_sh, P.q);
+            ntt_inv_finish(vb, P.ninv, P.ninv_sh, P.q);
+        }
+    } else {
+        ntt_fwd_pass(va, w, ws, P.q);
+        ntt_fwd_pass(vb, w, ws, P.q);
+        if (PASS == 2) {
+            ntt_fwd_finish(va, P.q);
+            ntt_fwd_finish(vb, P.q);
         }
     }
 #pragma unroll

@@ -838,15 +838,15 @@ template <int KS>
 __device__ __forceinline__ void ring_read(i8x16 (&b)[KS], uint32_t la) {
     i32x4 r[KS];
     if constexpr (KS == 1) {
-        asm volatile("ds_read_b128 %0, %1\n\ts_waitcnt lgkmcnt(0)" : "=v"(r[0]) : "v"(la) : "memory");
+        asm volatile("ds_read_b128 %0, %1\n\ts_waitcnt lgkmcnt(0)" : "=&v"(r[0]) : "v"(la) : "memory");
     } else if constexpr (KS == 2) {
         asm volatile("ds_read_b128 %0, %2\n\tds_read_b128 %1, %2 offset:1024\n\ts_waitcnt lgkmcnt(0)"
-                     : "=v"(r[0]), "=v"(r[1]) : "v"(la) : "memory");
+                     : "=&v"(r[0]), "=&v"(r[1]) : "v"(la) : "memory");
     } else {
         static_assert(KS == 4, "ring_read: KS in {1, 2, 4}");
         asm volatile("ds_read_b128 %0, %4\n\tds_read_b128 %1, %4 offset:1024\n\t"
                      "ds_read_b128 %2, %4 offset:2048\n\tds_read_b128 %3, %4 offset:3072\n\ts_waitcnt lgkmcnt(0)"
-                     : "=v"(r[0]), "=v"(r[1]), "=v"(r[2]), "=v"(r[3]) : "v"(la) : "memory");
+                     : "=&v"(r[0]), "=&v"(r[1]), "=&v"(r[2]), "=&v"(r[3]) : "v"(la) : "memory");
     }
 #pragma unroll
     for (int k = 0; k < KS; ++k) b[k] = __builtin_bit_cast(i8x16, r[k]);
@@ -874,7 +874,7 @@ __device__ __forceinline__ void stage_read(uint32_t (&w)[NI][4], uint32_t la) {
                      "ds_read_b32 %6, %8 offset:5792\n\t"
                      "ds_read_b32 %7, %8 offset:8368\n\t"
                      "s_waitcnt lgkmcnt(0)"
-                     : "=v"(w[0][0]), "=v"(w[0][1]), "=v"(w[0][2]), "=v"(w[0][3]), "=v"(w[1][0]), "=v"(w[1][1]), "=v"(w[1][2]), "=v"(w[1][3])
+                     : "=&v"(w[0][0]), "=&v"(w[0][1]), "=&v"(w[0][2]), "=&v"(w[0][3]), "=&v"(w[1][0]), "=&v"(w[1][1]), "=&v"(w[1][2]), "=&v"(w[1][3])
                      : "v"(la) : "memory");
     } else {
         static_assert(NI == 4, "stage_read: NI in {2, 4}");
@@ -895,7 +895,7 @@ __device__ __forceinline__ void stage_read(uint32_t (&w)[NI][4], uint32_t la) {
                      "ds_read_b32 %14, %16 offset:7072\n\t"
                      "ds_read_b32 %15, %16 offset:9648\n\t"
                      "s_waitcnt lgkmcnt(0)"
-                     : "=v"(w[0][0]), "=v"(w[0][1]), "=v"(w[0][2]), "=v"(w[0][3]), "=v"(w[1][0]), "=v"(w[1][1]), "=v"(w[1][2]), "=v"(w[1][3]), "=v"(w[2][0]), "=v"(w[2][1]), "=v"(w[2][2]), "=v"(w[2][3]), "=v"(w[3][0]), "=v"(w[3][1]), "=v"(w[3][2]), "=v"(w[3][3])
+                     : "=&v"(w[0][0]), "=&v"(w[0][1]), "=&v"(w[0][2]), "=&v"(w[0][3]), "=&v"(w[1][0]), "=&v"(w[1][1]), "=&v"(w[1][2]), "=&v"(w[1][3]), "=&v"(w[2][0]), "=&v"(w[2][1]), "=&v"(w[2][2]), "=&v"(w[2][3]), "=&v"(w[3][0]), "=&v"(w[3][1]), "=&v"(w[3][2]), "=&v"(w[3][3])
                      : "v"(la) : "memory");
     }
 }
@@ -1334,8 +1334,8 @@ __device__ __forceinline__ void dma_planes(uint32_t (*dst)[kPadN], const uint32_
     typedef __attribute__((address_space(1))) void glb_void;
     const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
 #pragma unroll
-    for (int b = wv; b < NP * kN / 64; b += kFusedThreads / 64) {
-        const int j = b >> 6, x0 = (b & 63) * 64;
+    for (int t = 0; t < NP * kN / 64 / (kFusedThreads / 64); ++t) {   // a trip count fixed at compile time
+        const int b = wv + t * (kFusedThreads / 64), j = b >> 6, x0 = (b & 63) * 64;
         __builtin_amdgcn_global_load_lds((glb_void*)(src + j * kN + x0 + lane), (lds_void*)(&dst[j][lds_pad(x0)]), 4, 0,
                                          0);
     }
@@ -1350,8 +1350,8 @@ __device__ __forceinline__ void dma_planes_flat(uint32_t (*dst)[kPadN], const ui
     typedef __attribute__((address_space(1))) void glb_void;
     const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
 #pragma unroll
-    for (int b = wv; b < NP * kN / 256; b += kFusedThreads / 64) {
-        const int j = b >> 4, x0 = (b & 15) * 256;
+    for (int t = 0; t < NP * kN / 256 / (kFusedThreads / 64); ++t) {
+        const int b = wv + t * (kFusedThreads / 64), j = b >> 4, x0 = (b & 15) * 256;
         __builtin_amdgcn_global_load_lds((glb_void*)(src + j * kN + x0 + 4 * lane), (lds_void*)(&dst[j][x0]), 16, 0,
                                          0);
     }
@@ -1368,8 +1368,8 @@ __device__ __forceinline__ void dma_planes_z(uint32_t (*dst)[kPadN], const uint3
     typedef __attribute__((address_space(1))) void glb_void;
     const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
 #pragma unroll
-    for (int b = wv; b < NP * kN / 64; b += kFusedThreads / 64) {
-        const int j = b >> 6, x0 = (b & 63) * 64, z = x0 + lane;
+    for (int t = 0; t < NP * kN / 64 / (kFusedThreads / 64); ++t) {
+        const int b = wv + t * (kFusedThreads / 64), j = b >> 6, x0 = (b & 63) * 64, z = x0 + lane;
         const uint32_t* src = blk + (size_t)j * 2 * n2 * kN + (size_t)(z >> 4) * 32 * n2 + i * 32 + ab * 16 + (z & 15);
         __builtin_amdgcn_global_load_lds((glb_void*)src, (lds_void*)(&dst[j][lds_pad(x0)]), 4, 0, 0);
     }
@@ -1382,8 +1382,8 @@ __device__ __forceinline__ void dma_planes_flat_z(uint32_t (*dst)[kPadN], const 
     typedef __attribute__((address_space(1))) void glb_void;
     const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
 #pragma unroll
-    for (int b = wv; b < NP * kN / 256; b += kFusedThreads / 64) {
-        const int j = b >> 4, x0 = (b & 15) * 256, z = x0 + 4 * lane;
+    for (int t = 0; t < NP * kN / 256 / (kFusedThreads / 64); ++t) {
+        const int b = wv + t * (kFusedThreads / 64), j = b >> 4, x0 = (b & 15) * 256, z = x0 + 4 * lane;
         const uint32_t* src = blk + (size_t)j * 2 * n2 * kN + (size_t)(z >> 4) * 32 * n2 + i * 32 + ab * 16 + (z & 15);
         __builtin_amdgcn_global_load_lds((glb_void*)src, (lds_void*)(&dst[j][x0]), 16, 0, 0);
     }
@@ -1419,7 +1419,14 @@ template <bool EXT, bool ZMAJ>
 __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_t* __restrict__ dbout,
         const KeyPtrs* __restrict__ kp, uint32_t* __restrict__ fd, uint32_t* __restrict__ ecoef, int r, int nq,
         int n1, int n2, QS S, Tabs T, int b0 = 0) {
-    __shared__ uint32_t s[8][kPadN];
+    // the planes 0..3 (digits), 4, 5 (a-row) and 6, 7 (b-row) as three LDS
+    // objects: the compiler then knows that reading one while a copy into
+    // another is in flight needs no wait
+    __shared__ __attribute__((aligned(16))) uint32_t sd[4][kPadN];
+    __shared__ __attribute__((aligned(16))) uint32_t sa[2][kPadN];
+    __shared__ __attribute__((aligned(16))) uint32_t sb[2][kPadN];
+    // plane i of the eight (i known at compile time where it is used)
+    auto plane = [&](int i) -> uint32_t* { return i < 4 ? sd[i] : i < 6 ? sa[i - 4] : sb[i - 6]; };
     // r: blocks of this launch (dbout holds their rows); b0: index of the
     // first of them in the outputs fd / ecoef / ext
     const int L = xcd_job(blockIdx.x, r * nq);
@@ -1452,7 +1459,7 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_
             const uint32_t kb0v[4] = {kb0.x, kb0.y, kb0.z, kb0.w}, kb1v[4] = {kb1.x, kb1.y, kb1.z, kb1.w};
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const uint64_t d0 = s[2 * p][lds_pad(x0) + i], d1 = s[2 * p + 1][lds_pad(x0) + i];
+                const uint64_t d0 = sd[2 * p][lds_pad(x0) + i], d1 = sd[2 * p + 1][lds_pad(x0) + i];
                 const uint64_t ta = d0 * ka0v[i] + d1 * ka1v[i];
                 const uint64_t tb = d0 * kb0v[i] + d1 * kb1v[i];
                 acc[0][p][4 * h + i] = sub_mod(acc[0][p][4 * h + i], mont_reduce(ta, P.q, P.mont_qinv), P.q);
@@ -1462,12 +1469,12 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_
     };
     // row i's a-planes into planes 4, 5 (padded), its b-planes into 6, 7 (flat)
     auto dma_a = [&](int i) {
-        if (ZMAJ) dma_planes_z<2>(s + 4, rows, i, 0, n2, tid);
-        else dma_planes<2>(s + 4, rows + (size_t)i * 4 * kN, tid);
+        if (ZMAJ) dma_planes_z<2>(sa, rows, i, 0, n2, tid);
+        else dma_planes<2>(sa, rows + (size_t)i * 4 * kN, tid);
     };
     auto dma_b = [&](int i) {
-        if (ZMAJ) dma_planes_flat_z<2>(s + 6, rows, i, 1, n2, tid);
-        else dma_planes_flat<2>(s + 6, rows + (size_t)i * 4 * kN + 2 * kN, tid);
+        if (ZMAJ) dma_planes_flat_z<2>(sb, rows, i, 1, n2, tid);
+        else dma_planes_flat<2>(sb, rows + (size_t)i * 4 * kN + 2 * kN, tid);
     };
     dma_a(n2 > 1 ? 1 : 0);
     dma_b(n2 > 1 ? 1 : 0);
@@ -1482,8 +1489,8 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_
 #pragma unroll
         for (int e = 0; e < PER; ++e) {
             const int px = perm_slot(4 * (tid + kFusedThreads * (e >> 2)) + (e & 3), g, T);
-            acc[1][0][e] = add_mod(acc[1][0][e], s[6][px], q1);
-            acc[1][1][e] = add_mod(acc[1][1][e], s[7][px], q2);
+            acc[1][0][e] = add_mod(acc[1][0][e], sb[0][px], q1);
+            acc[1][1][e] = add_mod(acc[1][1][e], sb[1][px], q2);
         }
         __builtin_amdgcn_sched_barrier(0);
         // the key switch of step c-1 (digits still in planes 0..3) rides along
@@ -1491,17 +1498,25 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_
         // while the transform computes.  Both halves end before the transform's
         // second barrier, so every thread is done with planes 0..3 before the
         // digits of row c overwrite them.
-        ntt_pass_lds<2, true>(s[4 + half], T.p[half], ht);
+        ntt_pass_lds<2, true>(sa[half], T.p[half], ht);
         if (c >= 2) mac(c - 1, 0);
         __syncthreads();
         __builtin_amdgcn_sched_barrier(0);
-        dma_b(next);     // every wave is past its b-gather
-        __builtin_amdgcn_sched_barrier(0);
-        ntt_pass_lds<1, true>(s[4 + half], T.p[half], ht);
+        ntt_pass_lds<1, true>(sa[half], T.p[half], ht);
         if (c >= 2) mac(c - 1, 1);
         __syncthreads();
-        ntt_pass_lds<0, true>(s[4 + half], T.p[half], ht);
-        __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
+        // every wave is past its b-gather: row c+1's b-planes are copied now.
+        // With a copy in flight the compiler makes the first use of any
+        // later vector load wait for the copy as well (vmcnt(0)); the next
+        // ones are the forward transform's pass-1 twiddles, two thirds of a
+        // step away (pass-0 twiddles are scalar loads)
+        dma_b(next);
+        __builtin_amdgcn_sched_barrier(0);
+        ntt_pass_lds<0, true>(sa[half], T.p[half], ht);
+        // from here to the top of the loop a copy is in flight: barriers for
+        // LDS data only (a __syncthreads() would wait for the copy)
+        lds_barrier();
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int e = 0; e < PER; ++e) {
@@ -1509,19 +1524,32 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_
             const uint32_t i0 = ((uint32_t)x * ginv) & (2 * kN - 1);
             const bool neg = i0 >= (uint32_t)kN;
             const int i = neg ? (int)i0 - kN : (int)i0;
-            uint64_t v = crt_q(s[4][lds_pad(i)], s[5][lds_pad(i)], T);
+            uint64_t v = crt_q(sa[0][lds_pad(i)], sa[1][lds_pad(i)], T);
             if (neg) v = v ? kQ - v : 0;
             const int32_t d0 = tfhe_digit<18, 20, 2>(v, 0), d1 = tfhe_digit<18, 20, 2>(v, 1);
-            s[0][lds_pad(x)] = digit_residue(d0, q1);
-            s[1][lds_pad(x)] = digit_residue(d1, q1);
-            s[2][lds_pad(x)] = digit_residue(d0, q2);
-            s[3][lds_pad(x)] = digit_residue(d1, q2);
+            sd[0][lds_pad(x)] = digit_residue(d0, q1);
+            sd[1][lds_pad(x)] = digit_residue(d1, q1);
+            sd[2][lds_pad(x)] = digit_residue(d0, q2);
+            sd[3][lds_pad(x)] = digit_residue(d1, q2);
         }
-        __syncthreads();
+        lds_barrier();
         __builtin_amdgcn_sched_barrier(0);
-        dma_a(next);
-        __builtin_amdgcn_sched_barrier(0);
-        ntt_fwd_r16x2_same(s[2 * half], s[2 * half + 1], T.p[half], ht);
+        // forward transforms of the digits; row c+1's a-planes are copied
+        // once the last pass's twiddles have arrived (as above)
+        ntt_pass_lds2<0, false, true>(sd[2 * half], T.p[half], sd[2 * half + 1], T.p[half], ht);
+        lds_barrier();
+        ntt_pass_lds2<1, false, true>(sd[2 * half], T.p[half], sd[2 * half + 1], T.p[half], ht);
+        lds_barrier();
+        {
+            uint32_t w[15], ws[15];
+            load_pass_twiddles(T.p[half].fw2, 2, ht, w, ws);
+            wait_vmem();
+            __builtin_amdgcn_sched_barrier(0);
+            dma_a(next);
+            __builtin_amdgcn_sched_barrier(0);
+            ntt_pass_lds2_tw<2, false>(sd[2 * half], sd[2 * half + 1], T.p[half], ht, w, ws);
+        }
+        lds_barrier();
         __builtin_amdgcn_sched_barrier(0);
     }
     if (n2 > 1) {   // the last step's key switch
@@ -1543,9 +1571,9 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_
                 uint32_t v[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    const uint32_t r0 = ab ? s[6 + p][x0 + i] : s[4 + p][lds_pad(x0) + i];   // b-planes unpadded
+                    const uint32_t r0 = ab ? sb[p][x0 + i] : sa[p][lds_pad(x0) + i];   // b-planes unpadded
                     v[i] = add_mod(acc[ab][p][4 * h + i], r0, T.p[p].q);
-                    s[ab * 2 + p][lds_pad(x0) + i] = v[i];
+                    sd[ab * 2 + p][lds_pad(x0) + i] = v[i];
                 }
                 if (fd)
                     *reinterpret_cast<uint4*>(fd + (((size_t)ob * 2 + ab) * 2 + p) * kN + x0) =
@@ -1554,14 +1582,14 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_
     }
     __syncthreads();
     __builtin_amdgcn_sched_barrier(0);
-    ntt_inv_r16x2(s[2 * half], T.p[0], s[2 * half + 1], T.p[1], ht);    // half = ab
+    ntt_inv_r16x2(sd[2 * half], T.p[0], sd[2 * half + 1], T.p[1], ht);    // half = ab
     if (!EXT) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int x0 = 4 * (tid + kFusedThreads * h);
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
-                const uint32_t* sp = &s[t][lds_pad(x0)];
+                const uint32_t* sp = &sd[t][lds_pad(x0)];
                 *reinterpret_cast<uint4*>(ecoef + ((size_t)ob * 4 + t) * kN + x0) =
                     make_uint4(sp[0], sp[1], sp[2], sp[3]);
             }
@@ -1578,7 +1606,7 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int src = 0; src < 2; ++src) {
-                const uint64_t v = crt_q(s[2 * src][lds_pad(x0) + i], s[2 * src + 1][lds_pad(x0) + i], T);
+                const uint64_t v = crt_q(sd[2 * src][lds_pad(x0) + i], sd[2 * src + 1][lds_pad(x0) + i], T);
                 dg[h][2 * src][i] = signed_digit<18, 20>(v, 0);
                 dg[h][2 * src + 1][i] = signed_digit<18, 20>(v, 1);
             }
@@ -1592,12 +1620,12 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_
 #pragma unroll
             for (int p = 0; p < 2; ++p)
 #pragma unroll
-                for (int i = 0; i < 4; ++i) s[2 * k + p][lds_pad(x0) + i] = digit_residue(dg[h][k][i], T.p[p].q);
+                for (int i = 0; i < 4; ++i) plane(2 * k + p)[lds_pad(x0) + i] = digit_residue(dg[h][k][i], T.p[p].q);
     }
     __syncthreads();
     __builtin_amdgcn_sched_barrier(0);
-    ntt_fwd_r16x2_same(s[0 + half], s[2 + half], T.p[half], ht);    // digits 0, 1 in prime `half`
-    ntt_fwd_r16x2_same(s[4 + half], s[6 + half], T.p[half], ht);    // digits 2, 3
+    ntt_fwd_r16x2_same(sd[half], sd[2 + half], T.p[half], ht);    // digits 0, 1 in prime `half`
+    ntt_fwd_r16x2_same(sa[half], sb[half], T.p[half], ht);    // digits 2, 3
     __builtin_amdgcn_sched_barrier(0);
     const uint32_t* rg = kp[qi].rgsw;   // [k][ab][p][N], Montgomery form
 #pragma unroll
@@ -1612,7 +1640,7 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     const uint4 kv = *reinterpret_cast<const uint4*>(rg + ((k * 2 + ab) * 2 + p) * kN + x0);
-                    const uint32_t* sp = &s[2 * k + p][lds_pad(x0)];
+                    const uint32_t* sp = &plane(2 * k + p)[lds_pad(x0)];
                     acc[0] += (uint64_t)sp[0] * kv.x;
                     acc[1] += (uint64_t)sp[1] * kv.y;
                     acc[2] += (uint64_t)sp[2] * kv.z;
