@@ -706,20 +706,30 @@ __global__ void __launch_bounds__(256) k_ct_limbs(const uint32_t* __restrict__ c
     }
     __syncthreads();
     const int lane = tid & 63, wv = tid >> 6;
-    const bool odd = lane & 1, hi = lane & 2;
-    for (int it = wv; it < 32 * 2 * (N1 / 64); it += 4) {
+    // 4x4 byte transpose in each lane quad by byte permutes (perm(a, b, sel)
+    // picks bytes of b as 0..3 and of a as 4..7): first with lane^1 the
+    // bytes [mine 0, its 0, mine 2, its 2] (even lane) or [its 1, mine 1,
+    // its 3, mine 3] (odd), then with lane^2 the half-words [mine, its]
+    // (lane & 2 clear) or [its high, mine high]
+    const uint32_t sel1 = (lane & 1) ? 0x03070105u : 0x06020400u;
+    const uint32_t sel2 = (lane & 2) ? 0x03020706u : 0x05040100u;
+    // this lane's part of the ql offset: limb v, rotations 4 (lane >> 2) ..
+    // (ql of the chunk is below 4 GiB: 32-bit offsets)
+    const uint32_t lane_off = (uint32_t)(lane & 3) * C * N1 + 4 * (lane >> 2);
+    constexpr int ITERS = 32 * 2 * (N1 / 64) / 4;   // per wave
+#pragma unroll 8
+    for (int t = 0; t < ITERS; ++t) {
+        const int it = wv + 4 * t;               // uniform over the wave
         const int zz = it & 31, ab = (it >> 5) & 1, kr = it >> 6;
         const uint32_t x = tile[(ab * N1 + kr * 64 + lane) * PITCH + zz];
         // byte v = limb v of this lane's word (x < 2^28)
         const uint32_t w = (x & 0x7Fu) | ((x << 1) & 0x7F00u) | ((x << 2) & 0x7F0000u) | ((x << 3) & 0x7F000000u);
-        // 4x4 byte transpose in each quad: swap bytes with lane^1, then half-words with lane^2
         const uint32_t pw = (uint32_t)__builtin_amdgcn_mov_dpp((int)w, 0xB1, 0xF, 0xF, false);   // quad_perm 1,0,3,2
-        const uint32_t xx = odd ? (((pw >> 8) & 0x00FF00FFu) | (w & 0xFF00FF00u))
-                                : ((w & 0x00FF00FFu) | ((pw & 0x00FF00FFu) << 8));
+        const uint32_t xx = __builtin_amdgcn_perm(pw, w, sel1);
         const uint32_t px = (uint32_t)__builtin_amdgcn_mov_dpp((int)xx, 0x4E, 0xF, 0xF, false);  // quad_perm 2,3,0,1
-        const uint32_t o = hi ? ((px >> 16) | (xx & 0xFFFF0000u)) : ((xx & 0xFFFFu) | (px << 16));
-        const int v = lane & 3, k4 = kr * 16 + (lane >> 2);
-        *reinterpret_cast<uint32_t*>(ql + ((((size_t)(z0 + zz) * 2 + p) * 4 + v) * C + q * 2 + ab) * N1 + 4 * k4) = o;
+        const uint32_t o = __builtin_amdgcn_perm(px, xx, sel2);
+        const uint32_t wave_off = ((((uint32_t)(z0 + zz) * 2 + p) * 4 * C + q * 2 + ab) * N1 + 64 * kr);
+        *reinterpret_cast<uint32_t*>(ql + (wave_off + lane_off)) = o;
     }
 }
 
