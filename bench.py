@@ -128,11 +128,13 @@ def cpu_baseline(r, n1, reps):
 MFMA_I8_PEAK_TOPS = 5000.0     # MI355X dense int8 MFMA, 5.0 POP/s
 
 
-def batched_roofline(r, n1, B, db3):
-    """the batched step's database pass (three kernels, timed live with HIP
-    events) against its bounds: k_db_mfma's int8-MFMA rate, and the pass's
-    algorithmic HBM bytes over the three kernels' time; VALU utilisation of
-    the transform kernels from the committed PMC summary"""
+def batched_roofline(r, n1, B, db3, direct=True):
+    """the batched step's database pass (its kernels, timed live with HIP
+    events) against its bounds: the MFMA kernel's int8 rate, and the pass's
+    algorithmic HBM bytes over the kernels' time; VALU utilisation of the
+    transform kernels from the committed PMC summary.  direct: the products
+    are written in the first-dimension layout by k_db_mfma_direct (no move
+    kernel); otherwise k_db_mfma + k_mfma_scatter"""
     if not db3:
         return None
     n2 = N // 2 // n1
@@ -144,11 +146,14 @@ def batched_roofline(r, n1, B, db3):
     # residue) in, the first-dimension products (4 B per residue) out
     alg = 8 * r * N * (N // 2) + B * n1 * 4 * N * 4 + B * R * 4 * N * 4
     pass_ms = limbs_ms + mfma_ms + move_ms
-    out = {"kernel": "k_db_mfma (batched int8-MFMA database pass)", "bound": "mfma",
+    name = "k_db_mfma_direct" if direct else "k_db_mfma"
+    kernels = {"k_ct_limbs": round(limbs_ms, 4), name: round(mfma_ms, 4)}
+    if not direct:
+        kernels["k_mfma_scatter"] = round(move_ms, 4)
+    out = {"kernel": name + " (batched int8-MFMA database pass)", "bound": "mfma",
            "achieved": round(tops, 1), "peak": MFMA_I8_PEAK_TOPS, "unit": "TOP/s (int8)",
            "frac": round(tops / MFMA_I8_PEAK_TOPS, 4), "ops_per_launch": ops, "avg_launch_ms": round(mfma_ms, 4),
-           "db_pass": {"kernels_ms": {"k_ct_limbs": round(limbs_ms, 4), "k_db_mfma": round(mfma_ms, 4),
-                                      "k_mfma_scatter": round(move_ms, 4)},
+           "db_pass": {"kernels_ms": kernels,
                        "algorithmic_bytes": alg, "achieved_GBps": round(alg / (pass_ms * 1e-3) / 1e9, 1),
                        "peak_GBps": HBM_PEAK_GBPS, "frac": round(alg / (pass_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)}}
     path = os.path.join(ROOT, "profiles", "batched_step_utilisation.json")
@@ -588,7 +593,7 @@ def main():
                            "p99": round(float(np.percentile(plain_lat, 99)), 4), "samples": len(plain_lat)},
         },
         "batched_over_plain": round(qps / plain_qps, 2),
-        "roofline_batched": batched_roofline(r, n1, B, db3),
+        "roofline_batched": batched_roofline(r, n1, B, db3, direct=bool(ctx.get("mfma_direct"))),
         "roofline": {"kernel": "k_db_product (single-query database pass)", "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic_from_profile(),

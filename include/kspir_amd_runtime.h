@@ -113,9 +113,10 @@ int kspir_time_db_pass(kspir_context* ctx, const kspir_database* db, const kspir
 /* HIP-event timing of the batched (int8-MFMA) database pass of nq >= 2
  * queries, one kernel at a time (after one first dimension has filled the
  * rotation words): ms3[0] the query limbs (k_ct_limbs), ms3[1] the MFMA
- * products (k_db_mfma, or k_db_mfma_direct with mfma_direct), ms3[2] their
- * move into the first-dimension layout (k_mfma_scatter; 0 work with
- * mfma_direct), each the average milliseconds per launch over iters. */
+ * products (with mfma_direct: k_db_mfma_direct writing the slot-group-major
+ * first-dimension layout of the batched pipeline; otherwise k_db_mfma),
+ * ms3[2] their move into the first-dimension layout (k_mfma_scatter; no work
+ * with mfma_direct), each the average milliseconds per launch over iters. */
 int kspir_time_db_pass_batch(kspir_context* ctx, const kspir_database* db, const kspir_keys* const* keys,
                              int32_t nq, const uint64_t* queries_dev, int iters, float* ms3);
 

@@ -1554,10 +1554,12 @@ int kspir_time_db_pass_batch(kspir_context* c, const kspir_database* db, const k
     HIP_OK(hipEventCreate(&e0));
     HIP_OK(hipEventCreate(&e1));
     const int part[3] = {1, 2, 4};
+    // the direct pass in the slot-group-major layout, as the batched pipeline runs it
+    const bool zmaj = c->mfma_direct != 0;
     for (int t = 0; t < 3; ++t) {
         HIP_OK(hipEventRecord(e0, c->stream));
         for (int i = 0; i < iters; ++i) {
-            rc = launch_db_batched(c, db, 0, nq, c->stream, part[t]);
+            rc = launch_db_batched(c, db, 0, nq, c->stream, part[t], 0, 0, zmaj);
             if (rc) return rc;
         }
         HIP_OK(hipEventRecord(e1, c->stream));
