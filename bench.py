@@ -205,7 +205,7 @@ def config_c4(args, log):
     ctx = kspir_amd.Context(0)
     perms = kspir_amd.compute_permutation_matrix(n1)
     rec = synthetic_records()
-    groups = []
+    groups, host_q = [], []
     seed = 5000
     for r, nq in plan:
         log(f"C4 database r={r}, {nq} clients")
@@ -219,6 +219,7 @@ def config_c4(args, log):
         qbuf, rbuf = ctx.alloc(nq * 4 * N * 8), ctx.alloc(nq * 2 * N * 8)
         qbuf.upload(np.stack(qs))
         groups.append((r, nq, db, keys, qbuf, rbuf))
+        host_q.append(np.stack(qs))
     total_q = sum(nq for _, nq in plan)
     out = {"config": "C4 uneven: 64 MB / 512 MB / 4 GB databases (r 4 / 32 / 256, N1 128) with 16 / 32 / 48 "
                      "queries of distinct clients, one step = all 96 answered back to back"}
@@ -270,6 +271,22 @@ def config_c4(args, log):
                               "responses_equal_sequential": True}
     for c in extra:
         c.close()
+    # one batch spanning the three databases: one launch per stage for all
+    # 96 queries (kspir_answer_multi_device), checked against the per-database
+    # batches, timed with HIP events
+    log("C4 one launch per stage across the three databases")
+    segs = [(db, keys) for r, nq, db, keys, qbuf, rbuf in groups]
+    mq, mr = ctx.alloc(total_q * 4 * N * 8), ctx.alloc(total_q * 2 * N * 8)
+    mq.upload(np.concatenate(host_q))
+    ctx.answer_multi_device(segs, mq, mr)
+    ctx.synchronize()
+    if not np.array_equal(mr.download(np.zeros((total_q, 2, N), dtype=np.uint64)), np.concatenate(want)):
+        raise RuntimeError("C4 responses of the batch spanning databases differ from the per-database batches")
+    ms = ctx.time_answer_multi(segs, mq, mr, 1, steps) / steps
+    out["mfma_one_launch"] = {"step_ms": round(ms, 3), "qps": round(total_q * 1000.0 / ms, 1),
+                              "responses_equal_per_database": True}
+    mq.free()
+    mr.free()
     for r, nq, db, keys, qbuf, rbuf in groups:
         for k in keys:
             k.close()

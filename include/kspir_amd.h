@@ -144,6 +144,25 @@ int kspir_answer_batch(kspir_context* ctx, const kspir_database* db, const kspir
 int kspir_answer_batch_device(kspir_context* ctx, const kspir_database* db, const kspir_keys* const* keys,
                               int32_t nq, const uint64_t* queries_dev, uint64_t* responses_dev, void* stream);
 
+/* Batched online answer spanning databases of uneven size: nseg databases
+ * (1..8, one N1, each with an int8-MFMA shape), seg_queries[s] consecutive
+ * queries against dbs[s], keys[q] per query (for its database's r and N1).
+ * One launch per stage serves them all: the baby steps of every query, per
+ * chunk of blocks one database pass over every database that has rows there
+ * and one giant-step launch over their (query, block) pairs, one launch per
+ * packing level over the queries deep enough for it.  queries
+ * [sum seg_queries][2][2][N] -> responses [sum seg_queries][2][N]; response i
+ * equals kspir_answer on query i alone against its own database.  Needs the
+ * default pipeline options (mfma_direct, giant_dma, ext_fused, giant_wgs_per_cu
+ * 1).  No counterpart in the reference, which answers one query against one
+ * database per call. */
+int kspir_answer_multi(kspir_context* ctx, int32_t nseg, const kspir_database* const* dbs,
+                       const int32_t* seg_queries, const kspir_keys* const* keys, const uint64_t* queries,
+                       uint64_t* responses);
+int kspir_answer_multi_device(kspir_context* ctx, int32_t nseg, const kspir_database* const* dbs,
+                              const int32_t* seg_queries, const kspir_keys* const* keys,
+                              const uint64_t* queries_dev, uint64_t* responses_dev, void* stream);
+
 /* matrix_vector_mul_bsgs_rns_crt_large (twosteps.h:55).  result: [r][2][N]
  * NTT form.  Like the reference, query[0] (a,b mod Q) is replaced by the
  * modulus-switched query in NTT form (twosteps.cpp:1074-1086); query[1] is

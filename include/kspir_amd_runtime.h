@@ -105,6 +105,11 @@ int kspir_time_answer_batch(kspir_context* ctx, const kspir_database* db, const 
                             int32_t nq, const uint64_t* queries_dev, uint64_t* responses_dev, int warmup,
                             int iters, float* ms);
 
+/* same for a batch spanning databases (kspir_answer_multi_device per call) */
+int kspir_time_answer_multi(kspir_context* ctx, int32_t nseg, const kspir_database* const* dbs,
+                            const int32_t* seg_queries, const kspir_keys* const* keys, const uint64_t* queries_dev,
+                            uint64_t* responses_dev, int warmup, int iters, float* ms);
+
 /* HIP-event timing of the database-pass kernel alone (after one answer has
  * filled the rotation words): *ms_per_launch = average milliseconds. */
 int kspir_time_db_pass(kspir_context* ctx, const kspir_database* db, const kspir_keys* keys,

@@ -44,7 +44,8 @@ def _load():
                  "kspir_query_footprint_bytes", "kspir_decrypt_bsgs_total", "kspir_context_get",
                  "kspir_autotune", "kspir_client_generate", "kspir_client_draws", "kspir_client_encrypt",
                  "kspir_server_create_lanes", "kspir_gen_giant_keys_from_offline", "kspir_answer_device_noalloc",
-                 "kspir_inverse_encode", "kspir_time_db_pass_batch"):
+                 "kspir_inverse_encode", "kspir_time_db_pass_batch", "kspir_answer_multi",
+                 "kspir_answer_multi_device", "kspir_time_answer_multi"):
         getattr(L, name).restype = ctypes.c_int
     L.kspir_autotune.argtypes = [_vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                  ctypes.POINTER(ctypes.c_int32), ctypes.c_char_p, ctypes.c_size_t]
@@ -71,7 +72,7 @@ EXPORTS = (
     "kspir_query_footprint_bytes", "kspir_decrypt_bsgs_total", "kspir_context_get", "kspir_autotune",
     "kspir_client_generate", "kspir_client_draws", "kspir_client_encrypt", "kspir_server_create_lanes",
     "kspir_gen_giant_keys_from_offline", "kspir_answer_device_noalloc", "kspir_inverse_encode",
-    "kspir_time_db_pass_batch",
+    "kspir_time_db_pass_batch", "kspir_answer_multi", "kspir_answer_multi_device", "kspir_time_answer_multi",
 )
 
 
@@ -348,6 +349,37 @@ class Context:
         arr = (_vp * nq)(*[k.h.value for k in keys_list])
         ms = ctypes.c_float(0)
         _check(lib.kspir_time_answer_batch(self.h, db.h, arr, nq, ctypes.cast(_vp(queries_buf.ptr), _u64p),
+                                           ctypes.cast(_vp(resp_buf.ptr), _u64p), warmup, iters, ctypes.byref(ms)))
+        return ms.value
+
+    @staticmethod
+    def _multi_args(segments):
+        """segments: [(Database, [Keys, ...]), ...] -> ctypes arrays"""
+        dbs = (_vp * len(segments))(*[db.h.value for db, _ in segments])
+        counts = (ctypes.c_int32 * len(segments))(*[len(ks) for _, ks in segments])
+        keys = [k for _, ks in segments for k in ks]
+        arr = (_vp * len(keys))(*[k.h.value for k in keys])
+        return len(segments), dbs, counts, arr, len(keys)
+
+    def answer_multi(self, segments, queries):
+        """one batch spanning databases: segments [(Database, [Keys per query])],
+        queries [sum nq][2][2][N] in segment order -> responses [sum nq][2][N]"""
+        nseg, dbs, counts, arr, nq = self._multi_args(segments)
+        q = np.ascontiguousarray(queries, dtype=np.uint64)
+        assert q.shape == (nq, 2, 2, N)
+        resp = np.zeros((nq, 2, N), dtype=np.uint64)
+        _check(lib.kspir_answer_multi(self.h, nseg, dbs, counts, arr, _p64(q), _p64(resp)))
+        return resp
+
+    def answer_multi_device(self, segments, queries_buf, resp_buf, stream=None):
+        nseg, dbs, counts, arr, nq = self._multi_args(segments)
+        _check(lib.kspir_answer_multi_device(self.h, nseg, dbs, counts, arr, ctypes.cast(_vp(queries_buf.ptr), _u64p),
+                                             ctypes.cast(_vp(resp_buf.ptr), _u64p), _vp(stream)))
+
+    def time_answer_multi(self, segments, queries_buf, resp_buf, warmup, iters):
+        nseg, dbs, counts, arr, nq = self._multi_args(segments)
+        ms = ctypes.c_float(0)
+        _check(lib.kspir_time_answer_multi(self.h, nseg, dbs, counts, arr, ctypes.cast(_vp(queries_buf.ptr), _u64p),
                                            ctypes.cast(_vp(resp_buf.ptr), _u64p), warmup, iters, ctypes.byref(ms)))
         return ms.value
 

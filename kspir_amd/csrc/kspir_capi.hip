@@ -3,6 +3,7 @@
 // answer behind the C ABI of include/kspir_amd.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstdio>
@@ -145,6 +146,7 @@ struct kspir_context {
         void* resp;
         hipStream_t st;
         hipGraphExec_t exec;   // null: capture failed, run eagerly
+        std::vector<const void*> segs;   // a batch spanning databases: (database, query count) per segment
     };
     std::vector<Graph> graphs;
     std::mutex graph_mu;   // the cache is also dropped from threads that free handles
@@ -165,7 +167,8 @@ static void drop_graphs(kspir_context* c, const void* only = nullptr) {
     std::lock_guard<std::mutex> lock(c->graph_mu);
     std::vector<kspir_context::Graph> keep;
     for (auto& g : c->graphs) {
-        if (only && g.db != only && g.kp != only) { keep.push_back(g); continue; }
+        const bool names = g.db == only || g.kp == only || std::find(g.segs.begin(), g.segs.end(), only) != g.segs.end();
+        if (only && !names) { keep.push_back(g); continue; }
         if (g.exec) hipGraphExecDestroy(g.exec);
     }
     c->graphs.swap(keep);
@@ -381,8 +384,7 @@ size_t layout_workspace(kspir_context* c, int r, int n1, int nq, uint32_t* base,
 // carve the workspace for nq queries of shape (r, n1); grows on demand.
 // allow_chunk = false keeps whole-database buffers (callers that run the
 // first dimension alone).
-int carve_workspace(kspir_context* c, int r, int n1, int nq, bool allow_chunk = true) {
-    const int cb = allow_chunk ? chunk_blocks_for(c, r, n1, nq) : 0;
+int carve_workspace_cb(kspir_context* c, int r, int n1, int nq, int cb) {
     const size_t total = layout_workspace(c, r, n1, nq, nullptr, false, cb);
     if (total * 4 > c->ws_bytes) {
         if (c->ws) {
@@ -399,6 +401,10 @@ int carve_workspace(kspir_context* c, int r, int n1, int nq, bool allow_chunk = 
     layout_workspace(c, r, n1, nq, (uint32_t*)c->ws, false, cb);
     c->cur_cb = cb;
     return KSPIR_OK;
+}
+
+int carve_workspace(kspir_context* c, int r, int n1, int nq, bool allow_chunk = true) {
+    return carve_workspace_cb(c, r, n1, nq, allow_chunk ? chunk_blocks_for(c, r, n1, nq) : 0);
 }
 
 int grid1d(size_t total) {
@@ -460,11 +466,12 @@ int mfma_sweep_queries(const kspir_context* c, int n1) {
 // k_db_mfma_direct in either first-dimension layout
 template <int KS, int RTW>
 void launch_direct(bool zmaj, dim3 g, hipStream_t st, const int8_t* dbl, const int8_t* ql, uint32_t* out, int R, int Rt,
-                   int r0, int C, int n2, int nq, const QS& S, const MfmaConst& MC, const Tabs& T) {
+                   int r0, int C, int n2, int nq, const QS& S, const MfmaConst& MC, const Tabs& T,
+                   const DbSegs& SG = DbSegs{}) {
     if (zmaj)
-        hipLaunchKernelGGL((k_db_mfma_direct<KS, RTW, true>), g, dim3(512), 0, st, dbl, ql, out, R, Rt, r0, C, n2, nq, S, MC, T);
+        hipLaunchKernelGGL((k_db_mfma_direct<KS, RTW, true>), g, dim3(512), 0, st, dbl, ql, out, R, Rt, r0, C, n2, nq, S, MC, T, SG);
     else
-        hipLaunchKernelGGL((k_db_mfma_direct<KS, RTW, false>), g, dim3(512), 0, st, dbl, ql, out, R, Rt, r0, C, n2, nq, S, MC, T);
+        hipLaunchKernelGGL((k_db_mfma_direct<KS, RTW, false>), g, dim3(512), 0, st, dbl, ql, out, R, Rt, r0, C, n2, nq, S, MC, T, SG);
 }
 
 // Queries q_lo .. q_lo + nq (in sweeps of mfma_sweep_queries), database
@@ -474,29 +481,29 @@ void launch_direct(bool zmaj, dim3 g, hipStream_t st, const int8_t* dbl, const i
 // same nq and nb, so every call carves the scratch the same way.  zmaj
 // (direct pass only): dbout in the slot-group-major layout of the chunked
 // pipeline.
-int launch_db_batched(kspir_context* c, const kspir_database* dbc, int q_lo, int nq, hipStream_t st, int parts = 7,
-                      int b0 = 0, int nb = 0, bool zmaj = false) {
-    kspir_database* db = const_cast<kspir_database*>(dbc);
-    const int r = db->r, n1 = db->n1, n2 = db->n2, Rt = r * n2;
-    if (nb <= 0) nb = r;
-    const int R = nb * n2, r0 = b0 * n2;
-    const size_t dbwords = (size_t)r * kN * (kN / 2);
-    {
-        std::lock_guard<std::mutex> lock(db->limbs_mu);
-        if (!db->limbs) {
-            int8_t* l = nullptr;
-            HIP_OK(hipMalloc(&l, dbwords * 8));
-            hipLaunchKernelGGL(k_db_limbs, dim3(kN / 64, Rt), dim3(256), 0, st, db->words, l, n1, Rt);
-            LAUNCH_OK("k_db_limbs");
-            HIP_OK(hipStreamSynchronize(st));   // complete before another context's stream reads them
-            db->limbs = l;
-        }
-    }
-    const bool direct = c->mfma_direct != 0;
+// the database's 7-bit int8 limbs, built once, by the first batched call
+// of any context of its device
+int ensure_limbs(kspir_database* db, hipStream_t st) {
+    std::lock_guard<std::mutex> lock(db->limbs_mu);
+    if (db->limbs) return KSPIR_OK;
+    const int Rt = db->r * db->n2;
+    int8_t* l = nullptr;
+    HIP_OK(hipMalloc(&l, (size_t)db->r * kN * (kN / 2) * 8));
+    hipLaunchKernelGGL(k_db_limbs, dim3(kN / 64, Rt), dim3(256), 0, st, db->words, l, db->n1, Rt);
+    LAUNCH_OK("k_db_limbs");
+    HIP_OK(hipStreamSynchronize(st));   // complete before another context's stream reads them
+    db->limbs = l;
+    return KSPIR_OK;
+}
+
+// the MFMA pass's scratch for nq queries of N1 n1 in sweeps of
+// mfma_sweep_queries: the query limbs, and (tile-major pass) the products of
+// R rows; grows on demand
+int ensure_mws(kspir_context* c, int n1, int nq, int R, hipStream_t st, size_t* ql_bytes_out) {
     const int nqc_max = mfma_sweep_queries(c, n1);
     const int C_full = ((2 * (nq < nqc_max ? nq : nqc_max)) + 15) / 16 * 16;
     const size_t ql_bytes = (size_t)kN * 2 * 4 * C_full * n1;
-    const size_t mf_bytes = direct ? 0 : (size_t)2 * R * C_full * kN * 4;
+    const size_t mf_bytes = c->mfma_direct ? 0 : (size_t)2 * R * C_full * kN * 4;
     if (ql_bytes + mf_bytes > c->mws_bytes) {
         if (c->mws) {
             HIP_OK(hipStreamSynchronize(st));
@@ -509,23 +516,56 @@ int launch_db_batched(kspir_context* c, const kspir_database* dbc, int q_lo, int
         HIP_OK(hipMalloc(&c->mws, ql_bytes + mf_bytes));
         c->mws_bytes = ql_bytes + mf_bytes;
     }
-    int8_t* ql = (int8_t*)c->mws;
-    uint32_t* mf = (uint32_t*)((char*)c->mws + ql_bytes);
+    *ql_bytes_out = ql_bytes;
+    return KSPIR_OK;
+}
+
+// 2^(7w) * 2^32 mod q_p: the weights that recombine limb-pair sums into
+// Montgomery-form residues
+MfmaConst mfma_consts(const kspir_context* c) {
     MfmaConst MC;
     for (int p = 0; p < 2; ++p) {
         const u64 q = c->T.p[p].q;
         for (int w = 0; w < 7; ++w) MC.cw[p][w] = (uint32_t)(((u128)powmod(2, 7 * w, q) << 32) % q);
     }
+    return MC;
+}
+
+// query limbs of nqc queries from their rotation words (C columns)
+int launch_ct_limbs(int n1, const uint32_t* ctw, int8_t* ql, int nqc, int C, const QS& S, hipStream_t st) {
+    switch (n1) {
+        case 64: hipLaunchKernelGGL(k_ct_limbs<64>, dim3(kN / 32, C / 2, 2), dim3(256), 0, st, ctw, ql, nqc, C, S); break;
+        case 128: hipLaunchKernelGGL(k_ct_limbs<128>, dim3(kN / 32, C / 2, 2), dim3(256), 0, st, ctw, ql, nqc, C, S); break;
+        default: hipLaunchKernelGGL(k_ct_limbs<256>, dim3(kN / 32, C / 2, 2), dim3(256), 0, st, ctw, ql, nqc, C, S); break;
+    }
+    LAUNCH_OK("k_ct_limbs");
+    return KSPIR_OK;
+}
+
+int launch_db_batched(kspir_context* c, const kspir_database* dbc, int q_lo, int nq, hipStream_t st, int parts = 7,
+                      int b0 = 0, int nb = 0, bool zmaj = false) {
+    kspir_database* db = const_cast<kspir_database*>(dbc);
+    const int r = db->r, n1 = db->n1, n2 = db->n2, Rt = r * n2;
+    if (nb <= 0) nb = r;
+    const int R = nb * n2, r0 = b0 * n2;
+    int rc = ensure_limbs(db, st);
+    if (rc) return rc;
+    const bool direct = c->mfma_direct != 0;
+    const int nqc_max = mfma_sweep_queries(c, n1);
+    size_t ql_bytes = 0;
+    rc = ensure_mws(c, n1, nq, R, st, &ql_bytes);
+    if (rc) return rc;
+    int8_t* ql = (int8_t*)c->mws;
+    uint32_t* mf = (uint32_t*)((char*)c->mws + ql_bytes);
+    const MfmaConst MC = mfma_consts(c);
     const QS& S = c->S;
     for (int q0 = q_lo; q0 < q_lo + nq; q0 += nqc_max) {
         const int nqc = q_lo + nq - q0 < nqc_max ? q_lo + nq - q0 : nqc_max;
         const int C = (2 * nqc + 15) / 16 * 16;
-        if (parts & 1) switch (n1) {
-            case 64: hipLaunchKernelGGL(k_ct_limbs<64>, dim3(kN / 32, C / 2, 2), dim3(256), 0, st, c->ctw + q0 * S.ctw, ql, nqc, C, S); break;
-            case 128: hipLaunchKernelGGL(k_ct_limbs<128>, dim3(kN / 32, C / 2, 2), dim3(256), 0, st, c->ctw + q0 * S.ctw, ql, nqc, C, S); break;
-            default: hipLaunchKernelGGL(k_ct_limbs<256>, dim3(kN / 32, C / 2, 2), dim3(256), 0, st, c->ctw + q0 * S.ctw, ql, nqc, C, S); break;
+        if (parts & 1) {
+            rc = launch_ct_limbs(n1, c->ctw + q0 * S.ctw, ql, nqc, C, S, st);
+            if (rc) return rc;
         }
-        LAUNCH_OK("k_ct_limbs");
         if (direct && (parts & 2)) {
             // products straight into dbout: 16 slots x (16 RTW) rows per workgroup
             const int rtw = (n1 <= 128 && R % 32 == 0) ? 2 : 1;
@@ -564,6 +604,31 @@ int launch_db_batched(kspir_context* c, const kspir_database* dbc, int q_lo, int
     return KSPIR_OK;
 }
 
+// baby steps (twosteps.cpp:1206-1214, evalAutoSRNSSmallKeys) of nq queries
+// of N1 n1: the rotation words ctw of every query
+int enqueue_baby(kspir_context* c, const KeyPtrs* kp, int nq, int n1, const uint64_t* q_dev, hipStream_t st) {
+    const int n1h = n1 / 2;
+    const Tabs& T = c->T;
+    const QS& S = c->S;
+    hipLaunchKernelGGL(k_baby_prep, dim3(16, nq), dim3(kNttThreads), 0, st, q_dev, c->D0, c->bhat0, c->ctw, n1, S, T);
+    LAUNCH_OK("k_baby_prep");
+    // rotation N1/2-1 of the query first: its switched words and the
+    // un-switched intermediate that the second half rotates
+    hipLaunchKernelGGL(k_baby_rot, dim3(2 * nq), dim3(kNttThreads), 0, st, c->D0, c->bhat0, kp, c->ctw, c->mid, 1,
+                       n1h - 2, nq, 1, n1h - 2, S.D, S.bhat, S, T);
+    LAUNCH_OK("k_baby_rot(intermediate)");
+    hipLaunchKernelGGL(k_baby_mid_coef, dim3(3, nq), dim3(kNttThreads), 0, st, c->mid, c->icoef, S, T);
+    LAUNCH_OK("k_baby_mid_coef");
+    hipLaunchKernelGGL(k_baby_mid_digits, dim3(9, nq), dim3(kNttThreads), 0, st, c->icoef, c->D1, S, T);
+    LAUNCH_OK("k_baby_mid_digits");
+    // the other rotations, both halves, one read of each key: rotations
+    // 1 .. N1/2-2 of the query and N1/2 .. N1-1 (from the intermediate)
+    hipLaunchKernelGGL(k_baby_rot_pair, dim3(n1h * 2 * nq), dim3(kNttThreads), 0, st, c->D0, c->bhat0, c->D1,
+                       c->mid + 3 * kN, kp, c->ctw, n1h, nq, S, T);
+    LAUNCH_OK("k_baby_rot_pair");
+    return KSPIR_OK;
+}
+
 // packing tree on the NTT-domain planes at `cur` for nq queries; returns the
 // buffer that holds the packed RLWE of each query (answer.cpp:800-835)
 int enqueue_packing(kspir_context* c, const KeyPtrs* kp, int r, int nq, hipStream_t st, uint32_t** out) {
@@ -576,7 +641,8 @@ int enqueue_packing(kspir_context* c, const KeyPtrs* kp, int r, int nq, hipStrea
         const uint32_t g = (uint32_t)(kN / i + 1);
         const uint32_t ginv = (uint32_t)powmod(g, kN - 1, 2 * kN);   // odd g has order | N in (Z/2N)*
         if (c->fused_pack && nq >= c->fused_min_batch && (int64_t)i * nq >= c->pack_fused_min_wgs) {
-            hipLaunchKernelGGL(k_pack_level, dim3(i * nq), dim3(512), 0, st, cur, kp, li, nxt, i, g, ginv, S, T);
+            hipLaunchKernelGGL(k_pack_level, dim3(i * nq), dim3(512), 0, st, cur, kp, li, nxt, i, g, ginv, S, T,
+                               PackSegs{});
             LAUNCH_OK("k_pack_level");
         } else {
             hipLaunchKernelGGL(k_pack_coef, dim3(i, 2 * nq), dim3(kNttThreads), 0, st, cur, c->pcoef, i, S, T);
@@ -619,11 +685,11 @@ int enqueue_chunked(kspir_context* c, const kspir_database* db, const KeyPtrs* k
             if (zmaj)
                 hipLaunchKernelGGL((k_giant_fused_dma<true, true>), dim3(cb * nqc), dim3(kFusedThreads), 0, st,
                                    c->dbout + q0 * S.dbout, kp + q0, nullptr, c->extA + q0 * S.ext, cb, nqc, n1, n2, S,
-                                   T, b0);
+                                   T, b0, GiantSegs{});
             else
                 hipLaunchKernelGGL((k_giant_fused_dma<true, false>), dim3(cb * nqc), dim3(kFusedThreads), 0, st,
                                    c->dbout + q0 * S.dbout, kp + q0, nullptr, c->extA + q0 * S.ext, cb, nqc, n1, n2, S,
-                                   T, b0);
+                                   T, b0, GiantSegs{});
             LAUNCH_OK("k_giant_fused_dma");
         }
     }
@@ -631,6 +697,121 @@ int enqueue_chunked(kspir_context* c, const kspir_database* db, const KeyPtrs* k
     int rc = enqueue_packing(c, kp, r, nq, st, &packed);
     if (rc) return rc;
     hipLaunchKernelGGL(k_response, dim3(2, nq), dim3(kNttThreads), 0, st, packed, resp, S.ext, T);
+    LAUNCH_OK("k_response");
+    return KSPIR_OK;
+}
+
+// A batch spanning several databases of one N1 (segment s: queries
+// q_lo .. q_lo + nq of the batch, against database db), one launch per stage
+// and chunk for all of them: the baby steps of every query; per MFMA sweep
+// of queries their limbs; per chunk of cb blocks one database pass over the
+// chunk's rows of every database that still has blocks there, then one
+// giant-step launch over every (query, block) of that chunk; one packing
+// launch per level over the queries of every database deep enough for it;
+// one response launch.  Each segment's external products are written to the
+// buffer that is the packing tree's input when its first level comes round
+// (the levels ping-pong between extA and extB), so a database with smaller
+// r joins the tree at its own depth with no copy.
+struct Seg {
+    const kspir_database* db;
+    int q_lo, nq;
+};
+
+int enqueue_multi(kspir_context* c, const std::vector<Seg>& segs, const KeyPtrs* kp, int nq, const uint64_t* q_dev,
+                  uint64_t* resp, hipStream_t st) {
+    const int n1 = segs[0].db->n1, n2 = segs[0].db->n2, cb = c->cur_cb;
+    int rmax = 0;
+    for (const Seg& g : segs) rmax = g.db->r > rmax ? g.db->r : rmax;
+    const int lmax = ilog2(rmax);
+    const QS& S = c->S;
+    const Tabs& T = c->T;
+    int rc = enqueue_baby(c, kp, nq, n1, q_dev, st);
+    if (rc) return rc;
+    for (const Seg& g : segs) {
+        rc = ensure_limbs(const_cast<kspir_database*>(g.db), st);
+        if (rc) return rc;
+    }
+    size_t ql_bytes = 0;
+    rc = ensure_mws(c, n1, nq, 0, st, &ql_bytes);
+    if (rc) return rc;
+    int8_t* ql = (int8_t*)c->mws;
+    const MfmaConst MC = mfma_consts(c);
+    const int nqc_max = mfma_sweep_queries(c, n1);
+    for (int q0 = 0; q0 < nq; q0 += nqc_max) {
+        const int nqc = nq - q0 < nqc_max ? nq - q0 : nqc_max;
+        const int C = (2 * nqc + 15) / 16 * 16;
+        rc = launch_ct_limbs(n1, c->ctw + q0 * S.ctw, ql, nqc, C, S, st);
+        if (rc) return rc;
+        for (int b0 = 0; b0 < rmax; b0 += cb) {
+            // the segments with queries in this sweep and blocks in this chunk
+            bool rtw2 = n1 <= 128;
+            for (const Seg& g : segs) {
+                const int lo = g.q_lo > q0 ? g.q_lo : q0, hi = g.q_lo + g.nq < q0 + nqc ? g.q_lo + g.nq : q0 + nqc;
+                const int nb = g.db->r - b0 < cb ? g.db->r - b0 : cb;
+                if (lo < hi && nb > 0 && (nb * n2) % 32) rtw2 = false;
+            }
+            const int rtw = rtw2 ? 2 : 1;
+            DbSegs D{};
+            GiantSegs G{};
+            int dwg = 0, gwg = 0, k = 0;
+            for (const Seg& g : segs) {
+                const int lo = g.q_lo > q0 ? g.q_lo : q0, hi = g.q_lo + g.nq < q0 + nqc ? g.q_lo + g.nq : q0 + nqc;
+                const int r = g.db->r, nb = r - b0 < cb ? r - b0 : cb;
+                if (lo >= hi || nb <= 0) continue;
+                const int R = nb * n2;
+                D.dbl[k] = g.db->limbs;
+                D.Rt[k] = r * n2;
+                D.R[k] = R;
+                D.r0[k] = b0 * n2;
+                D.ct_lo[k] = 2 * (lo - q0) / 16;
+                D.ct_hi[k] = (2 * (hi - q0) + 15) / 16;
+                D.q_lo[k] = lo - q0;
+                D.q_hi[k] = hi - q0;
+                dwg += (kN / 16) * (R / (16 * rtw)) * 2;
+                D.wg_end[k] = dwg;
+                G.q_lo[k] = lo - q0;
+                G.nb[k] = nb;
+                G.out[k] = (((lmax - ilog2(r)) & 1) ? c->extB : c->extA) + q0 * S.ext;
+                gwg += nb * (hi - lo);
+                G.wg_end[k] = gwg;
+                ++k;
+            }
+            if (!k) continue;
+            D.n = G.n = k;
+            uint32_t* out = c->dbout + q0 * S.dbout;
+            const dim3 g(dwg);
+            if (n1 == 64 && rtw == 2) launch_direct<1, 2>(true, g, st, nullptr, ql, out, 0, 0, 0, C, n2, nqc, S, MC, T, D);
+            else if (n1 == 64) launch_direct<1, 1>(true, g, st, nullptr, ql, out, 0, 0, 0, C, n2, nqc, S, MC, T, D);
+            else if (n1 == 128 && rtw == 2) launch_direct<2, 2>(true, g, st, nullptr, ql, out, 0, 0, 0, C, n2, nqc, S, MC, T, D);
+            else if (n1 == 128) launch_direct<2, 1>(true, g, st, nullptr, ql, out, 0, 0, 0, C, n2, nqc, S, MC, T, D);
+            else launch_direct<4, 1>(true, g, st, nullptr, ql, out, 0, 0, 0, C, n2, nqc, S, MC, T, D);
+            LAUNCH_OK("k_db_mfma_direct(segments)");
+            hipLaunchKernelGGL((k_giant_fused_dma<true, true>), dim3(gwg), dim3(kFusedThreads), 0, st, out, kp + q0,
+                               nullptr, nullptr, 0, 0, n1, n2, S, T, b0, G);
+            LAUNCH_OK("k_giant_fused_dma(segments)");
+        }
+    }
+    // packing: level i runs for the queries of every database with r >= 2 i
+    uint32_t* cur = c->extA;
+    uint32_t* nxt = c->extB;
+    for (int i = rmax / 2; i > 0; i >>= 1) {
+        const int li = ilog2(i);
+        const uint32_t g = (uint32_t)(kN / i + 1);
+        const uint32_t ginv = (uint32_t)powmod(g, kN - 1, 2 * kN);
+        PackSegs P{};
+        int wg = 0, k = 0;
+        for (const Seg& sg : segs) {
+            if (sg.db->r < 2 * i) continue;
+            P.q_lo[k] = sg.q_lo;
+            wg += i * sg.nq;
+            P.wg_end[k++] = wg;
+        }
+        P.n = k;
+        hipLaunchKernelGGL(k_pack_level, dim3(wg), dim3(512), 0, st, cur, kp, li, nxt, i, g, ginv, S, T, P);
+        LAUNCH_OK("k_pack_level(segments)");
+        uint32_t* t = cur; cur = nxt; nxt = t;
+    }
+    hipLaunchKernelGGL(k_response, dim3(2, nq), dim3(kNttThreads), 0, st, cur, resp, S.ext, T);
     LAUNCH_OK("k_response");
     return KSPIR_OK;
 }
@@ -643,26 +824,11 @@ int enqueue_chunked(kspir_context* c, const kspir_database* db, const KeyPtrs* k
 int enqueue_answer(kspir_context* c, const kspir_database* db, const KeyPtrs* kp, int nq, const uint64_t* q_dev,
                    uint64_t* resp, hipStream_t st, uint64_t* fd_u64, uint64_t* ext_u64, bool first_dim_only,
                    bool allow_mfma = true) {
-    const int r = db->r, n1 = db->n1, n2 = db->n2, n1h = n1 / 2;
+    const int r = db->r, n1 = db->n1, n2 = db->n2;
     const Tabs& T = c->T;
     const QS& S = c->S;
-    // baby steps (twosteps.cpp:1206-1214, evalAutoSRNSSmallKeys)
-    hipLaunchKernelGGL(k_baby_prep, dim3(16, nq), dim3(kNttThreads), 0, st, q_dev, c->D0, c->bhat0, c->ctw, n1, S, T);
-    LAUNCH_OK("k_baby_prep");
-    // rotation N1/2-1 of the query first: its switched words and the
-    // un-switched intermediate that the second half rotates
-    hipLaunchKernelGGL(k_baby_rot, dim3(2 * nq), dim3(kNttThreads), 0, st, c->D0, c->bhat0, kp, c->ctw, c->mid, 1,
-                       n1h - 2, nq, 1, n1h - 2, S.D, S.bhat, S, T);
-    LAUNCH_OK("k_baby_rot(intermediate)");
-    hipLaunchKernelGGL(k_baby_mid_coef, dim3(3, nq), dim3(kNttThreads), 0, st, c->mid, c->icoef, S, T);
-    LAUNCH_OK("k_baby_mid_coef");
-    hipLaunchKernelGGL(k_baby_mid_digits, dim3(9, nq), dim3(kNttThreads), 0, st, c->icoef, c->D1, S, T);
-    LAUNCH_OK("k_baby_mid_digits");
-    // the other rotations, both halves, one read of each key: rotations
-    // 1 .. N1/2-2 of the query and N1/2 .. N1-1 (from the intermediate)
-    hipLaunchKernelGGL(k_baby_rot_pair, dim3(n1h * 2 * nq), dim3(kNttThreads), 0, st, c->D0, c->bhat0, c->D1,
-                       c->mid + 3 * kN, kp, c->ctw, n1h, nq, S, T);
-    LAUNCH_OK("k_baby_rot_pair");
+    int rc0 = enqueue_baby(c, kp, nq, n1, q_dev, st);
+    if (rc0) return rc0;
     // database pass (crt.cpp:643-768): int8-MFMA limb GEMM for a batch,
     // u64-accumulating VALU stream for a single query
     if (c->cur_cb > 0) {
@@ -681,11 +847,11 @@ int enqueue_answer(kspir_context* c, const kspir_database* db, const KeyPtrs* kp
         if (c->giant_dma && c->giant_wgs_per_cu == 1 && c->ext_fused && !first_dim_only) {
             // the ext output of a query has the ecoef stride's layout [r][ab][p][N]
             hipLaunchKernelGGL((k_giant_fused_dma<true, false>), dim3(r * nq), dim3(kFusedThreads), 0, st, c->dbout, kp,
-                               fd_u64 ? c->fd : nullptr, c->extA, r, nq, n1, n2, S, T);
+                               fd_u64 ? c->fd : nullptr, c->extA, r, nq, n1, n2, S, T, 0, GiantSegs{});
             ext_done = true;
         } else if (c->giant_dma && c->giant_wgs_per_cu == 1)
             hipLaunchKernelGGL((k_giant_fused_dma<false, false>), dim3(r * nq), dim3(kFusedThreads), 0, st, c->dbout, kp,
-                               fd_u64 ? c->fd : nullptr, c->ecoef, r, nq, n1, n2, S, T);
+                               fd_u64 ? c->fd : nullptr, c->ecoef, r, nq, n1, n2, S, T, 0, GiantSegs{});
         else if (c->giant_wgs_per_cu == 2)
             hipLaunchKernelGGL(k_giant_fused2, dim3(r * nq), dim3(kFusedThreads), 0, st, c->dbout, kp, c->fd, c->ecoef,
                                r, nq, n1, n2, S, T);
@@ -752,32 +918,33 @@ int check_pair(const kspir_context* c, const kspir_database* db, const kspir_key
 // a given (database, key table, batch size, buffers, stream) runs eagerly
 // (which also makes every lazy allocation) and then captures the same
 // launches; later calls replay the graph
-int run_answer(kspir_context* c, const kspir_database* db, const KeyPtrs* kp, int nq, const uint64_t* q,
-               uint64_t* resp, hipStream_t st) {
+template <class Enq>
+int run_graph(kspir_context* c, const kspir_database* db, const std::vector<const void*>& segs, const KeyPtrs* kp,
+              int nq, const uint64_t* q, uint64_t* resp, hipStream_t st, Enq enq) {
     if (st != c->stream) c->last_stream = st;
-    if (!c->use_graphs) return enqueue_answer(c, db, kp, nq, q, resp, st, nullptr, nullptr, false);
+    if (!c->use_graphs) return enq();
     bool found = false;
     hipGraphExec_t exec = nullptr;
     {
         std::lock_guard<std::mutex> lock(c->graph_mu);
         for (auto& g : c->graphs)
-            if (g.db == db && g.kp == kp && g.nq == nq && g.q == q && g.resp == resp && g.st == st) {
+            if (g.db == db && g.kp == kp && g.nq == nq && g.q == q && g.resp == resp && g.st == st && g.segs == segs) {
                 found = true;
                 exec = g.exec;
                 break;
             }
     }
     if (found) {
-        if (!exec) return enqueue_answer(c, db, kp, nq, q, resp, st, nullptr, nullptr, false);
+        if (!exec) return enq();
         HIP_OK(hipGraphLaunch(exec, st));
         return KSPIR_OK;
     }
-    int rc = enqueue_answer(c, db, kp, nq, q, resp, st, nullptr, nullptr, false);
+    int rc = enq();
     if (rc) return rc;
-    kspir_context::Graph g{db, kp, nq, q, resp, st, nullptr};
+    kspir_context::Graph g{db, kp, nq, q, resp, st, nullptr, segs};
     hipGraph_t graph = nullptr;
     if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) == hipSuccess) {
-        const int crc = enqueue_answer(c, db, kp, nq, q, resp, st, nullptr, nullptr, false);
+        const int crc = enq();
         const hipError_t e = hipStreamEndCapture(st, &graph);
         if (crc == KSPIR_OK && e == hipSuccess && graph) {
             if (hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0) != hipSuccess) g.exec = nullptr;
@@ -792,6 +959,12 @@ int run_answer(kspir_context* c, const kspir_database* db, const KeyPtrs* kp, in
     std::lock_guard<std::mutex> lock(c->graph_mu);
     c->graphs.push_back(g);
     return KSPIR_OK;
+}
+
+int run_answer(kspir_context* c, const kspir_database* db, const KeyPtrs* kp, int nq, const uint64_t* q,
+               uint64_t* resp, hipStream_t st) {
+    return run_graph(c, db, {}, kp, nq, q, resp, st,
+                     [&] { return enqueue_answer(c, db, kp, nq, q, resp, st, nullptr, nullptr, false); });
 }
 
 // device key-pointer table for a batch; re-uploaded only when it changes
@@ -818,6 +991,50 @@ int stage_batch_keys(kspir_context* c, const kspir_keys* const* keys, int nq) {
     for (int i = 0; i < nq; ++i) c->kp_host[i] = keys[i]->ptrs;
     HIP_OK(hipMemcpyAsync(c->kp_dev, c->kp_host, sizeof(KeyPtrs) * nq, hipMemcpyHostToDevice, c->stream));
     c->kp_count = nq;
+    return KSPIR_OK;
+}
+
+// checks a batch spanning databases (segment s: seg_nq[s] consecutive
+// queries against dbs[s], keys[q] per query), fills segs, carves the
+// workspace (per-query buffers sized for the deepest database) and stages
+// the key table
+int prepare_multi(kspir_context* c, int nseg, const kspir_database* const* dbs, const int32_t* seg_nq,
+                  const kspir_keys* const* keys, std::vector<Seg>& segs, std::vector<const void*>& sig, int* nq_out) {
+    if (!c || !dbs || !seg_nq || !keys) return fail(KSPIR_EINVAL, "null argument");
+    if (nseg < 1 || nseg > kMaxSeg)
+        return fail(KSPIR_EINVAL, "a batch spans 1 to " + std::to_string(kMaxSeg) + " databases");
+    if (!c->mfma_direct || !c->giant_dma || c->giant_wgs_per_cu != 1 || !c->ext_fused)
+        return fail(KSPIR_EINVAL, "a batch spanning databases runs the chunked int8-MFMA pipeline: options mfma_direct, "
+                                  "giant_dma, ext_fused on and giant_wgs_per_cu 1");
+    segs.clear();
+    sig.clear();
+    int nq = 0, rmax = 0;
+    for (int s = 0; s < nseg; ++s) {
+        const kspir_database* db = dbs[s];
+        if (!db) return fail(KSPIR_EINVAL, "null database handle");
+        if (db->n1 != dbs[0]->n1) return fail(KSPIR_EINVAL, "the databases of one batch share N1");
+        if (!mfma_shape(db)) return fail(KSPIR_EINVAL, "database shape has no int8-MFMA pass (N1 64..256, r*N2 % 16 == 0)");
+        if (seg_nq[s] < 1) return fail(KSPIR_EINVAL, "every database of the batch needs at least one query");
+        for (int q = nq; q < nq + seg_nq[s]; ++q) {
+            int rc = check_pair(c, db, keys[q]);
+            if (rc) return rc;
+        }
+        segs.push_back(Seg{db, nq, seg_nq[s]});
+        sig.push_back(db);
+        sig.push_back((const void*)(uintptr_t)seg_nq[s]);
+        nq += seg_nq[s];
+        rmax = db->r > rmax ? db->r : rmax;
+    }
+    const int n1 = dbs[0]->n1, n2 = dbs[0]->n2;
+    int cb = c->chunk_blocks > 0 && c->chunk_blocks < rmax ? c->chunk_blocks : rmax;
+    while ((cb * n2) % 16) cb *= 2;
+    if (cb > rmax) cb = rmax;
+    HIP_OK(hipSetDevice(c->device));
+    int rc = carve_workspace_cb(c, rmax, n1, nq, cb);
+    if (rc) return rc;
+    rc = stage_batch_keys(c, keys, nq);
+    if (rc) return rc;
+    *nq_out = nq;
     return KSPIR_OK;
 }
 
@@ -1246,6 +1463,63 @@ int kspir_answer_batch_device(kspir_context* c, const kspir_database* db, const 
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
     if (st != c->stream) HIP_OK(hipStreamSynchronize(c->stream));   // key table upload runs on the context stream
     return run_answer(c, db, c->kp_dev, nq, queries_dev, responses_dev, st);
+}
+
+int kspir_answer_multi_device(kspir_context* c, int32_t nseg, const kspir_database* const* dbs,
+                              const int32_t* seg_queries, const kspir_keys* const* keys, const uint64_t* queries_dev,
+                              uint64_t* responses_dev, void* stream) {
+    std::vector<Seg> segs;
+    std::vector<const void*> sig;
+    int nq = 0;
+    int rc = prepare_multi(c, nseg, dbs, seg_queries, keys, segs, sig, &nq);
+    if (rc) return rc;
+    if (!queries_dev || !responses_dev) return fail(KSPIR_EINVAL, "null buffer");
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    if (st != c->stream) HIP_OK(hipStreamSynchronize(c->stream));   // key table upload runs on the context stream
+    return run_graph(c, nullptr, sig, c->kp_dev, nq, queries_dev, responses_dev, st,
+                     [&] { return enqueue_multi(c, segs, c->kp_dev, nq, queries_dev, responses_dev, st); });
+}
+
+int kspir_answer_multi(kspir_context* c, int32_t nseg, const kspir_database* const* dbs, const int32_t* seg_queries,
+                       const kspir_keys* const* keys, const uint64_t* queries, uint64_t* responses) {
+    std::vector<Seg> segs;
+    std::vector<const void*> sig;
+    int nq = 0;
+    int rc = prepare_multi(c, nseg, dbs, seg_queries, keys, segs, sig, &nq);
+    if (rc) return rc;
+    if (!queries || !responses) return fail(KSPIR_EINVAL, "null buffer");
+    uint64_t* qd = c->q_dev + (size_t)nq * c->S.qin64;   // second half of the query area
+    uint64_t* rd = c->resp_dev + (size_t)nq * c->S.resp64;
+    HIP_OK(hipMemcpyAsync(qd, queries, (size_t)nq * 4 * kN * 8, hipMemcpyHostToDevice, c->stream));
+    rc = kspir_answer_multi_device(c, nseg, dbs, seg_queries, keys, qd, rd, nullptr);
+    if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(responses, rd, (size_t)nq * 2 * kN * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return KSPIR_OK;
+}
+
+int kspir_time_answer_multi(kspir_context* c, int32_t nseg, const kspir_database* const* dbs,
+                            const int32_t* seg_queries, const kspir_keys* const* keys, const uint64_t* queries_dev,
+                            uint64_t* responses_dev, int warmup, int iters, float* ms) {
+    if (!ms || iters < 1) return fail(KSPIR_EINVAL, "bad timing arguments");
+    for (int i = 0; i < warmup; ++i) {
+        int rc = kspir_answer_multi_device(c, nseg, dbs, seg_queries, keys, queries_dev, responses_dev, nullptr);
+        if (rc) return rc;
+    }
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    HIP_OK(hipEventRecord(e0, c->stream));
+    for (int i = 0; i < iters; ++i) {
+        int rc = kspir_answer_multi_device(c, nseg, dbs, seg_queries, keys, queries_dev, responses_dev, nullptr);
+        if (rc) return rc;
+    }
+    HIP_OK(hipEventRecord(e1, c->stream));
+    HIP_OK(hipEventSynchronize(e1));
+    HIP_OK(hipEventElapsedTime(ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return KSPIR_OK;
 }
 
 int kspir_answer_batch(kspir_context* c, const kspir_database* db, const kspir_keys* const* keys, int32_t nq,

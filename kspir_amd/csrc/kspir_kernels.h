@@ -31,6 +31,48 @@ struct KeyPtrs {
     const uint16_t* perms;     // [N1][N]
 };
 
+// A launch that spans several databases (one segment per database, all of
+// one N1): each segment owns a contiguous range of the launch's workgroups,
+// wg_end holding the running totals.
+constexpr int kMaxSeg = 8;
+
+// database-pass segments: database limbs and their row range of this chunk,
+// the column tiles and (sweep-relative) queries of the segment
+struct DbSegs {
+    int n;                        // 0: a single-database launch (the kernel's own arguments)
+    int wg_end[kMaxSeg];
+    const int8_t* dbl[kMaxSeg];
+    int Rt[kMaxSeg], R[kMaxSeg], r0[kMaxSeg];
+    int ct_lo[kMaxSeg], ct_hi[kMaxSeg], q_lo[kMaxSeg], q_hi[kMaxSeg];
+};
+
+// giant-step segments: first query, blocks of this chunk per query, and the
+// external-product output buffer of the segment's queries
+struct GiantSegs {
+    int n;                        // 0: a single-database launch
+    int wg_end[kMaxSeg];
+    int q_lo[kMaxSeg], nb[kMaxSeg];
+    uint32_t* out[kMaxSeg];
+};
+
+// packing-level segments: the queries of a level when databases of several
+// r share the launch (a level of half-size i runs for the queries whose r is
+// at least 2 i), one segment per database
+struct PackSegs {
+    int n;                        // 0: every query of the launch
+    int wg_end[kMaxSeg];
+    int q_lo[kMaxSeg];
+};
+
+// segment of workgroup job L (uniform over the workgroup); L becomes
+// segment-relative
+__device__ __forceinline__ int seg_of(int& L, const int* wg_end, int n) {
+    int s = 0;
+    while (s + 1 < n && L >= wg_end[s]) ++s;
+    if (s) L -= wg_end[s - 1];
+    return s;
+}
+
 // per-query strides (in 32-bit words unless noted) of the answer workspace;
 // query q of a batch uses base + q * stride for every buffer
 struct QS {
@@ -926,24 +968,41 @@ __device__ __forceinline__ void lds_barrier() {
 template <int KS, int RTW, bool ZMAJ>
 __global__ void __launch_bounds__(512) k_db_mfma_direct(const int8_t* __restrict__ dbl, const int8_t* __restrict__ ql,
                                                         uint32_t* __restrict__ dbout, int R, int Rt, int r0, int C,
-                                                        int n2, int nq, QS S, MfmaConst MC, Tabs T) {
+                                                        int n2, int nq, QS S, MfmaConst MC, Tabs T, DbSegs SG) {
     typedef __attribute__((address_space(3))) void lds_void;
     typedef __attribute__((address_space(1))) void glb_void;
     constexpr int n1 = KS * 64;
     constexpr int NS = KS == 4 ? 2 : 4;     // ring stages per wave (KS KiB each)
     __shared__ __attribute__((aligned(16))) int8_t ring[8][NS][KS * 1024];
     __shared__ uint32_t stage[2][16 * kMfmaZStride];   // column tile ct in stage[ct & 1]
+    // a launch spanning databases takes this workgroup's database, rows,
+    // column tiles and queries from its segment
+    int ct_lo = 0, ct_hi = C / 16, q_lo = 0, q_hi = nq;
+    int L;
+    if (SG.n) {
+        L = xcd_job(blockIdx.x, SG.wg_end[SG.n - 1]);
+        const int sg = seg_of(L, SG.wg_end, SG.n);
+        dbl = SG.dbl[sg];
+        R = SG.R[sg];
+        Rt = SG.Rt[sg];
+        r0 = SG.r0[sg];
+        ct_lo = SG.ct_lo[sg];
+        ct_hi = SG.ct_hi[sg];
+        q_lo = SG.q_lo[sg];
+        q_hi = SG.q_hi[sg];
+    } else {
+        L = xcd_job(blockIdx.x, (kN / 16) * (R / (16 * RTW)) * 2);
+    }
     const int ngroups = R / (16 * RTW);
-    const int L = xcd_job(blockIdx.x, (kN / 16) * ngroups * 2);
     const int rg = L % ngroups, p = (L / ngroups) & 1, zg = L / (2 * ngroups);
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lr = lane & 15, lk = (lane >> 4) * 16;
     const PrimeTab& P = T.p[p];
-    const int CT = C / 16, G = CT * 8;      // stages: g = (ct * 2 + zi) * 4 + v
+    const int G = (ct_hi - ct_lo) * 8;      // stages: g = ((ct - ct_lo) * 2 + zi) * 4 + v
     // this lane's query-limb fragment of stage g: slot 2 wv + zi, limb v,
     // column ct * 16 + lr, rotations lk .. lk + 15 of each 64-rotation chunk
     auto issue = [&](int g) {
-        const int ct = g >> 3, zi = (g >> 2) & 1, v = g & 3;
+        const int ct = ct_lo + (g >> 3), zi = (g >> 2) & 1, v = g & 3;
         const int z = zg * 16 + 2 * wv + zi;
         const int8_t* src = ql + ((size_t)z * 2 + p) * 4 * C * n1 + ((size_t)v * C + ct * 16 + lr) * n1 + lk;
 #pragma unroll
@@ -969,7 +1028,7 @@ __global__ void __launch_bounds__(512) k_db_mfma_direct(const int8_t* __restrict
     __builtin_amdgcn_sched_barrier(0);
     for (int g = 0; g < NS && g < G; ++g) issue(g);
     __builtin_amdgcn_sched_barrier(0);
-    for (int ct = 0; ct < CT; ++ct) {
+    for (int ct = ct_lo; ct < ct_hi; ++ct) {
 #pragma unroll
         for (int zi = 0; zi < 2; ++zi) {
             const int zl = 2 * wv + zi;
@@ -980,7 +1039,7 @@ __global__ void __launch_bounds__(512) k_db_mfma_direct(const int8_t* __restrict
                 for (int w = 0; w < 7; ++w) acc[t][w] = i32x4{0, 0, 0, 0};
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int g = (ct * 2 + zi) * 4 + v;
+                const int g = ((ct - ct_lo) * 2 + zi) * 4 + v;
                 const int after = G - 1 - g < NS - 1 ? G - 1 - g : NS - 1;   // stages issued after g
                 __builtin_amdgcn_sched_barrier(0);
                 if (after >= 3) wait_vmcnt<3 * KS>();
@@ -1031,7 +1090,7 @@ __global__ void __launch_bounds__(512) k_db_mfma_direct(const int8_t* __restrict
             uint32_t w[NI][4];
             stage_read<NI>(w, (uint32_t)(uintptr_t)(lds_void*)&stage[ct & 1][(4 * qz) * kMfmaZStride +
                                                                               rho0 * kMfmaRhoStride + col]);
-            if (q < nq) {
+            if (q >= q_lo && q < q_hi) {
 #pragma unroll
                 for (int i = 0; i < NI; ++i) {
                     const int rhog = (rg * RTW) * 16 + rho0 + 8 * i, blk = rhog / n2, row = rhog % n2;
@@ -1428,7 +1487,7 @@ __device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70)
 template <bool EXT, bool ZMAJ>
 __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_t* __restrict__ dbout,
         const KeyPtrs* __restrict__ kp, uint32_t* __restrict__ fd, uint32_t* __restrict__ ecoef, int r, int nq,
-        int n1, int n2, QS S, Tabs T, int b0 = 0) {
+        int n1, int n2, QS S, Tabs T, int b0, GiantSegs GS) {
     // the planes 0..3 (digits), 4, 5 (a-row) and 6, 7 (b-row) as three LDS
     // objects: the compiler then knows that reading one while a copy into
     // another is in flight needs no wait
@@ -1438,10 +1497,23 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_
     // plane i of the eight (i known at compile time where it is used)
     auto plane = [&](int i) -> uint32_t* { return i < 4 ? sd[i] : i < 6 ? sa[i - 4] : sb[i - 6]; };
     // r: blocks of this launch (dbout holds their rows); b0: index of the
-    // first of them in the outputs fd / ecoef / ext
-    const int L = xcd_job(blockIdx.x, r * nq);
-    const int blk = L % r, ob = b0 + blk;
-    const size_t qi = L / r;
+    // first of them in the outputs fd / ecoef / ext.  A launch spanning
+    // databases takes the query, the block count and the output buffer from
+    // the workgroup's segment.
+    int blk;
+    size_t qi;
+    if (GS.n) {
+        int L = xcd_job(blockIdx.x, GS.wg_end[GS.n - 1]);
+        const int sg = seg_of(L, GS.wg_end, GS.n);
+        blk = L % GS.nb[sg];
+        qi = GS.q_lo[sg] + L / GS.nb[sg];
+        ecoef = GS.out[sg];
+    } else {
+        const int L = xcd_job(blockIdx.x, r * nq);
+        blk = L % r;
+        qi = L / r;
+    }
+    const int ob = b0 + blk;
     dbout += qi * S.dbout;
     if (fd) fd += qi * S.fd;
     ecoef += qi * (EXT ? S.ext : S.ecoef);
@@ -1886,11 +1958,13 @@ __global__ void __launch_bounds__(kNttThreads) k_pack_digit(const uint32_t* __re
 // phase.  1-D grid of i * nq workgroups.
 __global__ void __launch_bounds__(512) k_pack_level(const uint32_t* __restrict__ ext,
         const KeyPtrs* __restrict__ kp, int li, uint32_t* __restrict__ next, int i, uint32_t g, uint32_t ginv, QS S,
-        Tabs T) {
+        Tabs T, PackSegs PS) {
     __shared__ uint32_t s[8][kPadN];
     constexpr int PER = kN / 512;
-    const int j = blockIdx.x % i;
-    const size_t qi = blockIdx.x / i;
+    int L = blockIdx.x;
+    const int sg = PS.n ? seg_of(L, PS.wg_end, PS.n) : 0;
+    const int j = L % i;
+    const size_t qi = (PS.n ? PS.q_lo[sg] : 0) + L / i;
     ext += qi * S.ext;
     next += qi * S.ext;
     const uint32_t* pkey = kp[qi].packing + (size_t)li * 4 * 2 * 2 * kN;   // [k][ab][p][N]

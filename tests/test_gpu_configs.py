@@ -131,6 +131,24 @@ def test_uneven_databases_batched(ctx, oracle, db4gb):
             s.check_decrypts(ctx, b)
             batches.append(b)
         small[0].check_oracle(oracle, batches[0], (0, 15))
+        # one batch spanning the three databases: one launch per stage
+        say("[gpu] one batch spanning the three databases")
+        segs = [(s.db, s.keys) for s in shapes]
+        allq = np.concatenate([s.queries for s in shapes])
+        multi = ctx.answer_multi(segs, allq)
+        assert np.array_equal(multi, np.concatenate(batches))
+        # the device-buffer entry point, eager then a graph replay
+        qb, rb = ctx.alloc(len(allq) * 4 * N * 8), ctx.alloc(len(allq) * 2 * N * 8)
+        try:
+            qb.upload(allq)
+            for rep in range(2):
+                rb.upload(np.zeros((len(allq), 2, N), dtype=np.uint64))
+                ctx.answer_multi_device(segs, qb, rb)
+                ctx.synchronize()
+                assert np.array_equal(rb.download(np.zeros((len(allq), 2, N), dtype=np.uint64)), multi), rep
+        finally:
+            qb.free()
+            rb.free()
         # overlapped: one context (stream) per database, all enqueued before any wait
         say("[gpu] three batches overlapped on three contexts")
         others = [kspir_amd.Context(0), kspir_amd.Context(0)]
@@ -230,3 +248,61 @@ def test_serving_rejects_double_wait(ctx, oracle):
         srv.close()
         keys.close()
         db.close()
+
+
+@pytest.mark.parametrize("chunk,sweep", [(32, 1 << 30), (1, 4), (2, 1 << 30)])
+def test_multi_database_batch(ctx, oracle, chunk, sweep):
+    """one batch spanning databases of r 1, 4 and 2 (in that order) with 3, 5
+    and 2 clients: every response equals its single answer against its own
+    database and decrypts, one per database equals the oracle's; with
+    chunk_blocks 1 and 2 and with MFMA sweeps of 4 queries (databases split
+    across sweeps); device buffers replayed through the captured graph"""
+    say(f"[gpu] batch spanning r=1/4/2 databases, chunk_blocks={chunk} mfma_max_chunk={sweep}")
+    shapes = [Shape(ctx, oracle, 1, 128, 3, 5000), Shape(ctx, oracle, 4, 128, 5, 6000),
+              Shape(ctx, oracle, 2, 128, 2, 7000)]
+    ctx.set("chunk_blocks", chunk)
+    ctx.set("mfma_max_chunk", sweep)
+    try:
+        segs = [(s.db, s.keys) for s in shapes]
+        allq = np.concatenate([s.queries for s in shapes])
+        multi = ctx.answer_multi(segs, allq)
+        qb, rb = ctx.alloc(len(allq) * 4 * N * 8), ctx.alloc(len(allq) * 2 * N * 8)
+        try:
+            qb.upload(allq)
+            for rep in range(3):
+                rb.upload(np.zeros((len(allq), 2, N), dtype=np.uint64))
+                ctx.answer_multi_device(segs, qb, rb)
+                ctx.synchronize()
+                assert np.array_equal(rb.download(np.zeros((len(allq), 2, N), dtype=np.uint64)), multi), rep
+        finally:
+            qb.free()
+            rb.free()
+    finally:
+        ctx.set("chunk_blocks", 32)
+        ctx.set("mfma_max_chunk", 1 << 30)
+    try:
+        q = 0
+        for s in shapes:
+            resp = multi[q:q + s.nq]
+            for c in range(s.nq):
+                assert np.array_equal(resp[c], s.single[c]), (s.r, c)
+            s.check_decrypts(ctx, resp)
+            if chunk == 32:
+                s.check_oracle(oracle, resp, (s.nq - 1,))
+            q += s.nq
+    finally:
+        for s in shapes:
+            s.close()
+
+
+def test_multi_database_rejects_mixed_n1(ctx, oracle):
+    """the databases of one batch share N1; a mismatch is an error"""
+    import kspir_amd
+    a = Shape(ctx, oracle, 2, 128, 1, 8000)
+    b = Shape(ctx, oracle, 2, 64, 1, 8100)
+    try:
+        with pytest.raises(kspir_amd.KspirError, match="N1"):
+            ctx.answer_multi([(a.db, a.keys), (b.db, b.keys)], np.concatenate([a.queries, b.queries]))
+    finally:
+        a.close()
+        b.close()
