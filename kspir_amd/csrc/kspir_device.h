@@ -60,27 +60,39 @@ struct Tabs {
 
 __device__ __forceinline__ uint32_t mulhi32(uint32_t a, uint32_t b) { return __umulhi(a, b); }
 
+// The conditional corrections below are written as unsigned minima: for
+// r < 2^32, min(r, r - q) (wrapping subtraction) equals r >= q ? r - q : r,
+// and it compiles to a subtract and a v_min instead of a compare, a
+// lane-mask select and the wait state the select needs after the compare.
+__device__ __forceinline__ uint32_t umin32(uint32_t a, uint32_t b) { return a < b ? a : b; }
+// a + b mod q for a, b < q
 __device__ __forceinline__ uint32_t add_mod(uint32_t a, uint32_t b, uint32_t q) {
     uint32_t s = a + b;
-    return s >= q ? s - q : s;
+    return umin32(s, s - q);
 }
+// a - b mod q for a, b < q < 2^31: a - b if it did not wrap, else a - b + q
 __device__ __forceinline__ uint32_t sub_mod(uint32_t a, uint32_t b, uint32_t q) {
-    return a >= b ? a - b : a + q - b;
+    uint32_t d = a - b;
+    return umin32(d, d + q);
 }
 // x * w mod q, x < 2^32, w < q, wp = floor(w * 2^32 / q)
 __device__ __forceinline__ uint32_t mul_shoup(uint32_t x, uint32_t w, uint32_t wp, uint32_t q) {
     uint32_t qh = mulhi32(x, wp);
     uint32_t r = x * w - qh * q;
-    return r >= q ? r - q : r;
+    return umin32(r, r - q);
 }
 // T * 2^-32 mod q for T < q * 2^32
 __device__ __forceinline__ uint32_t mont_reduce(uint64_t T, uint32_t q, uint32_t qinv) {
     uint32_t m = (uint32_t)T * qinv;
     uint64_t t = T + (uint64_t)m * q;
     uint32_t r = (uint32_t)(t >> 32);
-    return r >= q ? r - q : r;
+    return umin32(r, r - q);
 }
-__device__ __forceinline__ uint32_t neg_mod(uint32_t a, uint32_t q) { return a ? q - a : 0; }
+// -a mod q for a < q: q - a, and 0 for a = 0 (q - 0 = q reduces to 0)
+__device__ __forceinline__ uint32_t neg_mod(uint32_t a, uint32_t q) {
+    uint32_t t = q - a;
+    return umin32(t, t - q);
+}
 
 // value in [0, Q) from its residues (Garner): x1 + q1 * ((x2 - x1) * q1^-1 mod q2)
 __device__ __forceinline__ uint64_t crt_q(uint32_t x1, uint32_t x2, const Tabs& T) {
