@@ -711,23 +711,29 @@ __global__ void __launch_bounds__(256) k_db_limbs(const uint64_t* __restrict__ w
     }
 }
 
-// query limbs, per batch: ctw [q][k][ab][p][z] -> ql [z][p][v][col][k]
-// (7-bit limbs v = 0..3), col = q*2 + ab, cols padded with zeros to C; ctw
-// points at the chunk's first query, nq = queries in the chunk.  A workgroup
-// takes 32 slots of one query for one residue.  It loads 128-byte row pieces
-// with 16-byte loads (all issued before the first LDS write) into the tile
-// [ab*N1 + k][slot] of pitch 33 words, conflict-free both ways.  Then each wave
-// reads 64 consecutive rotations of one slot (one word per lane), packs the
-// four limbs of its word into bytes, and transposes 4x4 bytes across each lane
-// quad with DPP.  Lane 4j+v then holds limb v of rotations 4j..4j+3, so each
-// store instruction writes four 64-byte runs.  grid (N/32, C/2, 2 p)
+// Query limbs for the int8-MFMA database pass: the rotation words ctw
+// [q][k][ab][p][z] (u32 residues < 2^28) of one MFMA sweep become 7-bit limbs
+// ql [z][p][4 limbs][C][N1] (column q 2 + ab), so that every (slot, prime,
+// limb) is a row-major C x N1 int8 matrix.  The workgroup (32 slots, query
+// q, prime p) stages the rotation words of its 32 slots with 16-byte loads
+// (each 128-byte row of 32 slots by 8 lanes) in an LDS tile [slot][row]
+// (row = ab N1 + k) whose rows are XOR-swizzled by 8 (slot >> 2) within each
+// slot: every ds_write_b32 of the staging and every ds_read_b128 below hits
+// 64 distinct banks, and 4-row groups stay contiguous.  Then lane (slot,
+// 4-row group) reads its four words in one 16-byte read, extracts limb v of
+// each (bits 7v .. 7v + 6) and packs the four rotations into one dword per
+// limb: a wave's store of limb v covers 64 consecutive 4-row groups, i.e.
+// whole 2 N1-byte runs [q][ab][k] of the output (256 bytes at N1 128).
+// Columns of queries q >= nq (padding up to C) are written as zeros.
+// grid (N/32, C/2, 2 p)
 template <int N1>
 __global__ void __launch_bounds__(256) k_ct_limbs(const uint32_t* __restrict__ ctw, int8_t* __restrict__ ql,
                                                   int nq, int C, QS S) {
-    constexpr int ROWS = 2 * N1, PITCH = 33, PER = ROWS * 8 / 256;
-    __shared__ uint32_t tile[ROWS * PITCH];
+    constexpr int ROWS = 2 * N1, PER = ROWS * 8 / 256, GROUPS = ROWS / 4;
+    __shared__ __attribute__((aligned(16))) uint32_t tile[32 * ROWS];
     const int z0 = blockIdx.x * 32, q = blockIdx.y, p = blockIdx.z;
     const int tid = threadIdx.x;
+    auto swz = [](int slot, int row) { return slot * ROWS + (row ^ ((slot >> 2) << 3)); };
     uint4 buf[PER];
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
@@ -740,38 +746,26 @@ __global__ void __launch_bounds__(256) k_ct_limbs(const uint32_t* __restrict__ c
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
         const int e = tid + 256 * i, zq = e & 7, row = e >> 3;
-        uint32_t* t = tile + row * PITCH + 4 * zq;
-        t[0] = buf[i].x;
-        t[1] = buf[i].y;
-        t[2] = buf[i].z;
-        t[3] = buf[i].w;
+        tile[swz(4 * zq + 0, row)] = buf[i].x;
+        tile[swz(4 * zq + 1, row)] = buf[i].y;
+        tile[swz(4 * zq + 2, row)] = buf[i].z;
+        tile[swz(4 * zq + 3, row)] = buf[i].w;
     }
     __syncthreads();
-    const int lane = tid & 63, wv = tid >> 6;
-    // 4x4 byte transpose in each lane quad by byte permutes (perm(a, b, sel)
-    // picks bytes of b as 0..3 and of a as 4..7): first with lane^1 the
-    // bytes [mine 0, its 0, mine 2, its 2] (even lane) or [its 1, mine 1,
-    // its 3, mine 3] (odd), then with lane^2 the half-words [mine, its]
-    // (lane & 2 clear) or [its high, mine high]
-    const uint32_t sel1 = (lane & 1) ? 0x03070105u : 0x06020400u;
-    const uint32_t sel2 = (lane & 2) ? 0x03020706u : 0x05040100u;
-    // this lane's part of the ql offset: limb v, rotations 4 (lane >> 2) ..
     // (ql of the chunk is below 4 GiB: 32-bit offsets)
-    const uint32_t lane_off = (uint32_t)(lane & 3) * C * N1 + 4 * (lane >> 2);
-    constexpr int ITERS = 32 * 2 * (N1 / 64) / 4;   // per wave
-#pragma unroll 8
-    for (int t = 0; t < ITERS; ++t) {
-        const int it = wv + 4 * t;               // uniform over the wave
-        const int zz = it & 31, ab = (it >> 5) & 1, kr = it >> 6;
-        const uint32_t x = tile[(ab * N1 + kr * 64 + lane) * PITCH + zz];
-        // byte v = limb v of this lane's word (x < 2^28)
-        const uint32_t w = (x & 0x7Fu) | ((x << 1) & 0x7F00u) | ((x << 2) & 0x7F0000u) | ((x << 3) & 0x7F000000u);
-        const uint32_t pw = (uint32_t)__builtin_amdgcn_mov_dpp((int)w, 0xB1, 0xF, 0xF, false);   // quad_perm 1,0,3,2
-        const uint32_t xx = __builtin_amdgcn_perm(pw, w, sel1);
-        const uint32_t px = (uint32_t)__builtin_amdgcn_mov_dpp((int)xx, 0x4E, 0xF, 0xF, false);  // quad_perm 2,3,0,1
-        const uint32_t o = __builtin_amdgcn_perm(px, xx, sel2);
-        const uint32_t wave_off = ((((uint32_t)(z0 + zz) * 2 + p) * 4 * C + q * 2 + ab) * N1 + 64 * kr);
-        *reinterpret_cast<uint32_t*>(ql + (wave_off + lane_off)) = o;
+    const uint32_t qoff = (uint32_t)q * 2 * N1;
+#pragma unroll
+    for (int t = 0; t < 32 * GROUPS / 256; ++t) {
+        const int P = tid + 256 * t, zz = P / GROUPS, g = P % GROUPS;
+        const uint4 x = *reinterpret_cast<const uint4*>(&tile[swz(zz, 4 * g)]);
+        const uint32_t base = (((uint32_t)(z0 + zz) * 2 + p) * 4 * C) * N1 + qoff + 4 * g;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            // rotation 4g + i at byte i (x < 2^28: limbs of 7 bits)
+            const uint32_t o = __builtin_amdgcn_ubfe(x.x, 7 * v, 7) | (__builtin_amdgcn_ubfe(x.y, 7 * v, 7) << 8) |
+                               (__builtin_amdgcn_ubfe(x.z, 7 * v, 7) << 16) | (__builtin_amdgcn_ubfe(x.w, 7 * v, 7) << 24);
+            *reinterpret_cast<uint32_t*>(ql + (base + (uint32_t)v * C * N1)) = o;
+        }
     }
 }
 
