@@ -103,7 +103,10 @@ __device__ __forceinline__ uint64_t crt_q(uint32_t x1, uint32_t x2, const Tabs& 
 
 // residue mod q of a signed gadget digit r with |r| < q
 __host__ __device__ __forceinline__ uint32_t digit_residue(int32_t r, uint32_t q) {
-    return r < 0 ? (uint32_t)(r + (int32_t)q) : (uint32_t)r;
+    // |r| < q < 2^31: u + q wraps to r + q exactly when r < 0, and is then the
+    // smaller of the two; for r >= 0 it stays above u
+    const uint32_t u = (uint32_t)r, w = u + q;
+    return w < u ? w : u;
 }
 
 // ------------------------------------------------------ gadget digits

@@ -100,6 +100,23 @@ __device__ __forceinline__ uint64_t auto_coeff(const uint32_t* a1, const uint32_
     return neg ? (x ? kQ - x : 0) : x;
 }
 
+// the same from two padded LDS planes, with the sign of the negacyclic wrap
+// applied to the residues (-x mod q_p) before the composition: the
+// composition of (-x1, -x2) is Q - x, or 0 for x = 0, exactly the value the
+// 64-bit negation gives, in 32-bit operations
+__device__ __forceinline__ uint64_t auto_coeff_lds(const uint32_t* s1, const uint32_t* s2, int s, uint32_t ginv,
+                                                   const Tabs& T) {
+    const uint32_t i0 = ((uint32_t)s * ginv) & (2 * kN - 1);
+    const bool neg = i0 >= (uint32_t)kN;
+    const int i = (int)(i0 & (kN - 1));
+    uint32_t x1 = s1[lds_pad(i)], x2 = s2[lds_pad(i)];
+    if (neg) {
+        x1 = neg_mod(x1, kQ1);
+        x2 = neg_mod(x2, kQ2);
+    }
+    return crt_q(x1, x2, T);
+}
+
 // 5^e mod 2N
 __device__ __forceinline__ uint32_t pow5_mod2n(uint32_t e) {
     uint32_t r = 1, b = 5;
@@ -1319,11 +1336,7 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused(const uint32_t* _
 #pragma unroll
         for (int e = 0; e < PER; ++e) {
             const int x = tid + e * kFusedThreads;
-            const uint32_t i0 = ((uint32_t)x * ginv) & (2 * kN - 1);
-            const bool neg = i0 >= (uint32_t)kN;
-            const int i = neg ? (int)i0 - kN : (int)i0;
-            uint64_t v = crt_q(s[0][lds_pad(i)], s[1][lds_pad(i)], T);
-            if (neg) v = v ? kQ - v : 0;
+            const uint64_t v = auto_coeff_lds(s[0], s[1], x, ginv, T);
             const int32_t d0 = tfhe_digit<18, 20, 2>(v, 0), d1 = tfhe_digit<18, 20, 2>(v, 1);
             dg[0][e] = digit_residue(d0, q1);
             dg[1][e] = digit_residue(d1, q1);
@@ -1597,11 +1610,7 @@ __global__ void __launch_bounds__(kFusedThreads) k_giant_fused_dma(const uint32_
 #pragma unroll
         for (int e = 0; e < PER; ++e) {
             const int x = tid + e * kFusedThreads;
-            const uint32_t i0 = ((uint32_t)x * ginv) & (2 * kN - 1);
-            const bool neg = i0 >= (uint32_t)kN;
-            const int i = neg ? (int)i0 - kN : (int)i0;
-            uint64_t v = crt_q(sa[0][lds_pad(i)], sa[1][lds_pad(i)], T);
-            if (neg) v = v ? kQ - v : 0;
+            const uint64_t v = auto_coeff_lds(sa[0], sa[1], x, ginv, T);
             const int32_t d0 = tfhe_digit<18, 20, 2>(v, 0), d1 = tfhe_digit<18, 20, 2>(v, 1);
             sd[0][lds_pad(x)] = digit_residue(d0, q1);
             sd[1][lds_pad(x)] = digit_residue(d1, q1);
@@ -1772,11 +1781,7 @@ __global__ void __launch_bounds__(kFusedThreads) __attribute__((amdgpu_waves_per
 #pragma unroll
         for (int e = 0; e < PER; ++e) {
             const int x = tid + e * kFusedThreads;
-            const uint32_t i0 = ((uint32_t)x * ginv) & (2 * kN - 1);
-            const bool neg = i0 >= (uint32_t)kN;
-            const int i = neg ? (int)i0 - kN : (int)i0;
-            uint64_t v = crt_q(s[0][lds_pad(i)], s[1][lds_pad(i)], T);
-            if (neg) v = v ? kQ - v : 0;
+            const uint64_t v = auto_coeff_lds(s[0], s[1], x, ginv, T);
             const int32_t d0 = tfhe_digit<18, 20, 2>(v, 0), d1 = tfhe_digit<18, 20, 2>(v, 1);
             dg[0][e] = digit_residue(d0, q1);
             dg[1][e] = digit_residue(d1, q1);
@@ -1986,11 +1991,7 @@ __global__ void __launch_bounds__(512) k_pack_level(const uint32_t* __restrict__
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
         const int x = tid + e * 512;
-        const uint32_t i0 = ((uint32_t)x * ginv) & (2 * kN - 1);
-        const bool neg = i0 >= (uint32_t)kN;
-        const int ix = neg ? (int)i0 - kN : (int)i0;
-        uint64_t v = crt_q(s[0][lds_pad(ix)], s[1][lds_pad(ix)], T);
-        if (neg) v = v ? kQ - v : 0;
+        const uint64_t v = auto_coeff_lds(s[0], s[1], x, ginv, T);
         int32_t d[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) d[k] = tfhe_digit<14, 0, 4>(v, k);
