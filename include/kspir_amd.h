@@ -67,6 +67,19 @@ int kspir_database_create(kspir_context* ctx, const uint64_t* datacrt, int32_t r
                           int on_device, kspir_database** out);
 void kspir_database_destroy(kspir_database* db);
 
+/* No counterpart in the reference (a resident-memory choice).  A database
+ * that has served a batch holds two forms in HBM: its u64 words (4 B per
+ * residue, read by the single-query word pass) and its 7-bit int8 limbs
+ * (4 B per residue, read by the int8-MFMA pass).  kspir_database_compact
+ * builds the limbs if needed and frees the words, halving the footprint.
+ * Afterwards every answer takes the MFMA pass (one query: 0.62 ms against
+ * 0.50 ms at 256 MB, responses unchanged), kspir_database_export rebuilds
+ * the words from the limbs, and kspir_answer_device_noalloc and
+ * kspir_time_db_pass (hence kspir_autotune: tune first) reject the
+ * database.  Only shapes with an MFMA pass (N1 64..256, r*N2 % 16 == 0) can
+ * be compacted; no answer on this database may be in flight.  Idempotent. */
+int kspir_database_compact(kspir_database* db);
+
 /* Server preprocessing on the GPU: database_tobsgsntt (twosteps.h:22,
  * twosteps.cpp:42-179) followed by database_tocrt (crt.h:133,
  * crt.cpp:859-885), straight into a resident database handle.  records: r

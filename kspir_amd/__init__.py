@@ -45,7 +45,7 @@ def _load():
                  "kspir_autotune", "kspir_client_generate", "kspir_client_draws", "kspir_client_encrypt",
                  "kspir_server_create_lanes", "kspir_gen_giant_keys_from_offline", "kspir_answer_device_noalloc",
                  "kspir_inverse_encode", "kspir_time_db_pass_batch", "kspir_answer_multi",
-                 "kspir_answer_multi_device", "kspir_time_answer_multi"):
+                 "kspir_answer_multi_device", "kspir_time_answer_multi", "kspir_database_compact"):
         getattr(L, name).restype = ctypes.c_int
     L.kspir_autotune.argtypes = [_vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                  ctypes.POINTER(ctypes.c_int32), ctypes.c_char_p, ctypes.c_size_t]
@@ -73,6 +73,7 @@ EXPORTS = (
     "kspir_client_generate", "kspir_client_draws", "kspir_client_encrypt", "kspir_server_create_lanes",
     "kspir_gen_giant_keys_from_offline", "kspir_answer_device_noalloc", "kspir_inverse_encode",
     "kspir_time_db_pass_batch", "kspir_answer_multi", "kspir_answer_multi_device", "kspir_time_answer_multi",
+    "kspir_database_compact",
 )
 
 
@@ -149,6 +150,11 @@ class Database:
         out = np.zeros((self.r, N * N // 2), dtype=np.uint64)
         _check(lib.kspir_database_export(self.h, _p64(out)))
         return out
+
+    def compact(self):
+        """keep only the int8 limbs resident (frees the u64 words; every answer
+        then takes the MFMA pass)"""
+        _check(lib.kspir_database_compact(self.h))
 
     def close(self):
         if self.h:

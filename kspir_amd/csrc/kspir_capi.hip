@@ -836,8 +836,11 @@ int enqueue_answer(kspir_context* c, const kspir_database* db, const KeyPtrs* kp
             return fail(KSPIR_EINVAL, "internal: chunked workspace for an unchunked pipeline");
         return enqueue_chunked(c, db, kp, nq, resp, st);
     }
-    int rc = (allow_mfma && nq >= c->mfma_min_batch && mfma_shape(db)) ? launch_db_batched(c, db, 0, nq, st)
-                                                          : launch_db_product(c, db, nq, st);
+    // a database kept as limbs only (kspir_database_compact) always takes the MFMA pass
+    const bool mfma = allow_mfma && mfma_shape(db) && (nq >= c->mfma_min_batch || !db->words);
+    if (!mfma && !db->words)
+        return fail(KSPIR_EINVAL, "database kept as int8 limbs only (kspir_database_compact): this call needs its words");
+    int rc = mfma ? launch_db_batched(c, db, 0, nq, st) : launch_db_product(c, db, nq, st);
     if (rc) return rc;
     // giant steps (twosteps.cpp:599-677) and the first-dimension sum: one
     // fused workgroup per (query, block) for a batch; for few queries, one
@@ -1298,17 +1301,46 @@ int kspir_database_export(const kspir_database* db, uint64_t* datacrt) {
     HIP_OK(hipSetDevice(c->device));
     const size_t words = (size_t)kN * kN / 2;
     uint64_t* stage = nullptr;
+    uint64_t* wstage = nullptr;   // a block's words rebuilt from the limbs (compacted database)
     HIP_OK(hipMalloc(&stage, words * 8));
+    if (!db->words) {
+        hipError_t e = hipMalloc(&wstage, words * 8);
+        if (e != hipSuccess) { hipFree(stage); return fail_hip("hipMalloc(export)", e); }
+    }
     hipError_t e = hipSuccess;
     for (int b = 0; b < db->r && e == hipSuccess; ++b) {
+        const uint64_t* src = db->words ? db->words + words * b : wstage;
+        if (!db->words) {
+            hipLaunchKernelGGL(k_db_unlimbs, dim3(kN / 64, db->n2), dim3(256), 0, c->stream, db->limbs, wstage, db->n1,
+                               db->r * db->n2, b * db->n2);
+            e = hipGetLastError();
+            if (e != hipSuccess) break;
+        }
         hipLaunchKernelGGL(k_db_untranspose, dim3(kN / 32, db->n2 * db->n1 / 32), dim3(256), 0, c->stream, stage,
-                           db->words + words * b, db->n2 * db->n1);
+                           src, db->n2 * db->n1);
         e = hipGetLastError();
         if (e == hipSuccess) e = hipMemcpyAsync(datacrt + words * b, stage, words * 8, hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     }
     hipFree(stage);
+    if (wstage) hipFree(wstage);
     if (e != hipSuccess) return fail_hip("database_export", e);
+    return KSPIR_OK;
+}
+
+int kspir_database_compact(kspir_database* db) {
+    if (!db) return fail(KSPIR_EINVAL, "null database handle");
+    if (!db->words) return KSPIR_OK;
+    if (!mfma_shape(db))
+        return fail(KSPIR_EINVAL, "this database shape has no int8-MFMA pass (N1 64..256, r*N2 % 16 == 0): its words stay");
+    kspir_context* c = db->ctx;
+    HIP_OK(hipSetDevice(c->device));
+    int rc = ensure_limbs(db, c->stream);
+    if (rc) return rc;
+    // graphs captured with the word pass name the words
+    drop_graphs_everywhere(c->device, db);
+    HIP_OK(hipFree(db->words));
+    db->words = nullptr;
     return KSPIR_OK;
 }
 
@@ -1434,6 +1466,9 @@ int kspir_answer_device_noalloc(kspir_context* c, const kspir_database* db, cons
     if (rc) return rc;
     if (!query_dev || !response_dev) return fail(KSPIR_EINVAL, "null buffer");
     HIP_OK(hipSetDevice(c->device));
+    if (!db->words)
+        return fail(KSPIR_EINVAL, "database kept as int8 limbs only (kspir_database_compact): the no-allocation answer "
+                                  "runs the word pass");
     uint32_t* ptr = g_scratch_ptr[c->device];   // set when the context was created
     if (!ptr) return fail(KSPIR_EINVAL, "no scratch address for this device");
     // every buffer the kernels use is written before it is read, so stale
@@ -1783,6 +1818,8 @@ int kspir_time_db_pass(kspir_context* c, const kspir_database* db, const kspir_k
     if (!ms_per_launch || iters < 1) return fail(KSPIR_EINVAL, "bad timing arguments");
     int rc = check_pair(c, db, k, true);
     if (rc) return rc;
+    if (!db->words)
+        return fail(KSPIR_EINVAL, "database kept as int8 limbs only (kspir_database_compact): no word pass to time");
     HIP_OK(hipSetDevice(c->device));
     rc = carve_workspace(c, db->r, db->n1, 1, false);
     if (rc) return rc;

@@ -728,6 +728,39 @@ __global__ void __launch_bounds__(256) k_db_limbs(const uint64_t* __restrict__ w
     }
 }
 
+// the inverse of k_db_limbs for rows rho0 .. rho0 + gridDim.y: database words
+// [row - rho0][N1][N] (x mod q1 | x mod q2 << 32) from the limbs
+// [N][2 p][4 limbs][R][N1].  grid (N/64, rows)
+__global__ void __launch_bounds__(256) k_db_unlimbs(const int8_t* __restrict__ dbl, uint64_t* __restrict__ words,
+                                                    int n1, int R, int rho0) {
+    __shared__ uint64_t tile[64][65];
+    const int z0 = blockIdx.x * 64, rho = rho0 + blockIdx.y;
+    const int zz = threadIdx.x >> 2, c16 = threadIdx.x & 3;   // 64 slots x 4 chunks of 16 rotations
+    for (int kc = 0; kc < n1; kc += 64) {
+        uint32_t x[2][16];
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+#pragma unroll
+            for (int t = 0; t < 16; ++t) x[p][t] = 0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const i8x16 v = *reinterpret_cast<const i8x16*>(
+                    dbl + ((((size_t)(z0 + zz) * 2 + p) * 4 + u) * R + rho) * n1 + kc + c16 * 16);
+#pragma unroll
+                for (int t = 0; t < 16; ++t) x[p][t] |= (uint32_t)(uint8_t)v[t] << (7 * u);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 16; ++t) tile[c16 * 16 + t][zz] = x[0][t] | ((uint64_t)x[1][t] << 32);
+        __syncthreads();
+        for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+            const int kk = e >> 6, z = e & 63;
+            words[((size_t)blockIdx.y * n1 + kc + kk) * kN + z0 + z] = tile[kk][z];
+        }
+        __syncthreads();
+    }
+}
+
 // Query limbs for the int8-MFMA database pass: the rotation words ctw
 // [q][k][ab][p][z] (u32 residues < 2^28) of one MFMA sweep become 7-bit limbs
 // ql [z][p][4 limbs][C][N1] (column q 2 + ab), so that every (slot, prime,

@@ -758,3 +758,42 @@ def test_inverse_encode_on_gpu(ctx, oracle):
     bad[7] = 65537
     with pytest.raises(kspir_amd.KspirError, match="below p"):
         ctx.inverse_encode(bad)
+
+
+def test_database_compact_keeps_answers_and_export(ctx, oracle):
+    """a database kept as int8 limbs only (kspir_database_compact) gives the
+    same single and batched answers, exports the same words, and is rejected
+    by the entry points that need its words"""
+    import kspir_amd
+    r, n1 = 4, 128
+    say(f"[gpu] compacted database r={r} n1={n1}")
+    words = oracle.gen_db(r, n1, seed=21)
+    db = ctx.database(words, r, n1)
+    perms = oracle.perm_matrix(n1)
+    cls = [oracle.gen_client(r, n1, 50 + c, (31 * c + 3) % (N // 2), c % r) for c in range(3)]
+    keys = [ctx.keys(r, n1, cl["baby"], cl["giant"], cl["packing"], cl["rgsw"], perms) for cl in cls]
+    try:
+        qs = np.stack([cl["query"] for cl in cls])
+        single = [ctx.answer(db, keys[c], qs[c]) for c in range(3)]
+        batch = ctx.answer_batch(db, keys, qs)
+        db.compact()
+        db.compact()   # idempotent
+        assert np.array_equal(db.export(), words)
+        for c in range(3):
+            assert np.array_equal(ctx.answer(db, keys[c], qs[c]), single[c]), c
+        assert np.array_equal(ctx.answer_batch(db, keys, qs), batch)
+        assert np.array_equal(single[0], oracle.answer(r=r, n1=n1, db=words, query=qs[0], rgsw=cls[0]["rgsw"],
+                                                       baby=cls[0]["baby"], giant=cls[0]["giant"],
+                                                       packing=cls[0]["packing"], perms=perms))
+        qb, rb = ctx.alloc(4 * N * 8), ctx.alloc(2 * N * 8)
+        try:
+            qb.upload(qs[0])
+            with pytest.raises(kspir_amd.KspirError, match="limbs only"):
+                ctx.answer_device_noalloc(db, keys[0], qb, rb)
+        finally:
+            qb.free()
+            rb.free()
+    finally:
+        for k in keys:
+            k.close()
+        db.close()
