@@ -108,6 +108,16 @@ int kspir_inverse_encode(kspir_context* ctx, uint64_t* messages, int32_t count);
  * that kspir_database_create loads again. */
 int kspir_database_export(const kspir_database* db, uint64_t* datacrt);
 
+/* No counterpart in the reference (which keeps the preprocessed database in
+ * memory only).  The persisted form as a file: a 64-byte header (magic
+ * "KSPIRDB", format version 1, N, r, N1, words per block, FNV-1a 64 checksum
+ * of the words) followed by the kspir_database_export words.
+ * kspir_database_load rejects a file whose magic, version, N or block size
+ * differ, whose length does not match its r, or whose checksum fails, and
+ * then uploads it as kspir_database_create does (r and N1 from the header). */
+int kspir_database_save(const kspir_database* db, const char* path);
+int kspir_database_load(kspir_context* ctx, const char* path, kspir_database** out);
+
 /* Per-client evaluation keys, resident in HBM.  Replaces the autoKey and
  * permutations arguments of matrix_vector_mul_bsgs_rns_crt_large
  * (twosteps.cpp:1527), the query argument of externalProduct (lwe.cpp:761)

@@ -45,7 +45,8 @@ def _load():
                  "kspir_autotune", "kspir_client_generate", "kspir_client_draws", "kspir_client_encrypt",
                  "kspir_server_create_lanes", "kspir_gen_giant_keys_from_offline", "kspir_answer_device_noalloc",
                  "kspir_inverse_encode", "kspir_time_db_pass_batch", "kspir_answer_multi",
-                 "kspir_answer_multi_device", "kspir_time_answer_multi", "kspir_database_compact"):
+                 "kspir_answer_multi_device", "kspir_time_answer_multi", "kspir_database_compact",
+                 "kspir_database_save", "kspir_database_load"):
         getattr(L, name).restype = ctypes.c_int
     L.kspir_autotune.argtypes = [_vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                  ctypes.POINTER(ctypes.c_int32), ctypes.c_char_p, ctypes.c_size_t]
@@ -73,7 +74,7 @@ EXPORTS = (
     "kspir_client_generate", "kspir_client_draws", "kspir_client_encrypt", "kspir_server_create_lanes",
     "kspir_gen_giant_keys_from_offline", "kspir_answer_device_noalloc", "kspir_inverse_encode",
     "kspir_time_db_pass_batch", "kspir_answer_multi", "kspir_answer_multi_device", "kspir_time_answer_multi",
-    "kspir_database_compact",
+    "kspir_database_compact", "kspir_database_save", "kspir_database_load",
 )
 
 
@@ -151,6 +152,10 @@ class Database:
         _check(lib.kspir_database_export(self.h, _p64(out)))
         return out
 
+    def save(self, path):
+        """write the persisted form (header + datacrt words) to `path`"""
+        _check(lib.kspir_database_save(self.h, os.fsencode(path)))
+
     def compact(self):
         """keep only the int8 limbs resident (frees the u64 words; every answer
         then takes the MFMA pass)"""
@@ -160,6 +165,18 @@ class Database:
         if self.h:
             lib.kspir_database_destroy(self.h)
             self.h = None
+
+
+def _database_load(ctx, path):
+    import struct
+    with open(path, "rb") as f:
+        head = f.read(24)
+    h = _vp()
+    _check(lib.kspir_database_load(ctx.h, os.fsencode(path), ctypes.byref(h)))
+    db = Database.__new__(Database)
+    db.ctx, db.h = ctx, h
+    db.r, db.n1 = struct.unpack_from("<II", head, 16)
+    return db
 
 
 class Keys:
@@ -287,6 +304,10 @@ class Context:
 
     def database(self, datacrt, r, n1, on_device=False):
         return Database(self, datacrt, r, n1, on_device)
+
+    def database_load(self, path):
+        """a database written by Database.save (header checked, words uploaded)"""
+        return _database_load(self, path)
 
     def database_from_records(self, records, r, n1, block_stride=None, on_device=False):
         """preprocess records ([r][N][N/2] values < p, the sample_database_bsgs

@@ -1328,6 +1328,85 @@ int kspir_database_export(const kspir_database* db, uint64_t* datacrt) {
     return KSPIR_OK;
 }
 
+// Persisted database file: a 64-byte header, then the r blocks of words in
+// the reference datacrt layout (what kspir_database_export returns).
+//   0  char[8]  "KSPIRDB\0"
+//   8  u32      format version (1)
+//  12  u32      N
+//  16  u32      r
+//  20  u32      N1
+//  24  u64      words per block (N * N / 2)
+//  32  u64      FNV-1a 64 of the words (little-endian bytes)
+//  40  u8[24]   zero
+namespace {
+constexpr char kDbMagic[8] = {'K', 'S', 'P', 'I', 'R', 'D', 'B', '\0'};
+constexpr uint32_t kDbVersion = 1;
+struct DbFileHeader {
+    char magic[8];
+    uint32_t version, n, r, n1;
+    uint64_t words_per_block, fnv;
+    uint8_t zero[24];
+};
+static_assert(sizeof(DbFileHeader) == 64, "database file header is 64 bytes");
+static uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
+    const uint8_t* b = (const uint8_t*)p;
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 0x100000001b3ULL;
+    return h;
+}
+constexpr uint64_t kFnvBasis = 0xcbf29ce484222325ULL;
+}  // namespace
+
+int kspir_database_save(const kspir_database* db, const char* path) {
+    if (!db || !path) return fail(KSPIR_EINVAL, "null argument");
+    const size_t words = (size_t)kN * kN / 2;
+    std::vector<uint64_t> host(words * db->r);
+    int rc = kspir_database_export(db, host.data());
+    if (rc) return rc;
+    DbFileHeader h{};
+    std::memcpy(h.magic, kDbMagic, 8);
+    h.version = kDbVersion;
+    h.n = kN;
+    h.r = (uint32_t)db->r;
+    h.n1 = (uint32_t)db->n1;
+    h.words_per_block = words;
+    h.fnv = fnv1a(kFnvBasis, host.data(), host.size() * 8);
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(KSPIR_EINVAL, std::string("cannot open ") + path + " for writing");
+    const bool ok = std::fwrite(&h, sizeof h, 1, f) == 1 && std::fwrite(host.data(), 8, host.size(), f) == host.size();
+    const bool closed = std::fclose(f) == 0;
+    if (!ok || !closed) return fail(KSPIR_EINVAL, std::string("short write to ") + path);
+    return KSPIR_OK;
+}
+
+int kspir_database_load(kspir_context* c, const char* path, kspir_database** out) {
+    if (!c || !path || !out) return fail(KSPIR_EINVAL, "null argument");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(KSPIR_EINVAL, std::string("cannot open ") + path);
+    DbFileHeader h{};
+    if (std::fread(&h, sizeof h, 1, f) != 1) {
+        std::fclose(f);
+        return fail(KSPIR_EINVAL, std::string(path) + ": no database header");
+    }
+    const size_t words = (size_t)kN * kN / 2;
+    if (std::memcmp(h.magic, kDbMagic, 8) != 0 || h.version != kDbVersion) {
+        std::fclose(f);
+        return fail(KSPIR_EINVAL, std::string(path) + ": not a database file of this format version");
+    }
+    if (h.n != (uint32_t)kN || h.words_per_block != words || h.r == 0 || h.r > (uint32_t)kN || h.n1 == 0) {
+        std::fclose(f);
+        return fail(KSPIR_EINVAL, std::string(path) + ": header shape (N, r, N1, block words) does not fit N = 4096");
+    }
+    std::vector<uint64_t> host(words * h.r);
+    const size_t got = std::fread(host.data(), 8, host.size(), f);
+    const bool extra = std::fgetc(f) != EOF;
+    std::fclose(f);
+    if (got != host.size() || extra)
+        return fail(KSPIR_EINVAL, std::string(path) + ": file length does not match its header's r");
+    if (fnv1a(kFnvBasis, host.data(), host.size() * 8) != h.fnv)
+        return fail(KSPIR_EINVAL, std::string(path) + ": checksum mismatch (file corrupted)");
+    return kspir_database_create(c, host.data(), (int32_t)h.r, (int32_t)h.n1, 0, out);
+}
+
 int kspir_database_compact(kspir_database* db) {
     if (!db) return fail(KSPIR_EINVAL, "null database handle");
     if (!db->words) return KSPIR_OK;

@@ -797,3 +797,34 @@ def test_database_compact_keeps_answers_and_export(ctx, oracle):
         for k in keys:
             k.close()
         db.close()
+
+
+def test_database_file_round_trip(ctx, oracle, tmp_path):
+    """kspir_database_save / _load: the reloaded database answers and exports
+    the same words; a file with a wrong magic, a truncated file or a flipped
+    word is rejected"""
+    import kspir_amd
+    r, n1 = 2, 128
+    say(f"[gpu] database file round trip r={r} n1={n1}")
+    words = oracle.gen_db(r, n1, seed=33)
+    db = ctx.database(words, r, n1)
+    path = str(tmp_path / "db.kspir")
+    db.save(path)
+    db2 = ctx.database_load(path)
+    cl = oracle.gen_client(r, n1, 71, 12, 1)
+    keys = ctx.keys(r, n1, cl["baby"], cl["giant"], cl["packing"], cl["rgsw"], oracle.perm_matrix(n1))
+    try:
+        assert (db2.r, db2.n1) == (r, n1)
+        assert np.array_equal(db2.export(), words)
+        assert np.array_equal(ctx.answer(db2, keys, cl["query"]), ctx.answer(db, keys, cl["query"]))
+        raw = open(path, "rb").read()
+        bad = [b"XSPIRDB\0" + raw[8:], raw[:-8], raw[:64 + 800] + bytes([raw[64 + 800] ^ 1]) + raw[64 + 801:]]
+        for i, blob in enumerate(bad):
+            p2 = str(tmp_path / f"bad{i}.kspir")
+            open(p2, "wb").write(blob)
+            with pytest.raises(kspir_amd.KspirError):
+                ctx.database_load(p2)
+    finally:
+        keys.close()
+        db2.close()
+        db.close()
