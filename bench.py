@@ -375,12 +375,15 @@ def config_offline(args, log):
              "sample": f"one client (seed 100) by the oracle on the host, {cpu * 1e3:.0f} ms, equal to the GPU's words"}}
 
 
-def serve_rates(srv, db, keys, queries, rates, dur, seed, log, tag):
+def serve_rates(srv, targets, rates, dur, seed, log, tag, share=None):
     """Poisson arrivals at each offered rate for `dur` seconds through the
-    server; p50/p99 of the server's submit-to-response latency"""
+    server, each arrival sent to one of `targets` ((database, keys, queries)
+    triples) drawn with probabilities `share` (default: all to the first);
+    p50/p99 of the server's submit-to-response latency"""
     import threading
     res = {}
     rng = np.random.default_rng(seed)
+    share = share or [1.0] + [0.0] * (len(targets) - 1)
     for rate in rates:
         log(f"C5 {tag}: {rate} queries/s offered for {dur} s")
         tickets, done_ms = [], []
@@ -403,30 +406,41 @@ def serve_rates(srv, db, keys, queries, rates, dur, seed, log, tag):
         t_start = time.perf_counter()
         t_next = t_start
         c = 0
+        sent = [0] * len(targets)
         while t_next - t_start < dur:
             now = time.perf_counter()
             if now < t_next:
                 time.sleep(t_next - now)
+            g = int(rng.choice(len(targets), p=share))
+            db, keys, queries = targets[g]
             tickets.append(srv.submit(db, keys[c % len(keys)], queries[c % len(queries)]))
+            sent[g] += 1
             c += 1
             t_next += rng.exponential(1.0 / rate)
         stop_flag[0] = True
         th.join()
         wall = time.perf_counter() - t_start
-        nb = srv.stats()["batches"] - b0["batches"]
+        b1 = srv.stats()
+        nb = b1["batches"] - b0["batches"]
         res[str(rate)] = {"offered_qps": rate, "submitted": c, "achieved_qps": round(c / wall, 1),
                           "latency_ms": {"p50": round(float(np.percentile(done_ms, 50)), 3),
                                          "p99": round(float(np.percentile(done_ms, 99)), 3)},
                           "batches": nb, "mean_batch": round(c / max(nb, 1), 2)}
+        if len(targets) > 1:
+            res[str(rate)]["submitted_per_database"] = sent
+            res[str(rate)]["batches_spanning_databases"] = b1["spanning"] - b0["spanning"]
     return res
 
 
 def config_c5(args, log):
     """C5: serving loop.  A resident 4 GB database (r=256) and, memory
-    permitting, a 16 GB one (r=1024); 64 clients with their own keys; queries
-    arrive as a Poisson process at each offered rate and go through the
-    library's continuous-batching server (kspir_server_*), with one lane and
-    with two lanes (batches overlapping on the device).  Latency is
+    permitting, a 16 GB one (r=1024); 64 clients per database with their own
+    keys; queries arrive as a Poisson process at each offered rate and go
+    through the library's continuous-batching server (kspir_server_*): each
+    database alone, with one lane and with two lanes (batches overlapping on
+    the device), then both databases resident together with the arrivals
+    split evenly between them, one lane, with batches spanning the two
+    databases (option serve_multi) and without.  Latency is
     submit-to-response, measured by the server."""
     import kspir_amd
     n1 = 128
@@ -435,8 +449,10 @@ def config_c5(args, log):
     rec = synthetic_records()
     out = {"config": "C5 serving: Poisson arrivals, continuous batching (batch cap 128 at 4 GB, 64 at 16 GB; admission "
                      "by each batch's device footprint; 1 lane, and 2 lanes whose batches overlap on the device), "
-                     "64 clients with distinct keys, resident 4 GB (r=256) and 16 GB (r=1024) databases, N1=128",
+                     "64 clients with distinct keys per database, resident 4 GB (r=256) and 16 GB (r=1024) databases, "
+                     "N1=128; then both resident with arrivals split evenly, batches spanning the two databases or not",
            "databases": {}}
+    held = []
     for r, rates, dur, cap, lane_set in ((256, (200, 500, 1000, 2000), 2.0, 128, (1, 2)),
                                          (1024, (200, 500), 2.0, 64, (1,))):
         try:
@@ -456,11 +472,26 @@ def config_c5(args, log):
             # warm-up: full batches on every lane (database limbs, workspaces at full size)
             for t in [srv.submit(db, keys[c % 64], queries[c % 64]) for c in range(cap * lanes)]:
                 srv.wait(t)
-            by_lanes[f"lanes_{lanes}"] = serve_rates(srv, db, keys, queries, rates, dur, r, log, f"r={r} lanes={lanes}")
+            by_lanes[f"lanes_{lanes}"] = serve_rates(srv, [(db, keys, queries)], rates, dur, r, log,
+                                                     f"r={r} lanes={lanes}")
             srv.close()
         out["databases"][f"{r // 64} GB"] = {"r": r, "batch_cap": cap,
                                             "per_query_footprint_MB": round(kspir_amd.query_footprint(r, n1) / 2**20, 1),
                                             **by_lanes}
+        held.append((db, keys, queries))
+    if len(held) == 2:
+        mixed = {"share": [0.5, 0.5], "batch_cap": 64}
+        for multi in (1, 0):
+            ctx.set("serve_multi", multi)
+            srv = kspir_amd.Server(ctx, max_batch=64, lanes=1)
+            for t in [srv.submit(held[c % 2][0], held[c % 2][1][c % 64], held[c % 2][2][c % 64]) for c in range(64)]:
+                srv.wait(t)
+            mixed[f"serve_multi_{multi}"] = serve_rates(srv, held, (200, 500), 2.0, 77, log,
+                                                        f"both databases serve_multi={multi}", share=[0.5, 0.5])
+            srv.close()
+        ctx.set("serve_multi", 0)
+        out["both_resident"] = mixed
+    for db, keys, _ in held:
         for k in keys:
             k.close()
         db.close()

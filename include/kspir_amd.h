@@ -297,8 +297,15 @@ int kspir_client_encrypt(kspir_context* ctx, int32_t kind, const uint64_t* secre
  * and up to max_batch of that database's waiting queries, in arrival order,
  * as many as workspace_budget_bytes admits at the per-query device footprint
  * of the database's shape (0 = half of the device's free memory at
- * creation).  The server drives the context: while it exists, do not call
- * other functions on that context.  query ([2][2][N], host) and response
+ * creation).  With the context option "serve_multi" set to 1 (0 by default)
+ * and the options of kspir_answer_multi, waiting queries of other databases of the
+ * same N1 and at most as many blocks fill the rest of the batch in arrival
+ * order (at most 8 databases), each admitted while the whole batch fits the
+ * budget at the first database's per-query footprint, and the batch is
+ * answered as by kspir_answer_multi: one launch per stage.  A larger
+ * database never joins: its longer giant steps would delay every query of
+ * the batch.  The server drives the context:
+ * while it exists, do not call other functions on that context.  query ([2][2][N], host) and response
  * ([2][N], host) must stay valid until kspir_server_wait returns for the
  * ticket; latency_ms is submit-to-completion time.  Responses equal
  * kspir_answer's for the same query. */
@@ -315,6 +322,8 @@ int kspir_server_submit(kspir_server* server, const kspir_database* db, const ks
                         const uint64_t* query, uint64_t* response, uint64_t* ticket);
 int kspir_server_wait(kspir_server* server, uint64_t ticket, double* latency_ms);
 int kspir_server_stats(kspir_server* server, uint64_t* batches, uint64_t* answered, uint64_t* pending);
+/* batches so far that spanned more than one database */
+int kspir_server_spanning_batches(kspir_server* server, uint64_t* spanning);
 /* waits for every submitted query, then stops the worker */
 void kspir_server_destroy(kspir_server* server);
 

@@ -61,6 +61,15 @@ int kspir_device_count(int* count);
  *   "db_waves"        workgroup shape of the single-query database pass:
  *                     2, 4 (default), 8 or 16 waves on 64 slots, each wave
  *                     taking N2 / waves rows
+ *   "serve_multi"     1: a serving loop on this context fills a batch with
+ *                     waiting queries of several databases of one N1
+ *                     (kspir_server_create); 0 (default): one database per
+ *                     batch, which measured lower latency at equal
+ *                     throughput on mixed traffic
+ *   "serve_graphs"    0 (default): a serving loop launches each batch
+ *                     eagerly (its batch sizes vary, and a graph captured for
+ *                     each new size costs more than it saves); 1: it captures
+ *                     and replays graphs as the answer calls above do
  * Setting an option waits for the context stream and drops captured graphs. */
 int kspir_context_set(kspir_context* ctx, const char* key, int64_t value);
 int kspir_context_get(kspir_context* ctx, const char* key, int64_t* value);

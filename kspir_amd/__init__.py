@@ -46,7 +46,7 @@ def _load():
                  "kspir_server_create_lanes", "kspir_gen_giant_keys_from_offline", "kspir_answer_device_noalloc",
                  "kspir_inverse_encode", "kspir_time_db_pass_batch", "kspir_answer_multi",
                  "kspir_answer_multi_device", "kspir_time_answer_multi", "kspir_database_compact",
-                 "kspir_database_save", "kspir_database_load"):
+                 "kspir_database_save", "kspir_database_load", "kspir_server_spanning_batches"):
         getattr(L, name).restype = ctypes.c_int
     L.kspir_autotune.argtypes = [_vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                  ctypes.POINTER(ctypes.c_int32), ctypes.c_char_p, ctypes.c_size_t]
@@ -74,7 +74,7 @@ EXPORTS = (
     "kspir_client_generate", "kspir_client_draws", "kspir_client_encrypt", "kspir_server_create_lanes",
     "kspir_gen_giant_keys_from_offline", "kspir_answer_device_noalloc", "kspir_inverse_encode",
     "kspir_time_db_pass_batch", "kspir_answer_multi", "kspir_answer_multi_device", "kspir_time_answer_multi",
-    "kspir_database_compact", "kspir_database_save", "kspir_database_load",
+    "kspir_database_compact", "kspir_database_save", "kspir_database_load", "kspir_server_spanning_batches",
 )
 
 
@@ -235,7 +235,9 @@ class Server:
     def stats(self):
         b, a, p = ctypes.c_uint64(0), ctypes.c_uint64(0), ctypes.c_uint64(0)
         _check(lib.kspir_server_stats(self.h, ctypes.byref(b), ctypes.byref(a), ctypes.byref(p)))
-        return {"batches": b.value, "answered": a.value, "pending": p.value}
+        sp = ctypes.c_uint64(0)
+        _check(lib.kspir_server_spanning_batches(self.h, ctypes.byref(sp)))
+        return {"batches": b.value, "answered": a.value, "pending": p.value, "spanning": sp.value}
 
     def close(self):
         if self.h:

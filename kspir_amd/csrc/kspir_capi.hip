@@ -135,6 +135,8 @@ struct kspir_context {
     int giant_wgs_per_cu = 1;       // k_giant_fused (1) or its two-per-CU variant k_giant_fused2 (2)
     int giant_dma = 1;              // one per CU with rows prefetched by LDS-DMA (k_giant_fused_dma)
     int64_t pack_fused_min_wgs = 0; // a packing level is fused only with at least this many (pair, query) workgroups
+    int serve_multi = 0;            // the serving loop puts queries of several databases in one batch (one launch per stage)
+    int serve_graphs = 0;           // the serving loop replays captured graphs (1) or launches eagerly (0)
     // captured pipelines: one hipGraph per (database, key table, batch size,
     // query / response buffers, stream); dropped whenever a buffer they name
     // is reallocated or freed, or an option changes
@@ -1001,12 +1003,35 @@ int stage_batch_keys(kspir_context* c, const kspir_keys* const* keys, int nq) {
 // queries against dbs[s], keys[q] per query), fills segs, carves the
 // workspace (per-query buffers sized for the deepest database) and stages
 // the key table
+// blocks per chunk of a batch spanning databases: the context's chunk size,
+// widened to whole 16-row MFMA tiles, at most the largest database's r
+int multi_chunk_blocks(const kspir_context* c, int rmax, int n2) {
+    int cb = c->chunk_blocks > 0 && c->chunk_blocks < rmax ? c->chunk_blocks : rmax;
+    while ((cb * n2) % 16) cb *= 2;
+    return cb > rmax ? rmax : cb;
+}
+
+// Device bytes of a batch of nq queries spanning databases of N1 n1 whose
+// largest has rmax blocks: the workspace carved for that shape at the total
+// count, in chunks, and the query limbs of one sweep
+size_t multi_footprint_bytes(kspir_context* c, int rmax, int n1, int nq) {
+    const int n2 = kN / 2 / n1;
+    const int nqc = nq < mfma_sweep_queries(c, n1) ? nq : mfma_sweep_queries(c, n1);
+    return layout_workspace(c, rmax, n1, nq, nullptr, false, multi_chunk_blocks(c, rmax, n2)) * 4 +
+           (size_t)kN * 2 * 4 * ((2 * nqc + 15) / 16 * 16) * n1;
+}
+
+// whether the context's options run batches spanning databases
+bool multi_options(const kspir_context* c) {
+    return c->mfma_direct && c->giant_dma && c->giant_wgs_per_cu == 1 && c->ext_fused;
+}
+
 int prepare_multi(kspir_context* c, int nseg, const kspir_database* const* dbs, const int32_t* seg_nq,
                   const kspir_keys* const* keys, std::vector<Seg>& segs, std::vector<const void*>& sig, int* nq_out) {
     if (!c || !dbs || !seg_nq || !keys) return fail(KSPIR_EINVAL, "null argument");
     if (nseg < 1 || nseg > kMaxSeg)
         return fail(KSPIR_EINVAL, "a batch spans 1 to " + std::to_string(kMaxSeg) + " databases");
-    if (!c->mfma_direct || !c->giant_dma || c->giant_wgs_per_cu != 1 || !c->ext_fused)
+    if (!multi_options(c))
         return fail(KSPIR_EINVAL, "a batch spanning databases runs the chunked int8-MFMA pipeline: options mfma_direct, "
                                   "giant_dma, ext_fused on and giant_wgs_per_cu 1");
     segs.clear();
@@ -1029,9 +1054,7 @@ int prepare_multi(kspir_context* c, int nseg, const kspir_database* const* dbs, 
         rmax = db->r > rmax ? db->r : rmax;
     }
     const int n1 = dbs[0]->n1, n2 = dbs[0]->n2;
-    int cb = c->chunk_blocks > 0 && c->chunk_blocks < rmax ? c->chunk_blocks : rmax;
-    while ((cb * n2) % 16) cb *= 2;
-    if (cb > rmax) cb = rmax;
+    const int cb = multi_chunk_blocks(c, rmax, n2);
     HIP_OK(hipSetDevice(c->device));
     int rc = carve_workspace_cb(c, rmax, n1, nq, cb);
     if (rc) return rc;
@@ -1810,6 +1833,10 @@ int kspir_context_set(kspir_context* c, const char* key, int64_t value) {
         if (value != 2 && value != 4 && value != 8 && value != 16)
             return fail(KSPIR_EINVAL, "db_waves must be 2, 4, 8 or 16");
         c->db_waves = (int)value;
+    } else if (k == "serve_multi") {
+        c->serve_multi = value != 0;
+    } else if (k == "serve_graphs") {
+        c->serve_graphs = value != 0;
     } else {
         return fail(KSPIR_EINVAL, "unknown option " + k);
     }
@@ -1828,6 +1855,8 @@ int kspir_context_get(kspir_context* c, const char* key, int64_t* value) {
     else if (k == "chunk_blocks") *value = c->chunk_blocks;
     else if (k == "graphs") *value = c->use_graphs;
     else if (k == "db_waves") *value = c->db_waves;
+    else if (k == "serve_multi") *value = c->serve_multi;
+    else if (k == "serve_graphs") *value = c->serve_graphs;
     else if (k == "giant_wgs_per_cu") *value = c->giant_wgs_per_cu;
     else if (k == "giant_dma") *value = c->giant_dma;
     else if (k == "pack_fused_min_wgs") *value = c->pack_fused_min_wgs;
@@ -2022,35 +2051,80 @@ struct kspir_server {
     std::deque<Req> pending;
     std::unordered_map<uint64_t, Done> done;
     std::unordered_set<uint64_t> live;   // submitted and not yet collected by a wait
-    uint64_t next_id = 1, batches = 0, answered = 0;
+    uint64_t next_id = 1, batches = 0, answered = 0, spanning = 0;   // spanning: batches over several databases
     bool stop = false;
     std::vector<Lane> lanes;
+
+    // The next batch, taken from `pending` (lock held): the database of the
+    // oldest waiting query and as many of its queries, in arrival order, as
+    // the batch cap and the lane's workspace budget admit.  With the lane's
+    // option serve_multi, waiting queries of other databases of the same N1
+    // and at most as many blocks follow in arrival order (at most kMaxSeg
+    // databases per batch), each admitted while the batch's footprint at its
+    // total count fits the budget; such a batch is answered by one launch per
+    // stage (kspir_answer_multi).  Only smaller databases join: the batch
+    // then takes about as long as the first database's queries alone would,
+    // while a query of a larger database would make every query of the batch
+    // wait for its longer giant steps.  take is grouped by database, in the
+    // order of dbs / seg_n.
+    void pick(Lane& L, std::vector<Req>& take, std::vector<const kspir_database*>& dbs, std::vector<int32_t>& seg_n) {
+        take.clear();
+        dbs.clear();
+        seg_n.clear();
+        const kspir_database* db = pending.front().db;
+        // the largest batch whose device footprint fits the lane's budget
+        size_t cap = (size_t)max_batch;
+        while (cap > 1 && batch_footprint_bytes(L.ctx, db->r, db->n1, (int)cap) > budget) --cap;
+        for (auto it = pending.begin(); it != pending.end() && take.size() < cap;) {
+            if (it->db == db) {
+                take.push_back(*it);
+                it = pending.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        dbs.push_back(db);
+        seg_n.push_back((int32_t)take.size());
+        if (!L.ctx->serve_multi || !multi_options(L.ctx) || !mfma_shape(db)) return;
+        std::vector<std::vector<Req>> more;
+        int total = (int)take.size();
+        for (auto it = pending.begin(); it != pending.end() && total < max_batch;) {
+            const kspir_database* d = it->db;
+            size_t k = 0;
+            while (k < dbs.size() && dbs[k] != d) ++k;
+            const bool fits = d->n1 == db->n1 && d->r <= db->r && mfma_shape(d) &&
+                              (k < dbs.size() || (int)dbs.size() < kMaxSeg) &&
+                              multi_footprint_bytes(L.ctx, db->r, db->n1, total + 1) <= budget;
+            // the first database's queries left behind were cut by the cap
+            if (!fits || k == 0) {
+                ++it;
+                continue;
+            }
+            if (k == dbs.size()) {
+                dbs.push_back(d);
+                seg_n.push_back(0);
+                more.emplace_back();
+            }
+            more[k - 1].push_back(*it);
+            ++seg_n[k];
+            ++total;
+            it = pending.erase(it);
+        }
+        for (auto& m : more) take.insert(take.end(), m.begin(), m.end());
+    }
 
     void run(Lane& L) {
         std::vector<Req> take;
         std::vector<const kspir_keys*> keys;
+        std::vector<const kspir_database*> dbs;
+        std::vector<int32_t> seg_n;
         for (;;) {
             bool more = false;
             {
                 std::unique_lock<std::mutex> lock(mu);
                 cv_work.wait(lock, [&] { return stop || !pending.empty(); });
                 if (pending.empty()) return;   // stop requested and drained
-                // continuous batching: the database of the oldest waiting
-                // query, and as many of its queries (in arrival order) as the
-                // batch cap and the lane's workspace budget admit
-                const kspir_database* db = pending.front().db;
-                // the largest batch whose device footprint fits the lane's budget
-                size_t cap = (size_t)max_batch;
-                while (cap > 1 && batch_footprint_bytes(L.ctx, db->r, db->n1, (int)cap) > budget) --cap;
-                take.clear();
-                for (auto it = pending.begin(); it != pending.end() && take.size() < cap;) {
-                    if (it->db == db) {
-                        take.push_back(*it);
-                        it = pending.erase(it);
-                    } else {
-                        ++it;
-                    }
-                }
+                pick(L, take, dbs, seg_n);
                 more = !pending.empty();
             }
             if (more) cv_work.notify_one();   // another lane may take the rest
@@ -2060,7 +2134,17 @@ struct kspir_server {
                 std::memcpy(L.qh + (size_t)i * 4 * kN, take[i].query, 4 * kN * 8);
                 keys[i] = take[i].keys;
             }
-            const int rc = kspir_answer_batch(L.ctx, take[0].db, keys.data(), nq, L.qh, L.rh);
+            // batch sizes vary from batch to batch: without serve_graphs
+            // each batch is launched eagerly, since capturing and
+            // instantiating a graph for every new size costs more than the
+            // launches it saves (and now and then takes about a second)
+            const int graphs = L.ctx->use_graphs;
+            L.ctx->use_graphs = graphs && L.ctx->serve_graphs;
+            const int rc = dbs.size() == 1
+                               ? kspir_answer_batch(L.ctx, dbs[0], keys.data(), nq, L.qh, L.rh)
+                               : kspir_answer_multi(L.ctx, (int32_t)dbs.size(), dbs.data(), seg_n.data(), keys.data(),
+                                                    L.qh, L.rh);
+            L.ctx->use_graphs = graphs;
             const auto t1 = std::chrono::steady_clock::now();
             {
                 std::lock_guard<std::mutex> lock(mu);
@@ -2069,6 +2153,7 @@ struct kspir_server {
                     done[take[i].id] = Done{rc, std::chrono::duration<double, std::milli>(t1 - take[i].t0).count()};
                 }
                 ++batches;
+                if (dbs.size() > 1) ++spanning;
                 answered += (uint64_t)nq;
             }
             cv_done.notify_all();
@@ -2603,6 +2688,8 @@ int kspir_server_create_lanes(kspir_context* c, int32_t max_batch, size_t worksp
             L.ctx->giant_dma = c->giant_dma;
             L.ctx->use_graphs = c->use_graphs;
             L.ctx->db_waves = c->db_waves;
+            L.ctx->serve_multi = c->serve_multi;
+            L.ctx->serve_graphs = c->serve_graphs;
         }
         hipError_t e = hipHostMalloc(&L.qh, (size_t)max_batch * 4 * kN * 8, hipHostMallocDefault);
         if (e == hipSuccess) e = hipHostMalloc(&L.rh, (size_t)max_batch * 2 * kN * 8, hipHostMallocDefault);
@@ -2660,6 +2747,13 @@ int kspir_server_stats(kspir_server* s, uint64_t* batches, uint64_t* answered, u
     if (batches) *batches = s->batches;
     if (answered) *answered = s->answered;
     if (pending) *pending = s->pending.size();
+    return KSPIR_OK;
+}
+
+int kspir_server_spanning_batches(kspir_server* s, uint64_t* spanning) {
+    if (!s || !spanning) return fail(KSPIR_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lock(s->mu);
+    *spanning = s->spanning;
     return KSPIR_OK;
 }
 

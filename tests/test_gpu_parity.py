@@ -352,6 +352,61 @@ def test_serving_loop_mixed_databases(ctx, oracle, lanes):
             k.close()
 
 
+@pytest.mark.parametrize("lanes", [1, 2])
+def test_serving_loop_batches_span_databases(ctx, oracle, lanes):
+    """queries for three databases of one N1 (r 2, 8, 4) submitted interleaved
+    to the serving loop come back equal to the same queries answered alone
+    (and to the oracle for one of them), with at least one batch spanning
+    databases (one launch per stage: once the oldest waiting query is one
+    of the r=8 database's, the smaller databases' queries join its batch);
+    the same submissions with the option serve_multi off never span"""
+    import kspir_amd
+    say(f"[gpu] serving loop, batches spanning three databases, lanes={lanes}")
+    n1 = 128
+    perms = oracle.perm_matrix(n1)
+    dbs, clients, want = [], [], []
+    for g, (r, seed) in enumerate(((2, 71), (8, 72), (4, 73))):
+        dbw = oracle.gen_db(r, n1, seed=seed)
+        db = ctx.database(dbw, r, n1)
+        cl = [oracle.gen_client(r, n1, seed * 10 + c, (29 * c + 5) % (N // 2), c % r) for c in range(4)]
+        ks = [ctx.keys(r, n1, x["baby"], x["giant"], x["packing"], x["rgsw"], perms) for x in cl]
+        dbs.append((db, dbw, r))
+        clients.append((cl, ks))
+        want.append([ctx.answer(db, k, x["query"]) for x, k in zip(cl, ks)])
+    db1, dbw1, r1 = dbs[1]
+    x = clients[1][0][3]
+    w = oracle.answer(r=r1, n1=n1, db=dbw1, query=x["query"], rgsw=x["rgsw"], baby=x["baby"], giant=x["giant"],
+                      packing=x["packing"], perms=perms)
+    assert np.array_equal(want[1][3], w)
+    try:
+        for multi in (1, 0):
+            ctx.set("serve_multi", multi)
+            srv = kspir_amd.Server(ctx, max_batch=16, lanes=lanes)
+            try:
+                tickets = []
+                for c in range(4):
+                    for g in range(3):
+                        cl, ks = clients[g]
+                        tickets.append((g, c, srv.submit(dbs[g][0], ks[c], cl[c]["query"])))
+                for g, c, t in tickets:
+                    resp, ms = srv.wait(t)
+                    assert ms > 0
+                    assert np.array_equal(resp, want[g][c]), (multi, g, c)
+                st = srv.stats()
+                say(f"[gpu] serve_multi={multi}: {st}")
+                assert st["answered"] == 12 and st["pending"] == 0
+                assert (st["spanning"] >= 1) if multi else (st["spanning"] == 0)
+            finally:
+                srv.close()
+    finally:
+        ctx.set("serve_multi", 0)
+        for db, *_ in dbs:
+            db.close()
+        for cl, ks in clients:
+            for k in ks:
+                k.close()
+
+
 def test_concurrent_contexts_uneven_databases(ctx, oracle):
     """batches against two databases of uneven size (r 2 and 8), each
     enqueued on a context of its own before either is waited for, so they
