@@ -438,10 +438,12 @@ def config_c5(args, log):
     keys; queries arrive as a Poisson process at each offered rate and go
     through the library's continuous-batching server (kspir_server_*): each
     database alone, with one lane and with two lanes (batches overlapping on
-    the device), then both databases resident together with the arrivals
-    split evenly between them, one lane, with batches spanning the two
-    databases (option serve_multi) and without.  Latency is
-    submit-to-response, measured by the server."""
+    the device), the 16 GB one also with graph capture and replay in the
+    loop (option serve_graphs); then both databases resident together with
+    the arrivals split evenly between them, one lane, with batches spanning
+    the two databases (option serve_multi) and without; then the same over
+    four 256 MB databases.  Latency is submit-to-response, measured by the
+    server."""
     import kspir_amd
     n1 = 128
     ctx = kspir_amd.Context(0)
@@ -450,7 +452,8 @@ def config_c5(args, log):
     out = {"config": "C5 serving: Poisson arrivals, continuous batching (batch cap 128 at 4 GB, 64 at 16 GB; admission "
                      "by each batch's device footprint; 1 lane, and 2 lanes whose batches overlap on the device), "
                      "64 clients with distinct keys per database, resident 4 GB (r=256) and 16 GB (r=1024) databases, "
-                     "N1=128; then both resident with arrivals split evenly, batches spanning the two databases or not",
+                     "N1=128 (16 GB also with graph replay in the loop); then both resident with arrivals split "
+                     "evenly, and four 256 MB databases with arrivals split evenly, batches spanning databases or not",
            "databases": {}}
     held = []
     for r, rates, dur, cap, lane_set in ((256, (200, 500, 1000, 2000), 2.0, 128, (1, 2)),
@@ -467,14 +470,19 @@ def config_c5(args, log):
             keys.append(ctx.keys(r, n1, baby, giant, packing, rgsw, perms))
             queries.append(query)
         by_lanes = {}
-        for lanes in lane_set:
+        # the 16 GB database also with graph capture and replay in the loop
+        # (serve_graphs 1), against the default eager launches
+        runs = [(lanes, 0) for lanes in lane_set] + ([(1, 1)] if r == 1024 else [])
+        for lanes, graphs in runs:
+            ctx.set("serve_graphs", graphs)
             srv = kspir_amd.Server(ctx, max_batch=cap, lanes=lanes)
             # warm-up: full batches on every lane (database limbs, workspaces at full size)
             for t in [srv.submit(db, keys[c % 64], queries[c % 64]) for c in range(cap * lanes)]:
                 srv.wait(t)
-            by_lanes[f"lanes_{lanes}"] = serve_rates(srv, [(db, keys, queries)], rates, dur, r, log,
-                                                     f"r={r} lanes={lanes}")
+            by_lanes[f"lanes_{lanes}" + ("_serve_graphs" if graphs else "")] = serve_rates(
+                srv, [(db, keys, queries)], rates, dur, r, log, f"r={r} lanes={lanes} serve_graphs={graphs}")
             srv.close()
+        ctx.set("serve_graphs", 0)
         out["databases"][f"{r // 64} GB"] = {"r": r, "batch_cap": cap,
                                             "per_query_footprint_MB": round(kspir_amd.query_footprint(r, n1) / 2**20, 1),
                                             **by_lanes}
@@ -492,6 +500,32 @@ def config_c5(args, log):
         ctx.set("serve_multi", 0)
         out["both_resident"] = mixed
     for db, keys, _ in held:
+        for k in keys:
+            k.close()
+        db.close()
+    # four 256 MB databases (r=16) of distinct records, 32 clients each,
+    # arrivals split evenly: batches spanning databases or not
+    four = []
+    for d in range(4):
+        db = ctx.database_from_records(synthetic_records(100 + d), 16, n1, block_stride=0)
+        keys, queries = [], []
+        for c in range(32):
+            query, baby, giant, packing, rgsw = synthetic_client(16, n1, 5000 + 100 * d + c)
+            keys.append(ctx.keys(16, n1, baby, giant, packing, rgsw, perms))
+            queries.append(query)
+        four.append((db, keys, queries))
+    eq = {"share": [0.25] * 4, "batch_cap": 64}
+    for multi in (1, 0):
+        ctx.set("serve_multi", multi)
+        srv = kspir_amd.Server(ctx, max_batch=64, lanes=1)
+        for t in [srv.submit(four[c % 4][0], four[c % 4][1][c % 32], four[c % 4][2][c % 32]) for c in range(128)]:
+            srv.wait(t)
+        eq[f"serve_multi_{multi}"] = serve_rates(srv, four, (4000, 12000), 2.0, 16, log,
+                                                 f"four 256 MB databases serve_multi={multi}", share=[0.25] * 4)
+        srv.close()
+    ctx.set("serve_multi", 0)
+    out["four_256MB_databases"] = eq
+    for db, keys, _ in four:
         for k in keys:
             k.close()
         db.close()

@@ -67,9 +67,9 @@ int kspir_device_count(int* count);
  *                     batch, which measured lower latency at equal
  *                     throughput on mixed traffic
  *   "serve_graphs"    0 (default): a serving loop launches each batch
- *                     eagerly (its batch sizes vary, and a graph captured for
- *                     each new size costs more than it saves); 1: it captures
- *                     and replays graphs as the answer calls above do
+ *                     eagerly (its batch sizes vary, so most batches would be
+ *                     a new capture); 1: it captures and replays graphs as
+ *                     the answer calls above do
  * Setting an option waits for the context stream and drops captured graphs. */
 int kspir_context_set(kspir_context* ctx, const char* key, int64_t value);
 int kspir_context_get(kspir_context* ctx, const char* key, int64_t* value);

@@ -293,6 +293,19 @@ int build_tables(kspir_context* c) {
 }
 
 
+// (query, block) pairs that keep the GPU full with one fused giant-step
+// workgroup each (eight rounds of one per CU)
+constexpr int64_t kFusedFillWgs = 2048;
+
+// whether a batch of nq queries over r blocks takes the fused giant-step
+// kernel: from fused_min_batch queries, or whenever its (query, block)
+// pairs fill the GPU.  The unfused kernels keep every block's giant rows
+// (gcoef, gks) per query, 2.6 GiB at r = 1024, so a large database takes the
+// fused path even for a few queries (r 1024 from 2 queries, r 256 from 8).
+bool fused_batch(const kspir_context* c, int r, int nq) {
+    return nq >= c->fused_min_batch || (int64_t)r * nq >= kFusedFillWgs;
+}
+
 // Blocks per chunk of the batched pipeline for nq queries of shape (r, n1),
 // or 0 when that batch takes the unchunked path.  A chunked batch runs the
 // MFMA database pass, the giant steps and the external products one chunk of
@@ -302,7 +315,7 @@ int build_tables(kspir_context* c) {
 int chunk_blocks_for(const kspir_context* c, int r, int n1, int nq) {
     const int n2 = kN / 2 / n1;
     const bool mfma = n1 % 64 == 0 && n1 <= 256 && (r * n2) % 16 == 0;
-    if (!mfma || nq < c->mfma_min_batch || nq < c->fused_min_batch || !c->giant_dma || c->giant_wgs_per_cu != 1 ||
+    if (!mfma || nq < c->mfma_min_batch || !fused_batch(c, r, nq) || !c->giant_dma || c->giant_wgs_per_cu != 1 ||
         !c->ext_fused)
         return 0;
     int cb = c->chunk_blocks > 0 && c->chunk_blocks < r ? c->chunk_blocks : r;
@@ -642,7 +655,7 @@ int enqueue_packing(kspir_context* c, const KeyPtrs* kp, int r, int nq, hipStrea
         const int li = ilog2(i);
         const uint32_t g = (uint32_t)(kN / i + 1);
         const uint32_t ginv = (uint32_t)powmod(g, kN - 1, 2 * kN);   // odd g has order | N in (Z/2N)*
-        if (c->fused_pack && nq >= c->fused_min_batch && (int64_t)i * nq >= c->pack_fused_min_wgs) {
+        if (c->fused_pack && (c->cur_cb > 0 || nq >= c->fused_min_batch) && (int64_t)i * nq >= c->pack_fused_min_wgs) {
             hipLaunchKernelGGL(k_pack_level, dim3(i * nq), dim3(512), 0, st, cur, kp, li, nxt, i, g, ginv, S, T,
                                PackSegs{});
             LAUNCH_OK("k_pack_level");
@@ -834,7 +847,7 @@ int enqueue_answer(kspir_context* c, const kspir_database* db, const KeyPtrs* kp
     // database pass (crt.cpp:643-768): int8-MFMA limb GEMM for a batch,
     // u64-accumulating VALU stream for a single query
     if (c->cur_cb > 0) {
-        if (!allow_mfma || first_dim_only || nq < c->fused_min_batch)
+        if (!allow_mfma || first_dim_only)
             return fail(KSPIR_EINVAL, "internal: chunked workspace for an unchunked pipeline");
         return enqueue_chunked(c, db, kp, nq, resp, st);
     }
@@ -2083,6 +2096,13 @@ struct kspir_server {
                 ++it;
             }
         }
+        // the footprint is not monotonic in the batch size (a few queries of
+        // a small database take the unchunked path, with every block's rows
+        // per query): the batch actually taken must fit too
+        while (take.size() > 1 && batch_footprint_bytes(L.ctx, db->r, db->n1, (int)take.size()) > budget) {
+            pending.push_front(take.back());
+            take.pop_back();
+        }
         dbs.push_back(db);
         seg_n.push_back((int32_t)take.size());
         if (!L.ctx->serve_multi || !multi_options(L.ctx) || !mfma_shape(db)) return;
@@ -2134,10 +2154,9 @@ struct kspir_server {
                 std::memcpy(L.qh + (size_t)i * 4 * kN, take[i].query, 4 * kN * 8);
                 keys[i] = take[i].keys;
             }
-            // batch sizes vary from batch to batch: without serve_graphs
-            // each batch is launched eagerly, since capturing and
-            // instantiating a graph for every new size costs more than the
-            // launches it saves (and now and then takes about a second)
+            // batch sizes vary from batch to batch, so that most batches
+            // would be a new capture: without serve_graphs each batch is
+            // launched eagerly (the two measure alike at 4 and 16 GB)
             const int graphs = L.ctx->use_graphs;
             L.ctx->use_graphs = graphs && L.ctx->serve_graphs;
             const int rc = dbs.size() == 1

@@ -230,12 +230,14 @@ def test_batch_mfma_direct_matches_tile_major(ctx, oracle, r, n1, nq):
     db = ctx.database(db_words, r, n1)
     keys = [ctx.keys(r, n1, cl["baby"], cl["giant"], cl["packing"], cl["rgsw"], perms) for cl in clients]
     queries = np.stack([cl["query"] for cl in clients])
-    tile = ctx.answer_batch(db, keys, queries)
-    ctx.set("mfma_direct", 1)
+    prev = ctx.get("mfma_direct")
     try:
+        ctx.set("mfma_direct", 0)
+        tile = ctx.answer_batch(db, keys, queries)
+        ctx.set("mfma_direct", 1)
         direct = ctx.answer_batch(db, keys, queries)
     finally:
-        ctx.set("mfma_direct", 0)
+        ctx.set("mfma_direct", prev)
     assert np.array_equal(direct, tile)
     want0 = oracle.answer(r=r, n1=n1, db=db_words, query=clients[0]["query"], rgsw=clients[0]["rgsw"],
                           baby=clients[0]["baby"], giant=clients[0]["giant"], packing=clients[0]["packing"],
@@ -649,14 +651,16 @@ def test_client_generation_matches_oracle(ctx, oracle, r, n1, seeds):
         db.close()
 
 
-@pytest.mark.parametrize("r,nq", [(64, 16), (256, 2), (1024, 1)])
+@pytest.mark.parametrize("r,nq", [(64, 16), (256, 2), (256, 8), (1024, 2)])
 def test_full_size_round_trip(ctx, oracle, r, nq):
     """1 GB (r 64), 4 GB (r 256) and 16 GB (r 1024) databases preprocessed on the GPU from one
     planted record block repeated r times, clients generated on the GPU,
     answered one by one (VALU database pass) and as one batch (int8-MFMA pass;
-    at r 64 with 16 queries also the fused giant-step and packing kernels),
-    recovered on the GPU: the batch equals the single answers word for word
-    and every client recovers its planted message"""
+    at r 64 with 16 queries, r 256 with 8 and r 1024 with 2 the chunked
+    pipeline with the fused giant-step and packing kernels, which a large
+    database takes from a few queries on), recovered on the GPU: the batch
+    equals the single answers word for word and every client recovers its
+    planted message"""
     from tests.planted import planted_block, planted_message
     n1 = 128
     say(f"[gpu] full-size round trip r={r} ({r * 16} MB), {nq} clients")
