@@ -1012,10 +1012,6 @@ int stage_batch_keys(kspir_context* c, const kspir_keys* const* keys, int nq) {
     return KSPIR_OK;
 }
 
-// checks a batch spanning databases (segment s: seg_nq[s] consecutive
-// queries against dbs[s], keys[q] per query), fills segs, carves the
-// workspace (per-query buffers sized for the deepest database) and stages
-// the key table
 // blocks per chunk of a batch spanning databases: the context's chunk size,
 // widened to whole 16-row MFMA tiles, at most the largest database's r
 int multi_chunk_blocks(const kspir_context* c, int rmax, int n2) {
@@ -1039,6 +1035,10 @@ bool multi_options(const kspir_context* c) {
     return c->mfma_direct && c->giant_dma && c->giant_wgs_per_cu == 1 && c->ext_fused;
 }
 
+// checks a batch spanning databases (segment s: seg_nq[s] consecutive
+// queries against dbs[s], keys[q] per query), fills segs, carves the
+// workspace (per-query buffers sized for the deepest database) and stages
+// the key table
 int prepare_multi(kspir_context* c, int nseg, const kspir_database* const* dbs, const int32_t* seg_nq,
                   const kspir_keys* const* keys, std::vector<Seg>& segs, std::vector<const void*>& sig, int* nq_out) {
     if (!c || !dbs || !seg_nq || !keys) return fail(KSPIR_EINVAL, "null argument");
