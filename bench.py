@@ -449,15 +449,16 @@ def config_c5(args, log):
     ctx = kspir_amd.Context(0)
     perms = kspir_amd.compute_permutation_matrix(n1)
     rec = synthetic_records()
-    out = {"config": "C5 serving: Poisson arrivals, continuous batching (batch cap 128 at 4 GB, 64 at 16 GB; admission "
+    out = {"config": "C5 serving: Poisson arrivals (5 s per offered rate per database alone, 3 s in the mixed runs), "
+                     "continuous batching (batch cap 128 at 4 GB, 64 at 16 GB; admission "
                      "by each batch's device footprint; 1 lane, and 2 lanes whose batches overlap on the device), "
                      "64 clients with distinct keys per database, resident 4 GB (r=256) and 16 GB (r=1024) databases, "
                      "N1=128 (16 GB also with graph replay in the loop); then both resident with arrivals split "
                      "evenly, and four 256 MB databases with arrivals split evenly, batches spanning databases or not",
            "databases": {}}
     held = []
-    for r, rates, dur, cap, lane_set in ((256, (200, 500, 1000, 2000), 2.0, 128, (1, 2)),
-                                         (1024, (200, 500), 2.0, 64, (1,))):
+    for r, rates, dur, cap, lane_set in ((256, (200, 500, 1000, 2000), 5.0, 128, (1, 2)),
+                                         (1024, (200, 500), 5.0, 64, (1,))):
         try:
             db = ctx.database_from_records(rec, r, n1, block_stride=0)
         except kspir_amd.KspirError as e:
@@ -494,7 +495,7 @@ def config_c5(args, log):
             srv = kspir_amd.Server(ctx, max_batch=64, lanes=1)
             for t in [srv.submit(held[c % 2][0], held[c % 2][1][c % 64], held[c % 2][2][c % 64]) for c in range(64)]:
                 srv.wait(t)
-            mixed[f"serve_multi_{multi}"] = serve_rates(srv, held, (200, 500), 2.0, 77, log,
+            mixed[f"serve_multi_{multi}"] = serve_rates(srv, held, (200, 500), 3.0, 77, log,
                                                         f"both databases serve_multi={multi}", share=[0.5, 0.5])
             srv.close()
         ctx.set("serve_multi", 0)
@@ -520,7 +521,7 @@ def config_c5(args, log):
         srv = kspir_amd.Server(ctx, max_batch=64, lanes=1)
         for t in [srv.submit(four[c % 4][0], four[c % 4][1][c % 32], four[c % 4][2][c % 32]) for c in range(128)]:
             srv.wait(t)
-        eq[f"serve_multi_{multi}"] = serve_rates(srv, four, (4000, 12000), 2.0, 16, log,
+        eq[f"serve_multi_{multi}"] = serve_rates(srv, four, (4000, 12000), 3.0, 16, log,
                                                  f"four 256 MB databases serve_multi={multi}", share=[0.25] * 4)
         srv.close()
     ctx.set("serve_multi", 0)
