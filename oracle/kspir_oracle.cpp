@@ -17,6 +17,9 @@
 //     ComputeInverse is the exact inverse; outputs fully reduced.
 //   * NTT(N, q) uses MinimalPrimitiveRoot(2N, q): the smallest primitive
 //     2N-th root of unity mod q.
+//   * Where the CPU has AVX-512F/DQ the NTT runs 8 lanes at a time, as
+//     HEXL's AVX-512 path does; the lane form gives the scalar form's words
+//     (ko_ntt_set_vector switches between them for that check).
 //   * EltwiseAddMod: s = x + y, s >= q ? s - q : s.
 //     EltwiseSubMod: x >= y ? x - y : x + q - y.
 //     EltwiseMultMod / EltwiseFMAMod: (x*y [+ z]) mod q, fully reduced.
@@ -139,6 +142,44 @@ u64 min_root(u64 q) {
     throw std::runtime_error("no primitive root");
 }
 
+// 8 lanes of the scalar arithmetic above, for moduli below 2^62: the high
+// 64 bits of a 64 x 64 product from four 32 x 32 products (AVX-512 has no
+// 64-bit high multiply), Shoup's product as mul_shoup, and the corrections
+// r >= q ? r - q : r written as an unsigned min of r and r - q (which wraps
+// above r exactly when r < q).  Same words as the scalar code, lane by lane.
+#define KO_AVX512 __attribute__((target("avx512f,avx512dq")))
+KO_AVX512 inline __m512i v_mulhi64(__m512i a, __m512i b) {
+    const __m512i lo32 = _mm512_set1_epi64(0xffffffffLL);
+    const __m512i a1 = _mm512_srli_epi64(a, 32), b1 = _mm512_srli_epi64(b, 32);
+    const __m512i p00 = _mm512_mul_epu32(a, b), p01 = _mm512_mul_epu32(a, b1);
+    const __m512i p10 = _mm512_mul_epu32(a1, b), p11 = _mm512_mul_epu32(a1, b1);
+    __m512i mid = _mm512_add_epi64(_mm512_srli_epi64(p00, 32), _mm512_and_si512(p01, lo32));
+    mid = _mm512_add_epi64(mid, _mm512_and_si512(p10, lo32));
+    __m512i hi = _mm512_add_epi64(p11, _mm512_srli_epi64(p01, 32));
+    hi = _mm512_add_epi64(hi, _mm512_srli_epi64(p10, 32));
+    return _mm512_add_epi64(hi, _mm512_srli_epi64(mid, 32));
+}
+KO_AVX512 inline __m512i v_mul_shoup(__m512i x, __m512i w, __m512i wp, __m512i q) {
+    const __m512i qh = v_mulhi64(x, wp);
+    const __m512i r = _mm512_sub_epi64(_mm512_mullo_epi64(x, w), _mm512_mullo_epi64(qh, q));
+    return _mm512_min_epu64(r, _mm512_sub_epi64(r, q));
+}
+KO_AVX512 inline __m512i v_add(__m512i a, __m512i b, __m512i q) {
+    const __m512i s = _mm512_add_epi64(a, b);
+    return _mm512_min_epu64(s, _mm512_sub_epi64(s, q));
+}
+KO_AVX512 inline __m512i v_sub(__m512i a, __m512i b, __m512i q) {
+    const __m512i d = _mm512_sub_epi64(a, b);
+    return _mm512_min_epu64(d, _mm512_add_epi64(d, q));
+}
+
+// the NTT's lane form; off only to compare it with the scalar one
+bool g_ntt_vector = true;
+bool ntt_vector_usable() {
+    static const bool ok = __builtin_cpu_supports("avx512f") && __builtin_cpu_supports("avx512dq");
+    return ok;
+}
+
 // Negacyclic NTT with HEXL conventions (see header comment).
 struct Ntt {
     u64 q = 0, root = 0;
@@ -158,6 +199,14 @@ struct Ntt {
         ninvp = shoup_pre(ninv, q);
     }
     void forward(u64* a) const {
+        if (g_ntt_vector && ntt_vector_usable()) forward_v(a);
+        else forward_scalar(a);
+    }
+    void inverse(u64* a) const {
+        if (g_ntt_vector && ntt_vector_usable()) inverse_v(a);
+        else inverse_scalar(a);
+    }
+    void forward_scalar(u64* a) const {
         for (int i = 0; i < N; ++i) a[i] %= q;   // a value equal to q enters as 0
         int t = N;
         for (int m = 1; m < N; m <<= 1) {
@@ -174,7 +223,7 @@ struct Ntt {
             }
         }
     }
-    void inverse(u64* a) const {
+    void inverse_scalar(u64* a) const {
         for (int i = 0; i < N; ++i) a[i] %= q;
         int t = 1;
         for (int m = N / 2; m >= 1; m >>= 1) {
@@ -191,6 +240,121 @@ struct Ntt {
             t <<= 1;
         }
         for (int i = 0; i < N; ++i) a[i] = mul_shoup(a[i], ninv, ninvp, q);
+    }
+
+    // The same butterflies 8 lanes at a time.  Stages with t >= 8 take x and
+    // y as whole vectors; for t = 4, 2, 1 two vectors (16 words, 16 / 2t
+    // groups) are split into their x and y halves by a two-source permute,
+    // each lane given its group's twiddle, and put back by the inverse
+    // permute.
+    struct Lanes { __m512i x, y, back0, back1, tw; };
+    KO_AVX512 static Lanes lanes(int t) {
+        // x / y positions within 16 words, and the inverse permutes
+        Lanes L;
+        if (t == 4) {
+            L.x = _mm512_setr_epi64(0, 1, 2, 3, 8, 9, 10, 11);
+            L.y = _mm512_setr_epi64(4, 5, 6, 7, 12, 13, 14, 15);
+            L.back0 = _mm512_setr_epi64(0, 1, 2, 3, 8, 9, 10, 11);
+            L.back1 = _mm512_setr_epi64(4, 5, 6, 7, 12, 13, 14, 15);
+            L.tw = _mm512_setr_epi64(0, 0, 0, 0, 1, 1, 1, 1);
+        } else if (t == 2) {
+            L.x = _mm512_setr_epi64(0, 1, 4, 5, 8, 9, 12, 13);
+            L.y = _mm512_setr_epi64(2, 3, 6, 7, 10, 11, 14, 15);
+            L.back0 = _mm512_setr_epi64(0, 1, 8, 9, 2, 3, 10, 11);
+            L.back1 = _mm512_setr_epi64(4, 5, 12, 13, 6, 7, 14, 15);
+            L.tw = _mm512_setr_epi64(0, 0, 1, 1, 2, 2, 3, 3);
+        } else {
+            L.x = _mm512_setr_epi64(0, 2, 4, 6, 8, 10, 12, 14);
+            L.y = _mm512_setr_epi64(1, 3, 5, 7, 9, 11, 13, 15);
+            L.back0 = _mm512_setr_epi64(0, 8, 1, 9, 2, 10, 3, 11);
+            L.back1 = _mm512_setr_epi64(4, 12, 5, 13, 6, 14, 7, 15);
+            L.tw = _mm512_setr_epi64(0, 1, 2, 3, 4, 5, 6, 7);
+        }
+        return L;
+    }
+    // words >= q reduced as the scalar pre-pass does
+    KO_AVX512 void reduce_input(u64* a) const {
+        const __m512i vq = _mm512_set1_epi64((long long)q);
+        for (int i = 0; i < N; i += 8) {
+            const __m512i v = _mm512_loadu_si512(a + i);
+            if (_mm512_cmpge_epu64_mask(v, vq))
+                for (int j = i; j < i + 8; ++j) a[j] %= q;
+        }
+    }
+    KO_AVX512 void forward_v(u64* a) const {
+        reduce_input(a);
+        const __m512i vq = _mm512_set1_epi64((long long)q);
+        int t = N;
+        for (int m = 1; m < N; m <<= 1) {
+            t >>= 1;
+            if (t >= 8) {
+                for (int i = 0; i < m; ++i) {
+                    const __m512i w = _mm512_set1_epi64((long long)fw[m + i]);
+                    const __m512i wp = _mm512_set1_epi64((long long)fwp[m + i]);
+                    u64* x = a + 2 * i * t;
+                    u64* y = x + t;
+                    for (int j = 0; j < t; j += 8) {
+                        const __m512i u = _mm512_loadu_si512(x + j);
+                        const __m512i v = v_mul_shoup(_mm512_loadu_si512(y + j), w, wp, vq);
+                        _mm512_storeu_si512(x + j, v_add(u, v, vq));
+                        _mm512_storeu_si512(y + j, v_sub(u, v, vq));
+                    }
+                }
+            } else {
+                const Lanes L = lanes(t);
+                const int per = 16 / (2 * t);   // groups per 16 words
+                for (int b = 0; b < N; b += 16) {
+                    const __m512i A = _mm512_loadu_si512(a + b), B = _mm512_loadu_si512(a + b + 8);
+                    const int g0 = m + b / (2 * t);
+                    const __m512i w = _mm512_permutexvar_epi64(L.tw, _mm512_maskz_loadu_epi64((1u << per) - 1, &fw[g0]));
+                    const __m512i wp =
+                        _mm512_permutexvar_epi64(L.tw, _mm512_maskz_loadu_epi64((1u << per) - 1, &fwp[g0]));
+                    const __m512i u = _mm512_permutex2var_epi64(A, L.x, B);
+                    const __m512i v = v_mul_shoup(_mm512_permutex2var_epi64(A, L.y, B), w, wp, vq);
+                    const __m512i X = v_add(u, v, vq), Y = v_sub(u, v, vq);
+                    _mm512_storeu_si512(a + b, _mm512_permutex2var_epi64(X, L.back0, Y));
+                    _mm512_storeu_si512(a + b + 8, _mm512_permutex2var_epi64(X, L.back1, Y));
+                }
+            }
+        }
+    }
+    KO_AVX512 void inverse_v(u64* a) const {
+        reduce_input(a);
+        const __m512i vq = _mm512_set1_epi64((long long)q);
+        int t = 1;
+        for (int m = N / 2; m >= 1; m >>= 1) {
+            if (t >= 8) {
+                for (int i = 0; i < m; ++i) {
+                    const __m512i w = _mm512_set1_epi64((long long)iv[m + i]);
+                    const __m512i wp = _mm512_set1_epi64((long long)ivp[m + i]);
+                    u64* x = a + 2 * i * t;
+                    u64* y = x + t;
+                    for (int j = 0; j < t; j += 8) {
+                        const __m512i u = _mm512_loadu_si512(x + j), v = _mm512_loadu_si512(y + j);
+                        _mm512_storeu_si512(x + j, v_add(u, v, vq));
+                        _mm512_storeu_si512(y + j, v_mul_shoup(v_sub(u, v, vq), w, wp, vq));
+                    }
+                }
+            } else {
+                const Lanes L = lanes(t);
+                const int per = 16 / (2 * t);
+                for (int b = 0; b < N; b += 16) {
+                    const __m512i A = _mm512_loadu_si512(a + b), B = _mm512_loadu_si512(a + b + 8);
+                    const int g0 = m + b / (2 * t);
+                    const __m512i w = _mm512_permutexvar_epi64(L.tw, _mm512_maskz_loadu_epi64((1u << per) - 1, &iv[g0]));
+                    const __m512i wp =
+                        _mm512_permutexvar_epi64(L.tw, _mm512_maskz_loadu_epi64((1u << per) - 1, &ivp[g0]));
+                    const __m512i u = _mm512_permutex2var_epi64(A, L.x, B), v = _mm512_permutex2var_epi64(A, L.y, B);
+                    const __m512i X = v_add(u, v, vq), Y = v_mul_shoup(v_sub(u, v, vq), w, wp, vq);
+                    _mm512_storeu_si512(a + b, _mm512_permutex2var_epi64(X, L.back0, Y));
+                    _mm512_storeu_si512(a + b + 8, _mm512_permutex2var_epi64(X, L.back1, Y));
+                }
+            }
+            t <<= 1;
+        }
+        const __m512i ni = _mm512_set1_epi64((long long)ninv), nip = _mm512_set1_epi64((long long)ninvp);
+        for (int i = 0; i < N; i += 8)
+            _mm512_storeu_si512(a + i, v_mul_shoup(_mm512_loadu_si512(a + i), ni, nip, vq));
     }
 };
 
@@ -1320,6 +1484,10 @@ static const Ntt& ntt_sel(int modsel) {
 }
 
 void ko_ntt_forward(int modsel, uint64_t* a) { ntt_sel(modsel).forward(a); }
+int ko_ntt_set_vector(int on) {
+    g_ntt_vector = on != 0;
+    return ntt_vector_usable() ? 1 : 0;
+}
 void ko_ntt_inverse(int modsel, uint64_t* a) { ntt_sel(modsel).inverse(a); }
 uint64_t ko_min_root(uint64_t q) { return min_root(q); }
 uint64_t ko_crt_compose(uint64_t x, uint64_t y) { return crt_compose(x, y); }

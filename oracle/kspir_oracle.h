@@ -35,6 +35,9 @@ void ko_set_threads(int n);     /* OpenMP threads for offline preprocessing and 
 
 /* arithmetic building blocks */
 void ko_ntt_forward(int modsel, uint64_t* a);   /* modsel 0: Q (root crt.h:81), 1: bsMod, 2: p, 3: q1, 4: q2, 5: auxMod */
+/* NTTs 8 lanes at a time with AVX-512 (1, the default) or scalar (0); returns 1
+ * when the CPU has AVX-512F/DQ (otherwise the scalar form runs either way) */
+int ko_ntt_set_vector(int on);
 void ko_ntt_inverse(int modsel, uint64_t* a);
 uint64_t ko_min_root(uint64_t q);               /* minimal primitive 2N-th root of unity mod prime q */
 uint64_t ko_crt_compose(uint64_t x, uint64_t y);            /* crt.cpp:368 */

@@ -291,6 +291,12 @@ def db_product_vectorised():
     return bool(lib().ko_db_product_vectorised())
 
 
+def ntt_set_vector(on):
+    """NTTs 8 lanes at a time with AVX-512 (on) or scalar; True when this
+    host has AVX-512F/DQ (without it the scalar form runs either way)"""
+    return bool(lib().ko_ntt_set_vector(1 if on else 0))
+
+
 def db_product(db_block, cbuf, n1, n2):
     out = np.zeros((n2, 2 * N), dtype=np.uint64)
     lib().ko_db_product(p64(out), p64(np.ascontiguousarray(db_block)), p64(np.ascontiguousarray(cbuf)), n1, n2)

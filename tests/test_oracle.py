@@ -53,23 +53,56 @@ def test_ntt_roots(oracle):
         assert all(pow(x, N, q) != q - 1 for x in range(2, w))
 
 
+@pytest.mark.parametrize("vector", [True, False])
 @pytest.mark.parametrize("modsel,q,root", [(0, 66974689739603969, 3375402822066082),
                                            (1, 16760833, 3689), (2, 65537, 13),
                                            (3, 268369921, 62736), (4, 249561089, 32309)])
-def test_ntt_closed_form(oracle, modsel, q, root):
-    """slot i of ComputeForward holds a(root^(2*bitrev(i)+1)) (utils.cpp:1476-1490)."""
-    rng = np.random.default_rng(modsel)
-    a = rng.integers(0, q, N, dtype=np.uint64)
-    fa = oracle.ntt_forward(modsel, a)
-    ai = [int(v) for v in a]
-    for i in list(range(4)) + [1000, 2048, 4095]:
-        x = pow(root, 2 * bitrev(i) + 1, q)
-        acc, xp = 0, 1
-        for c in ai:
-            acc += c * xp
-            xp = xp * x % q
-        assert int(fa[i]) == acc % q, (modsel, i)
-    assert np.array_equal(oracle.ntt_inverse(modsel, fa), a)
+def test_ntt_closed_form(oracle, modsel, q, root, vector):
+    """slot i of ComputeForward holds a(root^(2*bitrev(i)+1)) (utils.cpp:1476-1490),
+    for the AVX-512 lane form and the scalar form of the oracle's NTT."""
+    if not oracle.ntt_set_vector(vector) and vector:
+        pytest.skip("host without AVX-512: the scalar NTT is the only one")
+    try:
+        rng = np.random.default_rng(modsel)
+        a = rng.integers(0, q, N, dtype=np.uint64)
+        fa = oracle.ntt_forward(modsel, a)
+        ai = [int(v) for v in a]
+        for i in list(range(4)) + [1000, 2048, 4095]:
+            x = pow(root, 2 * bitrev(i) + 1, q)
+            acc, xp = 0, 1
+            for c in ai:
+                acc += c * xp
+                xp = xp * x % q
+            assert int(fa[i]) == acc % q, (modsel, i)
+        assert np.array_equal(oracle.ntt_inverse(modsel, fa), a)
+    finally:
+        oracle.ntt_set_vector(True)
+
+
+def test_ntt_lanes_match_scalar(oracle):
+    """the oracle's AVX-512 NTT (the lane form its parity checks and the CPU
+    baseline run) gives the scalar NTT's words, forward and inverse, for
+    every modulus, on random inputs with 0, q - 1 and the non-canonical q
+    (which enters as 0)"""
+    if not oracle.ntt_set_vector(True):
+        pytest.skip("host without AVX-512: the scalar NTT is the only one")
+    qs = {0: oracle.Q, 1: oracle.BS, 2: oracle.P, 3: oracle.Q1, 4: oracle.Q2, 5: 268361729}   # 5: auxMod, crt.h:37
+    rng = np.random.default_rng(11)
+    try:
+        for modsel in range(6):
+            q = qs[modsel]
+            for trial in range(6):
+                a = rng.integers(0, q, N, dtype=np.uint64)
+                a[[0, 1, 4095]] = [0, q - 1, q]
+                a[rng.integers(0, N, 16)] = q
+                for fn in (oracle.ntt_forward, oracle.ntt_inverse):
+                    oracle.ntt_set_vector(True)
+                    lanes = fn(modsel, a)
+                    oracle.ntt_set_vector(False)
+                    scalar = fn(modsel, a)
+                    assert np.array_equal(lanes, scalar), (modsel, trial, fn.__name__)
+    finally:
+        oracle.ntt_set_vector(True)
 
 
 def test_ntt_negacyclic_convolution(oracle):
