@@ -111,6 +111,7 @@ def cpu_baseline(r, n1, reps):
     names = ("baby_steps", "db_pass", "giant_steps", "external_products", "packing")
     split = {n: round(float(np.median([st[i] for st in stages])) / 1e3, 2) for i, n in enumerate(names)}
     loop = "AVX-512 database loop (crt.cpp:660-768 shape)" if O.db_product_vectorised() else "scalar database loop"
+    loop += ", AVX-512 NTTs (8 lanes, as HEXL's AVX-512 path)" if O.ntt_set_vector(True) else ", scalar NTTs"
     single = {"value": round(1.0 / sec, 3), "unit": "queries/s", "cores": 1, "kind": "port",
               "sample": f"{reps} single-query online answers (median {sec * 1e3:.1f} ms) at r={r}, N1={n1} "
                         f"on oracle-generated test-pir inputs, seed 1, {loop}; host {model}, {ncpu} logical CPUs; "
