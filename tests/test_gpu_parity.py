@@ -383,20 +383,25 @@ def test_serving_loop_batches_span_databases(ctx, oracle, lanes):
     try:
         for multi in (1, 0):
             ctx.set("serve_multi", multi)
-            srv = kspir_amd.Server(ctx, max_batch=16, lanes=lanes)
+            # a cap above one database's 16 queries: a batch that starts
+            # with the r=8 database's queries has room for the smaller ones
+            srv = kspir_amd.Server(ctx, max_batch=64, lanes=lanes)
             try:
+                # four rounds of the twelve queries, so that batches form
+                # while earlier ones run whatever the relative speeds
                 tickets = []
-                for c in range(4):
-                    for g in range(3):
-                        cl, ks = clients[g]
-                        tickets.append((g, c, srv.submit(dbs[g][0], ks[c], cl[c]["query"])))
+                for rnd in range(4):
+                    for c in range(4):
+                        for g in range(3):
+                            cl, ks = clients[g]
+                            tickets.append((g, c, srv.submit(dbs[g][0], ks[c], cl[c]["query"])))
                 for g, c, t in tickets:
                     resp, ms = srv.wait(t)
                     assert ms > 0
                     assert np.array_equal(resp, want[g][c]), (multi, g, c)
                 st = srv.stats()
                 say(f"[gpu] serve_multi={multi}: {st}")
-                assert st["answered"] == 12 and st["pending"] == 0
+                assert st["answered"] == 48 and st["pending"] == 0
                 assert (st["spanning"] >= 1) if multi else (st["spanning"] == 0)
             finally:
                 srv.close()
