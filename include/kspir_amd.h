@@ -40,6 +40,11 @@ extern "C" {
 #define KSPIR_EINVAL (-1)   /* bad argument or shape */
 #define KSPIR_EHIP (-2)     /* HIP runtime error (message carries hipGetErrorString) */
 #define KSPIR_ENOMEM (-3)   /* device allocation failed */
+/* serving loop (kspir_server_*) */
+#define KSPIR_ECANCELED (-4) /* the request was cancelled before it ran */
+#define KSPIR_EBUDGET (-5)   /* one query of this shape needs more device workspace than a lane's budget */
+#define KSPIR_EBUSY (-6)     /* the request is already running or answered: too late to cancel */
+#define KSPIR_EINJECTED (-7) /* failure injected by the context option serve_fail_batches (failure-path tests) */
 
 #define KSPIR_N 4096
 
@@ -318,9 +323,20 @@ int kspir_server_create(kspir_context* ctx, int32_t max_batch, size_t workspace_
  * The budget is split evenly between the lanes. */
 int kspir_server_create_lanes(kspir_context* ctx, int32_t max_batch, size_t workspace_budget_bytes, int32_t lanes,
                               kspir_server** out);
+/* A query whose batch of one needs more device workspace than a lane's
+ * budget (kspir_batch_footprint_bytes with nq = 1) is rejected here with
+ * KSPIR_EBUDGET: no batch could ever hold it. */
 int kspir_server_submit(kspir_server* server, const kspir_database* db, const kspir_keys* keys,
                         const uint64_t* query, uint64_t* response, uint64_t* ticket);
+/* Returns the request's own result: KSPIR_ECANCELED after kspir_server_cancel,
+ * KSPIR_EINJECTED for a batch failed on purpose (serve_fail_batches), or the
+ * error of its batch's answer. */
 int kspir_server_wait(kspir_server* server, uint64_t ticket, double* latency_ms);
+/* Withdraws a request that no batch has taken yet: its wait then returns
+ * KSPIR_ECANCELED and its response buffer is left untouched.  KSPIR_EBUSY
+ * when a batch already holds it (it completes normally), KSPIR_EINVAL for a
+ * ticket never issued or already collected. */
+int kspir_server_cancel(kspir_server* server, uint64_t ticket);
 int kspir_server_stats(kspir_server* server, uint64_t* batches, uint64_t* answered, uint64_t* pending);
 /* batches so far that spanned more than one database */
 int kspir_server_spanning_batches(kspir_server* server, uint64_t* spanning);

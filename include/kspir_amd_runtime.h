@@ -70,6 +70,10 @@ int kspir_device_count(int* count);
  *                     eagerly (its batch sizes vary, so most batches would be
  *                     a new capture); 1: it captures and replays graphs as
  *                     the answer calls above do
+ *   "serve_fail_batches"  n >= 0 (default 0): the next n batches a serving
+ *                     loop runs on this context fail with KSPIR_EINJECTED,
+ *                     launching nothing (for tests of the failure path);
+ *                     set it before kspir_server_create
  * Setting an option waits for the context stream and drops captured graphs. */
 int kspir_context_set(kspir_context* ctx, const char* key, int64_t value);
 int kspir_context_get(kspir_context* ctx, const char* key, int64_t* value);
@@ -94,6 +98,11 @@ int kspir_autotune(kspir_context* ctx, const kspir_database* db, const kspir_key
  * 64).  The serving loop admits each batch by its exact footprint for the
  * batch size and the lane's options. */
 int kspir_query_footprint_bytes(int32_t r, int32_t n1, size_t* bytes);
+/* device bytes of one batch of nq queries of shape (r, N1) under ctx's
+ * options (NULL: the defaults): the figure the serving loop admits a batch
+ * by.  It is not monotonic in nq: a few queries of a large database can take
+ * the unchunked path, whose buffers hold every block's rows per query. */
+int kspir_batch_footprint_bytes(const kspir_context* ctx, int32_t r, int32_t n1, int32_t nq, size_t* bytes);
 int kspir_device_alloc(kspir_context* ctx, size_t bytes, void** out);
 int kspir_device_free(kspir_context* ctx, void* ptr);
 int kspir_memcpy_h2d(kspir_context* ctx, void* dst, const void* src, size_t bytes);

@@ -22,7 +22,12 @@ _vp = ctypes.c_void_p
 
 
 class KspirError(RuntimeError):
-    pass
+    """a library error; .code is the C-ABI return code (KSPIR_* in include/kspir_amd.h)"""
+    code = None
+
+
+# return codes (include/kspir_amd.h)
+EINVAL, EHIP, ENOMEM, ECANCELED, EBUDGET, EBUSY, EINJECTED = -1, -2, -3, -4, -5, -6, -7
 
 
 def _load():
@@ -46,7 +51,8 @@ def _load():
                  "kspir_server_create_lanes", "kspir_gen_giant_keys_from_offline", "kspir_answer_device_noalloc",
                  "kspir_inverse_encode", "kspir_time_db_pass_batch", "kspir_answer_multi",
                  "kspir_answer_multi_device", "kspir_time_answer_multi", "kspir_database_compact",
-                 "kspir_database_save", "kspir_database_load", "kspir_server_spanning_batches"):
+                 "kspir_database_save", "kspir_database_load", "kspir_server_spanning_batches",
+                 "kspir_server_cancel", "kspir_batch_footprint_bytes"):
         getattr(L, name).restype = ctypes.c_int
     L.kspir_autotune.argtypes = [_vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                  ctypes.POINTER(ctypes.c_int32), ctypes.c_char_p, ctypes.c_size_t]
@@ -75,12 +81,15 @@ EXPORTS = (
     "kspir_gen_giant_keys_from_offline", "kspir_answer_device_noalloc", "kspir_inverse_encode",
     "kspir_time_db_pass_batch", "kspir_answer_multi", "kspir_answer_multi_device", "kspir_time_answer_multi",
     "kspir_database_compact", "kspir_database_save", "kspir_database_load", "kspir_server_spanning_batches",
+    "kspir_server_cancel", "kspir_batch_footprint_bytes",
 )
 
 
 def _check(rc):
     if rc != 0:
-        raise KspirError(f"kspir error {rc}: {lib.kspir_last_error().decode()}")
+        e = KspirError(f"kspir error {rc}: {lib.kspir_last_error().decode()}")
+        e.code = rc
+        raise e
 
 
 def _p64(a):
@@ -204,6 +213,14 @@ def query_footprint(r, n1):
     return out.value
 
 
+def batch_footprint(r, n1, nq, ctx=None):
+    """device bytes of a batch of nq queries of shape (r, N1) under ctx's
+    options (None: the defaults), as the serving loop admits it"""
+    out = ctypes.c_size_t(0)
+    _check(lib.kspir_batch_footprint_bytes(ctx.h if ctx is not None else None, r, n1, nq, ctypes.byref(out)))
+    return out.value
+
+
 class Server:
     """Continuous-batching serving loop on a context (see kspir_server_* in
     include/kspir_amd.h).  While it runs, the context is driven by its worker
@@ -228,9 +245,17 @@ class Server:
 
     def wait(self, ticket):
         ms = ctypes.c_double(0)
-        _check(lib.kspir_server_wait(self.h, ctypes.c_uint64(ticket), ctypes.byref(ms)))
-        q, resp = self._live.pop(ticket)
+        rc = lib.kspir_server_wait(self.h, ctypes.c_uint64(ticket), ctypes.byref(ms))
+        # the library forgets the ticket whatever its result (an unknown
+        # ticket was never kept here)
+        q, resp = self._live.pop(ticket, (None, None))
+        _check(rc)
         return resp, ms.value
+
+    def cancel(self, ticket):
+        """withdraw a request no batch has taken yet (its wait then raises
+        KspirError with code ECANCELED); raises with EBUSY when too late"""
+        _check(lib.kspir_server_cancel(self.h, ctypes.c_uint64(ticket)))
 
     def stats(self):
         b, a, p = ctypes.c_uint64(0), ctypes.c_uint64(0), ctypes.c_uint64(0)

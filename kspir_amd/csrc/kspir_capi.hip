@@ -137,6 +137,7 @@ struct kspir_context {
     int64_t pack_fused_min_wgs = 0; // a packing level is fused only with at least this many (pair, query) workgroups
     int serve_multi = 0;            // the serving loop puts queries of several databases in one batch (one launch per stage)
     int serve_graphs = 0;           // the serving loop replays captured graphs (1) or launches eagerly (0)
+    int64_t serve_fail_batches = 0; // the serving loop's next batches that fail on purpose (KSPIR_EINJECTED)
     // captured pipelines: one hipGraph per (database, key table, batch size,
     // query / response buffers, stream); dropped whenever a buffer they name
     // is reallocated or freed, or an option changes
@@ -298,12 +299,16 @@ int build_tables(kspir_context* c) {
 constexpr int64_t kFusedFillWgs = 2048;
 
 // whether a batch of nq queries over r blocks takes the fused giant-step
-// kernel: from fused_min_batch queries, or whenever its (query, block)
-// pairs fill the GPU.  The unfused kernels keep every block's giant rows
-// (gcoef, gks) per query, 2.6 GiB at r = 1024, so a large database takes the
-// fused path even for a few queries (r 1024 from 2 queries, r 256 from 8).
+// kernel: from fused_min_batch queries, whenever its (query, block) pairs
+// fill the GPU, and from 4 queries at 64 blocks or more.  The unfused
+// kernels keep every block's giant rows (gcoef, gks) per query, 2.6 GiB at
+// r = 1024, so a large database takes the fused path even for a few queries
+// (r 1024 from 2).  Measured at 1 GB (r 64): 4 queries 2.02 ms fused against
+// 2.09 unfused, 8 queries 2.17 against 2.71; at 4 GB (r 256): 4 queries 7.25
+// against 7.10 ms, at a ninth of the workspace; 2 and 3 queries stay
+// unfused there (5.8 against 7.0 ms for 2).
 bool fused_batch(const kspir_context* c, int r, int nq) {
-    return nq >= c->fused_min_batch || (int64_t)r * nq >= kFusedFillWgs;
+    return nq >= c->fused_min_batch || (int64_t)r * nq >= kFusedFillWgs || (r >= 64 && nq >= 4);
 }
 
 // Blocks per chunk of the batched pipeline for nq queries of shape (r, n1),
@@ -1850,6 +1855,9 @@ int kspir_context_set(kspir_context* c, const char* key, int64_t value) {
         c->serve_multi = value != 0;
     } else if (k == "serve_graphs") {
         c->serve_graphs = value != 0;
+    } else if (k == "serve_fail_batches") {
+        if (value < 0) return fail(KSPIR_EINVAL, "serve_fail_batches must be >= 0");
+        c->serve_fail_batches = value;
     } else {
         return fail(KSPIR_EINVAL, "unknown option " + k);
     }
@@ -1870,6 +1878,7 @@ int kspir_context_get(kspir_context* c, const char* key, int64_t* value) {
     else if (k == "db_waves") *value = c->db_waves;
     else if (k == "serve_multi") *value = c->serve_multi;
     else if (k == "serve_graphs") *value = c->serve_graphs;
+    else if (k == "serve_fail_batches") *value = c->serve_fail_batches;
     else if (k == "giant_wgs_per_cu") *value = c->giant_wgs_per_cu;
     else if (k == "giant_dma") *value = c->giant_dma;
     else if (k == "pack_fused_min_wgs") *value = c->pack_fused_min_wgs;
@@ -2159,10 +2168,15 @@ struct kspir_server {
             // launched eagerly (the two measure alike at 4 and 16 GB)
             const int graphs = L.ctx->use_graphs;
             L.ctx->use_graphs = graphs && L.ctx->serve_graphs;
-            const int rc = dbs.size() == 1
-                               ? kspir_answer_batch(L.ctx, dbs[0], keys.data(), nq, L.qh, L.rh)
-                               : kspir_answer_multi(L.ctx, (int32_t)dbs.size(), dbs.data(), seg_n.data(), keys.data(),
-                                                    L.qh, L.rh);
+            int rc;
+            if (L.ctx->serve_fail_batches > 0) {   // a failure forced from the host: nothing is launched
+                --L.ctx->serve_fail_batches;
+                rc = KSPIR_EINJECTED;
+            } else {
+                rc = dbs.size() == 1 ? kspir_answer_batch(L.ctx, dbs[0], keys.data(), nq, L.qh, L.rh)
+                                     : kspir_answer_multi(L.ctx, (int32_t)dbs.size(), dbs.data(), seg_n.data(),
+                                                          keys.data(), L.qh, L.rh);
+            }
             L.ctx->use_graphs = graphs;
             const auto t1 = std::chrono::steady_clock::now();
             {
@@ -2662,6 +2676,15 @@ int kspir_query_footprint_bytes(int32_t r, int32_t n1, size_t* bytes) {
     return KSPIR_OK;
 }
 
+int kspir_batch_footprint_bytes(const kspir_context* c, int32_t r, int32_t n1, int32_t nq, size_t* bytes) {
+    if (!bytes || nq < 1 || !pow2(r) || r > kN || !pow2(n1) || n1 < 32 || n1 > kN / 2)
+        return fail(KSPIR_EINVAL, "bad shape or batch size");
+    static kspir_context defaults;   // the default options, no device state
+    // the footprint only reads the options (no buffer is laid out)
+    *bytes = batch_footprint_bytes(const_cast<kspir_context*>(c ? c : &defaults), r, n1, nq);
+    return KSPIR_OK;
+}
+
 int kspir_server_create_lanes(kspir_context* c, int32_t max_batch, size_t workspace_budget_bytes, int32_t lanes,
                               kspir_server** out) {
     if (!c || !out || max_batch < 1 || lanes < 1 || lanes > 8) return fail(KSPIR_EINVAL, "bad server arguments");
@@ -2709,6 +2732,7 @@ int kspir_server_create_lanes(kspir_context* c, int32_t max_batch, size_t worksp
             L.ctx->db_waves = c->db_waves;
             L.ctx->serve_multi = c->serve_multi;
             L.ctx->serve_graphs = c->serve_graphs;
+            L.ctx->serve_fail_batches = c->serve_fail_batches;
         }
         hipError_t e = hipHostMalloc(&L.qh, (size_t)max_batch * 4 * kN * 8, hipHostMallocDefault);
         if (e == hipSuccess) e = hipHostMalloc(&L.rh, (size_t)max_batch * 2 * kN * 8, hipHostMallocDefault);
@@ -2734,6 +2758,11 @@ int kspir_server_submit(kspir_server* s, const kspir_database* db, const kspir_k
     if (!s || !query || !response || !ticket) return fail(KSPIR_EINVAL, "null argument");
     int rc = check_pair(s->ctx, db, keys);
     if (rc) return rc;
+    const size_t one = batch_footprint_bytes(s->ctx, db->r, db->n1, 1);
+    if (one > s->budget)
+        return fail(KSPIR_EBUDGET, "one query of this database needs " + std::to_string(one) +
+                                       " bytes of device workspace, above the lane budget of " +
+                                       std::to_string(s->budget));
     {
         std::lock_guard<std::mutex> lock(s->mu);
         if (s->stop) return fail(KSPIR_EINVAL, "server is stopping");
@@ -2756,7 +2785,25 @@ int kspir_server_wait(kspir_server* s, uint64_t ticket, double* latency_ms) {
     s->done.erase(ticket);
     s->live.erase(ticket);
     if (latency_ms) *latency_ms = d.ms;
+    if (d.rc == KSPIR_ECANCELED) return fail(d.rc, "request was cancelled");
+    if (d.rc == KSPIR_EINJECTED) return fail(d.rc, "batch failed on purpose (serve_fail_batches)");
     if (d.rc != KSPIR_OK) return fail(d.rc, "batch answer failed in the serving loop");
+    return KSPIR_OK;
+}
+
+int kspir_server_cancel(kspir_server* s, uint64_t ticket) {
+    if (!s) return fail(KSPIR_EINVAL, "null server");
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        if (!s->live.count(ticket)) return fail(KSPIR_EINVAL, "unknown ticket, or ticket already collected");
+        auto it = std::find_if(s->pending.begin(), s->pending.end(),
+                               [&](const kspir_server::Req& q) { return q.id == ticket; });
+        if (it == s->pending.end()) return fail(KSPIR_EBUSY, "request already taken by a batch");
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - it->t0).count();
+        s->pending.erase(it);
+        s->done[ticket] = kspir_server::Done{KSPIR_ECANCELED, ms};
+    }
+    s->cv_done.notify_all();
     return KSPIR_OK;
 }
 

@@ -250,6 +250,79 @@ def test_serving_rejects_double_wait(ctx, oracle):
         db.close()
 
 
+def test_serving_failure_paths(ctx, oracle):
+    """failures forced from the host, every kernel exiting normally: a query
+    no batch could hold is rejected at submission (EBUDGET); requests that
+    no batch has taken yet are withdrawn (their waits raise ECANCELED, the
+    others answer correctly), a cancel too late or of an unknown ticket is an
+    error; a batch failed on purpose (serve_fail_batches) fails only its own
+    requests, and the loop answers the next ones"""
+    import kspir_amd
+    say("[gpu] serving loop failure paths")
+    r, n1 = 2, 128
+    db = ctx.database(oracle.gen_db(r, n1, seed=6), r, n1)
+    cl = oracle.gen_client(r, n1, 78, 11, 0)
+    keys = ctx.keys(r, n1, cl["baby"], cl["giant"], cl["packing"], cl["rgsw"], oracle.perm_matrix(n1))
+    want = ctx.answer(db, keys, cl["query"])
+    one = kspir_amd.batch_footprint(r, n1, 1, ctx)
+    try:
+        # admission: a budget below one query's workspace
+        srv = kspir_amd.Server(ctx, max_batch=4, budget_bytes=one // 2)
+        try:
+            with pytest.raises(kspir_amd.KspirError) as e:
+                srv.submit(db, keys, cl["query"])
+            assert e.value.code == kspir_amd.EBUDGET
+        finally:
+            srv.close()
+        # cancellation: one query per batch, so the late submissions queue
+        srv = kspir_amd.Server(ctx, max_batch=1)
+        try:
+            tickets = [srv.submit(db, keys, cl["query"]) for _ in range(16)]
+            cancelled = []
+            for t in tickets[-6:]:
+                try:
+                    srv.cancel(t)
+                    cancelled.append(t)
+                except kspir_amd.KspirError as e:
+                    assert e.code == kspir_amd.EBUSY   # already taken by a batch
+            assert len(cancelled) >= 2
+            for t in tickets:
+                if t in cancelled:
+                    with pytest.raises(kspir_amd.KspirError) as e:
+                        srv.wait(t)
+                    assert e.value.code == kspir_amd.ECANCELED
+                else:
+                    resp, _ = srv.wait(t)
+                    assert np.array_equal(resp, want)
+            st = srv.stats()
+            assert st["answered"] == 16 - len(cancelled) and st["pending"] == 0
+            # collected, or never issued
+            with pytest.raises(kspir_amd.KspirError) as e:
+                srv.cancel(tickets[0])
+            assert e.value.code == kspir_amd.EINVAL
+            with pytest.raises(kspir_amd.KspirError) as e:
+                srv.cancel(tickets[-1] + 1000)
+            assert e.value.code == kspir_amd.EINVAL
+        finally:
+            srv.close()
+        # a failed batch: its own requests fail, the next ones are answered
+        ctx.set("serve_fail_batches", 1)
+        srv = kspir_amd.Server(ctx, max_batch=1)
+        try:
+            with pytest.raises(kspir_amd.KspirError) as e:
+                srv.wait(srv.submit(db, keys, cl["query"]))
+            assert e.value.code == kspir_amd.EINJECTED
+            resp, _ = srv.wait(srv.submit(db, keys, cl["query"]))
+            assert np.array_equal(resp, want)
+        finally:
+            srv.close()
+        assert ctx.get("serve_fail_batches") == 0
+    finally:
+        ctx.set("serve_fail_batches", 0)
+        keys.close()
+        db.close()
+
+
 @pytest.mark.parametrize("chunk,sweep", [(32, 1 << 30), (1, 4), (2, 1 << 30)])
 def test_multi_database_batch(ctx, oracle, chunk, sweep):
     """one batch spanning databases of r 1, 4 and 2 (in that order) with 3, 5
